@@ -1,0 +1,80 @@
+"""Build recipe for the native parts of energyspincurrents_amd.
+
+Produces, in-tree:
+  energyspincurrents_amd/libesc.so   HIP kernels + C ABI (include/esc.h), gfx950
+  energyspincurrents_amd/LanczosCond drop-in CLI of the reference driver
+  oracle/libesc_oracle.so            CPU oracle (test infrastructure, built only
+                                     so that tests and the baseline can load it)
+
+Usage: python -m energyspincurrents_amd.build  [--no-cli] [--no-oracle]
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+INC = os.path.join(ROOT, "include")
+LIB = os.path.join(PKG, "libesc.so")
+CLI = os.path.join(PKG, "LanczosCond")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+# Packed FP32 instructions are disabled for every device pass.
+DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", f"-I{INC}", f"-I{CSRC}",
+          "-Wno-unused-command-line-argument"]
+
+LIB_SOURCES = ["esc_kernels.hip", "esc_capi.cpp", "esc_host.cpp"]
+CLI_SOURCES = ["facade/lanczoscond_main.cpp"]
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_lib():
+    objs = []
+    for src in LIB_SOURCES:
+        path = os.path.join(CSRC, src)
+        obj = os.path.join(CSRC, os.path.splitext(os.path.basename(src))[0] + ".o")
+        lang = ["-x", "hip"] if src.endswith(".hip") else []
+        _run([HIPCC, *COMMON, *DEVICE_FLAGS, *lang, "-c", path, "-o", obj])
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, *COMMON, "-shared", *objs, "-o", tmp])
+    os.replace(tmp, LIB)
+    for o in objs:
+        os.remove(o)
+    return LIB
+
+
+def build_cli():
+    srcs = [os.path.join(CSRC, s) for s in CLI_SOURCES]
+    if not all(os.path.exists(s) for s in srcs):
+        return None
+    tmp = CLI + ".tmp"
+    _run([HIPCC, "-O2", "-std=c++17", f"-I{INC}", f"-I{CSRC}", f"-I{os.path.join(CSRC, 'facade')}",
+          *srcs, "-o", tmp, f"-L{PKG}", "-lesc", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, CLI)
+    return CLI
+
+
+def build_oracle():
+    _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+
+
+def build(cli=True, oracle=True):
+    build_lib()
+    if cli:
+        build_cli()
+    if oracle:
+        build_oracle()
+
+
+if __name__ == "__main__":
+    build(cli="--no-cli" not in sys.argv, oracle="--no-oracle" not in sys.argv)

@@ -1,0 +1,1016 @@
+// esc_capi.cpp -- implementation of the C ABI declared in include/esc.h.
+//
+// Host orchestration only: device tables, workspaces, the Lanczos loop and the
+// FP64 small solves.  All vector work runs in the HIP kernels of
+// esc_kernels.hip; nothing here falls back to a CPU path.
+#include "esc.h"
+#include "esc_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <memory>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+}  // namespace
+
+namespace esc {
+void set_error(const char *msg) { g_err = msg; }
+void clear_error() { g_err.clear(); }
+}  // namespace esc
+
+namespace {
+
+struct EscError : std::runtime_error {
+    int code;
+    EscError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+inline void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw EscError(ESC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename F>
+int guarded(F &&f) {
+    try {
+        f();
+        g_err.clear();
+        return ESC_OK;
+    } catch (const EscError &e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_err = "host allocation failed";
+        return ESC_ERR_ALLOC;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return ESC_ERR_NUMERIC;
+    }
+}
+
+// ------------------------------------------------------------------------
+// FP64 symmetric tridiagonal eigensolver: implicit QL with Wilkinson-type
+// shifts (Golub & Van Loan, Matrix Computations, sec. 8.3), eigenvectors
+// accumulated by Givens rotations, results sorted ascending.
+// ------------------------------------------------------------------------
+int tridiag_ql(int n, const double *a, const double *b, double *evals, double *Z) {
+    std::vector<double> d(a, a + n), e(n, 0.0);
+    for (int i = 0; i + 1 < n; ++i) e[i] = b[i + 1];
+    const bool vec = Z != nullptr;
+    std::vector<double> V;
+    if (vec) {
+        V.assign((size_t)n * n, 0.0);
+        for (int i = 0; i < n; ++i) V[(size_t)i * n + i] = 1.0;   // column-major, V[col*n + row]
+    }
+    for (int l = 0; l < n; ++l) {
+        int iter = 0;
+        for (;;) {
+            int m = l;
+            for (; m < n - 1; ++m) {
+                const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+                if (std::fabs(e[m]) <= std::numeric_limits<double>::epsilon() * dd) break;
+            }
+            if (m == l) break;
+            if (++iter > 60) return -1;
+            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            double r = std::hypot(g, 1.0);
+            g = d[m] - d[l] + e[l] / (g + std::copysign(r, g));
+            double s = 1.0, c = 1.0, p = 0.0;
+            int i = m - 1;
+            bool early = false;
+            for (; i >= l; --i) {
+                const double f = s * e[i], bb = c * e[i];
+                r = std::hypot(f, g);
+                e[i + 1] = r;
+                if (r == 0.0) {
+                    d[i + 1] -= p;
+                    e[m] = 0.0;
+                    early = true;
+                    break;
+                }
+                s = f / r;
+                c = g / r;
+                g = d[i + 1] - p;
+                r = (d[i] - g) * s + 2.0 * c * bb;
+                p = s * r;
+                d[i + 1] = g + p;
+                g = c * r - bb;
+                if (vec) {
+                    double *ci = &V[(size_t)i * n], *cj = &V[(size_t)(i + 1) * n];
+                    for (int k = 0; k < n; ++k) {
+                        const double t = cj[k];
+                        cj[k] = s * ci[k] + c * t;
+                        ci[k] = c * ci[k] - s * t;
+                    }
+                }
+            }
+            if (early) continue;
+            d[l] -= p;
+            e[l] = g;
+            e[m] = 0.0;
+        }
+    }
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return d[x] < d[y]; });
+    for (int i = 0; i < n; ++i) evals[i] = d[idx[i]];
+    if (vec)
+        for (int j = 0; j < n; ++j) std::memcpy(Z + (size_t)j * n, &V[(size_t)idx[j] * n], sizeof(double) * n);
+    return 0;
+}
+
+// lowest eigenvalue only (used for the per-iteration convergence test)
+double tridiag_lowest(int n, const double *a, const double *b) {
+    std::vector<double> ev(n);
+    if (tridiag_ql(n, a, b, ev.data(), nullptr) != 0) throw EscError(ESC_ERR_NUMERIC, "tridiagonal QL did not converge");
+    return ev[0];
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------
+// Context
+// ------------------------------------------------------------------------
+struct esc_ctx {
+    int n_sites = 0;
+    int precision = ESC_C128;
+    uint64_t dim = 0;
+    size_t esz = 16;
+    int device = 0;
+    esc_model model{};
+    std::vector<int32_t> bonds[3];
+    std::vector<int32_t> neigh;
+    esc::HTables htab{};
+    esc::HTables *d_htab = nullptr;
+    std::map<int64_t, esc::CurrentTable *> d_cur;   // cached per (op, site)
+    double *d_partials = nullptr;                   // reduction workspace
+    size_t partials_cap = 0;
+    double *d_result = nullptr;                     // small device results
+    size_t result_cap = 0;
+    void *d_vecs = nullptr;                         // Lanczos basis / work vectors
+    size_t vecs_cap = 0;                            // in vectors
+    int basis_n = 0;                                // vectors valid in the kept basis
+    std::vector<double> Z;                          // last tridiagonal eigenvectors
+    std::vector<double> evals;                      // last Ritz values (ascending)
+    int z_n = 0;
+    bool basis_kept = false;
+    void *d_aux = nullptr;                          // dynamics work vectors
+    size_t aux_cap = 0;
+    esc::CurrentTable *d_combo = nullptr;           // combined-operator table
+
+    bool c64() const { return precision == ESC_C64; }
+
+    ~esc_ctx() {
+        if (d_htab) (void)hipFree(d_htab);
+        for (auto &kv : d_cur) (void)hipFree(kv.second);
+        if (d_partials) (void)hipFree(d_partials);
+        if (d_result) (void)hipFree(d_result);
+        if (d_vecs) (void)hipFree(d_vecs);
+        if (d_aux) (void)hipFree(d_aux);
+        if (d_combo) (void)hipFree(d_combo);
+    }
+    void ensure_aux(size_t nvec) {
+        if (nvec <= aux_cap) return;
+        if (d_aux) hip_check(hipFree(d_aux), "hipFree");
+        d_aux = nullptr;
+        aux_cap = 0;
+        if (hipMalloc(&d_aux, nvec * dim * esz) != hipSuccess)
+            throw EscError(ESC_ERR_ALLOC, "cannot allocate " + std::to_string(nvec) + " work vectors on the device");
+        aux_cap = nvec;
+    }
+    void *aux(size_t m) { return (char *)d_aux + m * dim * esz; }
+
+    void ensure_partials(size_t n) {
+        if (n <= partials_cap) return;
+        if (d_partials) hip_check(hipFree(d_partials), "hipFree");
+        d_partials = nullptr;
+        if (hipMalloc(&d_partials, n * sizeof(double)) != hipSuccess) throw EscError(ESC_ERR_ALLOC, "workspace allocation failed");
+        partials_cap = n;
+    }
+    void ensure_result(size_t n) {
+        if (n <= result_cap) return;
+        if (d_result) hip_check(hipFree(d_result), "hipFree");
+        d_result = nullptr;
+        if (hipMalloc(&d_result, n * sizeof(double)) != hipSuccess) throw EscError(ESC_ERR_ALLOC, "workspace allocation failed");
+        result_cap = n;
+    }
+    void ensure_vecs(size_t nvec) {
+        if (nvec <= vecs_cap) return;
+        if (d_vecs) hip_check(hipFree(d_vecs), "hipFree");
+        d_vecs = nullptr;
+        vecs_cap = 0;
+        basis_kept = false;
+        if (hipMalloc(&d_vecs, nvec * dim * esz) != hipSuccess)
+            throw EscError(ESC_ERR_ALLOC, "cannot allocate " + std::to_string(nvec) + " vectors of " +
+                                              std::to_string(dim * esz) + " bytes on the device");
+        vecs_cap = nvec;
+    }
+    void *vec(size_t m) { return (char *)d_vecs + m * dim * esz; }
+};
+
+namespace {
+
+void build_tables(esc_ctx *c) {
+    const esc_model &m = c->model;
+    esc::HTables &t = c->htab;
+    std::memset(&t, 0, sizeof(t));
+    const int N = m.n_sites;
+    t.n_sites = (uint32_t)N;
+    for (int k = 0; k < esc::kMaxSites; ++k) t.xpart[k] = t.ypart[k] = -1;
+    // per-family bond coefficients (gather form of src/Hamiltonian.h:113-158;
+    // J and Gamma are extensions)
+    const double J4 = 0.25 * m.J, G4 = 0.25 * m.Gamma;
+    t.p0[0] = 0.25 * m.K[0] + J4; t.p1[0] = -J4;                t.q[0] = 0.0;
+    t.p0[1] = J4;                 t.p1[1] = -0.25 * m.K[1] - J4; t.q[1] = 0.0;
+    t.p0[2] = J4;                 t.p1[2] = -J4;                t.q[2] = -G4;
+    t.kz4 = 0.25 * m.K[2];
+    t.j4 = J4;
+    t.bz = m.B[2];
+    t.bx2 = 0.5 * m.B[0];
+    t.by2 = 0.5 * m.B[1];
+    t.g4 = G4;
+    uint32_t nb = 0, nz = 0, nj = 0;
+    for (int f = 0; f < 3; ++f) {
+        const bool offdiag = (f == 0 && (m.K[0] != 0.0 || m.J != 0.0)) || (f == 1 && (m.K[1] != 0.0 || m.J != 0.0)) ||
+                             (f == 2 && (m.J != 0.0 || m.Gamma != 0.0));
+        for (int b = 0; b < m.n_bonds[f]; ++b) {
+            const int i = c->bonds[f][2 * b], j = c->bonds[f][2 * b + 1];
+            const uint32_t packed = (uint32_t)i | ((uint32_t)j << 8);
+            if (offdiag) {
+                if (nb >= (uint32_t)esc::kMaxBonds) throw EscError(ESC_ERR_UNSUPPORTED, "too many bonds");
+                t.bond[nb++] = packed | ((uint32_t)f << 16);
+            }
+            if (f == 2) t.zdiag[nz++] = packed;
+            if (m.J != 0.0) {
+                if (nj >= (uint32_t)esc::kMaxBonds) throw EscError(ESC_ERR_UNSUPPORTED, "too many bonds");
+                t.jdiag[nj++] = packed;
+            }
+            if (m.Gamma != 0.0 && f < 2) {
+                int32_t *part = f == 0 ? t.xpart : t.ypart;
+                if (part[i] >= 0 || part[j] >= 0)
+                    throw EscError(ESC_ERR_UNSUPPORTED, "Gamma needs at most one x and one y bond per site");
+                part[i] = j;
+                part[j] = i;
+            }
+        }
+    }
+    t.n_bonds = nb;
+    t.n_zdiag = nz;
+    t.n_jdiag = nj;
+    t.has_gamma_flip = m.Gamma != 0.0 ? 1u : 0u;
+    t.n_flip = (m.B[0] != 0.0 || m.B[1] != 0.0 || m.Gamma != 0.0) ? (uint32_t)N : 0u;
+}
+
+// Pauli-string table for one operator (src/Observables.h:161-389,429-521)
+void add_pair(esc::CurrentTable &t, std::initializer_list<esc::OpFactor> a, double wa,
+              std::initializer_list<esc::OpFactor> b, double wb) {
+    // weights are real here; complex scale factors are applied by scale_table
+    if (t.n_pairs >= esc::kMaxCurrentPairs) throw EscError(ESC_ERR_UNSUPPORTED, "too many current terms");
+    esc::CurrentPair &p = t.pair[t.n_pairs++];
+    std::memset(&p, 0, sizeof(p));
+    p.na = (int)a.size();
+    p.nb = (int)b.size();
+    int q = 0;
+    for (auto &f : a) p.fa[q++] = f;
+    q = 0;
+    for (auto &f : b) p.fb[q++] = f;
+    p.wa_re = wa;
+    p.wa_im = 0.0;
+    p.wb_re = wb;
+    p.wb_im = 0.0;
+}
+
+// append src scaled by the complex factor (cr + i ci) to dst
+void append_scaled(esc::CurrentTable &dst, const esc::CurrentTable &src, double cr, double ci) {
+    for (int k = 0; k < src.n_pairs; ++k) {
+        if (dst.n_pairs >= esc::kMaxCurrentPairs) throw EscError(ESC_ERR_UNSUPPORTED, "too many current terms");
+        esc::CurrentPair p = src.pair[k];
+        const double ar = p.wa_re, ai = p.wa_im, br = p.wb_re, bi = p.wb_im;
+        if (!(cr == 1.0 && ci == 0.0)) {
+            p.wa_re = cr * ar - ci * ai;
+            p.wa_im = cr * ai + ci * ar;
+            p.wb_re = cr * br - ci * bi;
+            p.wb_im = cr * bi + ci * br;
+        }
+        dst.pair[dst.n_pairs++] = p;
+    }
+}
+
+int nb_of(const esc_ctx *c, int site, int col) {
+    if (c->neigh.empty()) throw EscError(ESC_ERR_ARG, "current operators need the neighbour table (model.neigh)");
+    const int j = c->neigh[(size_t)site * 3 + col];
+    if (j < 0 || j >= c->n_sites) throw EscError(ESC_ERR_ARG, "site " + std::to_string(site) + " has no neighbour in column " + std::to_string(col));
+    return j;
+}
+
+// Factor lists are in application order: the factor acting first on the ket
+// comes first (ApplyOp chains, e.g. src/Observables.h:181-183).
+void add_site_current(const esc_ctx *c, esc::CurrentTable &t, int op, int si) {
+    const double Kx = c->model.K[0], Ky = c->model.K[1], Kz = c->model.K[2];
+    enum { X = 0, Y = 1, Z = 2 };
+    switch (op) {
+    case ESC_OP_SX: add_pair(t, {{X, si}}, 1.0, {}, 0.0); break;
+    case ESC_OP_SY: add_pair(t, {{Y, si}}, 1.0, {}, 0.0); break;
+    case ESC_OP_SZ: add_pair(t, {{Z, si}}, 1.0, {}, 0.0); break;
+    case ESC_OP_JSZ: {  // Kyy Sx_i Sy_y - Kxx Sy_i Sx_x   (:181-198)
+        const int xn = nb_of(c, si, 2), yn = nb_of(c, si, 1);
+        (void)nb_of(c, si, 0);
+        add_pair(t, {{Y, yn}, {X, si}}, Ky, {{X, xn}, {Y, si}}, Kx);
+        break;
+    }
+    case ESC_OP_JSY: {  // Kxx Sz_i Sx_x - Kzz Sx_i Sz_z   (:222-239)
+        const int xn = nb_of(c, si, 2), zn = nb_of(c, si, 0);
+        (void)nb_of(c, si, 1);
+        add_pair(t, {{X, xn}, {Z, si}}, Kx, {{Z, zn}, {X, si}}, Kz);
+        break;
+    }
+    case ESC_OP_JSX: {  // Kzz Sy_i Sz_z - Kyy Sz_i Sy_y   (:263-280)
+        const int yn = nb_of(c, si, 1), zn = nb_of(c, si, 0);
+        (void)nb_of(c, si, 2);
+        add_pair(t, {{Z, zn}, {Y, si}}, Kz, {{Y, yn}, {Z, si}}, Ky);
+        break;
+    }
+    case ESC_OP_JEX: {  // (KxKz) Sx_i Sy_ix Sz_ixpz - (KxKy) Sx_i Sz_ix Sy_ixpy   (:362-388)
+        const int ix = nb_of(c, si, 2);
+        const int ixpy = nb_of(c, ix, 1), ixpz = nb_of(c, ix, 0);
+        add_pair(t, {{Z, ixpz}, {Y, ix}, {X, si}}, Kx * Kz, {{Y, ixpy}, {Z, ix}, {X, si}}, Kx * Ky);
+        break;
+    }
+    case ESC_OP_JEY: {  // (KxKy) Sy_i Sz_iy Sx_iypx - (KyKz) Sy_i Sx_iy Sz_iypz   (:325-353)
+        const int iy = nb_of(c, si, 1);
+        const int iypx = nb_of(c, iy, 2), iypz = nb_of(c, iy, 0);
+        add_pair(t, {{X, iypx}, {Z, iy}, {Y, si}}, Kx * Ky, {{Z, iypz}, {X, iy}, {Y, si}}, Ky * Kz);
+        break;
+    }
+    default: throw EscError(ESC_ERR_ARG, "unknown operator " + std::to_string(op));
+    }
+}
+
+void host_current_table(const esc_ctx *c, int op, int site, esc::CurrentTable *t) {
+    std::memset(t, 0, sizeof(esc::CurrentTable));
+    if (site == ESC_SITE_TOTAL) {
+        if (op <= ESC_OP_SZ) throw EscError(ESC_ERR_ARG, "single-site operators need a site");
+        // Js totals over all sites (:664-699); Jex over even, Jey over odd sites (:448-470)
+        const int start = op == ESC_OP_JEY ? 1 : 0, step = (op == ESC_OP_JEX || op == ESC_OP_JEY) ? 2 : 1;
+        for (int s = start; s < c->n_sites; s += step) add_site_current(c, *t, op, s);
+    } else {
+        if (site < 0 || site >= c->n_sites) throw EscError(ESC_ERR_ARG, "site out of range");
+        add_site_current(c, *t, op, site);
+    }
+}
+
+esc::CurrentTable *current_table(esc_ctx *c, int op, int site) {
+    const int64_t key = (int64_t)op * 1000 + (site + 1);
+    auto it = c->d_cur.find(key);
+    if (it != c->d_cur.end()) return it->second;
+    auto t = std::make_unique<esc::CurrentTable>();
+    host_current_table(c, op, site, t.get());
+    esc::CurrentTable *d = nullptr;
+    if (hipMalloc(&d, sizeof(esc::CurrentTable)) != hipSuccess) throw EscError(ESC_ERR_ALLOC, "table allocation failed");
+    hip_check(hipMemcpy(d, t.get(), sizeof(esc::CurrentTable), hipMemcpyHostToDevice), "hipMemcpy");
+    c->d_cur[key] = d;
+    return d;
+}
+
+void check_ctx(const esc_ctx *c) {
+    if (!c) throw EscError(ESC_ERR_ARG, "null context");
+}
+void check_ptr(const void *p, const char *name) {
+    if (!p) throw EscError(ESC_ERR_ARG, std::string("null pointer: ") + name);
+}
+
+void sync_get(hipStream_t s, void *dst, const void *src, size_t n) {
+    hip_check(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, s), "hipMemcpyAsync D2H");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+void device_dot(esc_ctx *c, const void *x, const void *y, double out[2], hipStream_t s) {
+    c->ensure_partials(2 * esc::kRedBlocks);
+    c->ensure_result(4);
+    hip_check(esc::launch_dot(c->c64(), x, y, c->dim, c->d_partials, c->d_result, s), "dot");
+    sync_get(s, out, c->d_result, 2 * sizeof(double));
+}
+
+double device_norm2(esc_ctx *c, const void *x, hipStream_t s) {
+    c->ensure_partials(2 * esc::kRedBlocks);
+    c->ensure_result(4);
+    hip_check(esc::launch_norm2(c->c64(), x, c->dim, c->d_partials, c->d_result, s), "norm2");
+    double r;
+    sync_get(s, &r, c->d_result, sizeof(double));
+    return r;
+}
+
+// classical Gram-Schmidt pass against V_0..V_{k-1} (applied twice by callers)
+void cgs_pass(esc_ctx *c, int k, void *w, hipStream_t s) {
+    c->ensure_partials((size_t)2 * k * 256);
+    c->ensure_result((size_t)2 * k + 4);
+    hip_check(esc::launch_multidot(c->c64(), c->vec(0), k, w, c->dim, c->d_partials, c->d_result, s), "multidot");
+    hip_check(esc::launch_multi_axpy(c->c64(), c->vec(0), k, c->d_result, -1.0, w, c->dim, false, s), "multi_axpy");
+}
+
+// Three-vector Lanczos without reorthogonalisation from phi/|phi| (work holds
+// three device vectors distinct from phi).  alpha[0..it-1], beta[0..it].
+void cf_run(esc_ctx *ctx, const void *d_phi, int steps, void *work[3], double *alpha, double *beta, int *iters,
+            double *norm2, hipStream_t s) {
+    check_ptr(alpha, "alpha");
+    check_ptr(beta, "beta");
+    check_ptr(iters, "iters");
+    check_ptr(norm2, "norm2");
+    if (steps < 1 || steps > 100000) throw EscError(ESC_ERR_ARG, "steps out of range");
+    const bool c64 = ctx->c64();
+    const uint64_t dim = ctx->dim;
+    ctx->ensure_partials(2 * esc::kRedBlocks);
+    ctx->ensure_result(8);
+    const double n2 = device_norm2(ctx, d_phi, s);
+    if (!(n2 > 0.0) || !std::isfinite(n2)) throw EscError(ESC_ERR_NUMERIC, "start vector has zero or non-finite norm");
+    *norm2 = n2;
+    void *vp = work[0], *v = work[1], *w = work[2];
+    hip_check(hipMemcpyAsync(v, d_phi, dim * ctx->esz, hipMemcpyDeviceToDevice, s), "copy");
+    hip_check(esc::launch_scale(c64, v, 1.0 / std::sqrt(n2), dim, s), "scale");
+    beta[0] = 0.0;
+    int k = 0;
+    for (; k < steps; ++k) {
+        hip_check(c64 ? esc::launch_hpsi_c64(ctx->d_htab, v, w, dim, s)
+                      : esc::launch_hpsi_c128(ctx->d_htab, v, w, dim, s), "hpsi");
+        double a[2];
+        hip_check(esc::launch_dot(c64, v, w, dim, ctx->d_partials, ctx->d_result, s), "dot");
+        sync_get(s, a, ctx->d_result, 2 * sizeof(double));
+        if (!std::isfinite(a[0])) throw EscError(ESC_ERR_NUMERIC, "non-finite alpha");
+        alpha[k] = a[0];
+        hip_check(esc::launch_lanczos_update(c64, w, v, k > 0 ? vp : nullptr, ctx->d_result, beta[k], dim,
+                                             ctx->d_partials, ctx->d_result + 2, s), "update");
+        double b2;
+        sync_get(s, &b2, ctx->d_result + 2, sizeof(double));
+        const double b = std::sqrt(b2);
+        beta[k + 1] = b;
+        if (!std::isfinite(b)) throw EscError(ESC_ERR_NUMERIC, "non-finite beta");
+        if (b == 0.0) { ++k; break; }
+        hip_check(esc::launch_scale(c64, w, 1.0 / b, dim, s), "scale");
+        void *t = vp; vp = v; v = w; w = t;
+    }
+    *iters = k;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *esc_last_error(void) { return g_err.c_str(); }
+int esc_version(void) { return 100; }
+
+int esc_ctx_create(const esc_model *model, int precision, esc_ctx **out) {
+    return guarded([&] {
+        check_ptr(model, "model");
+        check_ptr(out, "out");
+        *out = nullptr;
+        const int N = model->n_sites;
+        if (N < 2 || N > esc::kMaxSites) throw EscError(ESC_ERR_ARG, "n_sites must be in [2, 32]");
+        if (precision != ESC_C128 && precision != ESC_C64) throw EscError(ESC_ERR_ARG, "precision must be ESC_C128 or ESC_C64");
+        for (double v : {model->K[0], model->K[1], model->K[2], model->J, model->Gamma, model->B[0], model->B[1], model->B[2]})
+            if (!std::isfinite(v)) throw EscError(ESC_ERR_NUMERIC, "non-finite coupling");
+        auto c = std::make_unique<esc_ctx>();
+        c->n_sites = N;
+        c->precision = precision;
+        c->dim = (uint64_t)1 << N;
+        c->esz = precision == ESC_C64 ? 8 : 16;
+        c->model = *model;
+        for (int f = 0; f < 3; ++f) {
+            const int nb = model->n_bonds[f];
+            if (nb < 0) throw EscError(ESC_ERR_ARG, "negative bond count");
+            if (nb > 0) check_ptr(model->bonds[f], "bonds");
+            c->bonds[f].assign(model->bonds[f], model->bonds[f] + 2 * nb);
+            for (int b = 0; b < nb; ++b) {
+                const int i = c->bonds[f][2 * b], j = c->bonds[f][2 * b + 1];
+                if (i < 0 || j < 0 || i >= N || j >= N || i == j) throw EscError(ESC_ERR_ARG, "invalid bond");
+            }
+            c->model.bonds[f] = nullptr;
+        }
+        if (model->neigh) c->neigh.assign(model->neigh, model->neigh + 3 * N);
+        c->model.neigh = nullptr;
+        hip_check(hipGetDevice(&c->device), "hipGetDevice");
+        build_tables(c.get());
+        if (hipMalloc(&c->d_htab, sizeof(esc::HTables)) != hipSuccess) throw EscError(ESC_ERR_ALLOC, "table allocation failed");
+        hip_check(hipMemcpy(c->d_htab, &c->htab, sizeof(esc::HTables), hipMemcpyHostToDevice), "hipMemcpy");
+        *out = c.release();
+    });
+}
+
+int esc_ctx_destroy(esc_ctx *ctx) {
+    return guarded([&] { delete ctx; });
+}
+
+int esc_ctx_info(const esc_ctx *ctx, int64_t *dim, int32_t *elem_bytes, int32_t *precision) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (dim) *dim = (int64_t)ctx->dim;
+        if (elem_bytes) *elem_bytes = (int32_t)ctx->esz;
+        if (precision) *precision = ctx->precision;
+    });
+}
+
+int esc_hpsi(esc_ctx *ctx, const void *d_in, void *d_out, void *stream) {
+    return guarded([&] {
+        check_ctx(ctx);
+        check_ptr(d_in, "d_in");
+        check_ptr(d_out, "d_out");
+        if (d_in == d_out) throw EscError(ESC_ERR_ARG, "d_in and d_out must not overlap");
+        hipStream_t s = (hipStream_t)stream;
+        hip_check(ctx->c64() ? esc::launch_hpsi_c64(ctx->d_htab, d_in, d_out, ctx->dim, s)
+                             : esc::launch_hpsi_c128(ctx->d_htab, d_in, d_out, ctx->dim, s),
+                  "hpsi launch");
+    });
+}
+
+int esc_apply_op(esc_ctx *ctx, int op, int site, const void *d_in, void *d_out, void *stream) {
+    return guarded([&] {
+        check_ctx(ctx);
+        check_ptr(d_in, "d_in");
+        check_ptr(d_out, "d_out");
+        if (d_in == d_out) throw EscError(ESC_ERR_ARG, "d_in and d_out must not overlap");
+        esc::CurrentTable *t = current_table(ctx, op, site);
+        hipStream_t s = (hipStream_t)stream;
+        hip_check(ctx->c64() ? esc::launch_current_c64(t, d_in, d_out, ctx->dim, s)
+                             : esc::launch_current_c128(t, d_in, d_out, ctx->dim, s),
+                  "current launch");
+    });
+}
+
+int esc_start_vector(esc_ctx *ctx, uint64_t seed, int kind, void *d_out, void *stream) {
+    return guarded([&] {
+        check_ctx(ctx);
+        check_ptr(d_out, "d_out");
+        hipStream_t s = (hipStream_t)stream;
+        if (kind == ESC_START_REF_UNIFORM) {
+            // mt19937_64(seed) -> uniform_real_distribution<double>(0,1) as
+            // generate_canonical with one 64-bit draw: double(x) * 2^-64
+            // (libc++ convention, no clamp); re drawn before im.
+            const uint64_t dim = ctx->dim;
+            std::vector<double> h(2 * dim);
+            std::mt19937_64 g(seed);
+            const double inv = 1.0 / 18446744073709551616.0;
+            double n2 = 0.0;
+            for (uint64_t i = 0; i < 2 * dim; ++i) {
+                const double u = (double)g() * inv;
+                h[i] = -1.0 + u * 2.0;
+            }
+            for (uint64_t i = 0; i < 2 * dim; ++i) n2 += h[i] * h[i];
+            const double nrm = std::sqrt(n2);
+            for (auto &x : h) x /= nrm;    // Eigen normalize(): divide by the 2-norm
+            void *tmp = nullptr;
+            if (hipMalloc(&tmp, 2 * dim * sizeof(double)) != hipSuccess) throw EscError(ESC_ERR_ALLOC, "staging allocation failed");
+            hipError_t e = hipMemcpyAsync(tmp, h.data(), 2 * dim * sizeof(double), hipMemcpyHostToDevice, s);
+            if (e == hipSuccess) e = esc::launch_convert_from_c128(ctx->c64(), tmp, d_out, dim, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            (void)hipFree(tmp);
+            hip_check(e, "start vector upload");
+        } else if (kind == ESC_START_GAUSSIAN) {
+            hip_check(esc::launch_gaussian(ctx->c64(), seed, d_out, ctx->dim, s), "gaussian");
+            ctx->ensure_partials(2 * esc::kRedBlocks);
+            ctx->ensure_result(4);
+            hip_check(esc::launch_norm2(ctx->c64(), d_out, ctx->dim, ctx->d_partials, ctx->d_result, s), "norm2");
+            hip_check(esc::launch_normalize(ctx->c64(), d_out, ctx->d_result, ctx->dim, s), "normalize");
+        } else {
+            throw EscError(ESC_ERR_ARG, "unknown start-vector kind");
+        }
+    });
+}
+
+int esc_dot(esc_ctx *ctx, const void *d_x, const void *d_y, double *out, void *stream) {
+    return guarded([&] {
+        check_ctx(ctx);
+        check_ptr(d_x, "d_x");
+        check_ptr(d_y, "d_y");
+        check_ptr(out, "out");
+        device_dot(ctx, d_x, d_y, out, (hipStream_t)stream);
+    });
+}
+
+int esc_axpby(esc_ctx *ctx, const double *a, const void *d_x, const double *b, void *d_y, void *stream) {
+    return guarded([&] {
+        check_ctx(ctx);
+        check_ptr(a, "a");
+        check_ptr(b, "b");
+        check_ptr(d_x, "d_x");
+        check_ptr(d_y, "d_y");
+        hip_check(esc::launch_axpby(ctx->c64(), a[0], a[1], d_x, b[0], b[1], d_y, ctx->dim, (hipStream_t)stream), "axpby");
+    });
+}
+
+int esc_upload(esc_ctx *ctx, const double *host, void *d_out, void *stream) {
+    return guarded([&] {
+        check_ctx(ctx);
+        check_ptr(host, "host");
+        check_ptr(d_out, "d_out");
+        hipStream_t s = (hipStream_t)stream;
+        if (!ctx->c64()) {
+            hip_check(hipMemcpyAsync(d_out, host, ctx->dim * 16, hipMemcpyHostToDevice, s), "upload");
+            hip_check(hipStreamSynchronize(s), "sync");
+            return;
+        }
+        void *tmp = nullptr;
+        if (hipMalloc(&tmp, ctx->dim * 16) != hipSuccess) throw EscError(ESC_ERR_ALLOC, "staging allocation failed");
+        hipError_t e = hipMemcpyAsync(tmp, host, ctx->dim * 16, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = esc::launch_convert_from_c128(true, tmp, d_out, ctx->dim, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        (void)hipFree(tmp);
+        hip_check(e, "upload");
+    });
+}
+
+int esc_download(esc_ctx *ctx, const void *d_in, double *host, void *stream) {
+    return guarded([&] {
+        check_ctx(ctx);
+        check_ptr(host, "host");
+        check_ptr(d_in, "d_in");
+        hipStream_t s = (hipStream_t)stream;
+        if (!ctx->c64()) {
+            hip_check(hipMemcpyAsync(host, d_in, ctx->dim * 16, hipMemcpyDeviceToHost, s), "download");
+            hip_check(hipStreamSynchronize(s), "sync");
+            return;
+        }
+        void *tmp = nullptr;
+        if (hipMalloc(&tmp, ctx->dim * 16) != hipSuccess) throw EscError(ESC_ERR_ALLOC, "staging allocation failed");
+        hipError_t e = esc::launch_convert_to_c128(true, d_in, tmp, ctx->dim, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(host, tmp, ctx->dim * 16, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        (void)hipFree(tmp);
+        hip_check(e, "download");
+    });
+}
+
+int esc_lanczos(esc_ctx *ctx, const void *d_v0, int steps, double eps, int mode, double *alpha, double *beta,
+                int *iters, double *evals, double *Z, void *d_gs, void *stream) {
+    return guarded([&] {
+        check_ctx(ctx);
+        check_ptr(d_v0, "d_v0");
+        check_ptr(alpha, "alpha");
+        check_ptr(beta, "beta");
+        check_ptr(iters, "iters");
+        if (steps < 2 || steps > 4096) throw EscError(ESC_ERR_ARG, "steps must be in [2, 4096]");
+        if (mode < ESC_LANC_ONLYEGS || mode > ESC_LANC_PLAIN) throw EscError(ESC_ERR_ARG, "unknown Lanczos mode");
+        const bool reortho = mode == ESC_LANC_GS || mode == ESC_LANC_ALL;
+        if ((mode == ESC_LANC_GS || mode == ESC_LANC_ALL) && !d_gs) throw EscError(ESC_ERR_ARG, "d_gs required for GS/All");
+        hipStream_t s = (hipStream_t)stream;
+        const bool c64 = ctx->c64();
+        const uint64_t dim = ctx->dim;
+        // storage: reortho modes keep V_0..V_steps; the others rotate three
+        // vectors.  The new vector is formed in place in the V_{k+1} slot.
+        const size_t nvec = reortho ? (size_t)steps + 1 : 3;
+        ctx->ensure_vecs(nvec);
+        ctx->basis_kept = false;
+        ctx->ensure_partials((size_t)2 * (steps + 2) * 256 + 2 * esc::kRedBlocks);
+        ctx->ensure_result((size_t)2 * (steps + 2) + 8);
+        double *d_alpha = ctx->d_result;   // alpha lives in result[0..1] during updates
+        auto V = [&](int m) -> void * { return reortho ? ctx->vec((size_t)m) : ctx->vec((size_t)(m % 3)); };
+
+        hip_check(hipMemcpyAsync(V(0), d_v0, dim * ctx->esz, hipMemcpyDeviceToDevice, s), "copy v0");
+        std::vector<double> ar, ai, bt;
+        bt.push_back(0.0);
+        double E0 = 1.0, D0 = 0.0;
+        int it = 0;
+        auto step = [&](int k) {
+            // w = H V_k; alpha_k = <V_k|w>; w -= alpha_k V_k + beta_k V_{k-1}
+            void *w = V(k + 1);
+            hip_check(c64 ? esc::launch_hpsi_c64(ctx->d_htab, V(k), w, dim, s)
+                          : esc::launch_hpsi_c128(ctx->d_htab, V(k), w, dim, s), "hpsi");
+            double a[2];
+            hip_check(esc::launch_dot(c64, V(k), w, dim, ctx->d_partials, d_alpha, s), "dot");
+            sync_get(s, a, d_alpha, 2 * sizeof(double));
+            if (!std::isfinite(a[0]) || !std::isfinite(a[1])) throw EscError(ESC_ERR_NUMERIC, "non-finite alpha");
+            ar.push_back(a[0]);
+            ai.push_back(a[1]);
+            hip_check(esc::launch_lanczos_update(c64, w, V(k), k > 0 ? V(k - 1) : nullptr, d_alpha, bt[k], dim,
+                                                 ctx->d_partials, ctx->d_result + 2, s), "update");
+            double n2;
+            sync_get(s, &n2, ctx->d_result + 2, sizeof(double));
+            const double b = std::sqrt(n2);
+            if (!std::isfinite(b)) throw EscError(ESC_ERR_NUMERIC, "non-finite beta");
+            if (b == 0.0) throw EscError(ESC_ERR_BREAKDOWN, "Lanczos breakdown: beta == 0 at step " + std::to_string(k));
+            bt.push_back(b);
+            hip_check(esc::launch_scale(c64, w, 1.0 / b, dim, s), "scale");
+            if (reortho && k > 0) {   // (src/DynLanczosSolver.h:105-119), no reorthogonalisation at k = 0
+                cgs_pass(ctx, k + 1, w, s);
+                cgs_pass(ctx, k + 1, w, s);
+                hip_check(esc::launch_norm2(c64, w, dim, ctx->d_partials, ctx->d_result + 4, s), "norm2");
+                hip_check(esc::launch_normalize(c64, w, ctx->d_result + 4, dim, s), "normalize");
+            }
+        };
+        step(0);
+        bool done = false;
+        while (!done) {
+            ++it;
+            step(it);
+            D0 = tridiag_lowest((int)ar.size(), ar.data(), bt.data());
+            const double diff = std::fabs(E0 - D0);
+            if (diff < eps) done = true;
+            E0 = D0;
+            if (it == steps - 1) done = true;
+        }
+        const int n = it + 1;
+        *iters = n;
+        for (int k = 0; k < n; ++k) alpha[k] = ar[k];
+        for (int k = 0; k <= n; ++k) beta[k] = bt[k];
+        ctx->Z.assign((size_t)n * n, 0.0);
+        std::vector<double> ev(n);
+        if (tridiag_ql(n, ar.data(), bt.data(), ev.data(), ctx->Z.data()) != 0)
+            throw EscError(ESC_ERR_NUMERIC, "tridiagonal QL did not converge");
+        ctx->z_n = n;
+        ctx->evals = ev;
+        if (evals) std::memcpy(evals, ev.data(), sizeof(double) * n);
+        if (Z) std::memcpy(Z, ctx->Z.data(), sizeof(double) * n * n);
+        if (reortho) {
+            // |gs> = sum_m Z(m,0) V_m  (src/DynLanczosSolver.h:213-223 for k = 0)
+            std::vector<double> c(2 * (size_t)n);
+            for (int m = 0; m < n; ++m) { c[2 * m] = ctx->Z[m]; c[2 * m + 1] = 0.0; }
+            hip_check(hipMemcpyAsync(ctx->d_result, c.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice, s), "coef");
+            hip_check(esc::launch_multi_axpy(c64, ctx->vec(0), n, ctx->d_result, 1.0, d_gs, dim, true, s), "gs");
+            hip_check(hipStreamSynchronize(s), "sync");
+            ctx->basis_kept = mode == ESC_LANC_ALL;
+            ctx->basis_n = n;
+        }
+    });
+}
+
+int esc_ritz_overlaps(esc_ctx *ctx, const void *d_vec, int n_states, double *out, void *stream) {
+    return guarded([&] {
+        check_ctx(ctx);
+        check_ptr(d_vec, "d_vec");
+        check_ptr(out, "out");
+        if (!ctx->basis_kept) throw EscError(ESC_ERR_ARG, "no Lanczos basis kept (run esc_lanczos with ESC_LANC_ALL)");
+        const int n = ctx->basis_n;
+        if (n_states < 0 || n_states > n) throw EscError(ESC_ERR_ARG, "n_states out of range");
+        hipStream_t s = (hipStream_t)stream;
+        ctx->ensure_partials((size_t)2 * n * 256);
+        ctx->ensure_result((size_t)2 * n + 8);
+        hip_check(esc::launch_multidot(ctx->c64(), ctx->vec(0), n, d_vec, ctx->dim, ctx->d_partials, ctx->d_result, s), "multidot");
+        std::vector<double> t(2 * (size_t)n);
+        sync_get(s, t.data(), ctx->d_result, sizeof(double) * 2 * n);
+        for (int k = 0; k < n_states; ++k) {
+            double re = 0.0, im = 0.0;
+            for (int m = 0; m < n; ++m) {
+                const double z = ctx->Z[(size_t)k * n + m];
+                re += z * t[2 * m];
+                im += z * t[2 * m + 1];
+            }
+            out[2 * k] = re;
+            out[2 * k + 1] = im;
+        }
+    });
+}
+
+int esc_ritz_vector(esc_ctx *ctx, int k, void *d_out, void *stream) {
+    return guarded([&] {
+        check_ctx(ctx);
+        check_ptr(d_out, "d_out");
+        if (!ctx->basis_kept) throw EscError(ESC_ERR_ARG, "no Lanczos basis kept (run esc_lanczos with ESC_LANC_ALL)");
+        const int n = ctx->basis_n;
+        if (k < 0 || k >= n) throw EscError(ESC_ERR_ARG, "state index out of range");
+        hipStream_t s = (hipStream_t)stream;
+        std::vector<double> c(2 * (size_t)n);
+        for (int m = 0; m < n; ++m) { c[2 * m] = ctx->Z[(size_t)k * n + m]; c[2 * m + 1] = 0.0; }
+        ctx->ensure_result((size_t)2 * n + 8);
+        hip_check(hipMemcpyAsync(ctx->d_result, c.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice, s), "coef");
+        hip_check(esc::launch_multi_axpy(ctx->c64(), ctx->vec(0), n, ctx->d_result, 1.0, d_out, ctx->dim, true, s), "ritz");
+        hip_check(hipStreamSynchronize(s), "sync");
+    });
+}
+
+int esc_lanczos_cf(esc_ctx *ctx, const void *d_phi, int steps, double *alpha, double *beta, int *iters,
+                   double *norm2, void *stream) {
+    return guarded([&] {
+        check_ctx(ctx);
+        check_ptr(d_phi, "d_phi");
+        ctx->ensure_vecs(3);
+        ctx->basis_kept = false;
+        void *work[3] = {ctx->vec(0), ctx->vec(1), ctx->vec(2)};
+        cf_run(ctx, d_phi, steps, work, alpha, beta, iters, norm2, (hipStream_t)stream);
+    });
+}
+
+int esc_dynamics(esc_ctx *ctx, int kind, int method, const void *d_gs, double e0, int cf_steps, double eta,
+                 const double *omega, int nw, double *table, double *expect, void *stream) {
+    return guarded([&] {
+        check_ctx(ctx);
+        check_ptr(d_gs, "d_gs");
+        check_ptr(omega, "omega");
+        check_ptr(table, "table");
+        if (kind != ESC_DYN_JSJS && kind != ESC_DYN_JEJE) throw EscError(ESC_ERR_ARG, "unknown dynamics kind");
+        if (method != ESC_SPEC_RITZ && method != ESC_SPEC_CF) throw EscError(ESC_ERR_ARG, "unknown spectral method");
+        if (nw < 1) throw EscError(ESC_ERR_ARG, "nw must be >= 1");
+        if (method == ESC_SPEC_RITZ && !ctx->basis_kept)
+            throw EscError(ESC_ERR_ARG, "Ritz spectra need the basis of an ESC_LANC_ALL run (LancType=All)");
+        if (method == ESC_SPEC_CF && (cf_steps < 1 || cf_steps > 100000)) throw EscError(ESC_ERR_ARG, "cf_steps out of range");
+        hipStream_t s = (hipStream_t)stream;
+        const bool js = kind == ESC_DYN_JSJS;
+        const std::vector<int> ops = js ? std::vector<int>{ESC_OP_JSX, ESC_OP_JSY, ESC_OP_JSZ}
+                                        : std::vector<int>{ESC_OP_JEX, ESC_OP_JEY};
+        const int nc = (int)ops.size();
+        using cplx = std::complex<double>;
+        ctx->ensure_aux(method == ESC_SPEC_CF ? 4 : 1);
+        ctx->ensure_partials(2 * esc::kRedBlocks);
+        ctx->ensure_result(8);
+        std::vector<esc::CurrentTable> tabs(nc);
+        for (int a = 0; a < nc; ++a) host_current_table(ctx, ops[a], ESC_SITE_TOTAL, &tabs[a]);
+        if (!ctx->d_combo && hipMalloc(&ctx->d_combo, sizeof(esc::CurrentTable)) != hipSuccess)
+            throw EscError(ESC_ERR_ALLOC, "table allocation failed");
+        // phi = (sum_a c_a J_a) |gs> into aux(0), from one combined gather table
+        auto apply_combo = [&](const std::vector<cplx> &coef) {
+            esc::CurrentTable t;
+            std::memset(&t, 0, sizeof(t));
+            for (int a = 0; a < nc; ++a)
+                if (coef[a] != cplx(0.0, 0.0)) append_scaled(t, tabs[a], coef[a].real(), coef[a].imag());
+            hip_check(hipMemcpy(ctx->d_combo, &t, sizeof(t), hipMemcpyHostToDevice), "hipMemcpy");
+            hip_check(ctx->c64() ? esc::launch_current_c64(ctx->d_combo, d_gs, ctx->aux(0), ctx->dim, s)
+                                 : esc::launch_current_c128(ctx->d_combo, d_gs, ctx->aux(0), ctx->dim, s),
+                      "current launch");
+        };
+        auto unit = [&](int a) {
+            std::vector<cplx> c(nc, cplx(0.0, 0.0));
+            c[a] = 1.0;
+            return c;
+        };
+        // <J_a> = <gs|J_a|gs>  (src/dynamics.h:250-253,370-372); t = sum of components
+        std::vector<cplx> ex(nc + 1, cplx(0.0, 0.0));
+        std::vector<std::vector<cplx>> O(nc + 1);
+        const int nst = method == ESC_SPEC_RITZ ? ctx->basis_n : 0;
+        for (int a = 0; a < nc; ++a) {
+            apply_combo(unit(a));
+            double d[2];
+            device_dot(ctx, d_gs, ctx->aux(0), d, s);
+            ex[a] = cplx(d[0], d[1]);
+            ex[nc] += ex[a];
+            if (method == ESC_SPEC_RITZ) {
+                // O(n,a) = <psi_n|J_a gs> = sum_m Z(m,n) <v_m|J_a gs>  (src/dynamics.h:255-263,375-383)
+                const int n = ctx->basis_n;
+                ctx->ensure_partials((size_t)2 * n * 256);
+                ctx->ensure_result((size_t)2 * n + 8);
+                hip_check(esc::launch_multidot(ctx->c64(), ctx->vec(0), n, ctx->aux(0), ctx->dim, ctx->d_partials,
+                                               ctx->d_result, s), "multidot");
+                std::vector<double> tv(2 * (size_t)n);
+                sync_get(s, tv.data(), ctx->d_result, sizeof(double) * 2 * n);
+                O[a].assign(nst, cplx(0.0, 0.0));
+                for (int k = 0; k < nst; ++k) {
+                    cplx acc(0.0, 0.0);
+                    for (int m = 0; m < n; ++m) acc += ctx->Z[(size_t)k * n + m] * cplx(tv[2 * m], tv[2 * m + 1]);
+                    O[a][k] = acc;
+                }
+            }
+        }
+        if (expect)
+            for (int a = 0; a <= nc; ++a) { expect[2 * a] = ex[a].real(); expect[2 * a + 1] = ex[a].imag(); }
+        // spectra I_ab(w) before the elastic subtraction
+        std::map<std::pair<int, int>, std::vector<cplx>> F;
+        const int T = nc;   // index of the total channel
+        std::vector<std::pair<int, int>> pairs;
+        if (js) pairs = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}, {T, T}};
+        else pairs = {{0, 0}, {1, 1}, {0, 1}, {1, 0}, {T, T}};
+        if (method == ESC_SPEC_RITZ) {
+            O[T].assign(nst, cplx(0.0, 0.0));
+            for (int a = 0; a < nc; ++a)
+                for (int k = 0; k < nst; ++k) O[T][k] += O[a][k];
+            const double Eg = ctx->evals[0];
+            for (auto pr : pairs) {
+                std::vector<cplx> f(nw);
+                for (int q = 0; q < nw; ++q) {
+                    cplx acc(0.0, 0.0);
+                    for (int k = 0; k < nst; ++k) {
+                        const cplx den(omega[q] - (ctx->evals[k] - Eg), -eta);
+                        acc = acc + std::conj(O[pr.first][k]) * O[pr.second][k] / den;
+                    }
+                    f[q] = acc;
+                }
+                F[pr] = f;
+            }
+        } else {
+            void *work[3] = {ctx->aux(1), ctx->aux(2), ctx->aux(3)};
+            std::vector<double> al(cf_steps + 1), be(cf_steps + 2);
+            auto cf = [&](const std::vector<cplx> &coef) {
+                apply_combo(coef);
+                int it = 0;
+                double n2 = 0.0;
+                cf_run(ctx, ctx->aux(0), cf_steps, work, al.data(), be.data(), &it, &n2, s);
+                std::vector<cplx> f(nw);
+                for (int q = 0; q < nw; ++q) {
+                    const cplx z(omega[q] + e0, -eta);
+                    cplx g(0.0, 0.0);
+                    for (int k = it - 1; k >= 0; --k) {
+                        const cplx tail = (k + 1 < it) ? be[k + 1] * be[k + 1] * g : cplx(0.0, 0.0);
+                        g = 1.0 / (z - al[k] - tail);
+                    }
+                    f[q] = n2 * g;
+                }
+                return f;
+            };
+            for (int a = 0; a < nc; ++a) F[{a, a}] = cf(unit(a));
+            std::vector<cplx> all(nc, cplx(1.0, 0.0));
+            F[{T, T}] = cf(all);
+            for (auto pr : pairs) {
+                const int a = pr.first, b = pr.second;
+                if (a == b || F.count(pr)) continue;
+                // polarisation: f_ab = 1/2 [(S1 - faa - fbb) - i (Si - faa - fbb)],
+                //               f_ba = 1/2 [(S1 - faa - fbb) + i (Si - faa - fbb)]
+                std::vector<cplx> c1(nc, cplx(0.0, 0.0)), ci(nc, cplx(0.0, 0.0));
+                c1[a] = 1.0; c1[b] = 1.0;
+                ci[a] = 1.0; ci[b] = cplx(0.0, 1.0);
+                const std::vector<cplx> S1 = (nc == 2) ? F[{T, T}] : cf(c1);
+                const std::vector<cplx> Si = cf(ci);
+                std::vector<cplx> fab(nw), fba(nw);
+                for (int q = 0; q < nw; ++q) {
+                    const cplx u = S1[q] - F[{a, a}][q] - F[{b, b}][q];
+                    const cplx v = Si[q] - F[{a, a}][q] - F[{b, b}][q];
+                    fab[q] = 0.5 * (u - cplx(0.0, 1.0) * v);
+                    fba[q] = 0.5 * (u + cplx(0.0, 1.0) * v);
+                }
+                F[{a, b}] = fab;
+                F[{b, a}] = fba;
+            }
+        }
+        // elastic subtraction conj(<J_a>) <J_b> / (w - i eta); none for JeJe yx
+        const int ncols = js ? 14 : 5;
+        for (int q = 0; q < nw; ++q) {
+            double *row = table + (size_t)q * (1 + ncols);
+            row[0] = omega[q];
+            const cplx dn1(omega[q], -eta);
+            int col = 1;
+            for (auto pr : pairs) {
+                cplx v = F[pr][q];
+                const bool yx = !js && pr.first == 1 && pr.second == 0;
+                if (!yx) v = v - std::conj(ex[pr.first]) * ex[pr.second] / dn1;
+                if (js) { row[col++] = v.real(); row[col++] = v.imag(); }
+                else row[col++] = v.imag();
+            }
+        }
+    });
+}
+
+int esc_tridiag_eig(int n, const double *a, const double *b, double *evals, double *Z) {
+    return guarded([&] {
+        if (n < 1 || n > 4096) throw EscError(ESC_ERR_ARG, "n must be in [1, 4096]");
+        check_ptr(a, "a");
+        check_ptr(b, "b");
+        check_ptr(evals, "evals");
+        if (tridiag_ql(n, a, b, evals, Z) != 0) throw EscError(ESC_ERR_NUMERIC, "tridiagonal QL did not converge");
+    });
+}
+
+int esc_cf_spectrum(int n, const double *a, const double *b, double norm2, double e0, double eta,
+                    const double *omega, int nw, const double *elastic, double *out) {
+    return guarded([&] {
+        if (n < 1) throw EscError(ESC_ERR_ARG, "n must be >= 1");
+        check_ptr(a, "a");
+        check_ptr(b, "b");
+        check_ptr(omega, "omega");
+        check_ptr(out, "out");
+        for (int q = 0; q < nw; ++q) {
+            const std::complex<double> z(omega[q] + e0, -eta);
+            std::complex<double> g(0.0, 0.0);
+            for (int k = n - 1; k >= 0; --k) {
+                const std::complex<double> tail = (k + 1 < n) ? b[k + 1] * b[k + 1] * g : std::complex<double>(0.0, 0.0);
+                g = 1.0 / (z - a[k] - tail);
+            }
+            std::complex<double> r = norm2 * g;
+            if (elastic) r -= std::complex<double>(elastic[0], elastic[1]) / std::complex<double>(omega[q], -eta);
+            out[2 * q] = r.real();
+            out[2 * q + 1] = r.imag();
+        }
+    });
+}
+
+int esc_lehmann_spectrum(int n, const double *evals, const double *Oa, const double *Ob, const double *elastic,
+                         double eta, const double *omega, int nw, double *out) {
+    return guarded([&] {
+        if (n < 1) throw EscError(ESC_ERR_ARG, "n must be >= 1");
+        check_ptr(evals, "evals");
+        check_ptr(Oa, "Oa");
+        check_ptr(Ob, "Ob");
+        check_ptr(omega, "omega");
+        check_ptr(out, "out");
+        const double Eg = evals[0];
+        for (int q = 0; q < nw; ++q) {
+            std::complex<double> acc(0.0, 0.0);
+            for (int k = 0; k < n; ++k) {
+                const std::complex<double> den(omega[q] - (evals[k] - Eg), -eta);
+                const std::complex<double> num = std::conj(std::complex<double>(Oa[2 * k], Oa[2 * k + 1])) *
+                                                 std::complex<double>(Ob[2 * k], Ob[2 * k + 1]);
+                acc = acc + num / den;
+            }
+            if (elastic) acc = acc - std::complex<double>(elastic[0], elastic[1]) / std::complex<double>(omega[q], -eta);
+            out[2 * q] = acc.real();
+            out[2 * q + 1] = acc.imag();
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,97 @@
+// esc_internal.h -- device tables and kernel launchers shared by the C ABI
+// (esc_capi.cpp) and the HIP kernels (esc_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace esc {
+
+constexpr int kMaxSites = 32;
+constexpr int kMaxBonds = 96;      // off-diagonal two-site terms
+constexpr int kMaxCurrentPairs = 96;
+
+// Matrix-element tables of H in gather form.  Row r, s_k = +1 if bit k of r
+// is 0 (spin up), sigma_ij = s_i s_j.
+//   diagonal:  sum over zdiag (+-kz4 by sigma, in order), then over jdiag
+//              (+-j4), then bz * (N - 2 popcount(r)) / 2
+//   bond b:    H[r, r ^ (e_i|e_j)] = p0[f] + p1[f] sigma_ij + i q[f] (s_i + s_j)
+//   flip k:    H[r, r ^ e_k] = bx2 + g4 s_{ypart(k)} - i s_k (by2 + g4 s_{xpart(k)})
+struct HTables {
+    uint32_t n_sites;
+    uint32_t n_bonds;       // off-diagonal bond terms
+    uint32_t n_zdiag;       // Kitaev z bonds on the diagonal (reference order)
+    uint32_t n_jdiag;       // bonds with a Heisenberg diagonal part (0 if J == 0)
+    uint32_t n_flip;        // 0 (no single-flip terms) or n_sites
+    uint32_t has_gamma_flip;
+    uint32_t bond[kMaxBonds];     // i | j << 8 | family << 16
+    uint32_t zdiag[kMaxBonds];    // i | j << 8
+    uint32_t jdiag[kMaxBonds];    // i | j << 8
+    int32_t xpart[kMaxSites];     // x-bond partner per site, -1 = none
+    int32_t ypart[kMaxSites];     // y-bond partner per site, -1 = none
+    double p0[3], p1[3], q[3];
+    double kz4, j4, bz;
+    double bx2, by2, g4;
+};
+
+// One factor of a Pauli-string term: op 0 = Sx, 1 = Sy, 2 = Sz at site.
+struct OpFactor {
+    int32_t op;
+    int32_t site;
+};
+
+// A current (or single-site) operator applied in gather form.  Each pair
+// contributes (wa * A - wb * B) psi with A, B products of up to three
+// single-site factors, applied first-to-last (factors[0] acts first), and
+// complex weights wa = wa_re + i wa_im (likewise wb).
+struct CurrentPair {
+    int32_t na, nb;             // number of factors (nb = 0: no B term)
+    OpFactor fa[3], fb[3];
+    double wa_re, wa_im, wb_re, wb_im;
+};
+
+struct CurrentTable {
+    int32_t n_pairs;
+    CurrentPair pair[kMaxCurrentPairs];
+};
+
+// Last-error slot of the C ABI (thread-local), shared by all host files.
+void set_error(const char *msg);
+void clear_error();
+
+// Launchers (esc_kernels.hip).  dim = 2^N.  All return hipError_t of the launch.
+hipError_t launch_hpsi_c128(const HTables *d_tab, const void *in, void *out, uint64_t dim, hipStream_t s);
+hipError_t launch_hpsi_c64(const HTables *d_tab, const void *in, void *out, uint64_t dim, hipStream_t s);
+hipError_t launch_current_c128(const CurrentTable *d_tab, const void *in, void *out, uint64_t dim, hipStream_t s);
+hipError_t launch_current_c64(const CurrentTable *d_tab, const void *in, void *out, uint64_t dim, hipStream_t s);
+
+// Reductions write nblk partial results to `partials`, then a single-block
+// pass writes the final FP64 result to `result`.  Deterministic.
+constexpr int kRedBlocks = 1024;
+constexpr int kRedThreads = 256;
+
+// result[0..1] = <x|y>
+hipError_t launch_dot(bool c64, const void *x, const void *y, uint64_t dim, double *partials, double *result, hipStream_t s);
+// y = a x + b y (complex a, b held on the host)
+hipError_t launch_axpby(bool c64, double ar, double ai, const void *x, double br, double bi, void *y, uint64_t dim, hipStream_t s);
+// w = w - alpha v - beta u ; result[0] = |w|^2     (alpha complex, beta real)
+hipError_t launch_lanczos_update(bool c64, void *w, const void *v, const void *u, const double *d_alpha,
+                                 double beta, uint64_t dim, double *partials, double *result, hipStream_t s);
+// x *= 1 / sqrt(result[0]) with result read on the device (no host round trip)
+hipError_t launch_normalize(bool c64, void *x, const double *d_norm2, uint64_t dim, hipStream_t s);
+// x *= scale (real)
+hipError_t launch_scale(bool c64, void *x, double scale, uint64_t dim, hipStream_t s);
+// result[0] = |x|^2
+hipError_t launch_norm2(bool c64, const void *x, uint64_t dim, double *partials, double *result, hipStream_t s);
+// coef[m] = <V_m | w> for m < k (V_m = base + m * dim), FP64; result has 2k doubles
+hipError_t launch_multidot(bool c64, const void *base, int k, const void *w, uint64_t dim, double *partials,
+                           double *result, hipStream_t s);
+// w = beta_w * w + sum_m c[m] V_m   (c on the device, complex FP64)
+hipError_t launch_multi_axpy(bool c64, const void *base, int k, const double *d_coef, double sign, void *w,
+                             uint64_t dim, bool overwrite, hipStream_t s);
+// Philox4x32-10 Gaussian start vector, N(0, 1/2) per component
+hipError_t launch_gaussian(bool c64, uint64_t seed, void *out, uint64_t dim, hipStream_t s);
+// convert FP64 complex device buffer to c64 (or copy for c128)
+hipError_t launch_convert_from_c128(bool c64, const void *src, void *dst, uint64_t dim, hipStream_t s);
+hipError_t launch_convert_to_c128(bool c64, const void *src, void *dst, uint64_t dim, hipStream_t s);
+
+}  // namespace esc

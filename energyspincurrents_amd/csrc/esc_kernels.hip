@@ -1,0 +1,586 @@
+// esc_kernels.hip -- HIP kernels for gfx950 (MI355X): matrix-free H.psi in gather
+// form, current operators, and the FP64 reductions of the Lanczos recursion.
+//
+// Vectors are interleaved complex (double2 for c128, float2 for c64) indexed by
+// the basis state r (bit k = site k, bit 0 = spin up).  Every output row is
+// owned by exactly one lane, so no atomics are used anywhere.
+#include "esc_internal.h"
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace esc {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kNumXcd = 8;
+
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup g runs on
+// XCD g % 8).  Remap so that each XCD walks one contiguous range of rows: the
+// partner rows r ^ m of masks below that range's size then stay in the same
+// XCD's L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t g, uint32_t nblk) {
+    if (nblk % kNumXcd != 0) return g;
+    uint32_t per = nblk / kNumXcd;
+    return (g % kNumXcd) * per + g / kNumXcd;
+}
+
+template <typename R> struct Vec2;
+template <> struct Vec2<double> { using T = double2; };
+template <> struct Vec2<float> { using T = float2; };
+
+__device__ __forceinline__ double sgn(uint32_t r, uint32_t k) { return ((r >> k) & 1u) ? -1.0 : 1.0; }
+
+// ---------------------------------------------------------------------------
+// H.psi, one row per lane.  The diagonal is accumulated in FP64 in the
+// reference order (z bonds, then Heisenberg bonds, then Bzz * Sz_total), so for
+// c128 every matrix element equals the reference's bit for bit.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_hpsi(const HTables *__restrict__ T,
+                                                   const typename Vec2<R>::T *__restrict__ in,
+                                                   typename Vec2<R>::T *__restrict__ out, uint64_t dim,
+                                                   uint32_t nblk_total, uint32_t block_base) {
+    using V2 = typename Vec2<R>::T;
+    const uint32_t g = xcd_block(blockIdx.x + block_base, nblk_total);
+    const uint64_t r64 = (uint64_t)g * kThreads + threadIdx.x;
+    if (r64 >= dim) return;
+    const uint32_t r = (uint32_t)r64;
+
+    double d = 0.0;
+    {
+        const uint32_t nz = T->n_zdiag;
+        const double kz4 = T->kz4;
+        for (uint32_t b = 0; b < nz; ++b) {
+            const uint32_t p = T->zdiag[b];
+            const uint32_t x = ((r >> (p & 0xffu)) ^ (r >> ((p >> 8) & 0xffu))) & 1u;
+            d += x ? -kz4 : kz4;
+        }
+        const uint32_t nj = T->n_jdiag;
+        const double j4 = T->j4;
+        for (uint32_t b = 0; b < nj; ++b) {
+            const uint32_t p = T->jdiag[b];
+            const uint32_t x = ((r >> (p & 0xffu)) ^ (r >> ((p >> 8) & 0xffu))) & 1u;
+            d += x ? -j4 : j4;
+        }
+        d += T->bz * (0.5 * (double)((int)T->n_sites - 2 * __popc(r)));
+    }
+    const V2 v0 = in[r];
+    const R dd = (R)d;
+    R ar = dd * v0.x, ai = dd * v0.y;
+
+    const uint32_t nb = T->n_bonds;
+#pragma unroll 4
+    for (uint32_t b = 0; b < nb; ++b) {
+        const uint32_t p = T->bond[b];
+        const uint32_t i = p & 0xffu, j = (p >> 8) & 0xffu, f = p >> 16;
+        const R si = ((r >> i) & 1u) ? R(-1) : R(1);
+        const R sj = ((r >> j) & 1u) ? R(-1) : R(1);
+        const R cr = (R)T->p0[f] + (R)T->p1[f] * (si * sj);
+        const R ci = (R)T->q[f] * (si + sj);
+        const V2 x = in[r ^ ((1u << i) | (1u << j))];
+        ar += cr * x.x - ci * x.y;
+        ai += cr * x.y + ci * x.x;
+    }
+
+    const uint32_t nf = T->n_flip;
+    const R bx2 = (R)T->bx2, by2 = (R)T->by2, g4 = (R)T->g4;
+    const bool gam = T->has_gamma_flip != 0;
+#pragma unroll 4
+    for (uint32_t k = 0; k < nf; ++k) {
+        const R sk = ((r >> k) & 1u) ? R(-1) : R(1);
+        R re = bx2, im = by2;
+        if (gam) {
+            const int32_t xp = T->xpart[k], yp = T->ypart[k];
+            if (xp >= 0) im += g4 * (((r >> xp) & 1u) ? R(-1) : R(1));
+            if (yp >= 0) re += g4 * (((r >> yp) & 1u) ? R(-1) : R(1));
+        }
+        im = -sk * im;
+        const V2 x = in[r ^ (1u << k)];
+        ar += re * x.x - im * x.y;
+        ai += re * x.y + im * x.x;
+    }
+    V2 o;
+    o.x = ar;
+    o.y = ai;
+    out[r] = o;
+}
+
+// ---------------------------------------------------------------------------
+// Pauli-string operators (single-site S^a, spin and energy currents).
+// Row r gathers from c = r ^ M where M flips the sites of the Sx/Sy factors;
+// the coefficient follows the factor sequence exactly as the reference applies
+// it (first factor acts first on c), so products are exact and the only
+// rounding is the coupling weight times the value.
+// ---------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ void string_term(const OpFactor *f, int n, uint32_t r, double w_re, double w_im,
+                                            const typename Vec2<R>::T *__restrict__ in, R &outr, R &outi) {
+    uint32_t m = 0;
+    for (int t = 0; t < n; ++t)
+        if (f[t].op != 2) m ^= 1u << f[t].site;
+    const uint32_t c = r ^ m;
+    uint32_t k = c;
+    // coefficient = w * (cr + i ci), built from exact factors +-1/2, +-i/2
+    double cr = 1.0, ci = 0.0;
+    for (int t = 0; t < n; ++t) {
+        const uint32_t s = f[t].site;
+        const uint32_t bit = (k >> s) & 1u;
+        if (f[t].op == 0) {
+            cr *= 0.5; ci *= 0.5; k ^= 1u << s;
+        } else if (f[t].op == 1) {
+            const double h = bit ? -0.5 : 0.5;   // multiply by i*h
+            const double nr = -ci * h, ni = cr * h;
+            cr = nr; ci = ni; k ^= 1u << s;
+        } else {
+            const double h = bit ? -0.5 : 0.5;
+            cr *= h; ci *= h;
+        }
+    }
+    const R wr = (R)(w_re * cr - w_im * ci), wi = (R)(w_re * ci + w_im * cr);
+    const typename Vec2<R>::T x = in[c];
+    outr = wr * x.x - wi * x.y;
+    outi = wr * x.y + wi * x.x;
+}
+
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_current(const CurrentTable *__restrict__ T,
+                                                      const typename Vec2<R>::T *__restrict__ in,
+                                                      typename Vec2<R>::T *__restrict__ out, uint64_t dim,
+                                                      uint32_t nblk_total, uint32_t block_base) {
+    const uint32_t g = xcd_block(blockIdx.x + block_base, nblk_total);
+    const uint64_t r64 = (uint64_t)g * kThreads + threadIdx.x;
+    if (r64 >= dim) return;
+    const uint32_t r = (uint32_t)r64;
+    R accr = 0, acci = 0;
+    const int np = T->n_pairs;
+    for (int p = 0; p < np; ++p) {
+        const CurrentPair &cp = T->pair[p];
+        R ar, ai;
+        string_term<R>(cp.fa, cp.na, r, cp.wa_re, cp.wa_im, in, ar, ai);
+        if (cp.nb > 0) {
+            R br, bi;
+            string_term<R>(cp.fb, cp.nb, r, cp.wb_re, cp.wb_im, in, br, bi);
+            ar = ar - br;
+            ai = ai - bi;
+        }
+        accr += ar;
+        acci += ai;
+    }
+    typename Vec2<R>::T o;
+    o.x = accr;
+    o.y = acci;
+    out[r] = o;
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic FP64 block reduction helpers
+// ---------------------------------------------------------------------------
+template <int NV>
+__device__ __forceinline__ void block_reduce_store(double (&v)[NV], double *dst) {
+    __shared__ double sh[NV][kThreads];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) sh[q][threadIdx.x] = v[q];
+    __syncthreads();
+    for (int off = kThreads / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+#pragma unroll
+            for (int q = 0; q < NV; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) dst[q] = sh[q][0];
+    }
+}
+
+// final pass: nvals independent sums, each of npart partials laid out
+// partials[part * nvals + q]; one block per group of values
+__global__ __launch_bounds__(kThreads) void k_reduce_final(const double *__restrict__ partials, int npart, int nvals,
+                                                           double *__restrict__ result) {
+    const int q = blockIdx.x;
+    double v[1] = {0.0};
+    for (int p = threadIdx.x; p < npart; p += kThreads) v[0] += partials[(size_t)p * nvals + q];
+    block_reduce_store<1>(v, result + q);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_dot_partial(const typename Vec2<R>::T *__restrict__ x,
+                                                          const typename Vec2<R>::T *__restrict__ y, uint64_t dim,
+                                                          double *__restrict__ partials) {
+    double v[2] = {0.0, 0.0};
+    const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+    for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < dim; i += stride) {
+        const double xr = x[i].x, xi = x[i].y, yr = y[i].x, yi = y[i].y;
+        v[0] += xr * yr + xi * yi;
+        v[1] += xr * yi - xi * yr;
+    }
+    block_reduce_store<2>(v, partials + 2 * blockIdx.x);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_norm2_partial(const typename Vec2<R>::T *__restrict__ x, uint64_t dim,
+                                                            double *__restrict__ partials) {
+    double v[1] = {0.0};
+    const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+    for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < dim; i += stride) {
+        const double xr = x[i].x, xi = x[i].y;
+        v[0] += xr * xr + xi * xi;
+    }
+    block_reduce_store<1>(v, partials + blockIdx.x);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_axpby(double ar, double ai, const typename Vec2<R>::T *__restrict__ x,
+                                                    double br, double bi, typename Vec2<R>::T *__restrict__ y,
+                                                    uint64_t dim) {
+    const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+    for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < dim; i += stride) {
+        const double xr = x[i].x, xi = x[i].y, yr = y[i].x, yi = y[i].y;
+        typename Vec2<R>::T o;
+        o.x = (R)((ar * xr - ai * xi) + (br * yr - bi * yi));
+        o.y = (R)((ar * xi + ai * xr) + (br * yi + bi * yr));
+        y[i] = o;
+    }
+}
+
+// w = w - alpha v - beta u  (as V2 - alpha*V1 - beta*V0, src/DynLanczosSolver.h:96),
+// partial |w|^2 of the updated w
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_lanczos_update(typename Vec2<R>::T *__restrict__ w,
+                                                             const typename Vec2<R>::T *__restrict__ v,
+                                                             const typename Vec2<R>::T *__restrict__ u,
+                                                             const double *__restrict__ d_alpha, double beta,
+                                                             uint64_t dim, double *__restrict__ partials) {
+    const double a_r = d_alpha[0], a_i = d_alpha[1];
+    double acc[1] = {0.0};
+    const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+    for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < dim; i += stride) {
+        const double vr = v[i].x, vi = v[i].y;
+        double wr = w[i].x - (a_r * vr - a_i * vi);
+        double wi = w[i].y - (a_r * vi + a_i * vr);
+        if (u != nullptr) {
+            wr -= beta * (double)u[i].x;
+            wi -= beta * (double)u[i].y;
+        }
+        typename Vec2<R>::T o;
+        o.x = (R)wr;
+        o.y = (R)wi;
+        w[i] = o;
+        acc[0] += (double)o.x * o.x + (double)o.y * o.y;
+    }
+    block_reduce_store<1>(acc, partials + blockIdx.x);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_normalize(typename Vec2<R>::T *__restrict__ x,
+                                                        const double *__restrict__ d_norm2, uint64_t dim) {
+    const double s = 1.0 / sqrt(d_norm2[0]);
+    const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+    for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < dim; i += stride) {
+        typename Vec2<R>::T o = x[i];
+        o.x = (R)(s * o.x);
+        o.y = (R)(s * o.y);
+        x[i] = o;
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_scale(typename Vec2<R>::T *__restrict__ x, double s, uint64_t dim) {
+    const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+    for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < dim; i += stride) {
+        typename Vec2<R>::T o = x[i];
+        o.x = (R)(s * o.x);
+        o.y = (R)(s * o.y);
+        x[i] = o;
+    }
+}
+
+// coef[m] = <V_m|w> partials: grid (nchunk, ceil(k/4)); 4 basis vectors per block
+constexpr int kMultiChunks = 256;
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_multidot_partial(const typename Vec2<R>::T *__restrict__ base, int k,
+                                                               const typename Vec2<R>::T *__restrict__ w,
+                                                               uint64_t dim, double *__restrict__ partials) {
+    const int m0 = blockIdx.y * 4;
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+    for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < dim; i += stride) {
+        const double wr = w[i].x, wi = w[i].y;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (m0 + q < k) {
+                const typename Vec2<R>::T x = base[(uint64_t)(m0 + q) * dim + i];
+                v[2 * q] += (double)x.x * wr + (double)x.y * wi;
+                v[2 * q + 1] += (double)x.x * wi - (double)x.y * wr;
+            }
+        }
+    }
+    // partial layout: partials[chunk * (2k) + 2m + {0,1}]
+    __shared__ double sh[8][kThreads];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) sh[q][threadIdx.x] = v[q];
+    __syncthreads();
+    for (int off = kThreads / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 8) {
+        const int q = threadIdx.x / 2;
+        if (m0 + q < k) partials[(size_t)blockIdx.x * (2 * k) + 2 * (m0 + q) + (threadIdx.x & 1)] = sh[threadIdx.x][0];
+    }
+}
+
+// w = (overwrite ? 0 : w) + sign * sum_m c[m] V_m
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_multi_axpy(const typename Vec2<R>::T *__restrict__ base, int k,
+                                                         const double *__restrict__ c, double sign,
+                                                         typename Vec2<R>::T *__restrict__ w, uint64_t dim,
+                                                         int overwrite) {
+    const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+    for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < dim; i += stride) {
+        double sr = 0.0, si = 0.0;
+        for (int m = 0; m < k; ++m) {
+            const typename Vec2<R>::T x = base[(uint64_t)m * dim + i];
+            const double cr = c[2 * m], ci = c[2 * m + 1];
+            sr += cr * x.x - ci * x.y;
+            si += cr * x.y + ci * x.x;
+        }
+        double wr = 0.0, wi = 0.0;
+        if (!overwrite) {
+            wr = w[i].x;
+            wi = w[i].y;
+        }
+        typename Vec2<R>::T o;
+        o.x = (R)(wr + sign * sr);
+        o.y = (R)(wi + sign * si);
+        w[i] = o;
+    }
+}
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) keyed by the seed,
+// counter = element index; two 53-bit uniforms -> Box-Muller, N(0, 1/2).
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+    for (int rnd = 0; rnd < 10; ++rnd) {
+        const uint64_t p0 = (uint64_t)M0 * c[0], p1 = (uint64_t)M1 * c[2];
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+        k0 += W0; k1 += W1;
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_gaussian(uint64_t seed, typename Vec2<R>::T *__restrict__ out,
+                                                       uint64_t dim) {
+    const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+    for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < dim; i += stride) {
+        uint32_t c[4] = {(uint32_t)i, (uint32_t)(i >> 32), 0u, 0u};
+        philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+        const uint64_t a = ((uint64_t)c[0] << 21) ^ (uint64_t)(c[1] >> 11);
+        const uint64_t b = ((uint64_t)c[2] << 21) ^ (uint64_t)(c[3] >> 11);
+        const double u1 = ((double)(a & ((1ull << 53) - 1)) + 0.5) * 0x1p-53;
+        const double u2 = ((double)(b & ((1ull << 53) - 1)) + 0.5) * 0x1p-53;
+        const double rad = sqrt(-log(u1));
+        const double th = 6.283185307179586476925286766559 * u2;
+        typename Vec2<R>::T o;
+        o.x = (R)(rad * cos(th));
+        o.y = (R)(rad * sin(th));
+        out[i] = o;
+    }
+}
+
+template <typename RS, typename RD>
+__global__ __launch_bounds__(kThreads) void k_convert(const typename Vec2<RS>::T *__restrict__ src,
+                                                      typename Vec2<RD>::T *__restrict__ dst, uint64_t dim) {
+    const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+    for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < dim; i += stride) {
+        typename Vec2<RD>::T o;
+        o.x = (RD)src[i].x;
+        o.y = (RD)src[i].y;
+        dst[i] = o;
+    }
+}
+
+inline uint32_t row_blocks(uint64_t dim) { return (uint32_t)((dim + kThreads - 1) / kThreads); }
+inline uint32_t stream_blocks(uint64_t dim) {
+    uint64_t b = (dim + kThreads - 1) / kThreads;
+    return (uint32_t)(b < 8192 ? b : 8192);
+}
+
+// A launch covers at most kMaxGrid blocks; larger row counts (N = 32) are
+// split into several launches that share one XCD mapping.
+constexpr uint32_t kMaxGrid = 1u << 22;
+
+}  // namespace
+
+template <typename R>
+static hipError_t launch_rows_hpsi(const HTables *t, const void *in, void *out, uint64_t dim, hipStream_t s) {
+    const uint32_t nblk = row_blocks(dim);
+    for (uint32_t base = 0; base < nblk; base += kMaxGrid) {
+        const uint32_t n = (nblk - base) < kMaxGrid ? (nblk - base) : kMaxGrid;
+        hipLaunchKernelGGL(k_hpsi<R>, dim3(n), dim3(kThreads), 0, s, t,
+                           (const typename Vec2<R>::T *)in, (typename Vec2<R>::T *)out, dim, nblk, base);
+    }
+    return hipGetLastError();
+}
+
+template <typename R>
+static hipError_t launch_rows_current(const CurrentTable *t, const void *in, void *out, uint64_t dim, hipStream_t s) {
+    const uint32_t nblk = row_blocks(dim);
+    for (uint32_t base = 0; base < nblk; base += kMaxGrid) {
+        const uint32_t n = (nblk - base) < kMaxGrid ? (nblk - base) : kMaxGrid;
+        hipLaunchKernelGGL(k_current<R>, dim3(n), dim3(kThreads), 0, s, t,
+                           (const typename Vec2<R>::T *)in, (typename Vec2<R>::T *)out, dim, nblk, base);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_hpsi_c128(const HTables *t, const void *in, void *out, uint64_t dim, hipStream_t s) {
+    return launch_rows_hpsi<double>(t, in, out, dim, s);
+}
+hipError_t launch_hpsi_c64(const HTables *t, const void *in, void *out, uint64_t dim, hipStream_t s) {
+    return launch_rows_hpsi<float>(t, in, out, dim, s);
+}
+hipError_t launch_current_c128(const CurrentTable *t, const void *in, void *out, uint64_t dim, hipStream_t s) {
+    return launch_rows_current<double>(t, in, out, dim, s);
+}
+hipError_t launch_current_c64(const CurrentTable *t, const void *in, void *out, uint64_t dim, hipStream_t s) {
+    return launch_rows_current<float>(t, in, out, dim, s);
+}
+
+#define ESC_DISPATCH(c64, KERNEL, ...)                                  \
+    do {                                                                \
+        if (c64) hipLaunchKernelGGL(KERNEL<float>, __VA_ARGS__);        \
+        else hipLaunchKernelGGL(KERNEL<double>, __VA_ARGS__);           \
+    } while (0)
+
+template <typename R> using V2p = typename Vec2<R>::T *;
+template <typename R> using CV2p = const typename Vec2<R>::T *;
+
+hipError_t launch_dot(bool c64, const void *x, const void *y, uint64_t dim, double *partials, double *result,
+                      hipStream_t s) {
+    if (c64)
+        hipLaunchKernelGGL(k_dot_partial<float>, dim3(kRedBlocks), dim3(kThreads), 0, s, (CV2p<float>)x,
+                           (CV2p<float>)y, dim, partials);
+    else
+        hipLaunchKernelGGL(k_dot_partial<double>, dim3(kRedBlocks), dim3(kThreads), 0, s, (CV2p<double>)x,
+                           (CV2p<double>)y, dim, partials);
+    hipLaunchKernelGGL(k_reduce_final, dim3(2), dim3(kThreads), 0, s, partials, kRedBlocks, 2, result);
+    return hipGetLastError();
+}
+
+hipError_t launch_norm2(bool c64, const void *x, uint64_t dim, double *partials, double *result, hipStream_t s) {
+    if (c64)
+        hipLaunchKernelGGL(k_norm2_partial<float>, dim3(kRedBlocks), dim3(kThreads), 0, s, (CV2p<float>)x, dim,
+                           partials);
+    else
+        hipLaunchKernelGGL(k_norm2_partial<double>, dim3(kRedBlocks), dim3(kThreads), 0, s, (CV2p<double>)x, dim,
+                           partials);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kThreads), 0, s, partials, kRedBlocks, 1, result);
+    return hipGetLastError();
+}
+
+hipError_t launch_axpby(bool c64, double ar, double ai, const void *x, double br, double bi, void *y, uint64_t dim,
+                        hipStream_t s) {
+    if (c64)
+        hipLaunchKernelGGL(k_axpby<float>, dim3(stream_blocks(dim)), dim3(kThreads), 0, s, ar, ai, (CV2p<float>)x, br,
+                           bi, (V2p<float>)y, dim);
+    else
+        hipLaunchKernelGGL(k_axpby<double>, dim3(stream_blocks(dim)), dim3(kThreads), 0, s, ar, ai, (CV2p<double>)x,
+                           br, bi, (V2p<double>)y, dim);
+    return hipGetLastError();
+}
+
+hipError_t launch_lanczos_update(bool c64, void *w, const void *v, const void *u, const double *d_alpha, double beta,
+                                 uint64_t dim, double *partials, double *result, hipStream_t s) {
+    if (c64)
+        hipLaunchKernelGGL(k_lanczos_update<float>, dim3(kRedBlocks), dim3(kThreads), 0, s, (V2p<float>)w,
+                           (CV2p<float>)v, (CV2p<float>)u, d_alpha, beta, dim, partials);
+    else
+        hipLaunchKernelGGL(k_lanczos_update<double>, dim3(kRedBlocks), dim3(kThreads), 0, s, (V2p<double>)w,
+                           (CV2p<double>)v, (CV2p<double>)u, d_alpha, beta, dim, partials);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kThreads), 0, s, partials, kRedBlocks, 1, result);
+    return hipGetLastError();
+}
+
+hipError_t launch_normalize(bool c64, void *x, const double *d_norm2, uint64_t dim, hipStream_t s) {
+    if (c64)
+        hipLaunchKernelGGL(k_normalize<float>, dim3(stream_blocks(dim)), dim3(kThreads), 0, s, (V2p<float>)x, d_norm2,
+                           dim);
+    else
+        hipLaunchKernelGGL(k_normalize<double>, dim3(stream_blocks(dim)), dim3(kThreads), 0, s, (V2p<double>)x,
+                           d_norm2, dim);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale(bool c64, void *x, double sc, uint64_t dim, hipStream_t s) {
+    if (c64)
+        hipLaunchKernelGGL(k_scale<float>, dim3(stream_blocks(dim)), dim3(kThreads), 0, s, (V2p<float>)x, sc, dim);
+    else
+        hipLaunchKernelGGL(k_scale<double>, dim3(stream_blocks(dim)), dim3(kThreads), 0, s, (V2p<double>)x, sc, dim);
+    return hipGetLastError();
+}
+
+hipError_t launch_multidot(bool c64, const void *base, int k, const void *w, uint64_t dim, double *partials,
+                           double *result, hipStream_t s) {
+    if (k <= 0) return hipSuccess;
+    dim3 grid(kMultiChunks, (k + 3) / 4);
+    if (c64)
+        hipLaunchKernelGGL(k_multidot_partial<float>, grid, dim3(kThreads), 0, s, (CV2p<float>)base, k,
+                           (CV2p<float>)w, dim, partials);
+    else
+        hipLaunchKernelGGL(k_multidot_partial<double>, grid, dim3(kThreads), 0, s, (CV2p<double>)base, k,
+                           (CV2p<double>)w, dim, partials);
+    hipLaunchKernelGGL(k_reduce_final, dim3(2 * k), dim3(kThreads), 0, s, partials, kMultiChunks, 2 * k, result);
+    return hipGetLastError();
+}
+
+hipError_t launch_multi_axpy(bool c64, const void *base, int k, const double *d_coef, double sign, void *w,
+                             uint64_t dim, bool overwrite, hipStream_t s) {
+    if (c64)
+        hipLaunchKernelGGL(k_multi_axpy<float>, dim3(stream_blocks(dim)), dim3(kThreads), 0, s, (CV2p<float>)base, k,
+                           d_coef, sign, (V2p<float>)w, dim, (int)overwrite);
+    else
+        hipLaunchKernelGGL(k_multi_axpy<double>, dim3(stream_blocks(dim)), dim3(kThreads), 0, s, (CV2p<double>)base,
+                           k, d_coef, sign, (V2p<double>)w, dim, (int)overwrite);
+    return hipGetLastError();
+}
+
+hipError_t launch_gaussian(bool c64, uint64_t seed, void *out, uint64_t dim, hipStream_t s) {
+    if (c64)
+        hipLaunchKernelGGL(k_gaussian<float>, dim3(stream_blocks(dim)), dim3(kThreads), 0, s, seed, (V2p<float>)out,
+                           dim);
+    else
+        hipLaunchKernelGGL(k_gaussian<double>, dim3(stream_blocks(dim)), dim3(kThreads), 0, s, seed,
+                           (V2p<double>)out, dim);
+    return hipGetLastError();
+}
+
+hipError_t launch_convert_from_c128(bool c64, const void *src, void *dst, uint64_t dim, hipStream_t s) {
+    if (c64)
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(stream_blocks(dim)), dim3(kThreads), 0, s,
+                           (CV2p<double>)src, (V2p<float>)dst, dim);
+    else
+        hipLaunchKernelGGL((k_convert<double, double>), dim3(stream_blocks(dim)), dim3(kThreads), 0, s,
+                           (CV2p<double>)src, (V2p<double>)dst, dim);
+    return hipGetLastError();
+}
+
+hipError_t launch_convert_to_c128(bool c64, const void *src, void *dst, uint64_t dim, hipStream_t s) {
+    if (c64)
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(stream_blocks(dim)), dim3(kThreads), 0, s,
+                           (CV2p<float>)src, (V2p<double>)dst, dim);
+    else
+        hipLaunchKernelGGL((k_convert<double, double>), dim3(stream_blocks(dim)), dim3(kThreads), 0, s,
+                           (CV2p<double>)src, (V2p<double>)dst, dim);
+    return hipGetLastError();
+}
+
+}  // namespace esc

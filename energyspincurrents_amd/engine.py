@@ -1,0 +1,288 @@
+"""Python mirror of the reference's operator interfaces over the C ABI.
+
+Classes follow the reference roles (src/Lattice.h, src/Hamiltonian.h,
+src/DynLanczosSolver.h, src/Observables.h, src/dynamics.h):
+
+  Honeycomb   Lattice::Honeycomb1 + Hamiltonian::Connectors  (esc_honeycomb / esc_connectors)
+  SpinModel   the coupling set (Kxx..Kzz, Bxx..Bzz, and the J / Gamma extensions)
+  Engine      one device context: H.psi (MatrixVectorMult), ApplyOp / ApplyJ*Total,
+              Lanczos_Nirav, overlaps, StartDynamics (esc_dynamics)
+
+Device vectors are torch tensors (complex128 for "c128", complex64 for "c64")
+on the current HIP device; torch supplies memory and the stream only, every
+operation runs in libesc.so.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._capi import (DYN_KINDS, ESC_C64, ESC_C128, LANC_MODES, OPS, SITE_TOTAL, SPEC_METHODS, START_KINDS,
+                    EscModel, EscParams, check, lib)
+
+
+def _np_ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _dev_ptr(x):
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return ctypes.c_void_p(x)
+    return ctypes.c_void_p(x.data_ptr())
+
+
+def _stream(stream):
+    if stream is not None:
+        return ctypes.c_void_p(stream if isinstance(stream, int) else stream.cuda_stream)
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Honeycomb:
+    """Honeycomb torus with Lx unit cells along a1 and Ly/2 along a2 (src/Lattice.h:434-576)."""
+
+    def __init__(self, Lx, Ly, periodic_x=True, periodic_y=True):
+        self.Lx, self.Ly = int(Lx), int(Ly)
+        self.periodic_x, self.periodic_y = bool(periodic_x), bool(periodic_y)
+        self.N = self.Lx * self.Ly
+        self.neigh = np.empty((self.N, 3), np.int32)
+        check(lib().esc_honeycomb(self.Lx, self.Ly, int(periodic_x), int(periodic_y), _np_ptr(self.neigh)))
+
+    def bonds(self, K=(1.0, 1.0, 1.0)):
+        """Bond families x, y, z as (n, 2) int32 arrays (src/Hamiltonian.h:230-328)."""
+        Kd = np.asarray(K, np.float64)
+        px = np.zeros((self.N, 2), np.int32)
+        py = np.zeros((self.N, 2), np.int32)
+        pz = np.zeros((self.N, 2), np.int32)
+        cnt = np.zeros(3, np.int32)
+        check(lib().esc_connectors(self.N, _np_ptr(self.neigh), _np_ptr(Kd), _np_ptr(px), _np_ptr(py), _np_ptr(pz),
+                                   _np_ptr(cnt)))
+        return [px[: cnt[0]].copy(), py[: cnt[1]].copy(), pz[: cnt[2]].copy()]
+
+
+def field111(h):
+    b = h / np.sqrt(3.0)
+    return (b, b, b)
+
+
+@dataclass
+class SpinModel:
+    lattice: Honeycomb
+    K: tuple = (1.0, 1.0, 1.0)
+    J: float = 0.0
+    Gamma: float = 0.0
+    B: tuple = (0.0, 0.0, 0.0)
+    skip_single_bond_family: bool = False   # reference rule, src/Hamiltonian.h:116,143,347
+
+    def struct(self):
+        """esc_model for this model; keeps the arrays it points into alive on self."""
+        # families exist for every lattice bond; a zero coupling only zeroes the weight
+        self._bonds = self.lattice.bonds((1.0, 1.0, 1.0))
+        if self.skip_single_bond_family:
+            self._bonds = [b if len(b) != 1 else b[:0] for b in self._bonds]
+        m = EscModel()
+        m.n_sites = self.lattice.N
+        for f in range(3):
+            self._bonds[f] = np.ascontiguousarray(self._bonds[f], np.int32)
+            m.n_bonds[f] = len(self._bonds[f])
+            m.bonds[f] = self._bonds[f].ctypes.data if len(self._bonds[f]) else None
+        m.neigh = self.lattice.neigh.ctypes.data
+        for f in range(3):
+            m.K[f] = float(self.K[f])
+            m.B[f] = float(self.B[f])
+        m.J = float(self.J)
+        m.Gamma = float(self.Gamma)
+        return m
+
+
+@dataclass
+class LanczosResult:
+    alpha: np.ndarray
+    beta: np.ndarray
+    iters: int
+    evals: np.ndarray
+    Z: np.ndarray          # (iters, iters), column k = eigenvector k
+
+
+class Engine:
+    """One esc_ctx: the device-side operator set of one model."""
+
+    def __init__(self, model: SpinModel, precision="c128"):
+        self.model = model
+        self.precision = precision
+        self._m = model.struct()
+        h = ctypes.c_void_p()
+        check(lib().esc_ctx_create(ctypes.byref(self._m), ESC_C64 if precision == "c64" else ESC_C128,
+                                   ctypes.byref(h)))
+        self._h = h
+        d, e, p = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
+        check(lib().esc_ctx_info(h, ctypes.byref(d), ctypes.byref(e), ctypes.byref(p)))
+        self.dim, self.elem_bytes = d.value, e.value
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            check(lib().esc_ctx_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- device vectors ---------------------------------------------------
+    @property
+    def torch_dtype(self):
+        import torch
+        return torch.complex64 if self.precision == "c64" else torch.complex128
+
+    def empty(self):
+        import torch
+        return torch.empty(self.dim, dtype=self.torch_dtype, device="cuda")
+
+    def upload(self, host: np.ndarray, out=None):
+        out = self.empty() if out is None else out
+        h = np.ascontiguousarray(host, np.complex128)
+        check(lib().esc_upload(self._h, _np_ptr(h), _dev_ptr(out), _stream(None)))
+        return out
+
+    def download(self, x) -> np.ndarray:
+        h = np.empty(self.dim, np.complex128)
+        check(lib().esc_download(self._h, _dev_ptr(x), _np_ptr(h), _stream(None)))
+        return h
+
+    # -- operators --------------------------------------------------------
+    def hpsi(self, x, out=None, stream=None):
+        """out = H x  (Hamiltonian::MatrixVectorMult, src/Hamiltonian.h:480-557)."""
+        out = self.empty() if out is None else out
+        check(lib().esc_hpsi(self._h, _dev_ptr(x), _dev_ptr(out), _stream(stream)))
+        return out
+
+    def apply_op(self, op: str, site: int, x, out=None, stream=None):
+        """Observables::ApplyOp / ApplyJ*Total (site = -1) (src/Observables.h:24-67,448-470,664-699)."""
+        out = self.empty() if out is None else out
+        check(lib().esc_apply_op(self._h, OPS[op], int(site), _dev_ptr(x), _dev_ptr(out), _stream(stream)))
+        return out
+
+    def apply_total(self, op: str, x, out=None, stream=None):
+        return self.apply_op(op, SITE_TOTAL, x, out, stream)
+
+    def start_vector(self, seed: int, kind="Uniform", out=None, stream=None):
+        out = self.empty() if out is None else out
+        check(lib().esc_start_vector(self._h, ctypes.c_uint64(seed), START_KINDS[kind], _dev_ptr(out),
+                                     _stream(stream)))
+        return out
+
+    def dot(self, x, y, stream=None) -> complex:
+        r = np.zeros(2)
+        check(lib().esc_dot(self._h, _dev_ptr(x), _dev_ptr(y), _np_ptr(r), _stream(stream)))
+        return complex(r[0], r[1])
+
+    def axpby(self, a: complex, x, b: complex, y, stream=None):
+        av = np.array([a.real, a.imag]) if isinstance(a, complex) else np.array([float(a), 0.0])
+        bv = np.array([b.real, b.imag]) if isinstance(b, complex) else np.array([float(b), 0.0])
+        check(lib().esc_axpby(self._h, _np_ptr(av), _dev_ptr(x), _np_ptr(bv), _dev_ptr(y), _stream(stream)))
+        return y
+
+    # -- Lanczos ----------------------------------------------------------
+    def lanczos(self, v0, steps: int, eps=0.0, mode="All", gs=None, stream=None) -> LanczosResult:
+        """DynLanczosSolver::Lanczos_Nirav (src/DynLanczosSolver.h:38-239)."""
+        a = np.zeros(steps + 1)
+        b = np.zeros(steps + 2)
+        ev = np.zeros(steps + 1)
+        Z = np.zeros((steps + 1) * (steps + 1))
+        it = ctypes.c_int()
+        check(lib().esc_lanczos(self._h, _dev_ptr(v0), int(steps), float(eps), LANC_MODES[mode], _np_ptr(a),
+                                _np_ptr(b), ctypes.byref(it), _np_ptr(ev), _np_ptr(Z), _dev_ptr(gs),
+                                _stream(stream)))
+        n = it.value
+        return LanczosResult(a[:n].copy(), b[: n + 1].copy(), n, ev[:n].copy(),
+                             Z[: n * n].reshape(n, n).T.copy())
+
+    def ritz_overlaps(self, vec, n_states, stream=None) -> np.ndarray:
+        out = np.zeros(2 * n_states)
+        check(lib().esc_ritz_overlaps(self._h, _dev_ptr(vec), int(n_states), _np_ptr(out), _stream(stream)))
+        return out[0::2] + 1j * out[1::2]
+
+    def ritz_vector(self, k, out=None, stream=None):
+        out = self.empty() if out is None else out
+        check(lib().esc_ritz_vector(self._h, int(k), _dev_ptr(out), _stream(stream)))
+        return out
+
+    def lanczos_cf(self, phi, steps, stream=None):
+        a = np.zeros(steps + 1)
+        b = np.zeros(steps + 2)
+        it = ctypes.c_int()
+        n2 = ctypes.c_double()
+        check(lib().esc_lanczos_cf(self._h, _dev_ptr(phi), int(steps), _np_ptr(a), _np_ptr(b), ctypes.byref(it),
+                                   ctypes.byref(n2), _stream(stream)))
+        n = it.value
+        return a[:n].copy(), b[: n + 1].copy(), n2.value
+
+    def dynamics(self, kind: str, method: str, gs, omega, eta, e0=0.0, cf_steps=200, stream=None):
+        """StartDynamics -> DynJsJs / DynJeJe without the file (src/dynamics.h:43-60,222-460).
+        Returns (table, expectations)."""
+        omega = np.ascontiguousarray(omega, np.float64)
+        ncols = 14 if kind == "JsJs" else 5
+        table = np.zeros((len(omega), 1 + ncols))
+        expect = np.zeros(2 * 4)
+        check(lib().esc_dynamics(self._h, DYN_KINDS[kind], SPEC_METHODS[method], _dev_ptr(gs), float(e0),
+                                 int(cf_steps), float(eta), _np_ptr(omega), len(omega), _np_ptr(table),
+                                 _np_ptr(expect), _stream(stream)))
+        nexp = 4 if kind == "JsJs" else 3
+        return table, (expect[0::2] + 1j * expect[1::2])[:nexp]
+
+
+# -- host FP64 small solves -------------------------------------------------
+def tridiag_eig(alpha, beta, vectors=True):
+    a = np.ascontiguousarray(np.real(alpha), np.float64)
+    b = np.ascontiguousarray(np.abs(beta), np.float64)
+    n = len(a)
+    ev = np.zeros(n)
+    Z = np.zeros(n * n) if vectors else None
+    check(lib().esc_tridiag_eig(n, _np_ptr(a), _np_ptr(b), _np_ptr(ev), _np_ptr(Z) if vectors else None))
+    return ev, (Z.reshape(n, n).T.copy() if vectors else None)
+
+
+def cf_spectrum(alpha, beta, norm2, e0, eta, omega, elastic=None):
+    a = np.ascontiguousarray(alpha, np.float64)
+    b = np.ascontiguousarray(beta, np.float64)
+    om = np.ascontiguousarray(omega, np.float64)
+    out = np.zeros(2 * len(om))
+    el = None if elastic is None else np.array([complex(elastic).real, complex(elastic).imag])
+    check(lib().esc_cf_spectrum(len(a), _np_ptr(a), _np_ptr(b), float(norm2), float(e0), float(eta), _np_ptr(om),
+                                len(om), _np_ptr(el) if el is not None else None, _np_ptr(out)))
+    return out[0::2] + 1j * out[1::2]
+
+
+def lehmann_spectrum(evals, Oa, Ob, eta, omega, elastic=None):
+    ev = np.ascontiguousarray(evals, np.float64)
+    oa = np.ascontiguousarray(np.column_stack([np.real(Oa), np.imag(Oa)]).ravel())
+    ob = np.ascontiguousarray(np.column_stack([np.real(Ob), np.imag(Ob)]).ravel())
+    om = np.ascontiguousarray(omega, np.float64)
+    out = np.zeros(2 * len(om))
+    el = None if elastic is None else np.array([complex(elastic).real, complex(elastic).imag])
+    check(lib().esc_lehmann_spectrum(len(ev), _np_ptr(ev), _np_ptr(oa), _np_ptr(ob),
+                                     _np_ptr(el) if el is not None else None, float(eta), _np_ptr(om), len(om),
+                                     _np_ptr(out)))
+    return out[0::2] + 1j * out[1::2]
+
+
+def load_params(path) -> dict:
+    """ConstVariables(inputfile) (src/ParametersEngine.h:53-229) as a dict."""
+    p = EscParams()
+    check(lib().esc_params_load(str(path).encode(), ctypes.byref(p)))
+    out = {}
+    for name, _ in EscParams._fields_:
+        v = getattr(p, name)
+        out[name] = v.decode() if isinstance(v, bytes) else v
+    return out
+
+
+def reference_omega():
+    """omega = domega * om, om = 0..1000, domega = 0.002 (src/dynamics.h:231-232,270-271)."""
+    return np.array([0.002 * k for k in range(1001)])

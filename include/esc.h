@@ -1,0 +1,231 @@
+/*
+ * esc.h -- C ABI of the MI355X Lanczos dynamical-correlation engine.
+ *
+ * Plain C types only: device buffers are passed as `void *` (caller-owned device
+ * memory holding interleaved complex numbers, complex<double> for ESC_C128
+ * contexts and complex<float> for ESC_C64), host arrays as typed pointers,
+ * streams as `void *` (a hipStream_t; NULL = the default stream).
+ *
+ * Every call returns ESC_OK (0) or a negative status and then sets a
+ * thread-local message readable with esc_last_error().  No C++ exception
+ * crosses this boundary.  Calls on one context must not run concurrently;
+ * distinct contexts are independent.
+ *
+ * Basis convention (as the reference): index r has bit k = site k and bit value
+ * 0 = spin up (S^z = +1/2), S = sigma/2 (src/QuantumBasis.h:34-74).
+ *
+ * Reference interfaces replaced (paths relative to the fengshi96/EnergySpinCurrents
+ * root) are named on each entry point.
+ */
+#ifndef ESC_H
+#define ESC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ESC_OK               0
+#define ESC_ERR_ARG         -1   /* invalid argument (std::invalid_argument in the reference) */
+#define ESC_ERR_HIP         -2   /* HIP runtime failure */
+#define ESC_ERR_ALLOC       -3   /* device or host allocation failed */
+#define ESC_ERR_BREAKDOWN   -4   /* Lanczos beta == 0 (invariant subspace) */
+#define ESC_ERR_NUMERIC     -5   /* non-finite input or eigensolver failure (runtime_error) */
+#define ESC_ERR_UNSUPPORTED -6   /* model shape the kernels do not handle */
+
+/* Vector storage precision of a context.  Scalars (alpha, beta, dots, norms,
+ * tridiagonal solve, spectra) are always FP64. */
+#define ESC_C128 0
+#define ESC_C64  1
+
+typedef struct esc_ctx esc_ctx;
+
+/* Spin model on an explicit bond graph.
+ * Replaces the data Hamiltonian::SetupConnectors/Connectors derives
+ * (src/Hamiltonian.h:27-32,44-54,230-328) plus Lattice::N1neigh_
+ * (src/Lattice.h:29).  Bond family f = 0 (x), 1 (y), 2 (z) carries the Kitaev
+ * term K[f] S^f_i S^f_j.  Extensions with no reference counterpart: Heisenberg J
+ * (all bonds) and Gamma (S^a_i S^b_j + S^b_i S^a_j on a f-bond, {a,b} the
+ * other two axes).  Field term: +B . S on every site (src/Hamiltonian.h:167-198,
+ * 363-375).  The diagonal is accumulated as the reference does: z bonds in
+ * the given order, then (if J != 0) all bonds, then Bzz * Sz_total. */
+typedef struct {
+    int32_t n_sites;              /* N, 2 <= N <= 32 */
+    int32_t n_bonds[3];           /* bonds per family (x, y, z) */
+    const int32_t *bonds[3];      /* family f: n_bonds[f] pairs (i, j), i != j */
+    const int32_t *neigh;         /* N x 3 row-major neighbour table, column 0 = z,
+                                     1 = y, 2 = x, -1 = none; needed by the current
+                                     operators only (may be NULL) */
+    double K[3];                  /* Kxx, Kyy, Kzz */
+    double J;                     /* Heisenberg coupling (extension) */
+    double Gamma;                 /* Gamma coupling (extension) */
+    double B[3];                  /* Bxx, Byy, Bzz */
+} esc_model;
+
+/* Operators for esc_apply_op (src/Observables.h:24-67 ApplyOp labels). */
+#define ESC_OP_SX   0   /* S^x_site            src/Observables.h:506-521 */
+#define ESC_OP_SY   1   /* S^y_site            src/Observables.h:488-503 */
+#define ESC_OP_SZ   2   /* S^z_site            src/Observables.h:429-445 */
+#define ESC_OP_JSX  3   /* spin current x      src/Observables.h:243-281 (site), :664-674 (total) */
+#define ESC_OP_JSY  4   /* spin current y      src/Observables.h:202-240, :676-686 */
+#define ESC_OP_JSZ  5   /* spin current z      src/Observables.h:161-199, :689-699 */
+#define ESC_OP_JEX  6   /* energy current x    src/Observables.h:358-389 (A site), :448-458 (total over even sites) */
+#define ESC_OP_JEY  7   /* energy current y    src/Observables.h:321-354 (B site), :460-470 (total over odd sites) */
+#define ESC_SITE_TOTAL (-1)
+
+/* Lanczos modes (src/DynLanczosSolver.h:38-239, LancType). */
+#define ESC_LANC_ONLYEGS 0   /* no reorthogonalisation, no state formed */
+#define ESC_LANC_GS      1   /* reorthogonalised; ground state formed */
+#define ESC_LANC_ALL     2   /* reorthogonalised; basis kept for Ritz overlaps */
+#define ESC_LANC_PLAIN   3   /* three-vector recursion, no basis kept (N=32 path) */
+
+/* Start vectors (src/DynLanczosSolver.h:58-59,465-474). */
+#define ESC_START_REF_UNIFORM  0  /* mt19937_64(seed), re/im = -1 + 2 U[0,1), drawn re then im */
+#define ESC_START_GAUSSIAN     1  /* counter-based (Philox4x32-10) N(0,1/2) per component */
+
+/* ---------------------------------------------------------------- errors -- */
+const char *esc_last_error(void);
+int esc_version(void);
+
+/* --------------------------------------------------------------- context -- */
+/* Build device tables for a model.  Replaces Hamiltonian(...) + TightB_Ham()
+ * + Diagonal() (src/Hamiltonian.h:20-25,94-206,331-377): no COO lists or
+ * diagonal are stored; matrix elements are formed on the fly. */
+int esc_ctx_create(const esc_model *model, int precision, esc_ctx **out);
+int esc_ctx_destroy(esc_ctx *ctx);
+/* Vector length (2^N) and bytes per element (16 or 8) of a context. */
+int esc_ctx_info(const esc_ctx *ctx, int64_t *dim, int32_t *elem_bytes, int32_t *precision);
+
+/* ---------------------------------------------------------------- kernels -- */
+/* d_out = H d_in.  Replaces Hamiltonian::MatrixVectorMult
+ * (src/Hamiltonian.h:480-557).  d_in and d_out must not overlap. */
+int esc_hpsi(esc_ctx *ctx, const void *d_in, void *d_out, void *stream);
+
+/* d_out = Op(site) d_in, or the site sum for site == ESC_SITE_TOTAL.
+ * Replaces Observables::ApplyOp (src/Observables.h:24-67) and
+ * ApplyJ{sx,sy,sz,ex,ey}Total (:448-470,664-699).  Current operators need
+ * model.neigh; a missing neighbour is ESC_ERR_ARG. */
+int esc_apply_op(esc_ctx *ctx, int op, int site, const void *d_in, void *d_out, void *stream);
+
+/* ------------------------------------------------------ vector utilities -- */
+/* Start vector, normalised.  Replaces DynLanczosSolver::Randomize + normalize
+ * (src/DynLanczosSolver.h:58-59,465-474). */
+int esc_start_vector(esc_ctx *ctx, uint64_t seed, int kind, void *d_out, void *stream);
+/* out[0] + i out[1] = <x|y> (conjugate-linear in x), FP64 accumulate,
+ * deterministic.  Replaces Eigen's VectorXcd::dot as used at
+ * src/DynLanczosSolver.h:76,95 and src/dynamics.h:250-262,370-381. */
+int esc_dot(esc_ctx *ctx, const void *d_x, const void *d_y, double *out, void *stream);
+/* d_y = a d_x + b d_y, complex a = (a[0], a[1]), b likewise. */
+int esc_axpby(esc_ctx *ctx, const double *a, const void *d_x, const double *b, void *d_y, void *stream);
+/* Copy host complex<double> data into a context-precision device vector and back. */
+int esc_upload(esc_ctx *ctx, const double *host, void *d_out, void *stream);
+int esc_download(esc_ctx *ctx, const void *d_in, double *host, void *stream);
+
+/* ---------------------------------------------------------------- Lanczos -- */
+/* Lanczos from the normalised d_v0.  Replaces DynLanczosSolver::Lanczos_Nirav
+ * (src/DynLanczosSolver.h:38-239).  alpha[0..iters-1], beta[0..iters]
+ * (beta[0] = 0) are FP64 and recorded before reorthogonalisation, as the
+ * reference does.  Exit at iteration steps-1 or when |dE0| < eps (:139-162).
+ * evals (iters, ascending) and Z (iters x iters, column-major, column k =
+ * eigenvector k) describe the final tridiagonal matrix.  For ESC_LANC_GS and
+ * ESC_LANC_ALL the ground state sum_m Z(m,0) v_m is written to d_gs (may be
+ * NULL otherwise).  ESC_LANC_ALL keeps the basis in the context for
+ * esc_ritz_overlaps / esc_ritz_vector.  Capacity: steps <= 4096. */
+int esc_lanczos(esc_ctx *ctx, const void *d_v0, int steps, double eps, int mode,
+                double *alpha, double *beta, int *iters, double *evals, double *Z,
+                void *d_gs, void *stream);
+
+/* O[n] = <psi_n|d_vec>, n < n_states, for the Ritz states of the last
+ * ESC_LANC_ALL run, computed as sum_m conj(Z(m,n)) <v_m|d_vec> without forming
+ * the states.  Replaces the PsiAll construction plus Overlap loops
+ * (src/DynLanczosSolver.h:208-234; src/dynamics.h:255-263,375-383).
+ * out is interleaved complex (2 * n_states doubles). */
+int esc_ritz_overlaps(esc_ctx *ctx, const void *d_vec, int n_states, double *out, void *stream);
+/* d_out = PsiAll[k] = sum_m Z(m,k) v_m (src/DynLanczosSolver.h:213-223). */
+int esc_ritz_vector(esc_ctx *ctx, int k, void *d_out, void *stream);
+
+/* Plain three-vector Lanczos (no reorthogonalisation, no basis) from
+ * d_phi / |d_phi| for the continued-fraction spectra; norm2 = |d_phi|^2.
+ * No reference counterpart (the reference uses Ritz-Lehmann sums). */
+int esc_lanczos_cf(esc_ctx *ctx, const void *d_phi, int steps, double *alpha, double *beta,
+                   int *iters, double *norm2, void *stream);
+
+/* ------------------------------------------------------------- dynamics -- */
+#define ESC_DYN_JSJS 0   /* DynJsJs: 14 value columns (Re, Im) xx yy zz xy xz yz tt */
+#define ESC_DYN_JEJE 1   /* DynJeJe: 5 value columns Im xx yy xy yx tt */
+#define ESC_SPEC_RITZ 0  /* Lehmann sum over the Ritz states of the last ESC_LANC_ALL run */
+#define ESC_SPEC_CF   1  /* continued fractions from J|gs> (cross terms by polarisation) */
+
+/* Full dynamical-correlation pipeline for one kind, replacing
+ * StartDynamics -> DynJsJs / DynJeJe (src/dynamics.h:43-60,222-460) minus the
+ * file writing: currents applied to d_gs, <J_a> and overlaps (RITZ) or
+ * Lanczos continued fractions (CF, cf_steps each), evaluated on omega.
+ * table is row-major nw x (1 + ncols): omega, then the columns in the
+ * reference's file order; ncols = 14 (JSJS) or 5 (JEJE).  The JeJe yx column
+ * carries no elastic subtraction (the reference subtracts an uninitialised
+ * Jez there, src/dynamics.h:368-369,424).  e0 is used by CF only (RITZ uses the
+ * lowest Ritz value, src/dynamics.h:234,357).  expect (may be NULL) receives
+ * <J_a> for a = x, y, (z,) t as interleaved complex. */
+int esc_dynamics(esc_ctx *ctx, int kind, int method, const void *d_gs, double e0, int cf_steps,
+                 double eta, const double *omega, int nw, double *table, double *expect, void *stream);
+
+/* ------------------------------------------------- lattice and parameters -- */
+/* Honeycomb torus neighbour table (N x 3 row-major, column 0 = z, 1 = y,
+ * 2 = x, -1 = none) for Lx unit cells along a1 and Ly/2 along a2, site numbering
+ * A = 2 (i Ly/2 + j), B = A + 1.  Replaces Lattice::Honeycomb1
+ * (src/Lattice.h:434-576).  Ly must be even. */
+int esc_honeycomb(int Lx, int Ly, int periodic_x, int periodic_y, int32_t *neigh);
+/* Bond families from a neighbour table, pairs (i < j) in lexicographic order;
+ * a family whose coupling K[f] is 0 is empty.  Replaces Hamiltonian::Connectors
+ * (src/Hamiltonian.h:230-328).  Each pairs_f holds up to N pairs. */
+int esc_connectors(int n_sites, const int32_t *neigh, const double *K, int32_t *pairs_x, int32_t *pairs_y,
+                   int32_t *pairs_z, int32_t *counts);
+
+/* Parameter file (key=value, first exact key match wins), replacing
+ * ConstVariables::Initialize / matchstring / matchstring2
+ * (src/ParametersEngine.h:53-229) including its defaults and checks.
+ * Extension keys (optional): J, Gamma, Eta, OmegaMin, OmegaMax, OmegaPoints,
+ * Precision (c128|c64), SpectralMode (Ritz|CF), StartVector (Uniform|Gaussian),
+ * CFSteps. */
+typedef struct {
+    int32_t Lx, Ly, NumberofSites, RandomSeed, LanczosSteps, Threads;
+    int32_t IsPeriodicX, IsPeriodicY, LocalSkw, SpinCurrDynamics, EnergyCurrDynamics;
+    double Kxx, Kyy, Kzz, Bxx, Byy, Bzz, Temperature, LanczosEps;
+    char Model[64], Solver[64], LancType[64];
+    /* extensions */
+    double J, Gamma, Eta, OmegaMin, OmegaMax;
+    int32_t OmegaPoints;      /* 0 = the reference grid omega = 0.002 k, k <= 1000 */
+    int32_t Precision;        /* ESC_C128 / ESC_C64 */
+    int32_t SpectralMode;     /* ESC_SPEC_RITZ / ESC_SPEC_CF */
+    int32_t StartVector;      /* ESC_START_* */
+    int32_t CFSteps;
+} esc_params;
+int esc_params_load(const char *path, esc_params *out);
+
+/* ------------------------------------------------ host FP64 small solves -- */
+/* Symmetric tridiagonal eigensolve (implicit QL, Wilkinson shift), n <= 4096:
+ * diag a[0..n-1], off-diagonal b[1..n-1] (b[0] ignored), as TriDiag builds T
+ * (src/DynLanczosSolver.h:371-414).  Eigenvalues ascending; Z column-major
+ * (may be NULL for eigenvalues only). */
+int esc_tridiag_eig(int n, const double *a, const double *b, double *evals, double *Z);
+
+/* Continued fraction: G(w) = norm2 / (z - a0 - b1^2/(z - a1 - ...)),
+ * z = w + e0 - i eta, for nw frequencies.  Optional elastic subtraction of
+ * |<J>|^2/(w - i eta) when elastic != NULL (elastic[0] + i elastic[1] =
+ * conj(<J_a>) <J_b>).  out: interleaved complex.  No reference counterpart. */
+int esc_cf_spectrum(int n, const double *a, const double *b, double norm2, double e0, double eta,
+                    const double *omega, int nw, const double *elastic, double *out);
+
+/* Lehmann sum over Ritz or exact states, as DynJsJs/DynJeJe evaluate it
+ * (src/dynamics.h:270-316,389-425):
+ * I(w) = sum_n conj(Oa[n]) Ob[n] / (w - (E_n - E_0) - i eta) - elastic/(w - i eta),
+ * E_0 = evals[0]; Oa, Ob, elastic interleaved complex; out interleaved. */
+int esc_lehmann_spectrum(int n, const double *evals, const double *Oa, const double *Ob,
+                         const double *elastic, double eta, const double *omega, int nw,
+                         double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ESC_H */
