@@ -75,6 +75,14 @@ def lib():
         return _LIB
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m energyspincurrents_amd.build`")
+    # torch ships its own HIP runtime (libamdhip64.so, soname libamdhip64.so.7);
+    # loading it before libesc.so makes libesc bind to that same runtime, so
+    # device pointers and streams from torch are valid in libesc.  Loading
+    # libesc first would put two HIP runtimes in one process.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     P, I, D, I64, U64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int64, ctypes.c_uint64
     sig = {

@@ -456,7 +456,11 @@ void cf_run(esc_ctx *ctx, const void *d_phi, int steps, void *work[3], double *a
         const double b = std::sqrt(b2);
         beta[k + 1] = b;
         if (!std::isfinite(b)) throw EscError(ESC_ERR_NUMERIC, "non-finite beta");
-        if (b == 0.0) { ++k; break; }
+        if (b <= 1e-13 * std::max(1.0, std::fabs(a[0]))) {   // Krylov space exhausted: the fraction terminates
+            beta[k + 1] = 0.0;
+            ++k;
+            break;
+        }
         hip_check(esc::launch_scale(c64, w, 1.0 / b, dim, s), "scale");
         void *t = vp; vp = v; v = w; w = t;
     }
@@ -697,7 +701,12 @@ int esc_lanczos(esc_ctx *ctx, const void *d_v0, int steps, double eps, int mode,
             sync_get(s, &n2, ctx->d_result + 2, sizeof(double));
             const double b = std::sqrt(n2);
             if (!std::isfinite(b)) throw EscError(ESC_ERR_NUMERIC, "non-finite beta");
-            if (b == 0.0) throw EscError(ESC_ERR_BREAKDOWN, "Lanczos breakdown: beta == 0 at step " + std::to_string(k));
+            // Invariant subspace: beta at rounding level of |alpha|.  With full
+            // reorthogonalisation the normalised remainder is re-orthogonalised
+            // into a fresh direction and the run continues (as the reference
+            // does); without it the recursion would continue on noise.
+            if (b == 0.0 || (!reortho && b <= 1e-13 * std::max(1.0, std::fabs(a[0]))))
+                throw EscError(ESC_ERR_BREAKDOWN, "Lanczos breakdown: beta ~ 0 at step " + std::to_string(k));
             bt.push_back(b);
             hip_check(esc::launch_scale(c64, w, 1.0 / b, dim, s), "scale");
             if (reortho && k > 0) {   // (src/DynLanczosSolver.h:105-119), no reorthogonalisation at k = 0

@@ -63,7 +63,8 @@ __global__ __launch_bounds__(kThreads) void k_hpsi(const HTables *__restrict__ T
             const uint32_t x = ((r >> (p & 0xffu)) ^ (r >> ((p >> 8) & 0xffu))) & 1u;
             d += x ? -j4 : j4;
         }
-        d += T->bz * (0.5 * (double)((int)T->n_sites - 2 * __popc(r)));
+        // __popc returns unsigned: keep the difference signed
+        d += T->bz * (0.5 * (double)((int)T->n_sites - 2 * (int)__popc(r)));
     }
     const V2 v0 = in[r];
     const R dd = (R)d;

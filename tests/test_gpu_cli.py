@@ -1,0 +1,70 @@
+"""The drop-in CLI (LanczosCond, the reference's main.cpp call order over the
+C++ class facade) on the reference's own parameter file: the written
+*_outSpectrum.real files match the reference's committed ones in format
+(byte layout) and values (the oracle's reproduction bar)."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import GOLDEN, ROOT
+
+pytestmark = pytest.mark.gpu
+
+CLI = os.path.join(ROOT, "energyspincurrents_amd", "LanczosCond")
+
+
+def _l1(tab, ref):
+    return np.abs(tab[:, 1:] - ref[:, 1:]).sum(0) / np.abs(ref[:, 1:]).sum(0)
+
+
+def _run(tmp_path, text):
+    if not os.path.exists(CLI):
+        pytest.fail("LanczosCond is not built (python -m energyspincurrents_amd.build)")
+    inp = tmp_path / "Midinput.inp"
+    inp.write_text(text)
+    r = subprocess.run([CLI, str(inp)], cwd=tmp_path, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    return r
+
+
+def _check_format(path):
+    raw = open(path, "rb").read()
+    assert raw.endswith(b" \n ")
+    rows = raw.split(b" \n ")
+    assert rows[-1] == b"" and len(rows) == 1002
+    assert all(b" \t " in r for r in rows[:-1])
+
+
+def test_cli_jeje_matches_committed_file(tmp_path):
+    text = open(os.path.join(GOLDEN, "Midinput.inp")).read()
+    _run(tmp_path, text)
+    out = tmp_path / "JeJe_outSpectrum.real"
+    _check_format(out)
+    tab = O.read_real(str(out))
+    ref = O.read_real(os.path.join(GOLDEN, "JeJe_outSpectrum.real"))
+    assert _l1(tab, ref).max() < 0.06
+
+
+def test_cli_jsjs_matches_committed_file(tmp_path):
+    text = open(os.path.join(GOLDEN, "Midinput.inp")).read()
+    text = text.replace("Bxx=0.3", "Bxx=0.0").replace("Byy=0.3", "Byy=0.0").replace("Bzz=0.3", "Bzz=0.0")
+    text = text.replace("LocalSkw=0", "LocalSkw=1").replace("EnergyCurrDynamics=1", "EnergyCurrDynamics=0")
+    _run(tmp_path, text)
+    out = tmp_path / "JsJs_outSpectrum.real"
+    _check_format(out)
+    tab = O.read_real(str(out))
+    ref = O.read_real(os.path.join(GOLDEN, "JsJs_outSpectrum.real"))
+    assert _l1(tab, ref).max() < 5e-3
+
+
+def test_cli_missing_key_fails_cleanly(tmp_path):
+    text = "\n".join(ln for ln in open(os.path.join(GOLDEN, "Midinput.inp")).read().splitlines()
+                     if not ln.startswith("Kzz"))
+    inp = tmp_path / "bad.inp"
+    inp.write_text(text)
+    r = subprocess.run([CLI, str(inp)], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "Kzz= argument is missing" in r.stderr

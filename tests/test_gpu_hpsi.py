@@ -1,0 +1,124 @@
+"""H.psi on the device (esc_hpsi) against the oracle.
+
+Bar: bit-exact matrix elements for the reference model (columns H e_k against
+the oracle's COO restatement of TightB_Ham + Diagonal); FP64 vectors within
+1e-14 relative (norm-wise) of the oracle's gather form; c64 within 1e-6.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+esc = pytest.importorskip("energyspincurrents_amd")
+
+
+def engine_for(m: O.Model, precision="c128"):
+    lat = esc.Honeycomb(m.Lx, m.Ly, m.perX, m.perY)
+    return esc.Engine(esc.SpinModel(lat, K=m.K, J=m.J, Gamma=m.G, B=m.B), precision)
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+REF_MODELS = [
+    dict(Lx=3, Ly=4, K=(1.0, 1.0, 1.0), B=(0.3, 0.3, 0.3)),             # Midinput.inp
+    dict(Lx=2, Ly=6, K=(-1.0, 0.7, 0.3), B=(0.1, -0.2, 0.3)),          # non-dyadic couplings
+    dict(Lx=3, Ly=4, K=(-1.0, -1.0, -1.0), B=(0.0, 0.0, 0.0)),         # pure Kitaev
+]
+
+
+@pytest.mark.parametrize("kw", REF_MODELS)
+def test_matrix_elements_bit_exact(kw):
+    m = O.Model(**kw)
+    print(f"[hpsi bit-exact] {kw}", flush=True)
+    H = O.CooHamiltonian(m).dense()
+    eng = engine_for(m)
+    rng = np.random.default_rng(1)
+    cols = sorted(set([0, 1, m.dim - 1] + list(rng.integers(0, m.dim, 61))))
+    for k in cols:
+        e = np.zeros(m.dim, np.complex128)
+        e[k] = 1.0
+        got = eng.download(eng.hpsi(eng.upload(e)))
+        assert np.array_equal(got, H[:, k]), f"column {k}"
+
+
+@pytest.mark.parametrize("J,G,B", [(0.1, 0.2, O.field111(0.2)), (0.0, 0.3, (0.0, 0.0, 0.1)),
+                                   (0.25, 0.0, (0.2, -0.1, 0.0))])
+def test_extension_matrix_elements(J, G, B):
+    m = O.Model(3, 4, K=(-1.0, -1.0, -1.0), J=J, G=G, B=B)
+    print(f"[hpsi ext elements] J={J} G={G}", flush=True)
+    eng = engine_for(m)
+    rng = np.random.default_rng(2)
+    for k in [0, m.dim - 1] + list(rng.integers(0, m.dim, 30)):
+        e = np.zeros(m.dim, np.complex128)
+        e[k] = 1.0
+        got = eng.download(eng.hpsi(eng.upload(e)))
+        ref = O.hpsi_gather(m, e)
+        assert np.abs(got - ref).max() <= 1e-15
+
+
+@pytest.mark.parametrize("Lx,Ly", [(3, 4), (2, 8), (3, 6), (2, 10)])
+@pytest.mark.parametrize("ext", [False, True])
+def test_random_vector_c128(Lx, Ly, ext):
+    kw = dict(K=(-1.0, -1.0, -1.0), J=0.1, G=0.2, B=O.field111(0.2)) if ext else dict(K=(1.0, 0.9, 1.1), B=(0.3, 0.2, 0.1))
+    m = O.Model(Lx, Ly, **kw)
+    print(f"[hpsi c128] N={m.N} ext={ext}", flush=True)
+    rng = np.random.default_rng(m.N)
+    v = rng.standard_normal(m.dim) + 1j * rng.standard_normal(m.dim)
+    eng = engine_for(m)
+    got = eng.download(eng.hpsi(eng.upload(v)))
+    assert rel(got, O.hpsi_gather(m, v)) < 1e-14
+
+
+@pytest.mark.parametrize("Lx,Ly", [(3, 4), (2, 10)])
+def test_random_vector_c64(Lx, Ly):
+    m = O.Model(Lx, Ly, K=(-1.0, -1.0, -1.0), J=0.1, G=0.2, B=O.field111(0.2))
+    print(f"[hpsi c64] N={m.N}", flush=True)
+    rng = np.random.default_rng(5)
+    v = rng.standard_normal(m.dim) + 1j * rng.standard_normal(m.dim)
+    v = (v.astype(np.complex64)).astype(np.complex128)     # exactly representable input
+    eng = engine_for(m, "c64")
+    got = eng.download(eng.hpsi(eng.upload(v)))
+    assert rel(got, O.hpsi_gather(m, v)) < 1e-6
+
+
+def test_headline_size_n24_against_oracle_and_properties():
+    """24-site K-J-Gamma-h model at full size: parity with the oracle gather
+    and the size-independent properties (Hermiticity, linearity)."""
+    import torch
+    m = O.Model(3, 8, K=(-1.0, -1.0, -1.0), J=0.1, G=0.2, B=O.field111(0.2))
+    print("[hpsi N=24]", flush=True)
+    eng = engine_for(m)
+    rng = np.random.default_rng(24)
+    x = rng.standard_normal(m.dim) + 1j * rng.standard_normal(m.dim)
+    dx = eng.upload(x)
+    hx = eng.hpsi(dx)
+    got = eng.download(hx)
+    assert rel(got, O.hpsi_gather(m, x)) < 1e-14
+    del got
+    y = torch.randn(m.dim, dtype=torch.complex128, device="cuda")
+    hy = eng.hpsi(y)
+    a = eng.dot(dx, hy)
+    b = eng.dot(y, hx)
+    assert abs(a - b.conjugate()) < 1e-12 * abs(a)
+    z = (2.0 - 0.5j) * dx + 0.25j * y
+    hz = eng.hpsi(z)
+    lin = (2.0 - 0.5j) * hx + 0.25j * hy
+    assert (torch.linalg.vector_norm(hz - lin) / torch.linalg.vector_norm(lin)).item() < 1e-14
+    # c64 at the same size
+    eng64 = engine_for(m, "c64")
+    h64 = eng64.hpsi(dx.to(torch.complex64))
+    assert (torch.linalg.vector_norm(h64.to(torch.complex128) - hx) / torch.linalg.vector_norm(hx)).item() < 1e-6
+
+
+def test_error_paths():
+    m = O.Model(3, 4)
+    eng = engine_for(m)
+    x = eng.empty()
+    with pytest.raises(esc.EscError):
+        eng.hpsi(x, out=x)                 # in-place is rejected
+    with pytest.raises(esc.EscError):
+        eng.apply_op("Jex", 99, x)         # site out of range

@@ -1,0 +1,145 @@
+"""Lanczos, Ritz overlaps and spectra on the device against the oracle, full
+diagonalisation and the reference's committed spectra.
+
+Tolerances: start vector 1e-15; alpha/beta of the first steps 1e-10 (later
+steps of a reorthogonalised run amplify rounding, so only leading
+coefficients are compared); E0 1e-12 (c128) and 1e-6 relative (c64); Ritz
+spectra against the committed files at the oracle's own reproduction bar;
+CF spectra against the exact Lehmann sum within 1e-6 of the peak.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+esc = pytest.importorskip("energyspincurrents_amd")
+
+
+def engine_for(m: O.Model, precision="c128"):
+    lat = esc.Honeycomb(m.Lx, m.Ly, m.perX, m.perY)
+    return esc.Engine(esc.SpinModel(lat, K=m.K, J=m.J, Gamma=m.G, B=m.B), precision)
+
+
+def test_start_vector_matches_reference_generator():
+    m = O.Model(3, 4)
+    eng = engine_for(m)
+    got = eng.download(eng.start_vector(3, "Uniform"))
+    ref = O.start_vector_ref(3, m.dim)
+    assert np.abs(got - ref).max() < 1e-15
+
+
+def test_gaussian_start_vector_is_normalised_and_seeded():
+    m = O.Model(2, 8)
+    eng = engine_for(m)
+    a = eng.download(eng.start_vector(7, "Gaussian"))
+    b = eng.download(eng.start_vector(7, "Gaussian"))
+    c = eng.download(eng.start_vector(8, "Gaussian"))
+    assert np.array_equal(a, b) and not np.allclose(a, c)
+    assert abs(np.linalg.norm(a) - 1) < 1e-14
+    # N(0, 1/2) per component before normalisation
+    s = a * np.sqrt(m.dim)
+    assert abs(np.var(s.real) - 0.5) < 0.02 and abs(np.var(s.imag) - 0.5) < 0.02
+
+
+def test_reference_mode_lanczos_matches_oracle():
+    m = O.Model(3, 4, K=(1.0, 1.0, 1.0), B=(0.3, 0.3, 0.3))
+    print("[lanczos ref N=12]", flush=True)
+    eng = engine_for(m)
+    v0 = eng.start_vector(3, "Uniform")
+    gs = eng.empty()
+    res = eng.lanczos(v0, 100, 0.0, "All", gs)
+    coo = O.CooHamiltonian(m)
+    ref = O.lanczos_ref(coo.apply, O.start_vector_ref(3, m.dim), 100)
+    assert res.iters == ref.iters == 100
+    assert np.abs(res.alpha[:12] - ref.alpha[:12].real).max() < 1e-10
+    assert np.abs(res.beta[:12] - ref.beta[:12].real).max() < 1e-10
+    assert abs(res.evals[0] - ref.evals[0]) < 1e-12
+    H = coo.dense()
+    e0 = np.linalg.eigvalsh(H)[0]
+    assert abs(res.evals[0] - e0) < 1e-10
+    g = eng.download(gs)
+    assert abs(abs(np.vdot(g, ref.gs)) - 1) < 1e-9
+    # gs is an eigenvector of H
+    assert np.linalg.norm(H @ g - e0 * g) < 1e-6
+
+
+def test_ritz_overlaps_match_formed_ritz_vectors():
+    m = O.Model(2, 4, K=(-1.0, -1.0, -1.0), B=O.field111(0.2))
+    eng = engine_for(m)
+    v0 = eng.start_vector(5, "Uniform")
+    gs = eng.empty()
+    res = eng.lanczos(v0, 40, 0.0, "All", gs)
+    vec = eng.apply_total("Jex", gs)
+    ov = eng.ritz_overlaps(vec, res.iters)
+    for k in (0, 1, 7, res.iters - 1):
+        psi = eng.ritz_vector(k)
+        assert abs(eng.dot(psi, vec) - ov[k]) < 1e-12
+
+
+def _l1(tab, ref):
+    return np.abs(tab[:, 1:] - ref[:, 1:]).sum(0) / np.abs(ref[:, 1:]).sum(0)
+
+
+@pytest.mark.parametrize("kind,B,tol", [("JsJs", (0.0, 0.0, 0.0), 5e-3), ("JeJe", (0.3, 0.3, 0.3), 0.06)])
+def test_ritz_spectra_reproduce_committed_files(kind, B, tol):
+    """Reference pipeline on the device: Midinput.inp (B as listed), 100 steps,
+    seed 3, reference grid and eta; compared with the reference's own files."""
+    m = O.Model(3, 4, K=(1.0, 1.0, 1.0), B=B)
+    print(f"[ritz {kind}]", flush=True)
+    eng = engine_for(m)
+    gs = eng.empty()
+    eng.lanczos(eng.start_vector(3, "Uniform"), 100, 0.0, "All", gs)
+    tab, _ = eng.dynamics(kind, "Ritz", gs, esc.reference_omega(), 0.005)
+    ref = O.read_real(os.path.join(GOLDEN, f"{kind}_outSpectrum.real"))
+    assert _l1(tab, ref).max() < tol
+
+
+def _exact(m):
+    H = O.hamiltonian_dense_kron(m)
+    e, U = np.linalg.eigh(H)
+    return e, U
+
+
+@pytest.mark.parametrize("kind", ["JeJe", "JsJs"])
+def test_cf_spectra_equal_exact_lehmann(kind):
+    m = O.Model(2, 4, K=(-1.0, -1.0, -1.0), J=0.1, G=0.2, B=O.field111(0.2))
+    e, U = _exact(m)
+    gs = U[:, 0]
+    w = np.linspace(0.0, 6.0, 400)
+    eta = 0.1
+    states = U.T.copy()
+    exact = (O.jeje_ref if kind == "JeJe" else O.jsjs_ref)(m, e, states, gs, omega=w, eta=eta)
+    eng = engine_for(m)
+    tab, ex = eng.dynamics(kind, "CF", eng.upload(gs), w, eta, e0=e[0], cf_steps=150)
+    scale = np.abs(exact[:, 1:]).max()
+    assert np.abs(tab[:, 1:] - exact[:, 1:]).max() < 1e-6 * scale
+    # Ritz mode with a long reorthogonalised run reaches the same exact sum
+    gsd = eng.empty()
+    eng.lanczos(eng.start_vector(3, "Uniform"), 200, 0.0, "All", gsd)
+    tab2, _ = eng.dynamics(kind, "Ritz", gsd, w, eta)
+    assert np.abs(tab2[:, 1:] - exact[:, 1:]).max() < 1e-6 * scale
+
+
+def test_c64_ground_energy_error():
+    m = O.Model(2, 8, K=(-1.0, -1.0, -1.0), J=0.1, G=0.2, B=O.field111(0.2))
+    print("[c64 E0 N=16]", flush=True)
+    e128 = engine_for(m)
+    r128 = e128.lanczos(e128.start_vector(9, "Gaussian"), 150, 0.0, "OnlyEgs")
+    e64 = engine_for(m, "c64")
+    r64 = e64.lanczos(e64.start_vector(9, "Gaussian"), 150, 0.0, "OnlyEgs")
+    assert abs(r64.evals[0] - r128.evals[0]) / abs(r128.evals[0]) < 1e-6
+
+
+def test_lanczos_breakdown_is_reported():
+    """An eigenvector start gives beta = 0 after one step: ESC_ERR_BREAKDOWN."""
+    m = O.Model(2, 4, K=(-1.0, -1.0, -1.0))
+    e, U = _exact(m)
+    eng = engine_for(m)
+    with pytest.raises(esc.EscError) as ei:
+        eng.lanczos(eng.upload(U[:, 0]), 10, 0.0, "OnlyEgs")
+    assert ei.value.code in (-4, -5)
