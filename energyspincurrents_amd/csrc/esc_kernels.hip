@@ -49,6 +49,9 @@ __global__ __launch_bounds__(kThreads) void k_hpsi(const HTables *__restrict__ T
 
     double d = 0.0;
     {
+        // no FMA contraction: Bzz * Sz_total is rounded before it is added, as
+        // the reference's separate complex operators do (src/Hamiltonian.h:374)
+#pragma clang fp contract(off)
         const uint32_t nz = T->n_zdiag;
         const double kz4 = T->kz4;
         for (uint32_t b = 0; b < nz; ++b) {

@@ -3,7 +3,7 @@ diagonalisation and the reference's committed spectra.
 
 Tolerances: start vector 1e-15; alpha/beta of the first steps 1e-10 (later
 steps of a reorthogonalised run amplify rounding, so only leading
-coefficients are compared); E0 1e-12 (c128) and 1e-6 relative (c64); Ritz
+coefficients are compared); E0 at the convergence level of the run (1e-11 once converged) and 1e-6 relative for c64; Ritz
 spectra against the committed files at the oracle's own reproduction bar;
 CF spectra against the exact Lehmann sum within 1e-6 of the peak.
 """
@@ -58,14 +58,21 @@ def test_reference_mode_lanczos_matches_oracle():
     assert res.iters == ref.iters == 100
     assert np.abs(res.alpha[:12] - ref.alpha[:12].real).max() < 1e-10
     assert np.abs(res.beta[:12] - ref.beta[:12].real).max() < 1e-10
-    assert abs(res.evals[0] - ref.evals[0]) < 1e-12
+    # the ground state is two-fold degenerate: after the reference's 100 steps
+    # E0 is converged to ~2.4e-7 only, and rounding differences move it by
+    # ~1e-9; at 200 steps it is converged to ~1e-14
     H = coo.dense()
     e0 = np.linalg.eigvalsh(H)[0]
-    assert abs(res.evals[0] - e0) < 1e-10
+    assert abs(res.evals[0] - ref.evals[0]) < 1e-8
+    assert abs(res.evals[0] - e0) < 1e-6
+    res2 = eng.lanczos(v0, 200, 0.0, "All", gs)
+    ref2 = O.lanczos_ref(coo.apply, O.start_vector_ref(3, m.dim), 200)
+    assert abs(res2.evals[0] - e0) < 1e-11
+    assert abs(res2.evals[0] - ref2.evals[0]) < 1e-11
     g = eng.download(gs)
-    assert abs(abs(np.vdot(g, ref.gs)) - 1) < 1e-9
-    # gs is an eigenvector of H
+    # |gs> is the normalised projection of v0 on the ground eigenspace
     assert np.linalg.norm(H @ g - e0 * g) < 1e-6
+    assert abs(abs(np.vdot(g, ref2.gs)) - 1) < 1e-9
 
 
 def test_ritz_overlaps_match_formed_ritz_vectors():
