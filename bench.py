@@ -99,10 +99,13 @@ def main():
         eng = esc.Engine(model, prec)
         x = eng.start_vector(1234, "Gaussian")
         y = eng.empty()
+        tile_bits, _ = eng.autotune(x, y, reps=5)
         ms, wall = time_hpsi(eng, x, y, args.steps, args.warmup, args.copy_calib and prec == "c128")
         bytes_alg = 2 * eng.dim * eng.elem_bytes
         res[prec] = {"ms": ms, "wall": wall, "bytes": bytes_alg, "gbs": bytes_alg / (ms * 1e-3) / 1e9,
-                     "hpsi_per_s": 1e3 / ms}
+                     "hpsi_per_s": 1e3 / ms, "tile_bits": tile_bits,
+                     "kernel": f"k_hpsi_tile<{'float' if prec == 'c64' else 'double'}, {tile_bits}>"
+                               if tile_bits else f"k_hpsi<{'float' if prec == 'c64' else 'double'}>"}
         del x, y, eng
         torch.cuda.empty_cache()
 
@@ -124,10 +127,10 @@ def main():
                                "full 2^24 basis", "n_sites": 24, "dim": 1 << 24, "precision": "c128"},
         "roofline": {"bound": "hbm", "achieved": r["gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": r["gbs"] / PEAK_HBM_GBS, "traffic": None,
-                     "kernel": "k_hpsi<double>", "bytes_per_launch": r["bytes"]},
+                     "kernel": r["kernel"], "bytes_per_launch": r["bytes"]},
         "c64": {"hpsi_per_s": res["c64"]["hpsi_per_s"], "ms_per_hpsi": res["c64"]["ms"],
                 "achieved_GBs": res["c64"]["gbs"], "frac": res["c64"]["gbs"] / PEAK_HBM_GBS,
-                "bytes_per_launch": res["c64"]["bytes"], "kernel": "k_hpsi<float>"},
+                "bytes_per_launch": res["c64"]["bytes"], "kernel": res["c64"]["kernel"]},
     }
     if args.spectrum:
         out["spectrum"] = spectrum_timing(esc, model)

@@ -169,6 +169,9 @@ struct esc_ctx {
     void *d_aux = nullptr;                          // dynamics work vectors
     size_t aux_cap = 0;
     esc::CurrentTable *d_combo = nullptr;           // combined-operator table
+    std::map<int, esc::GTables *> d_gtab;           // tiled-kernel tables per tile size
+    std::map<int, std::pair<int, int>> gcount;      // (uniform terms, all terms) per tile size
+    int tile_bits = 0;                              // 0: one-row-per-lane kernel
 
     bool c64() const { return precision == ESC_C64; }
 
@@ -180,6 +183,7 @@ struct esc_ctx {
         if (d_vecs) (void)hipFree(d_vecs);
         if (d_aux) (void)hipFree(d_aux);
         if (d_combo) (void)hipFree(d_combo);
+        for (auto &kv : d_gtab) (void)hipFree(kv.second);
     }
     void ensure_aux(size_t nvec) {
         if (nvec <= aux_cap) return;
@@ -271,6 +275,104 @@ void build_tables(esc_ctx *c) {
     t.n_jdiag = nj;
     t.has_gamma_flip = m.Gamma != 0.0 ? 1u : 0u;
     t.n_flip = (m.B[0] != 0.0 || m.B[1] != 0.0 || m.Gamma != 0.0) ? (uint32_t)N : 0u;
+}
+
+// Tables of the tiled kernel for tile size TB (see esc::GTables).
+bool kz_sum_exact(double kz4, int nz) {
+    // every partial sum +-kz4 * m, |m| <= nz, must be exactly representable
+    for (int m = 1; m <= nz; ++m) {
+        const double p = kz4 * (double)m;
+        if (std::fma(kz4, (double)m, -p) != 0.0) return false;
+    }
+    return true;
+}
+
+void build_gtables(const esc_ctx *c, int TB, esc::GTables &g) {
+    const esc_model &m = c->model;
+    const esc::HTables &h = c->htab;
+    std::memset(&g, 0, sizeof(g));
+    const int N = m.n_sites;
+    g.n_sites = (uint32_t)N;
+    g.tile_bits = (uint32_t)TB;
+    std::vector<esc::GTerm> cls[esc::kNumClasses];
+    auto add_term = [&](uint32_t mask, int b0, int b1, int b2, const double (*coef)[2]) {
+        esc::GTerm t;
+        std::memset(&t, 0, sizeof(t));
+        t.mask = mask;
+        t.bits = (uint32_t)b0 | ((uint32_t)b1 << 8) | ((uint32_t)b2 << 16);
+        bool cplx = false;
+        for (int e = 0; e < 8; ++e) {
+            t.coef[e][0] = coef[e][0];
+            t.coef[e][1] = coef[e][1];
+            if (coef[e][1] != 0.0) cplx = true;
+        }
+        const bool far = (mask >> TB) != 0;
+        const bool lane = b0 < 6 || b1 < 6 || b2 < 6;
+        cls[(cplx ? 1 : 0) | (far ? 2 : 0) | (lane ? 4 : 0)].push_back(t);
+    };
+    // bond terms (same values as HTables: p0 + p1 s_i s_j + i q (s_i + s_j))
+    for (uint32_t b = 0; b < h.n_bonds; ++b) {
+        const uint32_t p = h.bond[b];
+        const int i = p & 0xff, j = (p >> 8) & 0xff, f = p >> 16;
+        double coef[8][2];
+        for (int e = 0; e < 8; ++e) {
+            const double si = (e & 1) ? -1.0 : 1.0, sj = (e & 2) ? -1.0 : 1.0;
+            coef[e][0] = h.p0[f] + h.p1[f] * (si * sj);
+            coef[e][1] = h.q[f] * (si + sj);
+        }
+        add_term((1u << i) | (1u << j), i, j, i, coef);
+    }
+    // single flips (bx2 + g4 s_q - i s_k (by2 + g4 s_p))
+    for (uint32_t k = 0; k < h.n_flip; ++k) {
+        const int xp = h.has_gamma_flip ? h.xpart[k] : -1, yp = h.has_gamma_flip ? h.ypart[k] : -1;
+        double coef[8][2];
+        for (int e = 0; e < 8; ++e) {
+            const double sk = (e & 1) ? -1.0 : 1.0, sp = (e & 2) ? -1.0 : 1.0, sq = (e & 4) ? -1.0 : 1.0;
+            double re = h.bx2, im = h.by2;
+            if (xp >= 0) im += h.g4 * sp;
+            if (yp >= 0) re += h.g4 * sq;
+            coef[e][0] = re;
+            coef[e][1] = -sk * im;
+        }
+        add_term(1u << k, (int)k, xp >= 0 ? xp : (int)k, yp >= 0 ? yp : (int)k, coef);
+    }
+    size_t n = 0;
+    for (int q = 0; q < esc::kNumClasses; ++q) {
+        g.cls_begin[q] = (uint32_t)n;
+        if (n + cls[q].size() > (size_t)esc::kMaxTerms) throw EscError(ESC_ERR_UNSUPPORTED, "too many terms");
+        for (auto &t : cls[q]) g.term[n++] = t;
+    }
+    g.cls_begin[esc::kNumClasses] = (uint32_t)n;
+    // diagonal
+    g.kz4 = h.kz4;
+    g.j4 = h.j4;
+    g.bz = h.bz;
+    g.nz = h.n_zdiag;
+    g.nj = h.n_jdiag;
+    g.diag_closed = kz_sum_exact(h.kz4, (int)h.n_zdiag) ? 1u : 0u;
+    auto groups = [&](const uint32_t *list, uint32_t nl, uint32_t *gd, uint32_t *gm, uint32_t &ng) {
+        ng = 0;
+        for (uint32_t b = 0; b < nl; ++b) {
+            uint32_t i = list[b] & 0xff, j = (list[b] >> 8) & 0xff;
+            if (i > j) std::swap(i, j);
+            const uint32_t d = j - i;
+            uint32_t q = 0;
+            while (q < ng && gd[q] != d) ++q;
+            if (q == ng) {
+                if (ng >= (uint32_t)esc::kMaxDistGroups) throw EscError(ESC_ERR_UNSUPPORTED, "too many bond distances");
+                gd[ng] = d;
+                gm[ng] = 0;
+                ++ng;
+            }
+            if (gm[q] & (1u << i)) {   // two bonds of one group on the same lower site: no closed form
+                g.diag_closed = 0;
+                return;
+            }
+            gm[q] |= 1u << i;
+        }
+    };
+    groups(h.zdiag, h.n_zdiag, g.zgrp_d, g.zgrp_m, g.n_zgrp);
+    groups(h.jdiag, h.n_jdiag, g.jgrp_d, g.jgrp_m, g.n_jgrp);
 }
 
 // Pauli-string table for one operator (src/Observables.h:161-389,429-521)
@@ -412,6 +514,57 @@ double device_norm2(esc_ctx *c, const void *x, hipStream_t s) {
     return r;
 }
 
+esc::GTables *gtab(esc_ctx *c, int TB) {
+    auto it = c->d_gtab.find(TB);
+    if (it != c->d_gtab.end()) return it->second;
+    auto g = std::make_unique<esc::GTables>();
+    build_gtables(c, TB, *g);
+    esc::GTables *d = nullptr;
+    if (hipMalloc(&d, sizeof(esc::GTables)) != hipSuccess) throw EscError(ESC_ERR_ALLOC, "table allocation failed");
+    hip_check(hipMemcpy(d, g.get(), sizeof(esc::GTables), hipMemcpyHostToDevice), "hipMemcpy");
+    c->d_gtab[TB] = d;
+    c->gcount[TB] = {(int)g->cls_begin[4], (int)g->cls_begin[esc::kNumClasses]};
+    return d;
+}
+
+// tile sizes the tiled kernel is built for (rows per workgroup = 2^TB)
+std::vector<int> tile_candidates(const esc_ctx *c) {
+    std::vector<int> v;
+    for (int tb : {11, 12, 13})
+        if (c->n_sites >= tb && (c->c64() || tb <= 12)) v.push_back(tb);
+    return v;
+}
+
+// default tile before tuning: 2^11 c128 rows (32 KiB) or 2^12 c64 rows;
+// systems below 2^11 rows use the one-row-per-lane kernel
+int default_tile_bits(const esc_ctx *c) {
+    const int tb = c->c64() ? 12 : 11;
+    if (c->n_sites >= tb) return tb;
+    return c->n_sites >= 11 ? 11 : 0;
+}
+
+void launch_hpsi_with(esc_ctx *c, int tb, const void *in, void *out, hipStream_t s) {
+    if (tb > 0) {
+        esc::GTables *g = gtab(c, tb);
+        const auto cnt = c->gcount[tb];
+        hip_check(esc::launch_hpsi_tile(c->c64(), g, c->d_htab, tb, cnt.first, cnt.second, in, out, c->dim, s),
+                  "hpsi tile launch");
+    } else {
+        hip_check(c->c64() ? esc::launch_hpsi_c64(c->d_htab, in, out, c->dim, s)
+                           : esc::launch_hpsi_c128(c->d_htab, in, out, c->dim, s), "hpsi launch");
+    }
+}
+
+// H.psi on the device with the context's kernel choice
+void run_hpsi(esc_ctx *c, const void *in, void *out, hipStream_t s) {
+    if (c->tile_bits > 0) {
+        launch_hpsi_with(c, c->tile_bits, in, out, s);
+    } else {
+        hip_check(c->c64() ? esc::launch_hpsi_c64(c->d_htab, in, out, c->dim, s)
+                           : esc::launch_hpsi_c128(c->d_htab, in, out, c->dim, s), "hpsi launch");
+    }
+}
+
 // classical Gram-Schmidt pass against V_0..V_{k-1} (applied twice by callers)
 void cgs_pass(esc_ctx *c, int k, void *w, hipStream_t s) {
     c->ensure_partials((size_t)2 * k * 256);
@@ -442,8 +595,7 @@ void cf_run(esc_ctx *ctx, const void *d_phi, int steps, void *work[3], double *a
     beta[0] = 0.0;
     int k = 0;
     for (; k < steps; ++k) {
-        hip_check(c64 ? esc::launch_hpsi_c64(ctx->d_htab, v, w, dim, s)
-                      : esc::launch_hpsi_c128(ctx->d_htab, v, w, dim, s), "hpsi");
+        run_hpsi(ctx, v, w, s);
         double a[2];
         hip_check(esc::launch_dot(c64, v, w, dim, ctx->d_partials, ctx->d_result, s), "dot");
         sync_get(s, a, ctx->d_result, 2 * sizeof(double));
@@ -507,6 +659,7 @@ int esc_ctx_create(const esc_model *model, int precision, esc_ctx **out) {
         build_tables(c.get());
         if (hipMalloc(&c->d_htab, sizeof(esc::HTables)) != hipSuccess) throw EscError(ESC_ERR_ALLOC, "table allocation failed");
         hip_check(hipMemcpy(c->d_htab, &c->htab, sizeof(esc::HTables), hipMemcpyHostToDevice), "hipMemcpy");
+        c->tile_bits = default_tile_bits(c.get());
         *out = c.release();
     });
 }
@@ -530,10 +683,7 @@ int esc_hpsi(esc_ctx *ctx, const void *d_in, void *d_out, void *stream) {
         check_ptr(d_in, "d_in");
         check_ptr(d_out, "d_out");
         if (d_in == d_out) throw EscError(ESC_ERR_ARG, "d_in and d_out must not overlap");
-        hipStream_t s = (hipStream_t)stream;
-        hip_check(ctx->c64() ? esc::launch_hpsi_c64(ctx->d_htab, d_in, d_out, ctx->dim, s)
-                             : esc::launch_hpsi_c128(ctx->d_htab, d_in, d_out, ctx->dim, s),
-                  "hpsi launch");
+        run_hpsi(ctx, d_in, d_out, (hipStream_t)stream);
     });
 }
 
@@ -687,8 +837,7 @@ int esc_lanczos(esc_ctx *ctx, const void *d_v0, int steps, double eps, int mode,
         auto step = [&](int k) {
             // w = H V_k; alpha_k = <V_k|w>; w -= alpha_k V_k + beta_k V_{k-1}
             void *w = V(k + 1);
-            hip_check(c64 ? esc::launch_hpsi_c64(ctx->d_htab, V(k), w, dim, s)
-                          : esc::launch_hpsi_c128(ctx->d_htab, V(k), w, dim, s), "hpsi");
+            run_hpsi(ctx, V(k), w, s);
             double a[2];
             hip_check(esc::launch_dot(c64, V(k), w, dim, ctx->d_partials, d_alpha, s), "dot");
             sync_get(s, a, d_alpha, 2 * sizeof(double));
@@ -961,6 +1110,52 @@ int esc_dynamics(esc_ctx *ctx, int kind, int method, const void *d_gs, double e0
                 else row[col++] = v.imag();
             }
         }
+    });
+}
+
+int esc_autotune(esc_ctx *ctx, const void *d_in, void *d_out, int reps, int *tile_bits, double *ms, void *stream) {
+    return guarded([&] {
+        check_ctx(ctx);
+        check_ptr(d_in, "d_in");
+        check_ptr(d_out, "d_out");
+        if (d_in == d_out) throw EscError(ESC_ERR_ARG, "d_in and d_out must not overlap");
+        if (reps < 1) reps = 1;
+        hipStream_t s = (hipStream_t)stream;
+        std::vector<int> cand = tile_candidates(ctx);
+        cand.push_back(0);   // one-row-per-lane kernel
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        hip_check(hipEventCreate(&e1), "hipEventCreate");
+        int best = ctx->tile_bits;
+        float best_ms = 1e30f;
+        for (int tb : cand) {
+            launch_hpsi_with(ctx, tb, d_in, d_out, s);   // warm-up (also builds tables)
+            hip_check(hipEventRecord(e0, s), "hipEventRecord");
+            for (int r = 0; r < reps; ++r) launch_hpsi_with(ctx, tb, d_in, d_out, s);
+            hip_check(hipEventRecord(e1, s), "hipEventRecord");
+            hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+            float t = 0.0f;
+            hip_check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
+            t /= (float)reps;
+            if (t < best_ms) { best_ms = t; best = tb; }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        ctx->tile_bits = best;
+        if (tile_bits) *tile_bits = best;
+        if (ms) *ms = best_ms;
+    });
+}
+
+int esc_set_tile_bits(esc_ctx *ctx, int tile_bits) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (tile_bits != 0) {
+            const auto cand = tile_candidates(ctx);
+            if (std::find(cand.begin(), cand.end(), tile_bits) == cand.end())
+                throw EscError(ESC_ERR_ARG, "tile size not available for this context");
+        }
+        ctx->tile_bits = tile_bits;
     });
 }
 

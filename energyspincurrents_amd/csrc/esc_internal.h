@@ -33,6 +33,36 @@ struct HTables {
     double bx2, by2, g4;
 };
 
+// Tiled gather form (k_hpsi_tile).  Every off-diagonal term is
+//   H[r, r ^ mask] = coef[ bit(r,b0) | bit(r,b1) << 1 | bit(r,b2) << 2 ]
+// with an 8-entry complex table built on the host (unused selector bits repeat
+// b0).  Terms are grouped by class = cplx | far << 1 | lane << 2, so the
+// terms whose selector bits are all >= 6 (uniform across a wave) come first:
+//   cplx  some table entry has a non-zero imaginary part
+//   far   mask has bits at or above the tile bits (partner outside the LDS tile)
+//   lane  a selector bit lies below bit 6 (coefficient differs across a wave)
+constexpr int kMaxTerms = 160;
+constexpr int kMaxDistGroups = 32;
+constexpr int kNumClasses = 8;
+
+struct GTerm {
+    uint32_t mask;
+    uint32_t bits;          // b0 | b1 << 8 | b2 << 16
+    double coef[8][2];
+};
+
+struct GTables {
+    uint32_t n_sites, tile_bits;
+    uint32_t cls_begin[kNumClasses + 1];
+    // diagonal: closed form when the Kitaev-z sum is exact in any order
+    uint32_t diag_closed;           // 1: popcount form, 0: sequential (HTables path)
+    uint32_t n_zgrp, n_jgrp, nz, nj;
+    uint32_t zgrp_d[kMaxDistGroups], zgrp_m[kMaxDistGroups];
+    uint32_t jgrp_d[kMaxDistGroups], jgrp_m[kMaxDistGroups];
+    double kz4, j4, bz;
+    GTerm term[kMaxTerms];
+};
+
 // One factor of a Pauli-string term: op 0 = Sx, 1 = Sy, 2 = Sz at site.
 struct OpFactor {
     int32_t op;
@@ -59,6 +89,12 @@ void set_error(const char *msg);
 void clear_error();
 
 // Launchers (esc_kernels.hip).  dim = 2^N.  All return hipError_t of the launch.
+// Tiled H.psi: 2^tile_bits rows per workgroup staged in LDS; h is the
+// sequential-diagonal table used when g->diag_closed == 0.
+// n_uniform = number of terms in classes 0..3, n_terms = all terms (for the
+// LDS size); tile_bits in {11, 12} (c128) or {11, 12, 13} (c64).
+hipError_t launch_hpsi_tile(bool c64, const GTables *d_g, const HTables *d_h, int tile_bits, int n_uniform,
+                            int n_terms, const void *in, void *out, uint64_t dim, hipStream_t s);
 hipError_t launch_hpsi_c128(const HTables *d_tab, const void *in, void *out, uint64_t dim, hipStream_t s);
 hipError_t launch_hpsi_c64(const HTables *d_tab, const void *in, void *out, uint64_t dim, hipStream_t s);
 hipError_t launch_current_c128(const CurrentTable *d_tab, const void *in, void *out, uint64_t dim, hipStream_t s);

@@ -111,6 +111,174 @@ __global__ __launch_bounds__(kThreads) void k_hpsi(const HTables *__restrict__ T
 }
 
 // ---------------------------------------------------------------------------
+// Tiled H.psi.  A workgroup owns 2^TB consecutive rows (the tile), stages them
+// in LDS, and each wave walks row groups of 64 consecutive rows: lane = bits
+// 0..5, everything above is uniform across the wave (r_hi).  Per term:
+//  * partner: tile-local masks read LDS, others global memory through a
+//    uniform base pointer (in + (r_hi ^ mask_hi)) plus a lane offset;
+//  * coefficient: 8-entry table indexed by three bits of the row; for terms
+//    whose selector bits are all >= 6 the index is uniform (scalar unit);
+//  * real-coefficient terms use two FMAs, complex ones four.
+// Output rows are written once each (no atomics).
+// ---------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ double hpsi_diag(const GTables *__restrict__ G, const HTables *__restrict__ H,
+                                            uint32_t r) {
+#pragma clang fp contract(off)
+    double d;
+    if (G->diag_closed) {
+        // Kitaev-z and Heisenberg sums by counting anti-aligned bonds; used only
+        // when +-Kz/4 partial sums are exact, so the value equals the
+        // reference's bond-by-bond accumulation bit for bit
+        uint32_t cz = 0;
+        const uint32_t nzg = G->n_zgrp;
+        for (uint32_t q = 0; q < nzg; ++q) cz += __popc((r ^ (r >> G->zgrp_d[q])) & G->zgrp_m[q]);
+        d = G->kz4 * (double)((int)G->nz - 2 * (int)cz);
+        if (G->nj) {
+            uint32_t cj = 0;
+            const uint32_t njg = G->n_jgrp;
+            for (uint32_t q = 0; q < njg; ++q) cj += __popc((r ^ (r >> G->jgrp_d[q])) & G->jgrp_m[q]);
+            d = d + G->j4 * (double)((int)G->nj - 2 * (int)cj);
+        }
+    } else {
+        d = 0.0;
+        const uint32_t nz = H->n_zdiag;
+        const double kz4 = H->kz4;
+        for (uint32_t b = 0; b < nz; ++b) {
+            const uint32_t p = H->zdiag[b];
+            d += (((r >> (p & 0xffu)) ^ (r >> ((p >> 8) & 0xffu))) & 1u) ? -kz4 : kz4;
+        }
+        const uint32_t nj = H->n_jdiag;
+        const double j4 = H->j4;
+        for (uint32_t b = 0; b < nj; ++b) {
+            const uint32_t p = H->jdiag[b];
+            d += (((r >> (p & 0xffu)) ^ (r >> ((p >> 8) & 0xffu))) & 1u) ? -j4 : j4;
+        }
+    }
+    d = d + G->bz * (0.5 * (double)((int)G->n_sites - 2 * (int)__popc(r)));
+    return d;
+}
+
+// Terms of one class for all NG row groups of a wave.  Row group k of wave w
+// is g = w + 4k, rows r = base | g << 6 | lane.
+template <typename R, int TB, int CLS>
+__device__ __forceinline__ void hpsi_class(const GTables *__restrict__ G, uint32_t base, uint32_t wave,
+                                           uint32_t lane, const typename Vec2<R>::T *__restrict__ in,
+                                           const typename Vec2<R>::T *s_in, const typename Vec2<R>::T *s_cu,
+                                           const typename Vec2<R>::T *s_ct, R *ar, R *ai) {
+    using V2 = typename Vec2<R>::T;
+    constexpr int NGRP = 1 << (TB - 6), NG = NGRP / 4;
+    constexpr uint32_t TMASK = (1u << TB) - 1u;
+    constexpr bool CPLX = (CLS & 1) != 0, FAR = (CLS & 2) != 0, LANE = (CLS & 4) != 0;
+    const uint32_t t0 = G->cls_begin[CLS], t1 = G->cls_begin[CLS + 1];
+    const uint32_t nU = G->cls_begin[4];
+    for (uint32_t t = t0; t < t1; ++t) {
+        const uint32_t m = G->term[t].mask;
+        const uint32_t mlo = m & 63u, mhi = m & ~63u;
+        const uint32_t loff = lane ^ mlo;
+        uint32_t lp = 0, b0 = 0, b1 = 0, b2 = 0;
+        if (LANE) {
+            const uint32_t bits = G->term[t].bits;
+            b0 = bits & 0xffu; b1 = (bits >> 8) & 0xffu; b2 = bits >> 16;
+            // selector bits below 6 come from the lane, the rest per row group
+            lp = (b0 < 6 ? ((lane >> b0) & 1u) : 0u) | (b1 < 6 ? (((lane >> b1) & 1u) << 1) : 0u) |
+                 (b2 < 6 ? (((lane >> b2) & 1u) << 2) : 0u);
+        }
+#pragma unroll
+        for (int k = 0; k < NG; ++k) {
+            const uint32_t g = wave + 4u * (uint32_t)k;
+            const uint32_t r_hi = base | (g << 6);
+            V2 x;
+            if (FAR) {
+                const V2 *p = in + (size_t)(r_hi ^ mhi);
+                x = p[loff];
+            } else {
+                x = s_in[(((g << 6) ^ mhi) & TMASK) | loff];
+            }
+            V2 c;
+            if (LANE) {
+                const uint32_t up = (b0 >= 6 ? ((r_hi >> b0) & 1u) : 0u) | (b1 >= 6 ? (((r_hi >> b1) & 1u) << 1) : 0u) |
+                                    (b2 >= 6 ? (((r_hi >> b2) & 1u) << 2) : 0u);
+                c = s_ct[(t - nU) * 8u + (lp | up)];
+            } else {
+                c = s_cu[t * NGRP + g];
+            }
+            if (CPLX) {
+                ar[k] += c.x * x.x - c.y * x.y;
+                ai[k] += c.x * x.y + c.y * x.x;
+            } else {
+                ar[k] += c.x * x.x;
+                ai[k] += c.x * x.y;
+            }
+        }
+    }
+}
+
+template <typename R, int TB>
+__global__ __launch_bounds__(kThreads) void k_hpsi_tile(const GTables *__restrict__ G, const HTables *__restrict__ H,
+                                                        const typename Vec2<R>::T *__restrict__ in,
+                                                        typename Vec2<R>::T *__restrict__ out, uint32_t nblk_total,
+                                                        uint32_t block_base) {
+    using V2 = typename Vec2<R>::T;
+    constexpr int NGRP = 1 << (TB - 6), NG = NGRP / 4;
+    constexpr uint32_t NROW = 1u << TB;
+    extern __shared__ unsigned char esc_lds[];
+    V2 *s_in = reinterpret_cast<V2 *>(esc_lds);                 // the tile
+    V2 *s_cu = s_in + NROW;                                     // uniform-term coefficients [nU][NGRP]
+    const uint32_t nU = G->cls_begin[4], nT = G->cls_begin[kNumClasses];
+    V2 *s_ct = s_cu + nU * NGRP;                                // lane-term tables [nT - nU][8]
+    const uint32_t tile = xcd_block(blockIdx.x + block_base, nblk_total);
+    const uint32_t base = tile << TB;
+    const V2 *src = in + (size_t)base;
+    for (uint32_t i = threadIdx.x; i < NROW; i += kThreads) s_in[i] = src[i];
+    for (uint32_t e = threadIdx.x; e < nU * NGRP; e += kThreads) {
+        const uint32_t t = e / NGRP, g = e % NGRP;
+        const uint32_t r_hi = base | (g << 6), bits = G->term[t].bits;
+        const uint32_t idx = ((r_hi >> (bits & 0xffu)) & 1u) | (((r_hi >> ((bits >> 8) & 0xffu)) & 1u) << 1) |
+                             (((r_hi >> (bits >> 16)) & 1u) << 2);
+        V2 c;
+        c.x = (R)G->term[t].coef[idx][0];
+        c.y = (R)G->term[t].coef[idx][1];
+        s_cu[e] = c;
+    }
+    for (uint32_t e = threadIdx.x; e < (nT - nU) * 8u; e += kThreads) {
+        V2 c;
+        c.x = (R)G->term[nU + e / 8u].coef[e % 8u][0];
+        c.y = (R)G->term[nU + e / 8u].coef[e % 8u][1];
+        s_ct[e] = c;
+    }
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    R ar[NG], ai[NG];
+#pragma unroll
+    for (int k = 0; k < NG; ++k) {
+        const uint32_t g = wave + 4u * (uint32_t)k;
+        const V2 v0 = s_in[(g << 6) | lane];
+        const R dd = (R)hpsi_diag<R>(G, H, base | (g << 6) | lane);
+        ar[k] = dd * v0.x;
+        ai[k] = dd * v0.y;
+    }
+    hpsi_class<R, TB, 0>(G, base, wave, lane, in, s_in, s_cu, s_ct, ar, ai);
+    hpsi_class<R, TB, 1>(G, base, wave, lane, in, s_in, s_cu, s_ct, ar, ai);
+    hpsi_class<R, TB, 2>(G, base, wave, lane, in, s_in, s_cu, s_ct, ar, ai);
+    hpsi_class<R, TB, 3>(G, base, wave, lane, in, s_in, s_cu, s_ct, ar, ai);
+    hpsi_class<R, TB, 4>(G, base, wave, lane, in, s_in, s_cu, s_ct, ar, ai);
+    hpsi_class<R, TB, 5>(G, base, wave, lane, in, s_in, s_cu, s_ct, ar, ai);
+    hpsi_class<R, TB, 6>(G, base, wave, lane, in, s_in, s_cu, s_ct, ar, ai);
+    hpsi_class<R, TB, 7>(G, base, wave, lane, in, s_in, s_cu, s_ct, ar, ai);
+    // streamed result: a non-temporal store keeps it from displacing the
+    // input rows that other tiles still gather from
+    typedef R nvec2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+    for (int k = 0; k < NG; ++k) {
+        const uint32_t r = base | ((wave + 4u * (uint32_t)k) << 6) | lane;
+        nvec2 o = {ar[k], ai[k]};
+        __builtin_nontemporal_store(o, reinterpret_cast<nvec2 *>(out + r));
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Pauli-string operators (single-site S^a, spin and energy currents).
 // Row r gathers from c = r ^ M where M flips the sites of the Sx/Sy factors;
 // the coefficient follows the factor sequence exactly as the reference applies
@@ -424,6 +592,9 @@ constexpr uint32_t kMaxGrid = 1u << 22;
 
 }  // namespace
 
+template <typename R> using V2p = typename Vec2<R>::T *;
+template <typename R> using CV2p = const typename Vec2<R>::T *;
+
 template <typename R>
 static hipError_t launch_rows_hpsi(const HTables *t, const void *in, void *out, uint64_t dim, hipStream_t s) {
     const uint32_t nblk = row_blocks(dim);
@@ -446,6 +617,39 @@ static hipError_t launch_rows_current(const CurrentTable *t, const void *in, voi
     return hipGetLastError();
 }
 
+template <typename R, int TB>
+static hipError_t launch_tile_tb(const GTables *g, const HTables *h, const void *in, void *out, uint32_t ntiles,
+                                 size_t lds, hipStream_t s) {
+    if (lds > 65536) {   // tiles above 64 KiB need the raised dynamic-LDS limit
+        hipError_t e = hipFuncSetAttribute((const void *)k_hpsi_tile<R, TB>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    for (uint32_t base = 0; base < ntiles; base += kMaxGrid) {
+        const uint32_t n = (ntiles - base) < kMaxGrid ? (ntiles - base) : kMaxGrid;
+        hipLaunchKernelGGL((k_hpsi_tile<R, TB>), dim3(n), dim3(kThreads), lds, s, g, h, (CV2p<R>)in, (V2p<R>)out,
+                           ntiles, base);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_hpsi_tile(bool c64, const GTables *g, const HTables *h, int tile_bits, int n_uniform, int n_terms,
+                            const void *in, void *out, uint64_t dim, hipStream_t s) {
+    const uint64_t ntiles64 = dim >> tile_bits;
+    if (ntiles64 == 0 || ntiles64 > 0x7fffffffull) return hipErrorInvalidValue;
+    const uint32_t ntiles = (uint32_t)ntiles64;
+    const size_t esz = c64 ? 8 : 16;
+    const size_t ngrp = (size_t)1 << (tile_bits - 6);
+    const size_t lds = esz * (((size_t)1 << tile_bits) + (size_t)n_uniform * ngrp + (size_t)(n_terms - n_uniform) * 8);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    switch (tile_bits) {
+    case 11: return c64 ? launch_tile_tb<float, 11>(g, h, in, out, ntiles, lds, s) : launch_tile_tb<double, 11>(g, h, in, out, ntiles, lds, s);
+    case 12: return c64 ? launch_tile_tb<float, 12>(g, h, in, out, ntiles, lds, s) : launch_tile_tb<double, 12>(g, h, in, out, ntiles, lds, s);
+    case 13: return c64 ? launch_tile_tb<float, 13>(g, h, in, out, ntiles, lds, s) : hipErrorInvalidValue;
+    default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_hpsi_c128(const HTables *t, const void *in, void *out, uint64_t dim, hipStream_t s) {
     return launch_rows_hpsi<double>(t, in, out, dim, s);
 }
@@ -465,8 +669,6 @@ hipError_t launch_current_c64(const CurrentTable *t, const void *in, void *out, 
         else hipLaunchKernelGGL(KERNEL<double>, __VA_ARGS__);           \
     } while (0)
 
-template <typename R> using V2p = typename Vec2<R>::T *;
-template <typename R> using CV2p = const typename Vec2<R>::T *;
 
 hipError_t launch_dot(bool c64, const void *x, const void *y, uint64_t dim, double *partials, double *result,
                       hipStream_t s) {

@@ -162,6 +162,17 @@ class Engine:
         check(lib().esc_hpsi(self._h, _dev_ptr(x), _dev_ptr(out), _stream(stream)))
         return out
 
+    def autotune(self, x, out=None, reps=5, stream=None):
+        """Time every H.psi tile size on x and keep the fastest (esc_autotune)."""
+        out = self.empty() if out is None else out
+        tb, ms = ctypes.c_int(), ctypes.c_double()
+        check(lib().esc_autotune(self._h, _dev_ptr(x), _dev_ptr(out), int(reps), ctypes.byref(tb),
+                                 ctypes.byref(ms), _stream(stream)))
+        return tb.value, ms.value
+
+    def set_tile_bits(self, tile_bits: int):
+        check(lib().esc_set_tile_bits(self._h, int(tile_bits)))
+
     def apply_op(self, op: str, site: int, x, out=None, stream=None):
         """Observables::ApplyOp / ApplyJ*Total (site = -1) (src/Observables.h:24-67,448-470,664-699)."""
         out = self.empty() if out is None else out

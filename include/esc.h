@@ -97,6 +97,15 @@ int esc_ctx_destroy(esc_ctx *ctx);
 /* Vector length (2^N) and bytes per element (16 or 8) of a context. */
 int esc_ctx_info(const esc_ctx *ctx, int64_t *dim, int32_t *elem_bytes, int32_t *precision);
 
+/* Kernel autotuner (no reference counterpart): times esc_hpsi with every
+ * tile size the context supports (rows per workgroup 2^11, 2^12, 2^13 for
+ * c64; 0 = one row per lane) over `reps` launches on d_in -> d_out, keeps the
+ * fastest for all later H.psi calls of the context and reports it.  d_out is
+ * overwritten. */
+int esc_autotune(esc_ctx *ctx, const void *d_in, void *d_out, int reps, int *tile_bits, double *ms, void *stream);
+/* Select the H.psi kernel explicitly (tile_bits as reported by esc_autotune). */
+int esc_set_tile_bits(esc_ctx *ctx, int tile_bits);
+
 /* ---------------------------------------------------------------- kernels -- */
 /* d_out = H d_in.  Replaces Hamiltonian::MatrixVectorMult
  * (src/Hamiltonian.h:480-557).  d_in and d_out must not overlap. */
