@@ -26,7 +26,7 @@ EXPORTS = [
     "esc_apply_op", "esc_start_vector", "esc_dot", "esc_axpby", "esc_upload", "esc_download", "esc_lanczos",
     "esc_ritz_overlaps", "esc_ritz_vector", "esc_lanczos_cf", "esc_dynamics", "esc_honeycomb",
     "esc_connectors", "esc_params_load", "esc_tridiag_eig", "esc_cf_spectrum", "esc_lehmann_spectrum",
-    "esc_autotune", "esc_set_tile_bits",
+    "esc_autotune", "esc_set_tile_bits", "esc_kernel_configs",
 ]
 
 
@@ -112,6 +112,7 @@ def lib():
         "esc_lehmann_spectrum": ([I, P, P, P, P, D, P, I, P], I),
         "esc_autotune": ([P, P, P, I, P, P, P], I),
         "esc_set_tile_bits": ([P, I], I),
+        "esc_kernel_configs": ([P, P, I, P], I),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -169,9 +169,13 @@ struct esc_ctx {
     void *d_aux = nullptr;                          // dynamics work vectors
     size_t aux_cap = 0;
     esc::CurrentTable *d_combo = nullptr;           // combined-operator table
-    std::map<int, esc::GTables *> d_gtab;           // tiled-kernel tables per tile size
-    std::map<int, std::pair<int, int>> gcount;      // (uniform terms, all terms) per tile size
-    int tile_bits = 0;                              // 0: one-row-per-lane kernel
+    struct PassDev {
+        esc::GTables *d;
+        int n_uniform, n_terms;
+    };
+    std::map<int, std::vector<PassDev>> d_pass;     // windowed tiled kernel per configuration
+    int tile_bits = 0;                              // kernel configuration: 0 = one row per lane,
+                                                    // else 100 * tile bits + window bits
 
     bool c64() const { return precision == ESC_C64; }
 
@@ -183,7 +187,8 @@ struct esc_ctx {
         if (d_vecs) (void)hipFree(d_vecs);
         if (d_aux) (void)hipFree(d_aux);
         if (d_combo) (void)hipFree(d_combo);
-        for (auto &kv : d_gtab) (void)hipFree(kv.second);
+        for (auto &kv : d_pass)
+            for (auto &p : kv.second) (void)hipFree(p.d);
     }
     void ensure_aux(size_t nvec) {
         if (nvec <= aux_cap) return;
@@ -287,31 +292,30 @@ bool kz_sum_exact(double kz4, int nz) {
     return true;
 }
 
-void build_gtables(const esc_ctx *c, int TB, esc::GTables &g) {
-    const esc_model &m = c->model;
+struct HostTerm {
+    uint32_t mask;
+    int b0, b1, b2;
+    double coef[8][2];
+    bool cplx;
+};
+
+// all off-diagonal terms of H in the 8-entry-table form (values as HTables)
+std::vector<HostTerm> host_terms(const esc_ctx *c) {
     const esc::HTables &h = c->htab;
-    std::memset(&g, 0, sizeof(g));
-    const int N = m.n_sites;
-    g.n_sites = (uint32_t)N;
-    g.tile_bits = (uint32_t)TB;
-    std::vector<esc::GTerm> cls[esc::kNumClasses];
-    auto add_term = [&](uint32_t mask, int b0, int b1, int b2, const double (*coef)[2]) {
-        esc::GTerm t;
-        std::memset(&t, 0, sizeof(t));
+    std::vector<HostTerm> out;
+    auto push = [&](uint32_t mask, int b0, int b1, int b2, const double (*coef)[2]) {
+        HostTerm t;
         t.mask = mask;
-        t.bits = (uint32_t)b0 | ((uint32_t)b1 << 8) | ((uint32_t)b2 << 16);
-        bool cplx = false;
+        t.b0 = b0; t.b1 = b1; t.b2 = b2;
+        t.cplx = false;
         for (int e = 0; e < 8; ++e) {
             t.coef[e][0] = coef[e][0];
             t.coef[e][1] = coef[e][1];
-            if (coef[e][1] != 0.0) cplx = true;
+            if (coef[e][1] != 0.0) t.cplx = true;
         }
-        const bool far = (mask >> TB) != 0;
-        const bool lane = b0 < 6 || b1 < 6 || b2 < 6;
-        cls[(cplx ? 1 : 0) | (far ? 2 : 0) | (lane ? 4 : 0)].push_back(t);
+        out.push_back(t);
     };
-    // bond terms (same values as HTables: p0 + p1 s_i s_j + i q (s_i + s_j))
-    for (uint32_t b = 0; b < h.n_bonds; ++b) {
+    for (uint32_t b = 0; b < h.n_bonds; ++b) {          // p0 + p1 s_i s_j + i q (s_i + s_j)
         const uint32_t p = h.bond[b];
         const int i = p & 0xff, j = (p >> 8) & 0xff, f = p >> 16;
         double coef[8][2];
@@ -320,10 +324,9 @@ void build_gtables(const esc_ctx *c, int TB, esc::GTables &g) {
             coef[e][0] = h.p0[f] + h.p1[f] * (si * sj);
             coef[e][1] = h.q[f] * (si + sj);
         }
-        add_term((1u << i) | (1u << j), i, j, i, coef);
+        push((1u << i) | (1u << j), i, j, i, coef);
     }
-    // single flips (bx2 + g4 s_q - i s_k (by2 + g4 s_p))
-    for (uint32_t k = 0; k < h.n_flip; ++k) {
+    for (uint32_t k = 0; k < h.n_flip; ++k) {            // bx2 + g4 s_q - i s_k (by2 + g4 s_p)
         const int xp = h.has_gamma_flip ? h.xpart[k] : -1, yp = h.has_gamma_flip ? h.ypart[k] : -1;
         double coef[8][2];
         for (int e = 0; e < 8; ++e) {
@@ -334,45 +337,122 @@ void build_gtables(const esc_ctx *c, int TB, esc::GTables &g) {
             coef[e][0] = re;
             coef[e][1] = -sk * im;
         }
-        add_term(1u << k, (int)k, xp >= 0 ? xp : (int)k, yp >= 0 ? yp : (int)k, coef);
+        push(1u << k, (int)k, xp >= 0 ? xp : (int)k, yp >= 0 ? yp : (int)k, coef);
     }
-    size_t n = 0;
-    for (int q = 0; q < esc::kNumClasses; ++q) {
-        g.cls_begin[q] = (uint32_t)n;
-        if (n + cls[q].size() > (size_t)esc::kMaxTerms) throw EscError(ESC_ERR_UNSUPPORTED, "too many terms");
-        for (auto &t : cls[q]) g.term[n++] = t;
-    }
-    g.cls_begin[esc::kNumClasses] = (uint32_t)n;
-    // diagonal
-    g.kz4 = h.kz4;
-    g.j4 = h.j4;
-    g.bz = h.bz;
-    g.nz = h.n_zdiag;
-    g.nj = h.n_jdiag;
-    g.diag_closed = kz_sum_exact(h.kz4, (int)h.n_zdiag) ? 1u : 0u;
-    auto groups = [&](const uint32_t *list, uint32_t nl, uint32_t *gd, uint32_t *gm, uint32_t &ng) {
-        ng = 0;
-        for (uint32_t b = 0; b < nl; ++b) {
-            uint32_t i = list[b] & 0xff, j = (list[b] >> 8) & 0xff;
-            if (i > j) std::swap(i, j);
-            const uint32_t d = j - i;
-            uint32_t q = 0;
-            while (q < ng && gd[q] != d) ++q;
-            if (q == ng) {
-                if (ng >= (uint32_t)esc::kMaxDistGroups) throw EscError(ESC_ERR_UNSUPPORTED, "too many bond distances");
-                gd[ng] = d;
-                gm[ng] = 0;
-                ++ng;
+    return out;
+}
+
+// Passes of the windowed tiled kernel for tile bits TB and window width w
+// (w >= n_sites: one pass over one window = the whole vector).
+std::vector<esc::GTables> build_passes(const esc_ctx *c, int TB, int w) {
+    const int N = c->n_sites;
+    if (w > N) w = N;
+    if (TB > w || TB < 7 || N - w > 16 || w - TB > 16) throw EscError(ESC_ERR_UNSUPPORTED, "tile/window shape");
+    const esc::HTables &h = c->htab;
+    std::vector<HostTerm> terms = host_terms(c);
+    std::vector<bool> done(terms.size(), false);
+    std::vector<esc::GTables> passes;
+    size_t left = terms.size();
+    bool first = true;
+    while (first || left > 0) {
+        // window bits: bits 0..5 always; the first pass takes the low w bits,
+        // later ones grow greedily around the highest remaining masks
+        uint32_t W = 0;
+        if (first) {
+            W = (w >= 32) ? 0xffffffffu : ((1u << w) - 1u);
+        } else {
+            W = 63u;
+            std::vector<size_t> order;
+            for (size_t t = 0; t < terms.size(); ++t)
+                if (!done[t]) order.push_back(t);
+            std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b2) { return terms[a].mask > terms[b2].mask; });
+            for (size_t t : order) {
+                const uint32_t nw = W | terms[t].mask;
+                if (__builtin_popcount(nw) <= w) W = nw;
             }
-            if (gm[q] & (1u << i)) {   // two bonds of one group on the same lower site: no closed form
-                g.diag_closed = 0;
-                return;
-            }
-            gm[q] |= 1u << i;
+            for (int b = 6; b < N && __builtin_popcount(W) < w; ++b) W |= 1u << b;   // fill up
         }
-    };
-    groups(h.zdiag, h.n_zdiag, g.zgrp_d, g.zgrp_m, g.n_zgrp);
-    groups(h.jdiag, h.n_jdiag, g.jgrp_d, g.jgrp_m, g.n_jgrp);
+        std::vector<int> wb, cb;   // window bits ascending, complement bits ascending
+        for (int b = 0; b < N; ++b) ((W >> b) & 1u ? wb : cb).push_back(b);
+        const int ww = (int)wb.size();
+        esc::GTables g;
+        std::memset(&g, 0, sizeof(g));
+        g.n_sites = (uint32_t)N;
+        g.tile_bits = (uint32_t)TB;
+        g.first_pass = first ? 1u : 0u;
+        g.n_windows = 1u << (N - ww);
+        g.blocks_per_window = 1u << (ww - TB);
+        g.xcd_windows = (g.n_windows % 8u == 0u) ? 1u : 0u;
+        auto scat = [](const std::vector<int> &bits, int from, uint32_t v) {
+            uint32_t r = 0;
+            for (int q = 0; from + q < (int)bits.size() && (v >> q); ++q)
+                if ((v >> q) & 1u) r |= 1u << bits[from + q];
+            return r;
+        };
+        for (uint32_t gi = 0; gi < (1u << (TB - 6)); ++gi) g.tscat[gi] = scat(wb, 6, gi);
+        for (uint32_t v = 0; v < 256; ++v) {
+            g.bscat[0][v] = scat(wb, TB, v);
+            g.bscat[1][v] = scat(wb, TB + 8, v);
+            g.wscat[0][v] = scat(cb, 0, v);
+            g.wscat[1][v] = scat(cb, 8, v);
+        }
+        std::vector<esc::GTerm> cls[esc::kNumClasses];
+        for (size_t t = 0; t < terms.size(); ++t) {
+            if (done[t] || (terms[t].mask & ~W) != 0) continue;
+            done[t] = true;
+            --left;
+            const HostTerm &ht = terms[t];
+            esc::GTerm gt;
+            std::memset(&gt, 0, sizeof(gt));
+            gt.mask = ht.mask;
+            gt.bits = (uint32_t)ht.b0 | ((uint32_t)ht.b1 << 8) | ((uint32_t)ht.b2 << 16);
+            for (int q = 0; q < ww; ++q)
+                if ((ht.mask >> wb[q]) & 1u) gt.local |= 1u << q;
+            std::memcpy(gt.coef, ht.coef, sizeof(gt.coef));
+            const bool far = (gt.local >> TB) != 0;
+            const bool lane = ht.b0 < 6 || ht.b1 < 6 || ht.b2 < 6;
+            cls[(ht.cplx ? 1 : 0) | (far ? 2 : 0) | (lane ? 4 : 0)].push_back(gt);
+        }
+        size_t n = 0;
+        for (int q = 0; q < esc::kNumClasses; ++q) {
+            g.cls_begin[q] = (uint32_t)n;
+            if (n + cls[q].size() > (size_t)esc::kMaxTerms) throw EscError(ESC_ERR_UNSUPPORTED, "too many terms");
+            for (auto &t : cls[q]) g.term[n++] = t;
+        }
+        g.cls_begin[esc::kNumClasses] = (uint32_t)n;
+        if (!first && n == 0) throw EscError(ESC_ERR_UNSUPPORTED, "window construction made no progress");
+        // diagonal (used by the first pass only)
+        g.kz4 = h.kz4;
+        g.j4 = h.j4;
+        g.bz = h.bz;
+        g.nz = h.n_zdiag;
+        g.nj = h.n_jdiag;
+        g.diag_closed = kz_sum_exact(h.kz4, (int)h.n_zdiag) ? 1u : 0u;
+        auto groups = [&](const uint32_t *list, uint32_t nl, uint32_t *gd, uint32_t *gm, uint32_t &ng) {
+            ng = 0;
+            for (uint32_t b = 0; b < nl; ++b) {
+                uint32_t i = list[b] & 0xff, j = (list[b] >> 8) & 0xff;
+                if (i > j) std::swap(i, j);
+                const uint32_t d = j - i;
+                uint32_t q = 0;
+                while (q < ng && gd[q] != d) ++q;
+                if (q == ng) {
+                    if (ng >= (uint32_t)esc::kMaxDistGroups) { g.diag_closed = 0; return; }
+                    gd[ng] = d;
+                    gm[ng] = 0;
+                    ++ng;
+                }
+                if (gm[q] & (1u << i)) { g.diag_closed = 0; return; }   // two bonds on one lower site
+                gm[q] |= 1u << i;
+            }
+        };
+        groups(h.zdiag, h.n_zdiag, g.zgrp_d, g.zgrp_m, g.n_zgrp);
+        groups(h.jdiag, h.n_jdiag, g.jgrp_d, g.jgrp_m, g.n_jgrp);
+        passes.push_back(g);
+        first = false;
+        if (passes.size() > 8) throw EscError(ESC_ERR_UNSUPPORTED, "too many window passes");
+    }
+    return passes;
 }
 
 // Pauli-string table for one operator (src/Observables.h:161-389,429-521)
@@ -514,41 +594,58 @@ double device_norm2(esc_ctx *c, const void *x, hipStream_t s) {
     return r;
 }
 
-esc::GTables *gtab(esc_ctx *c, int TB) {
-    auto it = c->d_gtab.find(TB);
-    if (it != c->d_gtab.end()) return it->second;
-    auto g = std::make_unique<esc::GTables>();
-    build_gtables(c, TB, *g);
-    esc::GTables *d = nullptr;
-    if (hipMalloc(&d, sizeof(esc::GTables)) != hipSuccess) throw EscError(ESC_ERR_ALLOC, "table allocation failed");
-    hip_check(hipMemcpy(d, g.get(), sizeof(esc::GTables), hipMemcpyHostToDevice), "hipMemcpy");
-    c->d_gtab[TB] = d;
-    c->gcount[TB] = {(int)g->cls_begin[4], (int)g->cls_begin[esc::kNumClasses]};
-    return d;
+const std::vector<esc_ctx::PassDev> &passes_for(esc_ctx *c, int config) {
+    auto it = c->d_pass.find(config);
+    if (it != c->d_pass.end()) return it->second;
+    const std::vector<esc::GTables> host = build_passes(c, (config / 100) % 100, config % 100);
+    std::vector<esc_ctx::PassDev> dev;
+    for (const auto &g : host) {
+        esc::GTables *d = nullptr;
+        if (hipMalloc(&d, sizeof(esc::GTables)) != hipSuccess) {
+            for (auto &p : dev) (void)hipFree(p.d);
+            throw EscError(ESC_ERR_ALLOC, "table allocation failed");
+        }
+        hip_check(hipMemcpy(d, &g, sizeof(esc::GTables), hipMemcpyHostToDevice), "hipMemcpy");
+        dev.push_back({d, (int)g.cls_begin[4], (int)g.cls_begin[esc::kNumClasses]});
+    }
+    return c->d_pass[config] = dev;
 }
 
-// tile sizes the tiled kernel is built for (rows per workgroup = 2^TB)
+// kernel configurations: 10000 * waves per workgroup + 100 * tile bits +
+// window bits (window >= N: one pass); 0 = one row per lane
 std::vector<int> tile_candidates(const esc_ctx *c) {
     std::vector<int> v;
-    for (int tb : {11, 12, 13})
-        if (c->n_sites >= tb && (c->c64() || tb <= 12)) v.push_back(tb);
+    const int N = c->n_sites;
+    const std::vector<int> windows = c->c64() ? std::vector<int>{N, 18, 19} : std::vector<int>{N, 17, 18};
+    for (int waves : {4, 8}) {
+        for (int tb : {10, 11, 12}) {
+            if (N < tb || !esc::tile_shape_supported(c->c64(), tb, waves)) continue;
+            for (int w : windows) {
+                const int ww = w > N ? N : w;
+                if (ww < tb || N - ww > 16 || ww - tb > 16) continue;
+                const int cfg = 10000 * waves + 100 * tb + ww;
+                if (std::find(v.begin(), v.end(), cfg) == v.end()) v.push_back(cfg);
+            }
+        }
+    }
     return v;
 }
 
-// default tile before tuning: 2^11 c128 rows (32 KiB) or 2^12 c64 rows;
-// systems below 2^11 rows use the one-row-per-lane kernel
+// before tuning: 4 waves, 2^11-row tiles, windows of 2^18 rows (the fastest
+// shape measured at N = 24); systems below 2^11 rows use one row per lane
 int default_tile_bits(const esc_ctx *c) {
-    const int tb = c->c64() ? 12 : 11;
-    if (c->n_sites >= tb) return tb;
-    return c->n_sites >= 11 ? 11 : 0;
+    const int N = c->n_sites;
+    if (N < 11) return 0;
+    return 40000 + 100 * 11 + (N < 18 ? N : 18);
 }
 
-void launch_hpsi_with(esc_ctx *c, int tb, const void *in, void *out, hipStream_t s) {
-    if (tb > 0) {
-        esc::GTables *g = gtab(c, tb);
-        const auto cnt = c->gcount[tb];
-        hip_check(esc::launch_hpsi_tile(c->c64(), g, c->d_htab, tb, cnt.first, cnt.second, in, out, c->dim, s),
-                  "hpsi tile launch");
+void launch_hpsi_with(esc_ctx *c, int config, const void *in, void *out, hipStream_t s) {
+    if (config > 0) {
+        const int waves = config / 10000, tb = (config / 100) % 100;
+        for (const auto &p : passes_for(c, config))
+            hip_check(esc::launch_hpsi_tile(c->c64(), p.d, c->d_htab, tb, waves, p.n_uniform, p.n_terms, in, out,
+                                            c->dim, s),
+                      "hpsi tile launch");
     } else {
         hip_check(c->c64() ? esc::launch_hpsi_c64(c->d_htab, in, out, c->dim, s)
                            : esc::launch_hpsi_c128(c->d_htab, in, out, c->dim, s), "hpsi launch");
@@ -1144,6 +1241,18 @@ int esc_autotune(esc_ctx *ctx, const void *d_in, void *d_out, int reps, int *til
         ctx->tile_bits = best;
         if (tile_bits) *tile_bits = best;
         if (ms) *ms = best_ms;
+    });
+}
+
+int esc_kernel_configs(const esc_ctx *ctx, int32_t *configs, int32_t cap, int32_t *n) {
+    return guarded([&] {
+        check_ctx(ctx);
+        check_ptr(n, "n");
+        std::vector<int> v = tile_candidates(ctx);
+        v.insert(v.begin(), 0);
+        *n = (int32_t)v.size();
+        if (configs)
+            for (int32_t i = 0; i < cap && i < (int32_t)v.size(); ++i) configs[i] = v[i];
     });
 }
 

@@ -46,13 +46,28 @@ constexpr int kMaxDistGroups = 32;
 constexpr int kNumClasses = 8;
 
 struct GTerm {
-    uint32_t mask;
+    uint32_t mask;          // partner offset in row space
     uint32_t bits;          // b0 | b1 << 8 | b2 << 16
+    uint32_t local;         // mask in window coordinates
+    uint32_t pad;
     double coef[8][2];
 };
 
+// One pass of the windowed tiled H.psi.  The pass owns a window: an ordered
+// set of w row bits (bits 0..5 first).  Rows are enumerated as
+//   r = wscat(window id) | bscat(tile within window) | tscat[g] | lane
+// where tscat places the row-group index g of a tile on window bits 6..TB-1,
+// bscat the tile index on window bits TB..w-1 and wscat the window id on the
+// bits outside the window (each as two byte-indexed lookup tables).  Terms
+// whose mask lies inside the window are applied by this pass; the first pass
+// also adds the diagonal and overwrites the output, later passes accumulate.
 struct GTables {
     uint32_t n_sites, tile_bits;
+    uint32_t first_pass;            // 1: acc = diag * psi, else acc = out
+    uint32_t n_windows, blocks_per_window, xcd_windows;
+    uint32_t tscat[128];
+    uint32_t bscat[2][256];
+    uint32_t wscat[2][256];
     uint32_t cls_begin[kNumClasses + 1];
     // diagonal: closed form when the Kitaev-z sum is exact in any order
     uint32_t diag_closed;           // 1: popcount form, 0: sequential (HTables path)
@@ -92,9 +107,12 @@ void clear_error();
 // Tiled H.psi: 2^tile_bits rows per workgroup staged in LDS; h is the
 // sequential-diagonal table used when g->diag_closed == 0.
 // n_uniform = number of terms in classes 0..3, n_terms = all terms (for the
-// LDS size); tile_bits in {11, 12} (c128) or {11, 12, 13} (c64).
-hipError_t launch_hpsi_tile(bool c64, const GTables *d_g, const HTables *d_h, int tile_bits, int n_uniform,
-                            int n_terms, const void *in, void *out, uint64_t dim, hipStream_t s);
+// LDS size).
+hipError_t launch_hpsi_tile(bool c64, const GTables *d_g, const HTables *d_h, int tile_bits, int waves,
+                            int n_uniform, int n_terms, const void *in, void *out, uint64_t dim, hipStream_t s);
+// tile shapes the tiled kernel is compiled for: 2^10..2^12 rows, 4 or 8 waves
+// per workgroup, 2 to 8 row groups of 64 per wave
+bool tile_shape_supported(bool c64, int tile_bits, int waves);
 hipError_t launch_hpsi_c128(const HTables *d_tab, const void *in, void *out, uint64_t dim, hipStream_t s);
 hipError_t launch_hpsi_c64(const HTables *d_tab, const void *in, void *out, uint64_t dim, hipStream_t s);
 hipError_t launch_current_c128(const CurrentTable *d_tab, const void *in, void *out, uint64_t dim, hipStream_t s);

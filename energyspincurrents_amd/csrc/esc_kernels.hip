@@ -161,20 +161,23 @@ __device__ __forceinline__ double hpsi_diag(const GTables *__restrict__ G, const
 
 // Terms of one class for all NG row groups of a wave.  Row group k of wave w
 // is g = w + 4k, rows r = base | g << 6 | lane.
-template <typename R, int TB, int CLS>
+template <typename R, int TB, int WPB, int CLS>
 __device__ __forceinline__ void hpsi_class(const GTables *__restrict__ G, uint32_t base, uint32_t wave,
                                            uint32_t lane, const typename Vec2<R>::T *__restrict__ in,
                                            const typename Vec2<R>::T *s_in, const typename Vec2<R>::T *s_cu,
-                                           const typename Vec2<R>::T *s_ct, R *ar, R *ai) {
+                                           const typename Vec2<R>::T *s_ct, const uint32_t *s_rhi, R *ar, R *ai) {
     using V2 = typename Vec2<R>::T;
-    constexpr int NGRP = 1 << (TB - 6), NG = NGRP / 4;
+    constexpr int NGRP = 1 << (TB - 6), NG = NGRP / WPB;
     constexpr uint32_t TMASK = (1u << TB) - 1u;
     constexpr bool CPLX = (CLS & 1) != 0, FAR = (CLS & 2) != 0, LANE = (CLS & 4) != 0;
     const uint32_t t0 = G->cls_begin[CLS], t1 = G->cls_begin[CLS + 1];
     const uint32_t nU = G->cls_begin[4];
+    // two terms per iteration: their loads are in flight together
+#pragma unroll 2
     for (uint32_t t = t0; t < t1; ++t) {
-        const uint32_t m = G->term[t].mask;
+        const uint32_t m = G->term[t].mask;          // row-space mask
         const uint32_t mlo = m & 63u, mhi = m & ~63u;
+        const uint32_t mloc = G->term[t].local;      // mask in window coordinates (near terms)
         const uint32_t loff = lane ^ mlo;
         uint32_t lp = 0, b0 = 0, b1 = 0, b2 = 0;
         if (LANE) {
@@ -186,14 +189,14 @@ __device__ __forceinline__ void hpsi_class(const GTables *__restrict__ G, uint32
         }
 #pragma unroll
         for (int k = 0; k < NG; ++k) {
-            const uint32_t g = wave + 4u * (uint32_t)k;
-            const uint32_t r_hi = base | (g << 6);
+            const uint32_t g = wave + (uint32_t)WPB * (uint32_t)k;
+            const uint32_t r_hi = s_rhi[g];
             V2 x;
             if (FAR) {
                 const V2 *p = in + (size_t)(r_hi ^ mhi);
                 x = p[loff];
             } else {
-                x = s_in[(((g << 6) ^ mhi) & TMASK) | loff];
+                x = s_in[(((g << 6) ^ (mloc & ~63u)) & TMASK) | loff];
             }
             V2 c;
             if (LANE) {
@@ -214,26 +217,42 @@ __device__ __forceinline__ void hpsi_class(const GTables *__restrict__ G, uint32
     }
 }
 
-template <typename R, int TB>
-__global__ __launch_bounds__(kThreads) void k_hpsi_tile(const GTables *__restrict__ G, const HTables *__restrict__ H,
+template <typename R, int TB, int WPB>
+__global__ __launch_bounds__(WPB * 64) void k_hpsi_tile(const GTables *__restrict__ G, const HTables *__restrict__ H,
                                                         const typename Vec2<R>::T *__restrict__ in,
                                                         typename Vec2<R>::T *__restrict__ out, uint32_t nblk_total,
                                                         uint32_t block_base) {
     using V2 = typename Vec2<R>::T;
-    constexpr int NGRP = 1 << (TB - 6), NG = NGRP / 4;
+    constexpr int NGRP = 1 << (TB - 6), NG = NGRP / WPB;
+    constexpr int NT = WPB * 64;
     constexpr uint32_t NROW = 1u << TB;
     extern __shared__ unsigned char esc_lds[];
     V2 *s_in = reinterpret_cast<V2 *>(esc_lds);                 // the tile
     V2 *s_cu = s_in + NROW;                                     // uniform-term coefficients [nU][NGRP]
     const uint32_t nU = G->cls_begin[4], nT = G->cls_begin[kNumClasses];
     V2 *s_ct = s_cu + nU * NGRP;                                // lane-term tables [nT - nU][8]
-    const uint32_t tile = xcd_block(blockIdx.x + block_base, nblk_total);
-    const uint32_t base = tile << TB;
-    const V2 *src = in + (size_t)base;
-    for (uint32_t i = threadIdx.x; i < NROW; i += kThreads) s_in[i] = src[i];
-    for (uint32_t e = threadIdx.x; e < nU * NGRP; e += kThreads) {
+    __shared__ uint32_t s_rhi[NGRP];                            // first row of each row group
+    // block -> (window, tile within window); with xcd_windows every window
+    // is walked by one XCD (workgroups are dealt round-robin over 8 XCDs)
+    const uint32_t b = blockIdx.x + block_base;
+    const uint32_t bpw = G->blocks_per_window;
+    uint32_t wi, tl;
+    if (G->xcd_windows) {
+        const uint32_t seq = b / kNumXcd;
+        wi = (seq / bpw) * kNumXcd + b % kNumXcd;
+        tl = seq % bpw;
+    } else {
+        wi = b / bpw;
+        tl = b % bpw;
+    }
+    const uint32_t base = G->wscat[0][wi & 0xffu] | G->wscat[1][(wi >> 8) & 0xffu] | G->bscat[0][tl & 0xffu] |
+                          G->bscat[1][(tl >> 8) & 0xffu];
+    for (uint32_t g = threadIdx.x; g < (uint32_t)NGRP; g += NT) s_rhi[g] = base | G->tscat[g];
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < NROW; i += NT) s_in[i] = in[s_rhi[i >> 6] | (i & 63u)];
+    for (uint32_t e = threadIdx.x; e < nU * NGRP; e += NT) {
         const uint32_t t = e / NGRP, g = e % NGRP;
-        const uint32_t r_hi = base | (g << 6), bits = G->term[t].bits;
+        const uint32_t r_hi = s_rhi[g], bits = G->term[t].bits;
         const uint32_t idx = ((r_hi >> (bits & 0xffu)) & 1u) | (((r_hi >> ((bits >> 8) & 0xffu)) & 1u) << 1) |
                              (((r_hi >> (bits >> 16)) & 1u) << 2);
         V2 c;
@@ -241,7 +260,7 @@ __global__ __launch_bounds__(kThreads) void k_hpsi_tile(const GTables *__restric
         c.y = (R)G->term[t].coef[idx][1];
         s_cu[e] = c;
     }
-    for (uint32_t e = threadIdx.x; e < (nT - nU) * 8u; e += kThreads) {
+    for (uint32_t e = threadIdx.x; e < (nT - nU) * 8u; e += NT) {
         V2 c;
         c.x = (R)G->term[nU + e / 8u].coef[e % 8u][0];
         c.y = (R)G->term[nU + e / 8u].coef[e % 8u][1];
@@ -251,28 +270,37 @@ __global__ __launch_bounds__(kThreads) void k_hpsi_tile(const GTables *__restric
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     R ar[NG], ai[NG];
+    if (G->first_pass) {
 #pragma unroll
-    for (int k = 0; k < NG; ++k) {
-        const uint32_t g = wave + 4u * (uint32_t)k;
-        const V2 v0 = s_in[(g << 6) | lane];
-        const R dd = (R)hpsi_diag<R>(G, H, base | (g << 6) | lane);
-        ar[k] = dd * v0.x;
-        ai[k] = dd * v0.y;
+        for (int k = 0; k < NG; ++k) {
+            const uint32_t g = wave + (uint32_t)WPB * (uint32_t)k;
+            const V2 v0 = s_in[(g << 6) | lane];
+            const R dd = (R)hpsi_diag<R>(G, H, s_rhi[g] | lane);
+            ar[k] = dd * v0.x;
+            ai[k] = dd * v0.y;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < NG; ++k) {
+            const V2 o = out[s_rhi[wave + (uint32_t)WPB * (uint32_t)k] | lane];
+            ar[k] = o.x;
+            ai[k] = o.y;
+        }
     }
-    hpsi_class<R, TB, 0>(G, base, wave, lane, in, s_in, s_cu, s_ct, ar, ai);
-    hpsi_class<R, TB, 1>(G, base, wave, lane, in, s_in, s_cu, s_ct, ar, ai);
-    hpsi_class<R, TB, 2>(G, base, wave, lane, in, s_in, s_cu, s_ct, ar, ai);
-    hpsi_class<R, TB, 3>(G, base, wave, lane, in, s_in, s_cu, s_ct, ar, ai);
-    hpsi_class<R, TB, 4>(G, base, wave, lane, in, s_in, s_cu, s_ct, ar, ai);
-    hpsi_class<R, TB, 5>(G, base, wave, lane, in, s_in, s_cu, s_ct, ar, ai);
-    hpsi_class<R, TB, 6>(G, base, wave, lane, in, s_in, s_cu, s_ct, ar, ai);
-    hpsi_class<R, TB, 7>(G, base, wave, lane, in, s_in, s_cu, s_ct, ar, ai);
+    hpsi_class<R, TB, WPB, 0>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_rhi, ar, ai);
+    hpsi_class<R, TB, WPB, 1>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_rhi, ar, ai);
+    hpsi_class<R, TB, WPB, 2>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_rhi, ar, ai);
+    hpsi_class<R, TB, WPB, 3>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_rhi, ar, ai);
+    hpsi_class<R, TB, WPB, 4>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_rhi, ar, ai);
+    hpsi_class<R, TB, WPB, 5>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_rhi, ar, ai);
+    hpsi_class<R, TB, WPB, 6>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_rhi, ar, ai);
+    hpsi_class<R, TB, WPB, 7>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_rhi, ar, ai);
     // streamed result: a non-temporal store keeps it from displacing the
     // input rows that other tiles still gather from
     typedef R nvec2 __attribute__((ext_vector_type(2)));
 #pragma unroll
     for (int k = 0; k < NG; ++k) {
-        const uint32_t r = base | ((wave + 4u * (uint32_t)k) << 6) | lane;
+        const uint32_t r = s_rhi[wave + (uint32_t)WPB * (uint32_t)k] | lane;
         nvec2 o = {ar[k], ai[k]};
         __builtin_nontemporal_store(o, reinterpret_cast<nvec2 *>(out + r));
     }
@@ -617,37 +645,52 @@ static hipError_t launch_rows_current(const CurrentTable *t, const void *in, voi
     return hipGetLastError();
 }
 
-template <typename R, int TB>
+template <typename R, int TB, int WPB>
 static hipError_t launch_tile_tb(const GTables *g, const HTables *h, const void *in, void *out, uint32_t ntiles,
                                  size_t lds, hipStream_t s) {
     if (lds > 65536) {   // tiles above 64 KiB need the raised dynamic-LDS limit
-        hipError_t e = hipFuncSetAttribute((const void *)k_hpsi_tile<R, TB>,
+        hipError_t e = hipFuncSetAttribute((const void *)k_hpsi_tile<R, TB, WPB>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
     for (uint32_t base = 0; base < ntiles; base += kMaxGrid) {
         const uint32_t n = (ntiles - base) < kMaxGrid ? (ntiles - base) : kMaxGrid;
-        hipLaunchKernelGGL((k_hpsi_tile<R, TB>), dim3(n), dim3(kThreads), lds, s, g, h, (CV2p<R>)in, (V2p<R>)out,
-                           ntiles, base);
+        hipLaunchKernelGGL((k_hpsi_tile<R, TB, WPB>), dim3(n), dim3(WPB * 64), lds, s, g, h, (CV2p<R>)in,
+                           (V2p<R>)out, ntiles, base);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_hpsi_tile(bool c64, const GTables *g, const HTables *h, int tile_bits, int n_uniform, int n_terms,
-                            const void *in, void *out, uint64_t dim, hipStream_t s) {
+// each wave holds 2^(TB-6)/WPB row groups in registers; beyond 8 groups the
+// unrolled term loop spills
+bool tile_shape_supported(bool c64, int tile_bits, int waves) {
+    (void)c64;
+    if (waves != 4 && waves != 8) return false;
+    const int groups = (1 << (tile_bits - 6)) / waves;
+    return tile_bits >= 10 && tile_bits <= 12 && groups >= 2 && groups <= 8;
+}
+
+hipError_t launch_hpsi_tile(bool c64, const GTables *g, const HTables *h, int tile_bits, int waves, int n_uniform,
+                            int n_terms, const void *in, void *out, uint64_t dim, hipStream_t s) {
     const uint64_t ntiles64 = dim >> tile_bits;
-    if (ntiles64 == 0 || ntiles64 > 0x7fffffffull) return hipErrorInvalidValue;
+    if (ntiles64 == 0 || ntiles64 > 0x7fffffffull || !tile_shape_supported(c64, tile_bits, waves))
+        return hipErrorInvalidValue;
     const uint32_t ntiles = (uint32_t)ntiles64;
     const size_t esz = c64 ? 8 : 16;
     const size_t ngrp = (size_t)1 << (tile_bits - 6);
     const size_t lds = esz * (((size_t)1 << tile_bits) + (size_t)n_uniform * ngrp + (size_t)(n_terms - n_uniform) * 8);
-    if (lds > 160 * 1024) return hipErrorInvalidValue;
-    switch (tile_bits) {
-    case 11: return c64 ? launch_tile_tb<float, 11>(g, h, in, out, ntiles, lds, s) : launch_tile_tb<double, 11>(g, h, in, out, ntiles, lds, s);
-    case 12: return c64 ? launch_tile_tb<float, 12>(g, h, in, out, ntiles, lds, s) : launch_tile_tb<double, 12>(g, h, in, out, ntiles, lds, s);
-    case 13: return c64 ? launch_tile_tb<float, 13>(g, h, in, out, ntiles, lds, s) : hipErrorInvalidValue;
-    default: return hipErrorInvalidValue;
-    }
+    if (lds + 512 > 160 * 1024) return hipErrorInvalidValue;
+#define ESC_TILE_CASE(TBV, WV)                                                                          \
+    if (tile_bits == TBV && waves == WV)                                                                \
+        return c64 ? launch_tile_tb<float, TBV, WV>(g, h, in, out, ntiles, lds, s)                      \
+                   : launch_tile_tb<double, TBV, WV>(g, h, in, out, ntiles, lds, s);
+    ESC_TILE_CASE(10, 4)
+    ESC_TILE_CASE(11, 4)
+    ESC_TILE_CASE(10, 8)
+    ESC_TILE_CASE(11, 8)
+    ESC_TILE_CASE(12, 8)
+#undef ESC_TILE_CASE
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_hpsi_c128(const HTables *t, const void *in, void *out, uint64_t dim, hipStream_t s) {

@@ -98,13 +98,19 @@ int esc_ctx_destroy(esc_ctx *ctx);
 int esc_ctx_info(const esc_ctx *ctx, int64_t *dim, int32_t *elem_bytes, int32_t *precision);
 
 /* Kernel autotuner (no reference counterpart): times esc_hpsi with every
- * tile size the context supports (rows per workgroup 2^11, 2^12, 2^13 for
- * c64; 0 = one row per lane) over `reps` launches on d_in -> d_out, keeps the
- * fastest for all later H.psi calls of the context and reports it.  d_out is
- * overwritten. */
+ * kernel configuration the context supports over `reps` launches on
+ * d_in -> d_out, keeps the fastest for all later H.psi calls of the context
+ * and reports it as tile_bits.  A configuration is 0 (one row per lane) or
+ * 10000 * V + 100 * TB + W: workgroups of V waves sharing 2^TB rows staged in
+ * LDS, walking windows of 2^W rows (one XCD per window) in as many passes as
+ * the bond graph needs.
+ * d_out is overwritten. */
 int esc_autotune(esc_ctx *ctx, const void *d_in, void *d_out, int reps, int *tile_bits, double *ms, void *stream);
-/* Select the H.psi kernel explicitly (tile_bits as reported by esc_autotune). */
+/* Select the H.psi kernel explicitly (a configuration as reported by
+ * esc_autotune or listed by esc_kernel_configs). */
 int esc_set_tile_bits(esc_ctx *ctx, int tile_bits);
+/* List the kernel configurations of a context (*n = count; up to cap written). */
+int esc_kernel_configs(const esc_ctx *ctx, int32_t *configs, int32_t cap, int32_t *n);
 
 /* ---------------------------------------------------------------- kernels -- */
 /* d_out = H d_in.  Replaces Hamiltonian::MatrixVectorMult

@@ -126,21 +126,21 @@ def test_error_paths():
         eng.apply_op("Jex", 99, x)         # site out of range
 
 
-def _variants(prec, N):
-    tbs = [0] + [tb for tb in (11, 12, 13) if tb <= N and (prec == "c64" or tb <= 12)]
-    return tbs
 
 
 @pytest.mark.parametrize("kw", [dict(K=(1.0, 1.0, 1.0), B=(0.3, 0.3, 0.3)),          # popcount diagonal
                                 dict(K=(-1.0, 0.7, 0.3), B=(0.1, -0.2, 0.3))])       # sequential diagonal
-def test_tiled_kernels_matrix_elements_bit_exact_n16(kw):
-    """Every H.psi kernel variant (one row per lane, tiles of 2^11 and 2^12
-    rows) gives the oracle's matrix elements bit for bit at N=16."""
-    m = O.Model(2, 8, **kw)
+def test_tiled_kernels_matrix_elements_bit_exact_n18(kw):
+    """Every H.psi kernel configuration (one row per lane, tiles of 2^11 and
+    2^12 rows, one pass or two passes over 2^17-row windows) gives the
+    oracle's matrix elements bit for bit at N=18."""
+    m = O.Model(3, 6, **kw)
     eng = engine_for(m)
     rng = np.random.default_rng(16)
-    cols = [0, m.dim - 1] + list(rng.integers(0, m.dim, 24))
-    for tb in _variants("c128", m.N):
+    cols = [0, m.dim - 1] + list(rng.integers(0, m.dim, 12))
+    configs = eng.kernel_configs()
+    assert 0 in configs and len(configs) >= 3
+    for tb in configs:
         eng.set_tile_bits(tb)
         print(f"[tile {tb}] {kw}", flush=True)
         for k in cols:
@@ -161,10 +161,11 @@ def test_tiled_kernels_random_vectors(prec, tol, Lx, Ly):
     ref = O.hpsi_gather(m, v)
     eng = engine_for(m, prec)
     x = eng.upload(v)
-    for tb in _variants(prec, m.N):
+    configs = eng.kernel_configs()
+    for tb in configs:
         eng.set_tile_bits(tb)
-        print(f"[tile {tb}] {prec} N={m.N}", flush=True)
-        assert rel(eng.download(eng.hpsi(x)), ref) < tol, f"tile {tb}"
+        print(f"[kernel config {tb}] {prec} N={m.N}", flush=True)
+        assert rel(eng.download(eng.hpsi(x)), ref) < tol, f"config {tb}"
     tb, ms = eng.autotune(x, reps=3)
-    assert tb in _variants(prec, m.N) and ms > 0
+    assert tb in configs and ms > 0
     assert rel(eng.download(eng.hpsi(x)), ref) < tol
