@@ -7,7 +7,12 @@ arithmetic).  Prints one JSON line.  The complex64 H.psi, the fused-step
 reductions and the CPU baseline (the oracle's OpenMP gather H.psi on the host
 cores, a bounded sample) are reported beside it.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--spectrum]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-spectrum]
+
+The spectrum block times the whole JeJe pipeline at the same size in both
+precisions: ground state (200 Lanczos steps, full reorthogonalisation, complex
+Gaussian start vector), the three energy-current vectors and the continued
+fractions (200 steps each) on omega = linspace(0, 6, 2000), eta = 0.05.
 """
 from __future__ import annotations
 
@@ -34,7 +39,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-reps", type=int, default=3)
-    ap.add_argument("--spectrum", action="store_true", help="also time the JeJe CF spectrum pipeline")
+    ap.add_argument("--no-spectrum", action="store_true", help="skip the JeJe spectrum timing")
+    ap.add_argument("--kernel-config", default=None,
+                    help="C128,C64: use these H.psi kernel configurations instead of autotuning (profiling runs)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run a plain device copy of one vector per step (HBM counter calibration)")
     return ap.parse_args()
@@ -60,6 +67,16 @@ def time_hpsi(eng, x, y, steps, warmup, copy_calib=False):
             y.copy_(x)
         torch.cuda.synchronize()
     return ms, wall
+
+
+# HBM bytes per H.psi (all passes) measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE on this workload, FETCH scaled by 2.0 and WRITE by 1.0 (the
+# factors that make a plain one-vector read / write read its own size); the
+# summary with the commands is profiles/hpsi_n24_hbm_traffic.md
+MEASURED_TRAFFIC = {
+    ("c128", 281116): 2716265472,   # 933039 KiB fetched x 2.0 + 786525 KiB written
+    ("c64", 381219): 1656249344,    # 611918 KiB fetched x 2.0 + 393595 KiB written
+}
 
 
 def cpu_baseline(reps):
@@ -93,20 +110,28 @@ def main():
 
     lat = esc.Honeycomb(LX, LY)
     model = esc.SpinModel(lat, K=(K, K, K), J=J, Gamma=G, B=esc.field111(H111))
-    out = {}
     res = {}
+    spec = {}
     for prec in ("c128", "c64"):
         eng = esc.Engine(model, prec)
         x = eng.start_vector(1234, "Gaussian")
         y = eng.empty()
-        tile_bits, _ = eng.autotune(x, y, reps=5)
+        if args.kernel_config:
+            config = int(args.kernel_config.split(",")[0 if prec == "c128" else 1])
+            eng.set_tile_bits(config)
+        else:
+            config, _ = eng.autotune(x, y, reps=5)
         ms, wall = time_hpsi(eng, x, y, args.steps, args.warmup, args.copy_calib and prec == "c128")
         bytes_alg = 2 * eng.dim * eng.elem_bytes
+        kname = f"k_hpsi_tile<{'float' if prec == 'c64' else 'double'}, TB, WPB>" if config else \
+            f"k_hpsi<{'float' if prec == 'c64' else 'double'}>"
         res[prec] = {"ms": ms, "wall": wall, "bytes": bytes_alg, "gbs": bytes_alg / (ms * 1e-3) / 1e9,
-                     "hpsi_per_s": 1e3 / ms, "tile_bits": tile_bits,
-                     "kernel": f"k_hpsi_tile<{'float' if prec == 'c64' else 'double'}, {tile_bits}>"
-                               if tile_bits else f"k_hpsi<{'float' if prec == 'c64' else 'double'}>"}
-        del x, y, eng
+                     "hpsi_per_s": 1e3 / ms, "config": config, "kernel": kname,
+                     "traffic": MEASURED_TRAFFIC.get((prec, config))}
+        del x, y
+        if not args.no_spectrum:
+            spec[prec] = spectrum_run(eng)
+        del eng
         torch.cuda.empty_cache()
 
     r = res["c128"]
@@ -124,16 +149,19 @@ def main():
         "dtype": "c128 (fp64 complex)",
         "data": "synthetic (seeded complex-Gaussian vector)",
         "config": {"workload": "H.psi, 24-site honeycomb 3x4 cells periodic, K=-1 J=0.1 Gamma=0.2 h[111]=0.2, "
-                               "full 2^24 basis", "n_sites": 24, "dim": 1 << 24, "precision": "c128"},
+                               "full 2^24 basis", "n_sites": 24, "dim": 1 << 24, "precision": "c128",
+                   "kernel_config": r["config"]},
         "roofline": {"bound": "hbm", "achieved": r["gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": r["gbs"] / PEAK_HBM_GBS, "traffic": None,
-                     "kernel": r["kernel"], "bytes_per_launch": r["bytes"]},
+                     "frac": r["gbs"] / PEAK_HBM_GBS, "traffic": r["traffic"],
+                     "kernel": r["kernel"], "bytes_per_launch": r["bytes"],
+                     "launch": "one H.psi = the passes of its kernel configuration, timed together"},
         "c64": {"hpsi_per_s": res["c64"]["hpsi_per_s"], "ms_per_hpsi": res["c64"]["ms"],
                 "achieved_GBs": res["c64"]["gbs"], "frac": res["c64"]["gbs"] / PEAK_HBM_GBS,
-                "bytes_per_launch": res["c64"]["bytes"], "kernel": res["c64"]["kernel"]},
+                "bytes_per_launch": res["c64"]["bytes"], "kernel": res["c64"]["kernel"],
+                "kernel_config": res["c64"]["config"], "traffic": res["c64"]["traffic"]},
     }
-    if args.spectrum:
-        out["spectrum"] = spectrum_timing(esc, model)
+    if spec:
+        out["spectrum"] = spectrum_summary(spec)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_reps)
     else:
@@ -141,28 +169,28 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def spectrum_timing(esc, model):
-    """E0 (200 steps, plain recursion) + JeJe CF spectrum (200 steps per run),
-    eta=0.05, omega in [0,6] at 2000 points, c128 and c64."""
+def spectrum_run(eng):
+    """Ground state (200 steps, GS mode) + JeJe CF spectrum (200 steps per
+    run), eta = 0.05, omega in [0, 6] at 2000 points, timed end to end."""
     import torch
     w = np.linspace(0.0, 6.0, 2000)
-    outs = {}
-    for prec in ("c128", "c64"):
-        eng = esc.Engine(model, prec)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        gs = eng.empty()
-        res = eng.lanczos(eng.start_vector(7, "Gaussian"), 200, 0.0, "GS", gs)
-        tab, _ = eng.dynamics("JeJe", "CF", gs, w, 0.05, e0=res.evals[0], cf_steps=200)
-        torch.cuda.synchronize()
-        outs[prec] = {"seconds": time.perf_counter() - t0, "E0": float(res.evals[0]), "table": tab}
-        del eng, gs
-        torch.cuda.empty_cache()
-    a, b = outs["c128"]["table"][:, 1:], outs["c64"]["table"][:, 1:]
-    return {"jeje_seconds_c128": outs["c128"]["seconds"], "jeje_seconds_c64": outs["c64"]["seconds"],
-            "E0_c128": outs["c128"]["E0"], "E0_c64": outs["c64"]["E0"],
-            "E0_rel_err_c64": abs(outs["c64"]["E0"] - outs["c128"]["E0"]) / abs(outs["c128"]["E0"]),
-            "spectral_weight_rel_err_c64": float(np.abs(a - b).sum() / np.abs(a).sum())}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gs = eng.empty()
+    lr = eng.lanczos(eng.start_vector(7, "Gaussian"), 200, 0.0, "GS", gs)
+    tab, _ = eng.dynamics("JeJe", "CF", gs, w, 0.05, e0=lr.evals[0], cf_steps=200)
+    torch.cuda.synchronize()
+    return {"seconds": time.perf_counter() - t0, "E0": float(lr.evals[0]), "table": tab}
+
+
+def spectrum_summary(spec):
+    a, b = spec["c128"]["table"][:, 1:], spec["c64"]["table"][:, 1:]
+    e128, e64 = spec["c128"]["E0"], spec["c64"]["E0"]
+    return {"jeje_seconds_c128": spec["c128"]["seconds"], "jeje_seconds_c64": spec["c64"]["seconds"],
+            "E0_c128": e128, "E0_c64": e64, "E0_rel_err_c64": abs(e64 - e128) / abs(e128),
+            "spectral_weight_rel_err_c64": float(np.abs(a - b).sum() / np.abs(a).sum()),
+            "workload": "GS 200 Lanczos steps (reorthogonalised) + JeJe xx, yy, xy, yx, tt by continued "
+                        "fractions, 200 steps each, eta=0.05, omega=linspace(0,6,2000)"}
 
 
 if __name__ == "__main__":

@@ -174,6 +174,7 @@ struct esc_ctx {
         int n_uniform, n_terms;
     };
     std::map<int, std::vector<PassDev>> d_pass;     // windowed tiled kernel per configuration
+    mutable std::map<int, bool> tile_fits;           // configuration fits the LDS
     int tile_bits = 0;                              // kernel configuration: 0 = one row per lane,
                                                     // else 100 * tile bits + window bits
 
@@ -342,43 +343,135 @@ std::vector<HostTerm> host_terms(const esc_ctx *c) {
     return out;
 }
 
-// Passes of the windowed tiled kernel for tile bits TB and window width w
-// (w >= n_sites: one pass over one window = the whole vector).
-std::vector<esc::GTables> build_passes(const esc_ctx *c, int TB, int w) {
+// Tile sites of one pass: the six lane sites plus TB - 6 sites of the window
+// W, added one at a time so that as many of the given masks as possible lie
+// inside the tile (ties: the site most masks touch, then the lowest site).
+uint32_t greedy_tile(int N, uint32_t W, const std::vector<uint32_t> &masks, int TB) {
+    uint32_t T = 63u;
+    while (__builtin_popcount(T) < TB) {
+        int best = -1;
+        long best_key = -1;
+        for (int s = 6; s < N; ++s) {
+            if (!((W >> s) & 1u) || ((T >> s) & 1u)) continue;
+            const uint32_t nt = T | (1u << s);
+            long inside = 0, touch = 0;
+            for (uint32_t m : masks) {
+                inside += (m & ~nt) == 0;
+                touch += (m >> s) & 1u;
+            }
+            const long key = inside * 1024 + touch;
+            if (key > best_key) { best_key = key; best = s; }
+        }
+        if (best < 0) break;
+        T |= 1u << best;
+    }
+    return T;
+}
+
+// A window of w sites holding the tile T and as many of the masks as fit
+// (highest masks first), filled up with the lowest free sites.
+uint32_t grow_window(int N, uint32_t T, const std::vector<uint32_t> &masks, int w) {
+    uint32_t W = T;
+    std::vector<uint32_t> order(masks);
+    std::sort(order.begin(), order.end(), [](uint32_t a, uint32_t b) { return a > b; });
+    for (uint32_t m : order)
+        if (__builtin_popcount(W | m) <= w) W |= m;
+    for (int b = 0; b < N && __builtin_popcount(W) < w; ++b) W |= 1u << b;
+    return W;
+}
+
+// Passes of the windowed tiled kernel for tile bits TB, window width w and at
+// most P planned passes (w >= n_sites: one pass over one window = the whole
+// vector).  Planning: pass 1 takes the low w sites as its window, later
+// passes are centred on the terms not yet covered; each pass picks the tile
+// sites that make the most of its terms "near" (partner inside the LDS tile).
+// Terms that are near in no planned pass are applied as "far" terms (partner
+// read from the L2-resident window) by a pass whose window holds them; terms
+// that fit no window open further passes.
+std::vector<esc::GTables> build_passes(const esc_ctx *c, int TB, int w, int P, int WPB) {
     const int N = c->n_sites;
     if (w > N) w = N;
     if (TB > w || TB < 7 || N - w > 16 || w - TB > 16) throw EscError(ESC_ERR_UNSUPPORTED, "tile/window shape");
+    if (TB - 6 > 7 || WPB < 1 || ((1 << (TB - 6)) % WPB) != 0) throw EscError(ESC_ERR_UNSUPPORTED, "tile/wave shape");
     const esc::HTables &h = c->htab;
-    std::vector<HostTerm> terms = host_terms(c);
-    std::vector<bool> done(terms.size(), false);
-    std::vector<esc::GTables> passes;
-    size_t left = terms.size();
-    bool first = true;
-    while (first || left > 0) {
-        // window bits: bits 0..5 always; the first pass takes the low w bits,
-        // later ones grow greedily around the highest remaining masks
-        uint32_t W = 0;
-        if (first) {
-            W = (w >= 32) ? 0xffffffffu : ((1u << w) - 1u);
+    const std::vector<HostTerm> terms = host_terms(c);
+    const uint32_t all = (N >= 32) ? 0xffffffffu : ((1u << N) - 1u);
+    struct Plan {
+        uint32_t W, T;
+        std::vector<size_t> near, far;
+    };
+    std::vector<Plan> plan;
+    std::vector<size_t> rem(terms.size());
+    for (size_t t = 0; t < terms.size(); ++t) rem[t] = t;
+    auto masks_of = [&](const std::vector<size_t> &ids) {
+        std::vector<uint32_t> m;
+        for (size_t t : ids) m.push_back(terms[t].mask);
+        return m;
+    };
+    auto take_near = [&](Plan &p) {
+        std::vector<size_t> keep;
+        for (size_t t : rem) ((terms[t].mask & ~p.T) == 0 ? p.near : keep).push_back(t);
+        rem.swap(keep);
+    };
+    if (w >= N) P = 1;
+    while (plan.empty() || (!rem.empty() && (int)plan.size() < P)) {
+        Plan p;
+        const std::vector<uint32_t> rm = masks_of(rem);
+        if (plan.empty()) {
+            p.W = (w >= N) ? all : ((1u << w) - 1u);
         } else {
-            W = 63u;
-            std::vector<size_t> order;
-            for (size_t t = 0; t < terms.size(); ++t)
-                if (!done[t]) order.push_back(t);
-            std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b2) { return terms[a].mask > terms[b2].mask; });
-            for (size_t t : order) {
-                const uint32_t nw = W | terms[t].mask;
-                if (__builtin_popcount(nw) <= w) W = nw;
-            }
-            for (int b = 6; b < N && __builtin_popcount(W) < w; ++b) W |= 1u << b;   // fill up
+            p.W = grow_window(N, greedy_tile(N, all, rm, TB), rm, w);
         }
-        std::vector<int> wb, cb;   // window bits ascending, complement bits ascending
-        for (int b = 0; b < N; ++b) ((W >> b) & 1u ? wb : cb).push_back(b);
+        p.T = greedy_tile(N, p.W, rm, TB);
+        take_near(p);
+        plan.push_back(std::move(p));
+    }
+    {   // far terms: the window with the fewest far terms so far
+        std::vector<size_t> keep;
+        for (size_t t : rem) {
+            Plan *best = nullptr;
+            for (auto &p : plan)
+                if ((terms[t].mask & ~p.W) == 0 && (!best || p.far.size() < best->far.size())) best = &p;
+            if (best) best->far.push_back(t);
+            else keep.push_back(t);
+        }
+        rem.swap(keep);
+    }
+    while (!rem.empty()) {   // leftovers: further passes
+        Plan p;
+        const std::vector<uint32_t> rm = masks_of(rem);
+        p.W = grow_window(N, 63u, rm, w);
+        p.T = greedy_tile(N, p.W, rm, TB);
+        take_near(p);
+        std::vector<size_t> keep;
+        for (size_t t : rem) ((terms[t].mask & ~p.W) == 0 ? p.far : keep).push_back(t);
+        if (p.near.empty() && p.far.empty()) throw EscError(ESC_ERR_UNSUPPORTED, "window construction made no progress");
+        rem.swap(keep);
+        plan.push_back(std::move(p));
+        if (plan.size() > 8) throw EscError(ESC_ERR_UNSUPPORTED, "too many window passes");
+    }
+
+    std::vector<esc::GTables> passes;
+    bool first = true;
+    for (const Plan &pl : plan) {
+        // window order: lane sites, the other tile sites, then the rest of
+        // the window; the complement (window id) ascending
+        std::vector<int> wb, cb;
+        for (int b = 0; b < 6; ++b) wb.push_back(b);
+        for (int b = 6; b < N; ++b)
+            if ((pl.T >> b) & 1u) wb.push_back(b);
+        for (int b = 6; b < N; ++b)
+            if (((pl.W & ~pl.T) >> b) & 1u) wb.push_back(b);
+        for (int b = 0; b < N; ++b)
+            if (!((pl.W >> b) & 1u)) cb.push_back(b);
         const int ww = (int)wb.size();
+        std::vector<size_t> mine(pl.near);
+        mine.insert(mine.end(), pl.far.begin(), pl.far.end());
         esc::GTables g;
         std::memset(&g, 0, sizeof(g));
         g.n_sites = (uint32_t)N;
         g.tile_bits = (uint32_t)TB;
+        g.waves = (uint32_t)WPB;
         g.first_pass = first ? 1u : 0u;
         g.n_windows = 1u << (N - ww);
         g.blocks_per_window = 1u << (ww - TB);
@@ -396,31 +489,62 @@ std::vector<esc::GTables> build_passes(const esc_ctx *c, int TB, int w) {
             g.wscat[0][v] = scat(cb, 0, v);
             g.wscat[1][v] = scat(cb, 8, v);
         }
+        // tile-group sites: group index bit q sits on site wb[6 + q]
+        const int ngb = TB - 6, ngrp = 1 << ngb, ng = ngrp / WPB;
+        uint32_t gsites = 0;
+        int gidx[32];
+        for (int b = 0; b < 32; ++b) gidx[b] = -1;
+        for (int q = 0; q < ngb; ++q) {
+            gsites |= 1u << wb[6 + q];
+            gidx[wb[6 + q]] = q;
+        }
         std::vector<esc::GTerm> cls[esc::kNumClasses];
-        for (size_t t = 0; t < terms.size(); ++t) {
-            if (done[t] || (terms[t].mask & ~W) != 0) continue;
-            done[t] = true;
-            --left;
+        for (size_t t : mine) {
             const HostTerm &ht = terms[t];
             esc::GTerm gt;
             std::memset(&gt, 0, sizeof(gt));
             gt.mask = ht.mask;
+            gt.mlo = ht.mask & 63u;
             gt.bits = (uint32_t)ht.b0 | ((uint32_t)ht.b1 << 8) | ((uint32_t)ht.b2 << 16);
-            for (int q = 0; q < ww; ++q)
-                if ((ht.mask >> wb[q]) & 1u) gt.local |= 1u << q;
             std::memcpy(gt.coef, ht.coef, sizeof(gt.coef));
-            const bool far = (gt.local >> TB) != 0;
-            const bool lane = ht.b0 < 6 || ht.b1 < 6 || ht.b2 < 6;
+            const bool far = (ht.mask & ~(63u | gsites)) != 0;
+            gt.mhib = ht.mask & ~(63u | gsites);
+            uint32_t mg = 0;     // mask on the group index
+            for (int q = 0; q < ngb; ++q)
+                if ((ht.mask >> wb[6 + q]) & 1u) mg |= 1u << q;
+            const int sel[3] = {ht.b0, ht.b1, ht.b2};
+            bool lane = false;
+            for (int q = 0; q < 3; ++q) {
+                if (sel[q] < 6) {
+                    gt.lp_pos[q] = (uint32_t)sel[q];
+                    gt.lp_w[q] = 1;
+                    lane = true;
+                } else if (gidx[sel[q]] < 0) {
+                    gt.bsel[q] = (uint32_t)sel[q] | (1u << 16);
+                }
+            }
+            for (int wv = 0; wv < WPB; ++wv) {
+                for (int k = 0; k < ng; ++k) {
+                    const uint32_t gg = (uint32_t)(wv + WPB * k), slot = (uint32_t)(wv * ng + k);
+                    gt.gofs[slot] = far ? g.tscat[gg ^ mg] : ((gg ^ mg) << 6);
+                    uint32_t up = 0;
+                    for (int q = 0; q < 3; ++q)
+                        if (sel[q] >= 6 && gidx[sel[q]] >= 0) up |= ((gg >> gidx[sel[q]]) & 1u) << q;
+                    gt.ugrp[slot] = up * (c->c64() ? 8u : 16u);   // bytes into the table
+                }
+            }
             cls[(ht.cplx ? 1 : 0) | (far ? 2 : 0) | (lane ? 4 : 0)].push_back(gt);
         }
         size_t n = 0;
         for (int q = 0; q < esc::kNumClasses; ++q) {
             g.cls_begin[q] = (uint32_t)n;
             if (n + cls[q].size() > (size_t)esc::kMaxTerms) throw EscError(ESC_ERR_UNSUPPORTED, "too many terms");
-            for (auto &t : cls[q]) g.term[n++] = t;
+            for (auto &t : cls[q]) {
+                if (q >= 4) t.ct_base = (uint32_t)(n - g.cls_begin[4]) * 8u;
+                g.term[n++] = t;
+            }
         }
         g.cls_begin[esc::kNumClasses] = (uint32_t)n;
-        if (!first && n == 0) throw EscError(ESC_ERR_UNSUPPORTED, "window construction made no progress");
         // diagonal (used by the first pass only)
         g.kz4 = h.kz4;
         g.j4 = h.j4;
@@ -450,7 +574,6 @@ std::vector<esc::GTables> build_passes(const esc_ctx *c, int TB, int w) {
         groups(h.jdiag, h.n_jdiag, g.jgrp_d, g.jgrp_m, g.n_jgrp);
         passes.push_back(g);
         first = false;
-        if (passes.size() > 8) throw EscError(ESC_ERR_UNSUPPORTED, "too many window passes");
     }
     return passes;
 }
@@ -597,7 +720,7 @@ double device_norm2(esc_ctx *c, const void *x, hipStream_t s) {
 const std::vector<esc_ctx::PassDev> &passes_for(esc_ctx *c, int config) {
     auto it = c->d_pass.find(config);
     if (it != c->d_pass.end()) return it->second;
-    const std::vector<esc::GTables> host = build_passes(c, (config / 100) % 100, config % 100);
+    const std::vector<esc::GTables> host = build_passes(c, (config / 100) % 100, config % 100, config / 100000, (config / 10000) % 10);
     std::vector<esc_ctx::PassDev> dev;
     for (const auto &g : host) {
         esc::GTables *d = nullptr;
@@ -611,37 +734,63 @@ const std::vector<esc_ctx::PassDev> &passes_for(esc_ctx *c, int config) {
     return c->d_pass[config] = dev;
 }
 
-// kernel configurations: 10000 * waves per workgroup + 100 * tile bits +
-// window bits (window >= N: one pass); 0 = one row per lane
+// whether every pass of a configuration fits the LDS of one CU (cached)
+bool tile_config_fits(const esc_ctx *c, int config) {
+    auto it = c->tile_fits.find(config);
+    if (it != c->tile_fits.end()) return it->second;
+    bool ok = true;
+    try {
+        for (const auto &g : build_passes(c, (config / 100) % 100, config % 100, config / 100000, (config / 10000) % 10))
+            if (esc::tile_lds_bytes(c->c64(), (int)g.tile_bits, (int)g.cls_begin[4],
+                                    (int)g.cls_begin[esc::kNumClasses]) > esc::kMaxTileLds)
+                ok = false;
+    } catch (const EscError &) {
+        ok = false;
+    }
+    c->tile_fits[config] = ok;
+    return ok;
+}
+
+// kernel configurations: 100000 * planned passes + 10000 * waves per
+// workgroup + 100 * tile bits + window bits (window >= N: one pass);
+// 0 = one row per lane
 std::vector<int> tile_candidates(const esc_ctx *c) {
     std::vector<int> v;
     const int N = c->n_sites;
-    const std::vector<int> windows = c->c64() ? std::vector<int>{N, 18, 19} : std::vector<int>{N, 17, 18};
-    for (int waves : {4, 8}) {
-        for (int tb : {10, 11, 12}) {
+    const std::vector<int> windows = c->c64() ? std::vector<int>{N, 17, 18, 19} : std::vector<int>{N, 16, 17, 18};
+    for (int waves : {4, 8, 16}) {
+        for (int tb : {10, 11, 12, 13}) {
             if (N < tb || !esc::tile_shape_supported(c->c64(), tb, waves)) continue;
             for (int w : windows) {
                 const int ww = w > N ? N : w;
                 if (ww < tb || N - ww > 16 || ww - tb > 16) continue;
-                const int cfg = 10000 * waves + 100 * tb + ww;
-                if (std::find(v.begin(), v.end(), cfg) == v.end()) v.push_back(cfg);
+                for (int P : {1, 2, 3}) {
+                    if ((ww == N) != (P == 1)) continue;
+                    const int cfg = 100000 * P + 10000 * waves + 100 * tb + ww;
+                    if (std::find(v.begin(), v.end(), cfg) == v.end() && tile_config_fits(c, cfg)) v.push_back(cfg);
+                }
             }
         }
     }
     return v;
 }
 
-// before tuning: 4 waves, 2^11-row tiles, windows of 2^18 rows (the fastest
-// shape measured at N = 24); systems below 2^11 rows use one row per lane
+// before tuning: the shapes measured fastest at N = 24 (c128: 8 waves,
+// 2^11-row tiles, 2^16-row windows, two planned passes; c64: 8 waves, 2^12-row
+// tiles, 2^19-row windows, three planned passes), else a single pass over
+// 2^11-row tiles; systems below 2^11 rows use one row per lane
 int default_tile_bits(const esc_ctx *c) {
     const int N = c->n_sites;
     if (N < 11) return 0;
-    return 40000 + 100 * 11 + (N < 18 ? N : 18);
+    const std::vector<int> cand = tile_candidates(c);
+    for (int cfg : {c->c64() ? 381219 : 281116, 100000 + 80000 + 100 * 11 + N, 100000 + 40000 + 100 * 10 + N})
+        if (std::find(cand.begin(), cand.end(), cfg) != cand.end()) return cfg;
+    return 0;
 }
 
 void launch_hpsi_with(esc_ctx *c, int config, const void *in, void *out, hipStream_t s) {
     if (config > 0) {
-        const int waves = config / 10000, tb = (config / 100) % 100;
+        const int waves = (config / 10000) % 10, tb = (config / 100) % 100;
         for (const auto &p : passes_for(c, config))
             hip_check(esc::launch_hpsi_tile(c->c64(), p.d, c->d_htab, tb, waves, p.n_uniform, p.n_terms, in, out,
                                             c->dim, s),

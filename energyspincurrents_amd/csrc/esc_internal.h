@@ -41,16 +41,35 @@ struct HTables {
 //   cplx  some table entry has a non-zero imaginary part
 //   far   mask has bits at or above the tile bits (partner outside the LDS tile)
 //   lane  a selector bit lies below bit 6 (coefficient differs across a wave)
-constexpr int kMaxTerms = 160;
+constexpr int kMaxTerms = 128;
 constexpr int kMaxDistGroups = 32;
 constexpr int kNumClasses = 8;
 
+constexpr int kMaxGroups = 128;    // row groups of 64 per tile (tiles up to 2^13 rows)
+
+// One off-diagonal term of a pass: row r gathers coef[sel(r)] * psi[r ^ mask]
+// where sel(r) collects the row bits at the selector sites b0, b1, b2
+// (selector bit j = bit b_j of r).  The selector bits are split by where
+// they live: lane sites (per lane, lp_pos/lp_w as bit-field extracts),
+// tile-group sites (per row group, ugrp) and the remaining sites of the row
+// (per tile, bsel as s_bfe operands offset | width << 16 on the tile base).
 struct GTerm {
     uint32_t mask;          // partner offset in row space
+    uint32_t mlo;           // mask & 63 (lane part)
+    uint32_t mhib;          // mask bits outside the lane and tile-group sites
+    uint32_t ct_base;       // lane-dependent terms: first entry of the table in LDS
+    uint32_t lp_pos[3];     // lane selector bit positions (width 0: not a lane site)
+    uint32_t lp_w[3];
+    uint32_t bsel[3];       // base selector extracts
     uint32_t bits;          // b0 | b1 << 8 | b2 << 16
-    uint32_t local;         // mask in window coordinates
-    uint32_t pad;
+    uint32_t pad[2];
     double coef[8][2];
+    // per row-group slot j = wave * (groups per wave) + k of the workgroup:
+    // near terms: tile element index of the partner group's first row;
+    // far terms: partner row bits on the tile-group sites
+    uint32_t gofs[kMaxGroups];
+    uint32_t ugrp[kMaxGroups];  // selector bits taken from the tile-group sites, as a byte
+                                // offset into the term's coefficient table
 };
 
 // One pass of the windowed tiled H.psi.  The pass owns a window: an ordered
@@ -62,7 +81,7 @@ struct GTerm {
 // whose mask lies inside the window are applied by this pass; the first pass
 // also adds the diagonal and overwrites the output, later passes accumulate.
 struct GTables {
-    uint32_t n_sites, tile_bits;
+    uint32_t n_sites, tile_bits, waves;
     uint32_t first_pass;            // 1: acc = diag * psi, else acc = out
     uint32_t n_windows, blocks_per_window, xcd_windows;
     uint32_t tscat[128];
@@ -110,9 +129,13 @@ void clear_error();
 // LDS size).
 hipError_t launch_hpsi_tile(bool c64, const GTables *d_g, const HTables *d_h, int tile_bits, int waves,
                             int n_uniform, int n_terms, const void *in, void *out, uint64_t dim, hipStream_t s);
-// tile shapes the tiled kernel is compiled for: 2^10..2^12 rows, 4 or 8 waves
-// per workgroup, 2 to 8 row groups of 64 per wave
+// tile shapes the tiled kernel is compiled for: 2^10..2^12 rows (2^13 for
+// c64), 4, 8 or 16 waves per workgroup, 2 to 8 row groups of 64 per wave
 bool tile_shape_supported(bool c64, int tile_bits, int waves);
+// dynamic LDS of one tiled workgroup: the tile, the uniform coefficients per
+// row group, the lane-dependent tables and their per-lane offsets
+size_t tile_lds_bytes(bool c64, int tile_bits, int n_uniform, int n_terms);
+constexpr size_t kMaxTileLds = 160 * 1024 - 512;   // 160 KiB per CU minus the static row-group table
 hipError_t launch_hpsi_c128(const HTables *d_tab, const void *in, void *out, uint64_t dim, hipStream_t s);
 hipError_t launch_hpsi_c64(const HTables *d_tab, const void *in, void *out, uint64_t dim, hipStream_t s);
 hipError_t launch_current_c128(const CurrentTable *d_tab, const void *in, void *out, uint64_t dim, hipStream_t s);

@@ -160,58 +160,49 @@ __device__ __forceinline__ double hpsi_diag(const GTables *__restrict__ G, const
 }
 
 // Terms of one class for all NG row groups of a wave.  Row group k of wave w
-// is g = w + 4k, rows r = base | g << 6 | lane.
+// is g = w + WPB * k, rows r = base | tscat[g] | lane; its per-term constants
+// sit in slot w * NG + k of the term record.
 template <typename R, int TB, int WPB, int CLS>
 __device__ __forceinline__ void hpsi_class(const GTables *__restrict__ G, uint32_t base, uint32_t wave,
                                            uint32_t lane, const typename Vec2<R>::T *__restrict__ in,
                                            const typename Vec2<R>::T *s_in, const typename Vec2<R>::T *s_cu,
-                                           const typename Vec2<R>::T *s_ct, const uint32_t *s_rhi, R *ar, R *ai) {
+                                           const typename Vec2<R>::T *s_ct, const uint32_t *s_lp, R *ar, R *ai) {
     using V2 = typename Vec2<R>::T;
     constexpr int NGRP = 1 << (TB - 6), NG = NGRP / WPB;
-    constexpr uint32_t TMASK = (1u << TB) - 1u;
     constexpr bool CPLX = (CLS & 1) != 0, FAR = (CLS & 2) != 0, LANE = (CLS & 4) != 0;
     const uint32_t t0 = G->cls_begin[CLS], t1 = G->cls_begin[CLS + 1];
     const uint32_t nU = G->cls_begin[4];
+    const uint32_t slot0 = wave * (uint32_t)NG;
     // two terms per iteration: their loads are in flight together
 #pragma unroll 2
     for (uint32_t t = t0; t < t1; ++t) {
-        const uint32_t m = G->term[t].mask;          // row-space mask
-        const uint32_t mlo = m & 63u, mhi = m & ~63u;
-        const uint32_t mloc = G->term[t].local;      // mask in window coordinates (near terms)
-        const uint32_t loff = lane ^ mlo;
-        uint32_t lp = 0, b0 = 0, b1 = 0, b2 = 0;
-        if (LANE) {
-            const uint32_t bits = G->term[t].bits;
-            b0 = bits & 0xffu; b1 = (bits >> 8) & 0xffu; b2 = bits >> 16;
-            // selector bits below 6 come from the lane, the rest per row group
-            lp = (b0 < 6 ? ((lane >> b0) & 1u) : 0u) | (b1 < 6 ? (((lane >> b1) & 1u) << 1) : 0u) |
-                 (b2 < 6 ? (((lane >> b2) & 1u) << 2) : 0u);
-        }
+        const GTerm &T = G->term[t];
+        // lane part of the partner row; far terms also fix the partner's
+        // sites outside the tile for the whole tile
+        const uint32_t loff = lane ^ T.mlo;
+        const V2 *src = FAR ? in + (size_t)(base ^ T.mhib) : s_in;
+        // lane-dependent coefficients: byte offset of this lane's entry in the
+        // term's table for row group 0 of the tile (set up per workgroup)
+        const uint32_t lpc = LANE ? s_lp[(t - nU) * 64u + lane] : 0u;
 #pragma unroll
         for (int k = 0; k < NG; ++k) {
-            const uint32_t g = wave + (uint32_t)WPB * (uint32_t)k;
-            const uint32_t r_hi = s_rhi[g];
-            V2 x;
-            if (FAR) {
-                const V2 *p = in + (size_t)(r_hi ^ mhi);
-                x = p[loff];
-            } else {
-                x = s_in[(((g << 6) ^ (mloc & ~63u)) & TMASK) | loff];
-            }
+            const uint32_t j = slot0 + (uint32_t)k;
+            const V2 x = src[T.gofs[j] + loff];
             V2 c;
             if (LANE) {
-                const uint32_t up = (b0 >= 6 ? ((r_hi >> b0) & 1u) : 0u) | (b1 >= 6 ? (((r_hi >> b1) & 1u) << 1) : 0u) |
-                                    (b2 >= 6 ? (((r_hi >> b2) & 1u) << 2) : 0u);
-                c = s_ct[(t - nU) * 8u + (lp | up)];
+                c = *reinterpret_cast<const V2 *>(reinterpret_cast<const unsigned char *>(s_ct) + lpc + T.ugrp[j]);
             } else {
-                c = s_cu[t * NGRP + g];
+                c = s_cu[t * NGRP + wave + (uint32_t)WPB * (uint32_t)k];
             }
+            // accumulate with fused multiply-adds (4 FMAs per complex term)
             if (CPLX) {
-                ar[k] += c.x * x.x - c.y * x.y;
-                ai[k] += c.x * x.y + c.y * x.x;
+                ar[k] = fma(c.x, x.x, ar[k]);
+                ar[k] = fma(-c.y, x.y, ar[k]);
+                ai[k] = fma(c.x, x.y, ai[k]);
+                ai[k] = fma(c.y, x.x, ai[k]);
             } else {
-                ar[k] += c.x * x.x;
-                ai[k] += c.x * x.y;
+                ar[k] = fma(c.x, x.x, ar[k]);
+                ai[k] = fma(c.x, x.y, ai[k]);
             }
         }
     }
@@ -231,6 +222,7 @@ __global__ __launch_bounds__(WPB * 64) void k_hpsi_tile(const GTables *__restric
     V2 *s_cu = s_in + NROW;                                     // uniform-term coefficients [nU][NGRP]
     const uint32_t nU = G->cls_begin[4], nT = G->cls_begin[kNumClasses];
     V2 *s_ct = s_cu + nU * NGRP;                                // lane-term tables [nT - nU][8]
+    uint32_t *s_lp = reinterpret_cast<uint32_t *>(s_ct + (nT - nU) * 8u);   // [nT - nU][64] byte offsets
     __shared__ uint32_t s_rhi[NGRP];                            // first row of each row group
     // block -> (window, tile within window); with xcd_windows every window
     // is walked by one XCD (workgroups are dealt round-robin over 8 XCDs)
@@ -266,6 +258,20 @@ __global__ __launch_bounds__(WPB * 64) void k_hpsi_tile(const GTables *__restric
         c.y = (R)G->term[nU + e / 8u].coef[e % 8u][1];
         s_ct[e] = c;
     }
+    // lane-dependent terms: table entry of each lane for row group 0, from the
+    // selector bits on lane sites (per lane) and on tile-base sites (per tile);
+    // the row group adds its own selector bits (ugrp, in bytes)
+    for (uint32_t e = threadIdx.x; e < (nT - nU) * 64u; e += NT) {
+        const GTerm &T = G->term[nU + e / 64u];
+        const uint32_t ln = e % 64u;
+        uint32_t sel = 0;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            sel |= ((ln >> T.lp_pos[q]) & T.lp_w[q]) << q;
+            sel |= ((base >> (T.bsel[q] & 0xffffu)) & (T.bsel[q] >> 16)) << q;
+        }
+        s_lp[e] = (T.ct_base + sel) * (uint32_t)sizeof(V2);
+    }
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -287,14 +293,14 @@ __global__ __launch_bounds__(WPB * 64) void k_hpsi_tile(const GTables *__restric
             ai[k] = o.y;
         }
     }
-    hpsi_class<R, TB, WPB, 0>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_rhi, ar, ai);
-    hpsi_class<R, TB, WPB, 1>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_rhi, ar, ai);
-    hpsi_class<R, TB, WPB, 2>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_rhi, ar, ai);
-    hpsi_class<R, TB, WPB, 3>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_rhi, ar, ai);
-    hpsi_class<R, TB, WPB, 4>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_rhi, ar, ai);
-    hpsi_class<R, TB, WPB, 5>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_rhi, ar, ai);
-    hpsi_class<R, TB, WPB, 6>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_rhi, ar, ai);
-    hpsi_class<R, TB, WPB, 7>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_rhi, ar, ai);
+    hpsi_class<R, TB, WPB, 0>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_lp, ar, ai);
+    hpsi_class<R, TB, WPB, 1>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_lp, ar, ai);
+    hpsi_class<R, TB, WPB, 2>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_lp, ar, ai);
+    hpsi_class<R, TB, WPB, 3>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_lp, ar, ai);
+    hpsi_class<R, TB, WPB, 4>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_lp, ar, ai);
+    hpsi_class<R, TB, WPB, 5>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_lp, ar, ai);
+    hpsi_class<R, TB, WPB, 6>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_lp, ar, ai);
+    hpsi_class<R, TB, WPB, 7>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_lp, ar, ai);
     // streamed result: a non-temporal store keeps it from displacing the
     // input rows that other tiles still gather from
     typedef R nvec2 __attribute__((ext_vector_type(2)));
@@ -661,13 +667,20 @@ static hipError_t launch_tile_tb(const GTables *g, const HTables *h, const void 
     return hipGetLastError();
 }
 
+size_t tile_lds_bytes(bool c64, int tile_bits, int n_uniform, int n_terms) {
+    const size_t esz = c64 ? 8 : 16;
+    const size_t ngrp = (size_t)1 << (tile_bits - 6);
+    return esz * (((size_t)1 << tile_bits) + (size_t)n_uniform * ngrp + (size_t)(n_terms - n_uniform) * 8) +
+           (size_t)(n_terms - n_uniform) * 64 * sizeof(uint32_t);
+}
+
 // each wave holds 2^(TB-6)/WPB row groups in registers; beyond 8 groups the
 // unrolled term loop spills
 bool tile_shape_supported(bool c64, int tile_bits, int waves) {
-    (void)c64;
-    if (waves != 4 && waves != 8) return false;
+    if (waves != 4 && waves != 8 && waves != 16) return false;
+    if (tile_bits < 10 || tile_bits > (c64 ? 13 : 12)) return false;
     const int groups = (1 << (tile_bits - 6)) / waves;
-    return tile_bits >= 10 && tile_bits <= 12 && groups >= 2 && groups <= 8;
+    return groups >= 2 && groups <= 8;
 }
 
 hipError_t launch_hpsi_tile(bool c64, const GTables *g, const HTables *h, int tile_bits, int waves, int n_uniform,
@@ -676,10 +689,8 @@ hipError_t launch_hpsi_tile(bool c64, const GTables *g, const HTables *h, int ti
     if (ntiles64 == 0 || ntiles64 > 0x7fffffffull || !tile_shape_supported(c64, tile_bits, waves))
         return hipErrorInvalidValue;
     const uint32_t ntiles = (uint32_t)ntiles64;
-    const size_t esz = c64 ? 8 : 16;
-    const size_t ngrp = (size_t)1 << (tile_bits - 6);
-    const size_t lds = esz * (((size_t)1 << tile_bits) + (size_t)n_uniform * ngrp + (size_t)(n_terms - n_uniform) * 8);
-    if (lds + 512 > 160 * 1024) return hipErrorInvalidValue;
+    const size_t lds = tile_lds_bytes(c64, tile_bits, n_uniform, n_terms);
+    if (lds > kMaxTileLds) return hipErrorInvalidValue;
 #define ESC_TILE_CASE(TBV, WV)                                                                          \
     if (tile_bits == TBV && waves == WV)                                                                \
         return c64 ? launch_tile_tb<float, TBV, WV>(g, h, in, out, ntiles, lds, s)                      \
@@ -689,6 +700,9 @@ hipError_t launch_hpsi_tile(bool c64, const GTables *g, const HTables *h, int ti
     ESC_TILE_CASE(10, 8)
     ESC_TILE_CASE(11, 8)
     ESC_TILE_CASE(12, 8)
+    ESC_TILE_CASE(11, 16)
+    ESC_TILE_CASE(12, 16)
+    if (tile_bits == 13 && waves == 16 && c64) return launch_tile_tb<float, 13, 16>(g, h, in, out, ntiles, lds, s);
 #undef ESC_TILE_CASE
     return hipErrorInvalidValue;
 }

@@ -174,11 +174,13 @@ class Engine:
         check(lib().esc_set_tile_bits(self._h, int(tile_bits)))
 
     def kernel_configs(self):
-        """H.psi kernel configurations: 0 (one row per lane) or 100 * tile bits + window bits."""
+        """H.psi kernel configurations: 0 (one row per lane) or
+        100000 * planned passes + 10000 * waves + 100 * tile bits + window bits."""
         n = ctypes.c_int32()
-        buf = (ctypes.c_int32 * 64)()
-        check(lib().esc_kernel_configs(self._h, buf, 64, ctypes.byref(n)))
-        return [buf[i] for i in range(min(n.value, 64))]
+        check(lib().esc_kernel_configs(self._h, None, 0, ctypes.byref(n)))
+        buf = (ctypes.c_int32 * max(n.value, 1))()
+        check(lib().esc_kernel_configs(self._h, buf, n.value, ctypes.byref(n)))
+        return [buf[i] for i in range(n.value)]
 
     def apply_op(self, op: str, site: int, x, out=None, stream=None):
         """Observables::ApplyOp / ApplyJ*Total (site = -1) (src/Observables.h:24-67,448-470,664-699)."""

@@ -131,9 +131,10 @@ def test_error_paths():
 @pytest.mark.parametrize("kw", [dict(K=(1.0, 1.0, 1.0), B=(0.3, 0.3, 0.3)),          # popcount diagonal
                                 dict(K=(-1.0, 0.7, 0.3), B=(0.1, -0.2, 0.3))])       # sequential diagonal
 def test_tiled_kernels_matrix_elements_bit_exact_n18(kw):
-    """Every H.psi kernel configuration (one row per lane, tiles of 2^11 and
-    2^12 rows, one pass or two passes over 2^17-row windows) gives the
-    oracle's matrix elements bit for bit at N=18."""
+    """Every H.psi kernel configuration (one row per lane; tiles of 2^10 to
+    2^12 rows with 4, 8 or 16 waves; one pass, or two and three planned passes
+    over 2^16- and 2^17-row windows) gives the oracle's matrix elements bit for
+    bit at N=18."""
     m = O.Model(3, 6, **kw)
     eng = engine_for(m)
     rng = np.random.default_rng(16)
