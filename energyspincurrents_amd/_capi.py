@@ -26,7 +26,7 @@ EXPORTS = [
     "esc_apply_op", "esc_start_vector", "esc_dot", "esc_axpby", "esc_upload", "esc_download", "esc_lanczos",
     "esc_ritz_overlaps", "esc_ritz_vector", "esc_lanczos_cf", "esc_dynamics", "esc_honeycomb",
     "esc_connectors", "esc_params_load", "esc_tridiag_eig", "esc_cf_spectrum", "esc_lehmann_spectrum",
-    "esc_autotune", "esc_set_tile_bits", "esc_kernel_configs",
+    "esc_autotune", "esc_set_tile_bits", "esc_kernel_configs", "esc_ground_state",
 ]
 
 
@@ -103,6 +103,7 @@ def lib():
         "esc_ritz_overlaps": ([P, P, I, P, P], I),
         "esc_ritz_vector": ([P, I, P, P], I),
         "esc_lanczos_cf": ([P, P, I, P, P, P, P, P], I),
+        "esc_ground_state": ([P, P, I, D, I, P, P, P, P, P], I),
         "esc_dynamics": ([P, I, I, P, D, I, D, P, I, P, P, P], I),
         "esc_honeycomb": ([I, I, I, I, P], I),
         "esc_connectors": ([I, P, P, P, P, P, P], I),

@@ -865,6 +865,59 @@ void cf_run(esc_ctx *ctx, const void *d_phi, int steps, void *work[3], double *a
     *iters = k;
 }
 
+// Lanczos without reorthogonalisation from the unit vector `start`, in the
+// context's vectors 0..2, with the exit rule of esc_lanczos (|dE0| < eps or
+// `steps` iterations; also when the Krylov space is exhausted).  Returns the
+// number n of alphas.  With coef != nullptr the same recursion is replayed
+// (deterministic kernels: the same vectors bit for bit) for exactly n_fixed
+// vectors, accumulating acc += coef[k] v_k.
+int plain_recursion(esc_ctx *c, const void *start, int steps, double eps, const double *coef, int n_fixed, void *acc,
+                    std::vector<double> &ar, std::vector<double> &bt, hipStream_t s) {
+    const bool c64 = c->c64();
+    const uint64_t dim = c->dim;
+    c->ensure_partials(2 * esc::kRedBlocks);
+    c->ensure_result(8);
+    void *vp = c->vec(0), *v = c->vec(1), *w = c->vec(2);
+    hip_check(hipMemcpyAsync(v, start, dim * c->esz, hipMemcpyDeviceToDevice, s), "copy start");
+    ar.clear();
+    bt.assign(1, 0.0);
+    double E0 = 1.0;
+    for (int k = 0;; ++k) {
+        if (coef) {
+            hip_check(esc::launch_axpby(c64, coef[k], 0.0, v, 1.0, 0.0, acc, dim, s), "accumulate");
+            if (k + 1 == n_fixed) return n_fixed;
+        }
+        run_hpsi(c, v, w, s);
+        double a[2];
+        hip_check(esc::launch_dot(c64, v, w, dim, c->d_partials, c->d_result, s), "dot");
+        sync_get(s, a, c->d_result, 2 * sizeof(double));
+        if (!std::isfinite(a[0]) || !std::isfinite(a[1])) throw EscError(ESC_ERR_NUMERIC, "non-finite alpha");
+        ar.push_back(a[0]);
+        hip_check(esc::launch_lanczos_update(c64, w, v, k > 0 ? vp : nullptr, c->d_result, bt[k], dim, c->d_partials,
+                                             c->d_result + 2, s), "update");
+        double n2;
+        sync_get(s, &n2, c->d_result + 2, sizeof(double));
+        const double b = std::sqrt(n2);
+        if (!std::isfinite(b)) throw EscError(ESC_ERR_NUMERIC, "non-finite beta");
+        bt.push_back(b);
+        const bool exhausted = b <= 1e-13 * std::max(1.0, std::fabs(a[0]));
+        if (!coef) {
+            const int n = k + 1;
+            bool done = exhausted || n == steps;
+            if (n >= 2) {
+                const double D0 = tridiag_lowest(n, ar.data(), bt.data());
+                if (std::fabs(E0 - D0) < eps) done = true;
+                E0 = D0;
+            }
+            if (done) return n;
+        } else if (exhausted) {
+            throw EscError(ESC_ERR_NUMERIC, "replayed recursion ended early");
+        }
+        hip_check(esc::launch_scale(c64, w, 1.0 / b, dim, s), "scale");
+        void *t = vp; vp = v; v = w; w = t;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -956,6 +1009,10 @@ int esc_start_vector(esc_ctx *ctx, uint64_t seed, int kind, void *d_out, void *s
             // mt19937_64(seed) -> uniform_real_distribution<double>(0,1) as
             // generate_canonical with one 64-bit draw: double(x) * 2^-64
             // (libc++ convention, no clamp); re drawn before im.
+            // Serial on the host as in the reference: bounded to 2^28 rows.
+            if (ctx->n_sites > 28)
+                throw EscError(ESC_ERR_UNSUPPORTED, "the reference (mt19937_64) start vector is limited to N <= 28; "
+                                                    "use the Gaussian start vector");
             const uint64_t dim = ctx->dim;
             std::vector<double> h(2 * dim);
             std::mt19937_64 g(seed);
@@ -1147,6 +1204,56 @@ int esc_lanczos(esc_ctx *ctx, const void *d_v0, int steps, double eps, int mode,
     });
 }
 
+int esc_ground_state(esc_ctx *ctx, const void *d_v0, int steps, double eps, int restarts, double *e0, double *resid,
+                     int *iters, void *d_gs, void *stream) {
+    return guarded([&] {
+        check_ctx(ctx);
+        check_ptr(d_v0, "d_v0");
+        check_ptr(d_gs, "d_gs");
+        if (d_v0 == d_gs) throw EscError(ESC_ERR_ARG, "d_v0 and d_gs must not overlap");
+        if (steps < 2 || steps > 4096) throw EscError(ESC_ERR_ARG, "steps must be in [2, 4096]");
+        if (restarts < 0 || restarts > 64) throw EscError(ESC_ERR_ARG, "restarts must be in [0, 64]");
+        hipStream_t s = (hipStream_t)stream;
+        const bool c64 = ctx->c64();
+        const uint64_t dim = ctx->dim;
+        ctx->ensure_vecs(4);
+        ctx->basis_kept = false;
+        void *start = ctx->vec(3);
+        hip_check(hipMemcpyAsync(start, d_v0, dim * ctx->esz, hipMemcpyDeviceToDevice, s), "copy v0");
+        const double n0 = device_norm2(ctx, start, s);
+        if (!(n0 > 0.0) || !std::isfinite(n0)) throw EscError(ESC_ERR_NUMERIC, "start vector has zero or non-finite norm");
+        hip_check(esc::launch_scale(c64, start, 1.0 / std::sqrt(n0), dim, s), "scale");
+        long total = 0;
+        for (int cyc = 0; cyc <= restarts; ++cyc) {
+            std::vector<double> ar, bt;
+            const int n = plain_recursion(ctx, start, steps, eps, nullptr, 0, nullptr, ar, bt, s);
+            total += n;
+            std::vector<double> ev(n), Z((size_t)n * n);
+            if (tridiag_ql(n, ar.data(), bt.data(), ev.data(), Z.data()) != 0)
+                throw EscError(ESC_ERR_NUMERIC, "tridiagonal QL did not converge");
+            std::vector<double> coef(Z.begin(), Z.begin() + n);   // column 0: the lowest Ritz pair
+            hip_check(hipMemsetAsync(d_gs, 0, dim * ctx->esz, s), "clear");
+            std::vector<double> ar2, bt2;
+            plain_recursion(ctx, start, steps, eps, coef.data(), n, d_gs, ar2, bt2, s);
+            total += n - 1;
+            const double ng = device_norm2(ctx, d_gs, s);
+            hip_check(esc::launch_scale(c64, d_gs, 1.0 / std::sqrt(ng), dim, s), "scale");
+            if (cyc < restarts) hip_check(hipMemcpyAsync(start, d_gs, dim * ctx->esz, hipMemcpyDeviceToDevice, s), "copy");
+        }
+        // Rayleigh quotient and residual |H gs - e gs| in FP64 reductions
+        void *w = ctx->vec(0);
+        run_hpsi(ctx, d_gs, w, s);
+        ++total;
+        double d[2];
+        device_dot(ctx, d_gs, w, d, s);
+        hip_check(esc::launch_axpby(c64, -d[0], 0.0, d_gs, 1.0, 0.0, w, dim, s), "residual");
+        const double r2 = device_norm2(ctx, w, s);
+        if (e0) *e0 = d[0];
+        if (resid) *resid = std::sqrt(r2);
+        if (iters) *iters = (int)total;
+    });
+}
+
 int esc_ritz_overlaps(esc_ctx *ctx, const void *d_vec, int n_states, double *out, void *stream) {
     return guarded([&] {
         check_ctx(ctx);
@@ -1222,7 +1329,12 @@ int esc_dynamics(esc_ctx *ctx, int kind, int method, const void *d_gs, double e0
                                         : std::vector<int>{ESC_OP_JEX, ESC_OP_JEY};
         const int nc = (int)ops.size();
         using cplx = std::complex<double>;
-        ctx->ensure_aux(method == ESC_SPEC_CF ? 4 : 1);
+        // continued fractions run in the context's Lanczos vectors unless those
+        // hold a kept basis (then in separate work vectors): at N = 32 the
+        // ground state, J|gs> and three work vectors are all that is resident
+        const bool cf_in_vecs = method == ESC_SPEC_CF && !ctx->basis_kept;
+        if (cf_in_vecs) ctx->ensure_vecs(3);
+        ctx->ensure_aux(method == ESC_SPEC_CF && !cf_in_vecs ? 4 : 1);
         ctx->ensure_partials(2 * esc::kRedBlocks);
         ctx->ensure_result(8);
         std::vector<esc::CurrentTable> tabs(nc);
@@ -1298,7 +1410,8 @@ int esc_dynamics(esc_ctx *ctx, int kind, int method, const void *d_gs, double e0
                 F[pr] = f;
             }
         } else {
-            void *work[3] = {ctx->aux(1), ctx->aux(2), ctx->aux(3)};
+            void *work[3] = {cf_in_vecs ? ctx->vec(0) : ctx->aux(1), cf_in_vecs ? ctx->vec(1) : ctx->aux(2),
+                             cf_in_vecs ? ctx->vec(2) : ctx->aux(3)};
             std::vector<double> al(cf_steps + 1), be(cf_steps + 2);
             auto cf = [&](const std::vector<cplx> &coef) {
                 apply_combo(coef);

@@ -223,6 +223,16 @@ class Engine:
         return LanczosResult(a[:n].copy(), b[: n + 1].copy(), n, ev[:n].copy(),
                              Z[: n * n].reshape(n, n).T.copy())
 
+    def ground_state(self, v0, steps=200, eps=0.0, restarts=0, out=None, stream=None):
+        """Ground state with 5 resident vectors (esc_ground_state): two-pass
+        Lanczos without reorthogonalisation, `restarts` restarts from the
+        previous state.  Returns (out, e0, residual norm, H.psi count)."""
+        out = self.empty() if out is None else out
+        e0, res, it = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        check(lib().esc_ground_state(self._h, _dev_ptr(v0), int(steps), float(eps), int(restarts), ctypes.byref(e0),
+                                     ctypes.byref(res), ctypes.byref(it), _dev_ptr(out), _stream(stream)))
+        return out, e0.value, res.value, it.value
+
     def ritz_overlaps(self, vec, n_states, stream=None) -> np.ndarray:
         out = np.zeros(2 * n_states)
         check(lib().esc_ritz_overlaps(self._h, _dev_ptr(vec), int(n_states), _np_ptr(out), _stream(stream)))

@@ -81,7 +81,7 @@ typedef struct {
 #define ESC_LANC_PLAIN   3   /* three-vector recursion, no basis kept (N=32 path) */
 
 /* Start vectors (src/DynLanczosSolver.h:58-59,465-474). */
-#define ESC_START_REF_UNIFORM  0  /* mt19937_64(seed), re/im = -1 + 2 U[0,1), drawn re then im */
+#define ESC_START_REF_UNIFORM  0  /* mt19937_64(seed), re/im = -1 + 2 U[0,1), drawn re then im (host, N <= 28) */
 #define ESC_START_GAUSSIAN     1  /* counter-based (Philox4x32-10) N(0,1/2) per component */
 
 /* ---------------------------------------------------------------- errors -- */
@@ -150,6 +150,20 @@ int esc_download(esc_ctx *ctx, const void *d_in, double *host, void *stream);
 int esc_lanczos(esc_ctx *ctx, const void *d_v0, int steps, double eps, int mode,
                 double *alpha, double *beta, int *iters, double *evals, double *Z,
                 void *d_gs, void *stream);
+
+/* Ground state with three resident Lanczos vectors plus the start and the
+ * result (the memory plan for N = 32: 5 vectors of 2^32 complex64 = 172 GB).
+ * Each cycle runs the recursion of esc_lanczos without reorthogonalisation
+ * from the normalised start (same exit rule), then replays it -- the kernels
+ * are deterministic, so the replay produces the same vectors -- accumulating
+ * |gs> = sum_m Z(m,0) v_m; later cycles restart from the previous |gs>
+ * (`restarts` extra cycles; each restart removes the drift that rounding in
+ * complex64 storage leaves in the Ritz vector).  e0 = <gs|H|gs>, resid =
+ * |H gs - e0 gs| (FP64 reductions), iters = H.psi applications in all.
+ * Replaces the ground-state use of Lanczos_Nirav("GS") where the basis
+ * cannot be stored (src/DynLanczosSolver.h:38-239). */
+int esc_ground_state(esc_ctx *ctx, const void *d_v0, int steps, double eps, int restarts, double *e0,
+                     double *resid, int *iters, void *d_gs, void *stream);
 
 /* O[n] = <psi_n|d_vec>, n < n_states, for the Ritz states of the last
  * ESC_LANC_ALL run, computed as sum_m conj(Z(m,n)) <v_m|d_vec> without forming

@@ -260,6 +260,73 @@ int orc_coo_apply(void *p, const double *in, double *out, int nthreads)
 /*             y bond (i,j): flip i gets G/4 s_j, flip j gets G/4 s_i.         */
 /* Families with one bond are skipped as in the COO form when skip1 != 0.      */
 /* ------------------------------------------------------------------------- */
+/* Value of the input at column c: direct indexing, or (keys != NULL) the
+ * entry of a sorted key list, for evaluating single rows of huge systems. */
+static inline const double *colval(const double *in, const int64_t *keys, int64_t nkeys, int64_t c)
+{
+    if (!keys) return in + 2 * c;
+    int64_t lo = 0, hi = nkeys - 1;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) / 2;
+        if (keys[mid] < c) lo = mid + 1; else hi = mid;
+    }
+    return in + 2 * lo;   /* the caller supplies every column a row needs */
+}
+
+static inline void hpsi_row(int N, const int *const *P, const int *nb, const double *K, double J, double G,
+                            const double *B, const double *in, const int64_t *keys, int64_t nkeys,
+                            int64_t r, double *out_re, double *out_im)
+{
+    double d = 0.0;
+    for (int b = 0; b < nb[2]; b++) {
+        int si = P[2][2 * b], sj = P[2][2 * b + 1];
+        d += (bitof(r, si) == bitof(r, sj)) ? 0.25 * K[2] : -0.25 * K[2];
+    }
+    if (J != 0.0)
+        for (int f = 0; f < 3; f++)
+            for (int b = 0; b < nb[f]; b++) {
+                int si = P[f][2 * b], sj = P[f][2 * b + 1];
+                d += (bitof(r, si) == bitof(r, sj)) ? 0.25 * J : -0.25 * J;
+            }
+    double szt = 0.5 * (double)(N - 2 * __builtin_popcountll((unsigned long long)r));
+    d += B[2] * szt;
+    const double *x = colval(in, keys, nkeys, r);
+    double re = d * x[0], im = d * x[1];
+    for (int f = 0; f < 3; f++)
+        for (int b = 0; b < nb[f]; b++) {
+            int si = P[f][2 * b], sj = P[f][2 * b + 1];
+            double s_i = bitof(r, si) ? -1.0 : 1.0, s_j = bitof(r, sj) ? -1.0 : 1.0;
+            double cr = 0.25 * J * (1.0 - s_i * s_j), ci = 0.0;
+            if (f == 0) cr += 0.25 * K[0];
+            if (f == 1) cr += -0.25 * K[1] * s_i * s_j;
+            if (f == 2) ci = -0.25 * G * (s_i + s_j);
+            const double *y = colval(in, keys, nkeys, r ^ (((int64_t)1 << si) | ((int64_t)1 << sj)));
+            re += cr * y[0] - ci * y[1];
+            im += cr * y[1] + ci * y[0];
+            if (G != 0.0 && f == 0) {            /* Gamma on x bond: single flips */
+                double gi = -0.25 * G * s_i * s_j;
+                const double *y1 = colval(in, keys, nkeys, r ^ ((int64_t)1 << si));
+                const double *y2 = colval(in, keys, nkeys, r ^ ((int64_t)1 << sj));
+                re += -gi * y1[1] - gi * y2[1];
+                im += gi * y1[0] + gi * y2[0];
+            }
+            if (G != 0.0 && f == 1) {            /* Gamma on y bond: single flips */
+                const double *y1 = colval(in, keys, nkeys, r ^ ((int64_t)1 << si));
+                const double *y2 = colval(in, keys, nkeys, r ^ ((int64_t)1 << sj));
+                re += 0.25 * G * s_j * y1[0] + 0.25 * G * s_i * y2[0];
+                im += 0.25 * G * s_j * y1[1] + 0.25 * G * s_i * y2[1];
+            }
+        }
+    for (int k = 0; k < N; k++) {
+        double sk = bitof(r, k) ? -1.0 : 1.0;
+        double cr = 0.5 * B[0], ci = -0.5 * B[1] * sk;
+        const double *y = colval(in, keys, nkeys, r ^ ((int64_t)1 << k));
+        re += cr * y[0] - ci * y[1];
+        im += cr * y[1] + ci * y[0];
+    }
+    *out_re = re; *out_im = im;
+}
+
 int orc_hpsi_gather(int N, const int *pairs_x, int nbx, const int *pairs_y, int nby,
                     const int *pairs_z, int nbz, const double *K, double J, double G,
                     const double *B, int skip1, const double *in, double *out)
@@ -269,53 +336,24 @@ int orc_hpsi_gather(int N, const int *pairs_x, int nbx, const int *pairs_y, int 
     int nb[3] = {nbx, nby, nbz};
     if (skip1) for (int f = 0; f < 3; f++) if (nb[f] == 1) nb[f] = 0;
 #pragma omp parallel for schedule(static)
-    for (int64_t r = 0; r < dim; r++) {
-        double d = 0.0;
-        for (int b = 0; b < nb[2]; b++) {
-            int si = P[2][2 * b], sj = P[2][2 * b + 1];
-            d += (bitof(r, si) == bitof(r, sj)) ? 0.25 * K[2] : -0.25 * K[2];
-        }
-        if (J != 0.0)
-            for (int f = 0; f < 3; f++)
-                for (int b = 0; b < nb[f]; b++) {
-                    int si = P[f][2 * b], sj = P[f][2 * b + 1];
-                    d += (bitof(r, si) == bitof(r, sj)) ? 0.25 * J : -0.25 * J;
-                }
-        double szt = 0.5 * (double)(N - 2 * __builtin_popcountll((unsigned long long)r));
-        d += B[2] * szt;
-        double re = d * in[2 * r], im = d * in[2 * r + 1];
-        for (int f = 0; f < 3; f++)
-            for (int b = 0; b < nb[f]; b++) {
-                int si = P[f][2 * b], sj = P[f][2 * b + 1];
-                double s_i = bitof(r, si) ? -1.0 : 1.0, s_j = bitof(r, sj) ? -1.0 : 1.0;
-                double cr = 0.25 * J * (1.0 - s_i * s_j), ci = 0.0;
-                if (f == 0) cr += 0.25 * K[0];
-                if (f == 1) cr += -0.25 * K[1] * s_i * s_j;
-                if (f == 2) ci = -0.25 * G * (s_i + s_j);
-                int64_t c = r ^ (((int64_t)1 << si) | ((int64_t)1 << sj));
-                re += cr * in[2 * c] - ci * in[2 * c + 1];
-                im += cr * in[2 * c + 1] + ci * in[2 * c];
-                if (G != 0.0 && f == 0) {            /* Gamma on x bond: single flips */
-                    double gi = -0.25 * G * s_i * s_j;
-                    int64_t c1 = r ^ ((int64_t)1 << si), c2 = r ^ ((int64_t)1 << sj);
-                    re += -gi * in[2 * c1 + 1] - gi * in[2 * c2 + 1];
-                    im += gi * in[2 * c1] + gi * in[2 * c2];
-                }
-                if (G != 0.0 && f == 1) {            /* Gamma on y bond: single flips */
-                    int64_t c1 = r ^ ((int64_t)1 << si), c2 = r ^ ((int64_t)1 << sj);
-                    re += 0.25 * G * s_j * in[2 * c1] + 0.25 * G * s_i * in[2 * c2];
-                    im += 0.25 * G * s_j * in[2 * c1 + 1] + 0.25 * G * s_i * in[2 * c2 + 1];
-                }
-            }
-        for (int k = 0; k < N; k++) {
-            double sk = bitof(r, k) ? -1.0 : 1.0;
-            double cr = 0.5 * B[0], ci = -0.5 * B[1] * sk;
-            int64_t c = r ^ ((int64_t)1 << k);
-            re += cr * in[2 * c] - ci * in[2 * c + 1];
-            im += cr * in[2 * c + 1] + ci * in[2 * c];
-        }
-        out[2 * r] = re; out[2 * r + 1] = im;
-    }
+    for (int64_t r = 0; r < dim; r++)
+        hpsi_row(N, P, nb, K, J, G, B, in, NULL, 0, r, &out[2 * r], &out[2 * r + 1]);
+    return 0;
+}
+
+/* The same rows for a list of row indices of a system too large to hold:
+ * keys (sorted, ascending) lists every column those rows read and vals
+ * (interleaved complex) their input values.  N may be up to 62. */
+int orc_hpsi_rows(int N, const int *pairs_x, int nbx, const int *pairs_y, int nby,
+                  const int *pairs_z, int nbz, const double *K, double J, double G,
+                  const double *B, int skip1, int64_t nrows, const int64_t *rows,
+                  int64_t nkeys, const int64_t *keys, const double *vals, double *out)
+{
+    const int *P[3] = {pairs_x, pairs_y, pairs_z};
+    int nb[3] = {nbx, nby, nbz};
+    if (skip1) for (int f = 0; f < 3; f++) if (nb[f] == 1) nb[f] = 0;
+    for (int64_t q = 0; q < nrows; q++)
+        hpsi_row(N, P, nb, K, J, G, B, vals, keys, nkeys, rows[q], &out[2 * q], &out[2 * q + 1]);
     return 0;
 }
 

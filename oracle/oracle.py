@@ -51,6 +51,7 @@ def lib():
         L.orc_coo_export.argtypes = [P, P, P, P, P]
         L.orc_coo_apply.argtypes = [P, P, P, i32]
         L.orc_hpsi_gather.argtypes = [i32, P, i32, P, i32, P, i32, P, f64, f64, P, i32, P, P]
+        L.orc_hpsi_rows.argtypes = [i32, P, i32, P, i32, P, i32, P, f64, f64, P, i32, i64, P, i64, P, P, P]
         L.orc_apply_site_op.argtypes = [i32, i32, i32, P, P]
         L.orc_current_js_site.argtypes = [i32, P, P, i32, i32, P, P]
         L.orc_current_je_site.argtypes = [i32, P, P, i32, i32, P, P]
@@ -144,6 +145,30 @@ def hpsi_gather(m: Model, v):
     B = np.asarray(m.B, np.float64)
     lib().orc_hpsi_gather(m.N, _p(px), len(px), _p(py), len(py), _p(pz), len(pz), _p(K),
                           float(m.J), float(m.G), _p(B), 1, _p(v), _p(out))
+    return out
+
+
+def row_columns(m: Model, rows):
+    """Sorted columns that rows of H.v read (the rows, their bond and single-
+    flip partners): the values orc_hpsi_rows needs."""
+    rows = np.asarray(rows, np.int64)
+    masks = [0] + [(1 << int(i)) | (1 << int(j)) for fam in m.bonds for i, j in fam] + [1 << k for k in range(m.N)]
+    cols = rows[:, None] ^ np.asarray(masks, np.int64)[None, :]
+    return np.unique(cols.ravel())
+
+
+def hpsi_rows(m: Model, rows, cols, vals):
+    """Rows of H.v (same elements as hpsi_gather) for a system too large to
+    hold: cols = row_columns(m, rows), vals = v[cols]."""
+    rows = np.ascontiguousarray(rows, np.int64)
+    cols = np.ascontiguousarray(cols, np.int64)
+    vals = np.ascontiguousarray(vals, np.complex128)
+    out = np.empty(len(rows), np.complex128)
+    px, py, pz = (np.ascontiguousarray(b, np.int32) for b in m.bonds)
+    K = np.asarray(m.K, np.float64)
+    B = np.asarray(m.B, np.float64)
+    lib().orc_hpsi_rows(m.N, _p(px), len(px), _p(py), len(py), _p(pz), len(pz), _p(K), float(m.J), float(m.G),
+                        _p(B), 1, len(rows), _p(rows), len(cols), _p(cols), _p(vals), _p(out))
     return out
 
 

@@ -150,3 +150,55 @@ def test_lanczos_breakdown_is_reported():
     with pytest.raises(esc.EscError) as ei:
         eng.lanczos(eng.upload(U[:, 0]), 10, 0.0, "OnlyEgs")
     assert ei.value.code in (-4, -5)
+
+
+@pytest.mark.parametrize("kw", [dict(Lx=3, Ly=4, K=(1.0, 1.0, 1.0), B=(0.3, 0.3, 0.3)),
+                                dict(Lx=2, Ly=6, K=(-1.0, -1.0, -1.0), J=0.1, G=0.2, B=O.field111(0.2))])
+def test_ground_state_two_pass_matches_exact_diagonalisation(kw):
+    """esc_ground_state (three resident Lanczos vectors, replayed recursion)
+    against dense diagonalisation at N=12: E0 and the state's energy
+    variance."""
+    m = O.Model(**kw)
+    print(f"[ground state N=12] {kw}", flush=True)
+    H = O.hamiltonian_dense_kron(m)
+    w, U = np.linalg.eigh(H)
+    eng = engine_for(m)
+    gs, e0, resid, it = eng.ground_state(eng.start_vector(5, "Gaussian"), steps=300, eps=1e-13, restarts=1)
+    assert abs(e0 - w[0]) < 1e-10 * max(1.0, abs(w[0]))
+    assert resid < 1e-5
+    g = eng.download(gs)
+    assert abs(np.linalg.norm(g) - 1.0) < 1e-12
+    hg = H @ g
+    assert np.linalg.norm(hg - w[0] * g) < 1e-5       # an eigenvector of the lowest eigenvalue
+    assert it > 0
+
+
+def test_ground_state_replay_equals_reorthogonalised_ritz_vector():
+    """For a short run (no loss of orthogonality yet) the replayed three-vector
+    ground state equals the reorthogonalised one."""
+    m = O.Model(2, 8, K=(-1.0, -1.0, -1.0), J=0.1, G=0.2, B=O.field111(0.2))
+    eng = engine_for(m)
+    v0 = eng.start_vector(11, "Gaussian")
+    gs_ref = eng.empty()
+    res = eng.lanczos(v0, 12, 0.0, "GS", gs_ref)
+    gs, e0, resid, it = eng.ground_state(v0, steps=12, eps=0.0, restarts=0)
+    a, b = eng.download(gs), eng.download(gs_ref)
+    assert np.linalg.norm(a - b) < 1e-9
+    assert it == 12 + 11 + 1
+
+
+def test_ground_state_c64_with_restarts_matches_c128():
+    """complex64 storage with FP64 scalars: E0 within 1e-6 relative of the
+    complex128 result after restarts; restarting lowers the residual."""
+    m = O.Model(2, 10, K=(-1.0, -1.0, -1.0), J=0.1, G=0.2, B=O.field111(0.2))
+    print("[ground state c64 N=20]", flush=True)
+    e128 = engine_for(m, "c128")
+    _, E128, r128, _ = e128.ground_state(e128.start_vector(3, "Gaussian"), steps=200, eps=1e-12, restarts=1)
+    del e128
+    e64 = engine_for(m, "c64")
+    v0 = e64.start_vector(3, "Gaussian")
+    _, E0a, ra, _ = e64.ground_state(v0, steps=200, eps=1e-12, restarts=0)
+    _, E0b, rb, _ = e64.ground_state(v0, steps=200, eps=1e-12, restarts=2)
+    assert abs(E0b - E128) < 1e-6 * abs(E128)
+    assert rb <= ra * 1.01
+    assert r128 < 1e-4

@@ -287,3 +287,18 @@ def test_real_file_format_roundtrip(tmp_path):
     out = tmp_path / "JeJe_outSpectrum.real"
     O.write_real(str(out), tab)
     assert out.read_bytes() == open(ref_path, "rb").read()
+
+
+@pytest.mark.parametrize("kw", [dict(Lx=2, Ly=6, K=(-1.0, 0.5, 0.7), J=0.1, G=0.2, B=(0.1, 0.2, 0.3)),
+                                dict(Lx=2, Ly=8, K=(1.0, 1.0, 1.0), B=(0.3, 0.3, 0.3))])
+def test_sampled_rows_equal_full_gather(kw):
+    """orc_hpsi_rows (rows of H.v from a sparse column list, used to check
+    systems too large to hold) reproduces orc_hpsi_gather row for row,
+    including repeated rows and rows 0 and dim-1."""
+    m = O.Model(**kw)
+    rng = np.random.default_rng(m.N)
+    v = rng.standard_normal(m.dim) + 1j * rng.standard_normal(m.dim)
+    full = O.hpsi_gather(m, v)
+    rows = np.concatenate([[0, m.dim - 1], rng.integers(0, m.dim, 64), [5, 5]])
+    cols = O.row_columns(m, rows)
+    assert np.array_equal(O.hpsi_rows(m, rows, cols, v[cols]), full[rows])
