@@ -3,11 +3,12 @@
 periodic; K=-1, J=0.1, Gamma=0.2, h=0.2).
 
 A step is one H.psi over the whole vector (complex128, the reference's
-arithmetic).  Prints one JSON line.  The complex64 H.psi, the fused-step
-reductions and the CPU baseline (the oracle's OpenMP gather H.psi on the host
-cores, a bounded sample) are reported beside it.
+arithmetic).  Prints one JSON line.  Reported beside it: the complex64 H.psi,
+the JeJe spectrum pipeline in both precisions, the 32-site complex64 H.psi
+(2^32 rows), a 64-instance batched parameter scan and the CPU baseline (the
+oracle's OpenMP gather H.psi on the host cores, a bounded sample).
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-spectrum]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-spectrum] [--no-n32] [--no-batch]
 
 The spectrum block times the whole JeJe pipeline at the same size in both
 precisions: ground state (200 Lanczos steps, full reorthogonalisation, complex
@@ -40,6 +41,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-spectrum", action="store_true", help="skip the JeJe spectrum timing")
+    ap.add_argument("--no-n32", action="store_true", help="skip the 32-site complex64 H.psi timing")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batched scan")
     ap.add_argument("--kernel-config", default=None,
                     help="C128,C64: use these H.psi kernel configurations instead of autotuning (profiling runs)")
     ap.add_argument("--copy-calib", action="store_true",
@@ -162,11 +165,67 @@ def main():
     }
     if spec:
         out["spectrum"] = spectrum_summary(spec)
+    if not args.no_n32:
+        out["n32"] = n32_run(esc)
+    if not args.no_batch:
+        out["batch"] = batch_run(esc)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_reps)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
+
+
+def n32_run(esc):
+    """H.psi at N=32 (4x4 cells, K=-1, h[111]=0.3; 2^32 rows, complex64, 34.4 GB
+    per vector): autotuned configuration, 5 timed applications."""
+    import torch
+    lat = esc.Honeycomb(4, 8)
+    model = esc.SpinModel(lat, K=(-1.0, -1.0, -1.0), B=esc.field111(0.3))
+    eng = esc.Engine(model, "c64")
+    x = eng.start_vector(1234, "Gaussian")
+    y = eng.empty()
+    config, _ = eng.autotune(x, y, reps=1)
+    ms, _ = time_hpsi(eng, x, y, 5, 1)
+    bytes_alg = 2 * eng.dim * eng.elem_bytes
+    del x, y, eng
+    torch.cuda.empty_cache()
+    return {"workload": "H.psi, 32-site honeycomb 4x4 cells periodic, K=-1 h[111]=0.3, full 2^32 basis, c64",
+            "hpsi_per_s": 1e3 / ms, "ms_per_hpsi": ms, "achieved_GBs": bytes_alg / (ms * 1e-3) / 1e9,
+            "frac": bytes_alg / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, "bytes_per_launch": bytes_alg,
+            "kernel_config": config}
+
+
+def batch_run(esc):
+    """Parameter scan of 64 instances solved in lockstep (one replayed HIP
+    graph per Lanczos step): 16 each of N = 16, 18, 20, 24; K = cos(phi),
+    J = sin(phi) on every bond, phi ~ U[0, 2 pi), field h ~ U[0, 0.5] along
+    [111]; complex64 storage, FP64 scalars; ground state (200 steps + replay)
+    and the JsJs spectrum (continued fractions, 150 steps per run, eta =
+    0.05, omega = linspace(0, 6, 2000))."""
+    import torch
+    rng = np.random.default_rng(2024)
+    geoms = [(2, 8)] * 16 + [(3, 6)] * 16 + [(2, 10)] * 16 + [(3, 8)] * 16
+    models = []
+    for lx, ly in geoms:
+        phi, h = rng.uniform(0, 2 * np.pi), rng.uniform(0, 0.5)
+        models.append(esc.SpinModel(esc.Honeycomb(lx, ly), K=(np.cos(phi),) * 3, J=float(np.sin(phi)),
+                                    B=esc.field111(h)))
+    b = esc.Batch(models, "c64")
+    b.autotune(reps=2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0, _, _, _ = b.ground_state(seed=1, steps=200)
+    t1 = time.perf_counter()
+    tab, _ = b.dynamics("JsJs", np.linspace(0.0, 6.0, 2000), 0.05, cf_steps=150)
+    t2 = time.perf_counter()
+    ok = bool(np.all(np.isfinite(e0)) and np.all(np.isfinite(tab)))
+    b.close()
+    torch.cuda.empty_cache()
+    return {"workload": "64 instances (16 each of N=16,18,20,24), K=cos(phi) J=sin(phi) h[111]~U[0,0.5], c64, "
+                        "GS 200 steps + JsJs CF 150 steps per run, eta=0.05, 2000 omegas",
+            "gs_seconds": t1 - t0, "jsjs_seconds": t2 - t1, "total_seconds": t2 - t0,
+            "instances_per_s": 64 / (t2 - t0), "finite": ok}
 
 
 def spectrum_run(eng):

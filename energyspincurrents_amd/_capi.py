@@ -27,6 +27,8 @@ EXPORTS = [
     "esc_ritz_overlaps", "esc_ritz_vector", "esc_lanczos_cf", "esc_dynamics", "esc_honeycomb",
     "esc_connectors", "esc_params_load", "esc_tridiag_eig", "esc_cf_spectrum", "esc_lehmann_spectrum",
     "esc_autotune", "esc_set_tile_bits", "esc_kernel_configs", "esc_ground_state",
+    "esc_batch_create", "esc_batch_destroy", "esc_batch_ground_state", "esc_batch_state", "esc_batch_dynamics",
+    "esc_batch_autotune",
 ]
 
 
@@ -104,6 +106,12 @@ def lib():
         "esc_ritz_vector": ([P, I, P, P], I),
         "esc_lanczos_cf": ([P, P, I, P, P, P, P, P], I),
         "esc_ground_state": ([P, P, I, D, I, P, P, P, P, P], I),
+        "esc_batch_create": ([I, P, I, P], I),
+        "esc_batch_destroy": ([P], I),
+        "esc_batch_ground_state": ([P, U64, I, I, P, P, P, P], I),
+        "esc_batch_state": ([P, I, P, P], I),
+        "esc_batch_autotune": ([P, I, P], I),
+        "esc_batch_dynamics": ([P, I, I, D, P, I, I, P, P], I),
         "esc_dynamics": ([P, I, I, P, D, I, D, P, I, P, P, P], I),
         "esc_honeycomb": ([I, I, I, I, P], I),
         "esc_connectors": ([I, P, P, P, P, P, P], I),

@@ -167,6 +167,21 @@ hipError_t launch_multi_axpy(bool c64, const void *base, int k, const double *d_
                              uint64_t dim, bool overwrite, hipStream_t s);
 // Philox4x32-10 Gaussian start vector, N(0, 1/2) per component
 hipError_t launch_gaussian(bool c64, uint64_t seed, void *out, uint64_t dim, hipStream_t s);
+
+// One instance of a batched Lanczos step: its three rotating vectors for this
+// step (vp = v_{k-1}, v = v_k, w = H v_k -> v_{k+1}), the accumulation target
+// of a replay, and its blocks in the flattened grid.
+struct BatchLane {
+    void *vp, *v, *w, *acc;
+    uint64_t dim;
+    uint32_t blk0, nblk;
+};
+constexpr int kBatchDot = 0;      // alpha[i][k] = <v|w>
+constexpr int kBatchUpdate = 1;   // w -= alpha v + beta_k vp; beta[i][k+1] = |w|; w /= beta; k += 1
+constexpr int kBatchAccum = 2;    // acc += coef[i][k] v
+hipError_t launch_batch_step(bool c64, int piece, const BatchLane *d_lanes, int n, uint32_t total_blocks,
+                             double *partials, double *alpha, int lda, double *beta, int ldb, const double *coef,
+                             int ldc, uint32_t *step, hipStream_t s);
 // convert FP64 complex device buffer to c64 (or copy for c128)
 hipError_t launch_convert_from_c128(bool c64, const void *src, void *dst, uint64_t dim, hipStream_t s);
 hipError_t launch_convert_to_c128(bool c64, const void *src, void *dst, uint64_t dim, hipStream_t s);

@@ -614,6 +614,147 @@ __global__ __launch_bounds__(kThreads) void k_convert(const typename Vec2<RS>::T
     }
 }
 
+// ---------------------------------------------------------------------------
+// Batched Lanczos steps over many independent systems (one flattened grid):
+// block b belongs to the instance i with lanes[i].blk0 <= b < blk0 + nblk and
+// walks that instance's rows with a stride of nblk * kThreads.  The step index
+// k is read from device memory (one counter for the batch), so the same
+// captured step can be replayed; alpha / beta / coefficients are per-instance
+// rows of device arrays.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int batch_instance(const BatchLane *__restrict__ L, int n, uint32_t b) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (L[mid].blk0 <= b) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_b_dot(const BatchLane *__restrict__ L, int n, double *__restrict__ partials) {
+    using V2 = typename Vec2<R>::T;
+    const BatchLane &l = L[batch_instance(L, n, blockIdx.x)];
+    const V2 *x = (const V2 *)l.v, *y = (const V2 *)l.w;
+    double v[2] = {0.0, 0.0};
+    const uint64_t stride = (uint64_t)l.nblk * kThreads;
+    for (uint64_t i = (uint64_t)(blockIdx.x - l.blk0) * kThreads + threadIdx.x; i < l.dim; i += stride) {
+        const double xr = x[i].x, xi = x[i].y, yr = y[i].x, yi = y[i].y;
+        v[0] += xr * yr + xi * yi;
+        v[1] += xr * yi - xi * yr;
+    }
+    block_reduce_store<2>(v, partials + 2 * blockIdx.x);
+}
+
+// one block per instance: alpha[i][k] = sum of its partials (fixed order)
+__global__ __launch_bounds__(kThreads) void k_b_alpha(const BatchLane *__restrict__ L, const double *__restrict__ partials,
+                                                      double *__restrict__ alpha, int lda,
+                                                      const uint32_t *__restrict__ step) {
+    const BatchLane &l = L[blockIdx.x];
+    double v[2] = {0.0, 0.0};
+    for (uint32_t p = threadIdx.x; p < l.nblk; p += kThreads) {
+        v[0] += partials[2 * (size_t)(l.blk0 + p)];
+        v[1] += partials[2 * (size_t)(l.blk0 + p) + 1];
+    }
+    block_reduce_store<2>(v, alpha + 2 * ((size_t)blockIdx.x * lda + *step));
+}
+
+// w -= alpha_k v + beta_k vp ; partial |w|^2
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_b_update(const BatchLane *__restrict__ L, int n,
+                                                       const double *__restrict__ alpha, int lda,
+                                                       const double *__restrict__ beta, int ldb,
+                                                       const uint32_t *__restrict__ step, double *__restrict__ partials) {
+    using V2 = typename Vec2<R>::T;
+    const int inst = batch_instance(L, n, blockIdx.x);
+    const BatchLane &l = L[inst];
+    const uint32_t k = *step;
+    const double ar = alpha[2 * ((size_t)inst * lda + k)], ai = alpha[2 * ((size_t)inst * lda + k) + 1];
+    const double b = beta[(size_t)inst * ldb + k];
+    const V2 *v = (const V2 *)l.v, *u = (const V2 *)l.vp;
+    V2 *w = (V2 *)l.w;
+    double acc[1] = {0.0};
+    const uint64_t stride = (uint64_t)l.nblk * kThreads;
+    for (uint64_t i = (uint64_t)(blockIdx.x - l.blk0) * kThreads + threadIdx.x; i < l.dim; i += stride) {
+        const double vr = v[i].x, vi = v[i].y;
+        double wr = (double)w[i].x - (ar * vr - ai * vi) - b * (double)u[i].x;
+        double wi = (double)w[i].y - (ar * vi + ai * vr) - b * (double)u[i].y;
+        V2 o;
+        o.x = (R)wr;
+        o.y = (R)wi;
+        w[i] = o;
+        wr = (double)o.x;
+        wi = (double)o.y;
+        acc[0] += wr * wr + wi * wi;
+    }
+    block_reduce_store<1>(acc, partials + blockIdx.x);
+}
+
+// one block per instance: beta[i][k+1] = |w|, or 0 when the Krylov space is
+// exhausted (|w| at rounding level of |alpha_k|) or the value is not finite
+__global__ __launch_bounds__(kThreads) void k_b_beta(const BatchLane *__restrict__ L, const double *__restrict__ partials,
+                                                     const double *__restrict__ alpha, int lda,
+                                                     double *__restrict__ beta, int ldb,
+                                                     const uint32_t *__restrict__ step) {
+    const BatchLane &l = L[blockIdx.x];
+    __shared__ double tot[1];
+    double v[1] = {0.0};
+    for (uint32_t p = threadIdx.x; p < l.nblk; p += kThreads) v[0] += partials[l.blk0 + p];
+    block_reduce_store<1>(v, tot);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t k = *step;
+        const double a = fabs(alpha[2 * ((size_t)blockIdx.x * lda + k)]);
+        double b = sqrt(tot[0]);
+        if (!(b > 1e-13 * (a > 1.0 ? a : 1.0)) || !isfinite(b)) b = 0.0;
+        beta[(size_t)blockIdx.x * ldb + k + 1] = b;
+    }
+}
+
+// w /= beta_{k+1} (zero when beta_{k+1} = 0)
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_b_scale(const BatchLane *__restrict__ L, int n,
+                                                      const double *__restrict__ beta, int ldb,
+                                                      const uint32_t *__restrict__ step) {
+    using V2 = typename Vec2<R>::T;
+    const int inst = batch_instance(L, n, blockIdx.x);
+    const BatchLane &l = L[inst];
+    const double b = beta[(size_t)inst * ldb + *step + 1];
+    const double sc = b > 0.0 ? 1.0 / b : 0.0;
+    V2 *w = (V2 *)l.w;
+    const uint64_t stride = (uint64_t)l.nblk * kThreads;
+    for (uint64_t i = (uint64_t)(blockIdx.x - l.blk0) * kThreads + threadIdx.x; i < l.dim; i += stride) {
+        V2 o = w[i];
+        o.x = (R)((double)o.x * sc);
+        o.y = (R)((double)o.y * sc);
+        w[i] = o;
+    }
+}
+
+// acc += coef[i][k] v  (replay of a recursion forming a Ritz vector)
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_b_accum(const BatchLane *__restrict__ L, int n,
+                                                      const double *__restrict__ coef, int ldc,
+                                                      const uint32_t *__restrict__ step) {
+    using V2 = typename Vec2<R>::T;
+    const int inst = batch_instance(L, n, blockIdx.x);
+    const BatchLane &l = L[inst];
+    const double c = coef[(size_t)inst * ldc + *step];
+    const V2 *v = (const V2 *)l.v;
+    V2 *a = (V2 *)l.acc;
+    const uint64_t stride = (uint64_t)l.nblk * kThreads;
+    for (uint64_t i = (uint64_t)(blockIdx.x - l.blk0) * kThreads + threadIdx.x; i < l.dim; i += stride) {
+        V2 o = a[i];
+        o.x = (R)((double)o.x + c * (double)v[i].x);
+        o.y = (R)((double)o.y + c * (double)v[i].y);
+        a[i] = o;
+    }
+}
+
+__global__ void k_b_next_step(uint32_t *step) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1u;
+}
+
 inline uint32_t row_blocks(uint64_t dim) { return (uint32_t)((dim + kThreads - 1) / kThreads); }
 inline uint32_t stream_blocks(uint64_t dim) {
     uint64_t b = (dim + kThreads - 1) / kThreads;
@@ -654,10 +795,15 @@ static hipError_t launch_rows_current(const CurrentTable *t, const void *in, voi
 template <typename R, int TB, int WPB>
 static hipError_t launch_tile_tb(const GTables *g, const HTables *h, const void *in, void *out, uint32_t ntiles,
                                  size_t lds, hipStream_t s) {
-    if (lds > 65536) {   // tiles above 64 KiB need the raised dynamic-LDS limit
+    // tiles above 64 KiB need the raised dynamic-LDS limit; raised once to
+    // the full 160 KiB, outside any stream capture (the first launch of a
+    // shape is an eager one)
+    static bool raised = false;
+    if (lds > 65536 && !raised) {
         hipError_t e = hipFuncSetAttribute((const void *)k_hpsi_tile<R, TB, WPB>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxTileLds);
         if (e != hipSuccess) return e;
+        raised = true;
     }
     for (uint32_t base = 0; base < ntiles; base += kMaxGrid) {
         const uint32_t n = (ntiles - base) < kMaxGrid ? (ntiles - base) : kMaxGrid;
@@ -843,6 +989,40 @@ hipError_t launch_convert_to_c128(bool c64, const void *src, void *dst, uint64_t
     else
         hipLaunchKernelGGL((k_convert<double, double>), dim3(stream_blocks(dim)), dim3(kThreads), 0, s,
                            (CV2p<double>)src, (V2p<double>)dst, dim);
+    return hipGetLastError();
+}
+
+// ---- batched Lanczos step pieces ----
+hipError_t launch_batch_step(bool c64, int piece, const BatchLane *d_lanes, int n, uint32_t total_blocks,
+                             double *partials, double *alpha, int lda, double *beta, int ldb, const double *coef,
+                             int ldc, uint32_t *step, hipStream_t s) {
+    const dim3 g(total_blocks), gi((uint32_t)n), t(kThreads);
+    switch (piece) {
+    case kBatchDot:
+        if (c64) hipLaunchKernelGGL(k_b_dot<float>, g, t, 0, s, d_lanes, n, partials);
+        else hipLaunchKernelGGL(k_b_dot<double>, g, t, 0, s, d_lanes, n, partials);
+        hipLaunchKernelGGL(k_b_alpha, gi, t, 0, s, d_lanes, (const double *)partials, alpha, lda, (const uint32_t *)step);
+        break;
+    case kBatchUpdate:
+        if (c64)
+            hipLaunchKernelGGL(k_b_update<float>, g, t, 0, s, d_lanes, n, (const double *)alpha, lda,
+                               (const double *)beta, ldb, (const uint32_t *)step, partials);
+        else
+            hipLaunchKernelGGL(k_b_update<double>, g, t, 0, s, d_lanes, n, (const double *)alpha, lda,
+                               (const double *)beta, ldb, (const uint32_t *)step, partials);
+        hipLaunchKernelGGL(k_b_beta, gi, t, 0, s, d_lanes, (const double *)partials, (const double *)alpha, lda, beta,
+                           ldb, (const uint32_t *)step);
+        if (c64) hipLaunchKernelGGL(k_b_scale<float>, g, t, 0, s, d_lanes, n, (const double *)beta, ldb, (const uint32_t *)step);
+        else hipLaunchKernelGGL(k_b_scale<double>, g, t, 0, s, d_lanes, n, (const double *)beta, ldb, (const uint32_t *)step);
+        hipLaunchKernelGGL(k_b_next_step, dim3(1), dim3(64), 0, s, step);
+        break;
+    case kBatchAccum:
+        if (c64) hipLaunchKernelGGL(k_b_accum<float>, g, t, 0, s, d_lanes, n, coef, ldc, (const uint32_t *)step);
+        else hipLaunchKernelGGL(k_b_accum<double>, g, t, 0, s, d_lanes, n, coef, ldc, (const uint32_t *)step);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

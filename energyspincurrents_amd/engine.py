@@ -268,6 +268,81 @@ class Engine:
 
 
 # -- host FP64 small solves -------------------------------------------------
+class Batch:
+    """Many models solved in lockstep (esc_batch_*): each Lanczos step of all
+    instances is one replayed HIP graph.  Instances may differ in size and
+    couplings; all share one precision."""
+
+    def __init__(self, models, precision="c64"):
+        import copy
+        self.models = list(models)
+        self.precision = precision
+        # one copy per entry: struct() keeps the bond arrays it points into on
+        # the model object, so a model listed twice must not share them
+        self._copies = [copy.copy(m) for m in self.models]
+        self._structs = [m.struct() for m in self._copies]
+        arr = (EscModel * len(self._structs))(*self._structs)
+        self._arr = arr
+        h = ctypes.c_void_p()
+        check(lib().esc_batch_create(len(self._structs), arr, ESC_C64 if precision == "c64" else ESC_C128,
+                                     ctypes.byref(h)))
+        self._h = h
+        self.n = len(self._structs)
+        self.e0 = None
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            check(lib().esc_batch_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def autotune(self, reps=3):
+        """Pick every instance's H.psi kernel (esc_batch_autotune); returns the configurations."""
+        cfg = np.zeros(self.n, np.int32)
+        check(lib().esc_batch_autotune(self._h, int(reps), _np_ptr(cfg)))
+        return cfg
+
+    def ground_state(self, seed=0, steps=200, use_graph=True):
+        """Returns (e0[n], iters[n], alpha (n, steps), beta (n, steps + 1))."""
+        e0 = np.zeros(self.n)
+        it = np.zeros(self.n, np.int32)
+        al = np.zeros((self.n, steps))
+        be = np.zeros((self.n, steps + 1))
+        check(lib().esc_batch_ground_state(self._h, int(seed), int(steps), int(bool(use_graph)), _np_ptr(e0),
+                                           _np_ptr(it), _np_ptr(al), _np_ptr(be)))
+        self.e0 = e0
+        return e0, it, al, be
+
+    def state(self, i, out=None, stream=None):
+        """Instance i's ground state as a device tensor."""
+        import torch
+        m = self.models[i]
+        if out is None:
+            out = torch.empty(1 << m.lattice.N, dtype=torch.complex64 if self.precision == "c64" else torch.complex128,
+                              device="cuda")
+        check(lib().esc_batch_state(self._h, int(i), _dev_ptr(out), _stream(stream)))
+        if stream is None:
+            torch.cuda.current_stream().synchronize()
+        return out
+
+    def dynamics(self, kind, omega, eta, cf_steps=150, use_graph=True):
+        """JsJs / JeJe spectra of every instance (continued fractions).
+        Returns (tables (n, nw, 1 + ncols), expectations (n, ncomp))."""
+        omega = np.ascontiguousarray(omega, np.float64)
+        ncols = 14 if kind == "JsJs" else 5
+        tables = np.zeros((self.n, len(omega), 1 + ncols))
+        expect = np.zeros((self.n, 8))
+        check(lib().esc_batch_dynamics(self._h, DYN_KINDS[kind], int(cf_steps), float(eta), _np_ptr(omega),
+                                       len(omega), int(bool(use_graph)), _np_ptr(tables), _np_ptr(expect)))
+        nexp = 4 if kind == "JsJs" else 3
+        return tables, (expect[:, 0::2] + 1j * expect[:, 1::2])[:, :nexp]
+
+
 def tridiag_eig(alpha, beta, vectors=True):
     a = np.ascontiguousarray(np.real(alpha), np.float64)
     b = np.ascontiguousarray(np.abs(beta), np.float64)

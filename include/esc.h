@@ -254,6 +254,36 @@ int esc_lehmann_spectrum(int n, const double *evals, const double *Oa, const dou
                          const double *elastic, double eta, const double *omega, int nw,
                          double *out);
 
+/* --------------------------------------------------------------- batches -- */
+/* Many independent systems (any sizes and couplings, one precision) solved in
+ * lockstep: each Lanczos step of all instances is replayed as one captured
+ * HIP graph (the instances' H.psi launches and four batched kernels that
+ * keep alpha / beta on the device).  No reference counterpart: the reference
+ * solves one system per process (main.cpp:41-112); these entry points run
+ * its ground-state + JsJs / JeJe pipeline for a whole parameter scan.
+ * Each instance holds 6 vectors.  The calls are synchronous. */
+typedef struct esc_batch esc_batch;
+int esc_batch_create(int n, const esc_model *models, int precision, esc_batch **out);
+int esc_batch_destroy(esc_batch *b);
+/* Ground states: `steps` lockstep steps without reorthogonalisation from
+ * Gaussian start vectors (seed + i, as esc_start_vector), then a replay
+ * forming |gs_i> = sum_k Z(k,0) v_k.  e0[i], iters[i] (recursion length; it
+ * ends early only on an exhausted Krylov space), alpha (n x steps) and beta
+ * (n x (steps + 1)) may be NULL.  use_graph = 0 issues the same launches
+ * without graphs (same results bit for bit). */
+int esc_batch_ground_state(esc_batch *b, uint64_t seed, int steps, int use_graph, double *e0, int *iters,
+                           double *alpha, double *beta);
+/* Tune every instance's H.psi kernel (esc_autotune on a Gaussian vector);
+ * configs (may be NULL) receives the n choices.  Clears the ground states. */
+int esc_batch_autotune(esc_batch *b, int reps, int *configs);
+/* Copy instance i's ground state into d_out (the instance's precision). */
+int esc_batch_state(esc_batch *b, int i, void *d_out, void *stream);
+/* JsJs / JeJe spectra of every instance by continued fractions (cf_steps per
+ * run), rows as esc_dynamics: tables is n x nw x (1 + 14) for JsJs or
+ * n x nw x (1 + 5) for JeJe; expect (may be NULL) n x 4 complex <J_a>. */
+int esc_batch_dynamics(esc_batch *b, int kind, int cf_steps, double eta, const double *omega, int nw,
+                       int use_graph, double *tables, double *expect);
+
 #ifdef __cplusplus
 }
 #endif

@@ -172,6 +172,19 @@ def hpsi_rows(m: Model, rows, cols, vals):
     return out
 
 
+def ground_pair(m: Model, tol=1e-13):
+    """Lowest eigenpair of the gather-form H by implicitly restarted Lanczos
+    (ARPACK through scipy.sparse.linalg.eigsh) on a LinearOperator over
+    hpsi_gather: the exact-diagonalisation check for N where a dense
+    eigensolve is slow (N = 12: dim 4096)."""
+    from scipy.sparse.linalg import LinearOperator, eigsh
+    op = LinearOperator((m.dim, m.dim), matvec=lambda v: hpsi_gather(m, v.ravel()), dtype=np.complex128)
+    rng = np.random.default_rng(12345)          # a generic start: no symmetry sector is missed
+    v0 = rng.standard_normal(m.dim) + 1j * rng.standard_normal(m.dim)
+    w, U = eigsh(op, k=1, which="SA", tol=tol, v0=v0)
+    return float(w[0]), U[:, 0]
+
+
 class CooHamiltonian:
     """Faithful COO restatement: TightB_Ham + Diagonal + MatrixVectorMult
     (src/Hamiltonian.h:94-206,331-377,480-557).  Reference model only (no J, G)."""

@@ -1,0 +1,90 @@
+"""Batches of independent systems solved in lockstep (esc_batch_*): each
+Lanczos step of all instances is one replayed HIP graph.
+
+Bars: graph replays equal the eager launches bit for bit, on every run;
+E0 of every instance against exact diagonalisation (N <= 12) within 1e-9 and
+against the single-system ground state (N = 16) within 1e-9 relative; CF
+spectra of every instance against the exact Lehmann sum within 1e-6 of the
+peak; complex64 batches within 1e-6 relative of complex128 in E0.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+esc = pytest.importorskip("energyspincurrents_amd")
+
+
+def spin_model(m: O.Model):
+    lat = esc.Honeycomb(m.Lx, m.Ly, m.perX, m.perY)
+    return esc.SpinModel(lat, K=m.K, J=m.J, Gamma=m.G, B=m.B)
+
+
+def kjh(phi, h):
+    """K = cos(phi), J = sin(phi) on every bond, field h along [111]."""
+    return dict(K=(np.cos(phi),) * 3, J=float(np.sin(phi)), B=O.field111(h))
+
+
+MIXED = [O.Model(2, 4, **kjh(0.3, 0.2)), O.Model(3, 4, K=(1.0, 1.0, 1.0), B=(0.3, 0.3, 0.3)),
+         O.Model(2, 6, K=(-1.0, -1.0, -1.0), J=0.1, G=0.2, B=O.field111(0.2)), O.Model(2, 8, **kjh(2.0, 0.4)),
+         O.Model(2, 4, **kjh(4.0, 0.1))]
+
+
+def test_batch_ground_states_graph_equals_eager_and_exact():
+    print("[batch gs] mixed N=8,12,12,16,8", flush=True)
+    b = esc.Batch([spin_model(m) for m in MIXED], "c128")
+    e_eager, it_e, al_e, be_e = b.ground_state(seed=21, steps=200, use_graph=False)
+    gs_eager = [b.state(i).cpu().numpy() for i in range(b.n)]
+    for rep in range(2):                      # first run captures, the second replays
+        e_g, it_g, al_g, be_g = b.ground_state(seed=21, steps=200, use_graph=True)
+        assert np.array_equal(e_g, e_eager) and np.array_equal(it_g, it_e)
+        assert np.array_equal(al_g, al_e) and np.array_equal(be_g, be_e)
+        for i in range(b.n):
+            assert np.array_equal(b.state(i).cpu().numpy(), gs_eager[i]), f"replay {rep} instance {i}"
+    for i, m in enumerate(MIXED):
+        if m.N <= 12:
+            w0, _ = O.ground_pair(m)
+            assert abs(e_eager[i] - w0) < 1e-9, f"instance {i}"
+            g = gs_eager[i]
+            assert abs(np.linalg.norm(g) - 1) < 1e-12
+            assert np.linalg.norm(O.hpsi_gather(m, g) - w0 * g) < 1e-5
+        else:
+            eng = esc.Engine(spin_model(m), "c128")
+            _, e1, _, _ = eng.ground_state(eng.start_vector(21 + i, "Gaussian"), steps=200, restarts=1)
+            assert abs(e_eager[i] - e1) < 1e-9 * abs(e1)
+
+
+@pytest.mark.parametrize("kind", ["JeJe", "JsJs"])
+def test_batch_cf_spectra_equal_exact_lehmann(kind):
+    ms = [O.Model(2, 4, K=(-1.0, -1.0, -1.0), J=0.1, G=0.2, B=O.field111(0.2)), O.Model(2, 4, **kjh(1.0, 0.3))]
+    print(f"[batch {kind}]", flush=True)
+    b = esc.Batch([spin_model(m) for m in ms], "c128")
+    b.ground_state(seed=4, steps=200)
+    w = np.linspace(0.0, 6.0, 300)
+    eta = 0.1
+    tab, ex = b.dynamics(kind, w, eta, cf_steps=150, use_graph=True)
+    tab_e, ex_e = b.dynamics(kind, w, eta, cf_steps=150, use_graph=False)
+    assert np.array_equal(tab, tab_e) and np.array_equal(ex, ex_e)
+    for i, m in enumerate(ms):
+        H = O.hamiltonian_dense_kron(m)
+        e, U = np.linalg.eigh(H)
+        exact = (O.jeje_ref if kind == "JeJe" else O.jsjs_ref)(m, e, U.T.copy(), U[:, 0], omega=w, eta=eta)
+        scale = np.abs(exact[:, 1:]).max()
+        assert np.abs(tab[i][:, 1:] - exact[:, 1:]).max() < 1e-6 * scale, f"instance {i}"
+
+
+def test_batch_c64_energies_match_c128():
+    """A small scan in the shape of the 64-instance study: sizes 16 and 18,
+    K = cos(phi), J = sin(phi), h along [111]."""
+    rng = np.random.default_rng(3)
+    ms = [O.Model(*geom, **kjh(rng.uniform(0, 2 * np.pi), rng.uniform(0, 0.5)))
+          for geom in [(2, 8), (3, 6), (2, 8), (3, 6)]]
+    print("[batch c64 vs c128]", flush=True)
+    e128, _, _, _ = esc.Batch([spin_model(m) for m in ms], "c128").ground_state(seed=9, steps=200)
+    b64 = esc.Batch([spin_model(m) for m in ms], "c64")
+    cfg = b64.autotune(reps=2)
+    assert len(cfg) == len(ms)
+    e64, _, _, _ = b64.ground_state(seed=9, steps=200)
+    assert np.all(np.abs(e64 - e128) < 1e-6 * np.abs(e128))

@@ -302,3 +302,15 @@ def test_sampled_rows_equal_full_gather(kw):
     rows = np.concatenate([[0, m.dim - 1], rng.integers(0, m.dim, 64), [5, 5]])
     cols = O.row_columns(m, rows)
     assert np.array_equal(O.hpsi_rows(m, rows, cols, v[cols]), full[rows])
+
+
+def test_sparse_ground_pair_equals_dense():
+    """The eigsh-based ground state (used to check N = 12 runs) equals the
+    dense eigensolve of the Kronecker-built H at N = 8 and 10."""
+    for m in (O.Model(2, 4, K=(-1.0, -1.0, -1.0), J=0.1, G=0.2, B=O.field111(0.2)),
+              O.Model(1, 10, K=(1.0, 0.8, 1.2), B=(0.3, 0.1, 0.2))):
+        H = O.hamiltonian_dense_kron(m)
+        w = np.linalg.eigvalsh(H)
+        e0, g = O.ground_pair(m)
+        assert abs(e0 - w[0]) < 1e-11
+        assert np.linalg.norm(H @ g - e0 * g) < 1e-8
