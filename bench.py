@@ -11,9 +11,10 @@ oracle's OpenMP gather H.psi on the host cores, a bounded sample).
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-spectrum] [--no-n32] [--no-batch]
 
 The spectrum block times the whole JeJe pipeline at the same size in both
-precisions: ground state (200 Lanczos steps, full reorthogonalisation, complex
-Gaussian start vector), the three energy-current vectors and the continued
-fractions (200 steps each) on omega = linspace(0, 6, 2000), eta = 0.05.
+precisions: ground state (200 Lanczos steps with one restart from the
+previous state, complex Gaussian start vector), the energy-current vectors and
+the continued fractions (200 steps each) on omega = linspace(0, 6, 2000),
+eta = 0.05, and compares complex64 with complex128.
 """
 from __future__ import annotations
 
@@ -133,7 +134,7 @@ def main():
                      "traffic": MEASURED_TRAFFIC.get((prec, config))}
         del x, y
         if not args.no_spectrum:
-            spec[prec] = spectrum_run(eng)
+            spec[prec] = spectrum_run(eng, None if prec == "c128" else spec["c128"]["gs"])
         del eng
         torch.cuda.empty_cache()
 
@@ -164,6 +165,7 @@ def main():
                 "kernel_config": res["c64"]["config"], "traffic": res["c64"]["traffic"]},
     }
     if spec:
+        spec["c128"]["gs"] = None
         out["spectrum"] = spectrum_summary(spec)
     if not args.no_n32:
         out["n32"] = n32_run(esc)
@@ -228,28 +230,63 @@ def batch_run(esc):
             "instances_per_s": 64 / (t2 - t0), "finite": ok}
 
 
-def spectrum_run(eng):
-    """Ground state (200 steps, GS mode) + JeJe CF spectrum (200 steps per
-    run), eta = 0.05, omega in [0, 6] at 2000 points, timed end to end."""
+def spectrum_run(eng, ref_gs=None):
+    """Ground state (200-step Lanczos with one restart from the previous state,
+    FP64 scalars) + the JeJe spectrum by continued fractions (200 steps per
+    run, eta = 0.05, omega in [0, 6] at 2000 points), timed end to end.
+
+    For the precision comparison the spectrum is also computed from ref_gs
+    (the complex128 ground state, converted), which separates the
+    continued-fraction stage from the choice among the quasi-degenerate
+    ground states of the torus, at 200 steps and at 400 steps (where the
+    eta = 0.05 spectrum has converged)."""
     import torch
     w = np.linspace(0.0, 6.0, 2000)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    gs = eng.empty()
-    lr = eng.lanczos(eng.start_vector(7, "Gaussian"), 200, 0.0, "GS", gs)
-    tab, _ = eng.dynamics("JeJe", "CF", gs, w, 0.05, e0=lr.evals[0], cf_steps=200)
+    gs, e0, resid, it = eng.ground_state(eng.start_vector(7, "Gaussian"), steps=200, eps=0.0, restarts=1)
+    tab, _ = eng.dynamics("JeJe", "CF", gs, w, 0.05, e0=e0, cf_steps=200)
     torch.cuda.synchronize()
-    return {"seconds": time.perf_counter() - t0, "E0": float(lr.evals[0]), "table": tab}
+    seconds = time.perf_counter() - t0
+    src = gs if ref_gs is None else ref_gs.to(gs.dtype)
+    ref_e0 = e0 if ref_gs is None else float(eng.dot(src, eng.hpsi(src)).real)
+    tab_200 = tab if ref_gs is None else eng.dynamics("JeJe", "CF", src, w, 0.05, e0=ref_e0, cf_steps=200)[0]
+    tab_400 = eng.dynamics("JeJe", "CF", src, w, 0.05, e0=ref_e0, cf_steps=400)[0]
+    cf = {op: eng.lanczos_cf(eng.apply_total(op, src), 20) for op in ("Jex", "Jey")}
+    return {"seconds": seconds, "E0": e0, "residual": resid, "hpsi": it, "table": tab, "table_200": tab_200,
+            "table_400": tab_400, "cf": cf, "gs": gs if ref_gs is None else None}
+
+
+def _l1(a, b):
+    return float(np.abs(a[:, 1:] - b[:, 1:]).sum() / np.abs(b[:, 1:]).sum())
 
 
 def spectrum_summary(spec):
-    a, b = spec["c128"]["table"][:, 1:], spec["c64"]["table"][:, 1:]
-    e128, e64 = spec["c128"]["E0"], spec["c64"]["E0"]
-    return {"jeje_seconds_c128": spec["c128"]["seconds"], "jeje_seconds_c64": spec["c64"]["seconds"],
+    c128, c64 = spec["c128"], spec["c64"]
+    e128, e64 = c128["E0"], c64["E0"]
+    # total spectral weight of a channel = |J_a gs|^2 (the zeroth moment)
+    wt = max(abs(c64["cf"][op][2] - c128["cf"][op][2]) / c128["cf"][op][2] for op in ("Jex", "Jey"))
+    lead = 0.0
+    for op in ("Jex", "Jey"):
+        a1, b1, _ = c128["cf"][op]
+        a2, b2, _ = c64["cf"][op]
+        lead = max(lead, np.max(np.abs(a2[:5] - a1[:5]) / np.abs(b1[1:6])),
+                   np.max(np.abs(b2[1:6] - b1[1:6]) / np.abs(b1[1:6])))
+    return {"jeje_seconds_c128": c128["seconds"], "jeje_seconds_c64": c64["seconds"],
             "E0_c128": e128, "E0_c64": e64, "E0_rel_err_c64": abs(e64 - e128) / abs(e128),
-            "spectral_weight_rel_err_c64": float(np.abs(a - b).sum() / np.abs(a).sum()),
-            "workload": "GS 200 Lanczos steps (reorthogonalised) + JeJe xx, yy, xy, yx, tt by continued "
-                        "fractions, 200 steps each, eta=0.05, omega=linspace(0,6,2000)"}
+            "gs_residual_c128": c128["residual"], "gs_residual_c64": c64["residual"],
+            "spectral_weight_rel_err_c64": wt,
+            "leading_alpha_beta_rel_err_c64": float(lead),
+            "spectrum_l1_rel_err_c64_400_steps": _l1(c64["table_400"], c128["table_400"]),
+            "spectrum_l1_rel_err_c64_200_steps": _l1(c64["table_200"], c128["table_200"]),
+            "spectrum_l1_c128_200_vs_400_steps": _l1(c128["table_200"], c128["table_400"]),
+            "spectrum_l1_rel_err_c64_own_gs": _l1(c64["table"], c128["table"]),
+            "workload": "timed: GS 200 Lanczos steps + 1 restart (FP64 scalars) + JeJe xx, yy, xy, yx, tt by "
+                        "continued fractions, 200 steps each, eta=0.05, omega=linspace(0,6,2000).  Errors of "
+                        "complex64 against complex128 from the complex128 ground state: weight = |J_a gs|^2, "
+                        "leading alpha/beta = first 5 of the Jex and Jey recursions (relative to beta), L1 of the "
+                        "JeJe table at 400 steps (converged at eta=0.05) and at 200 steps (not converged: see the "
+                        "complex128 200-vs-400 figure); own_gs compares the two timed pipelines end to end"}
 
 
 if __name__ == "__main__":

@@ -18,7 +18,7 @@ SITE_TOTAL = -1
 LANC_MODES = {"OnlyEgs": 0, "GS": 1, "All": 2, "Plain": 3}
 START_KINDS = {"Uniform": 0, "Gaussian": 1}
 DYN_KINDS = {"JsJs": 0, "JeJe": 1}
-SPEC_METHODS = {"Ritz": 0, "CF": 1}
+SPEC_METHODS = {"Ritz": 0, "CF": 1, "CFReorth": 2}
 
 # every entry point declared in include/esc.h
 EXPORTS = [

@@ -823,7 +823,7 @@ void cgs_pass(esc_ctx *c, int k, void *w, hipStream_t s) {
 // Three-vector Lanczos without reorthogonalisation from phi/|phi| (work holds
 // three device vectors distinct from phi).  alpha[0..it-1], beta[0..it].
 void cf_run(esc_ctx *ctx, const void *d_phi, int steps, void *work[3], double *alpha, double *beta, int *iters,
-            double *norm2, hipStream_t s) {
+            double *norm2, hipStream_t s, bool reortho = false) {
     check_ptr(alpha, "alpha");
     check_ptr(beta, "beta");
     check_ptr(iters, "iters");
@@ -836,12 +836,26 @@ void cf_run(esc_ctx *ctx, const void *d_phi, int steps, void *work[3], double *a
     const double n2 = device_norm2(ctx, d_phi, s);
     if (!(n2 > 0.0) || !std::isfinite(n2)) throw EscError(ESC_ERR_NUMERIC, "start vector has zero or non-finite norm");
     *norm2 = n2;
-    void *vp = work[0], *v = work[1], *w = work[2];
+    // with reortho the vectors v_0..v_steps are kept in the context's vectors
+    // and every new one is orthogonalised against all before it (two
+    // classical Gram-Schmidt passes)
+    if (reortho) {
+        ctx->ensure_vecs((size_t)steps + 1);
+        ctx->basis_kept = false;
+        ctx->ensure_partials((size_t)2 * (steps + 2) * 256 + 2 * esc::kRedBlocks);
+        ctx->ensure_result((size_t)2 * (steps + 2) + 8);
+    }
+    void *vp = reortho ? nullptr : work[0], *v = reortho ? ctx->vec(0) : work[1], *w = reortho ? ctx->vec(1) : work[2];
     hip_check(hipMemcpyAsync(v, d_phi, dim * ctx->esz, hipMemcpyDeviceToDevice, s), "copy");
     hip_check(esc::launch_scale(c64, v, 1.0 / std::sqrt(n2), dim, s), "scale");
     beta[0] = 0.0;
     int k = 0;
     for (; k < steps; ++k) {
+        if (reortho) {
+            vp = k > 0 ? ctx->vec((size_t)k - 1) : nullptr;
+            v = ctx->vec((size_t)k);
+            w = ctx->vec((size_t)k + 1);
+        }
         run_hpsi(ctx, v, w, s);
         double a[2];
         hip_check(esc::launch_dot(c64, v, w, dim, ctx->d_partials, ctx->d_result, s), "dot");
@@ -861,6 +875,15 @@ void cf_run(esc_ctx *ctx, const void *d_phi, int steps, void *work[3], double *a
             break;
         }
         hip_check(esc::launch_scale(c64, w, 1.0 / b, dim, s), "scale");
+        if (reortho) {
+            if (k > 0) {
+                cgs_pass(ctx, k + 1, w, s);
+                cgs_pass(ctx, k + 1, w, s);
+                hip_check(esc::launch_norm2(c64, w, dim, ctx->d_partials, ctx->d_result + 4, s), "norm2");
+                hip_check(esc::launch_normalize(c64, w, ctx->d_result + 4, dim, s), "normalize");
+            }
+            continue;
+        }
         void *t = vp; vp = v; v = w; w = t;
     }
     *iters = k;
@@ -1430,11 +1453,15 @@ int esc_dynamics(esc_ctx *ctx, int kind, int method, const void *d_gs, double e0
         check_ptr(omega, "omega");
         check_ptr(table, "table");
         if (kind != ESC_DYN_JSJS && kind != ESC_DYN_JEJE) throw EscError(ESC_ERR_ARG, "unknown dynamics kind");
-        if (method != ESC_SPEC_RITZ && method != ESC_SPEC_CF) throw EscError(ESC_ERR_ARG, "unknown spectral method");
+        if (method != ESC_SPEC_RITZ && method != ESC_SPEC_CF && method != ESC_SPEC_CF_REORTH)
+            throw EscError(ESC_ERR_ARG, "unknown spectral method");
+        const bool reortho = method == ESC_SPEC_CF_REORTH;
+        if (reortho) method = ESC_SPEC_CF;
         if (nw < 1) throw EscError(ESC_ERR_ARG, "nw must be >= 1");
         if (method == ESC_SPEC_RITZ && !ctx->basis_kept)
             throw EscError(ESC_ERR_ARG, "Ritz spectra need the basis of an ESC_LANC_ALL run (LancType=All)");
-        if (method == ESC_SPEC_CF && (cf_steps < 1 || cf_steps > 100000)) throw EscError(ESC_ERR_ARG, "cf_steps out of range");
+        if (method == ESC_SPEC_CF && (cf_steps < 1 || cf_steps > (reortho ? 4096 : 100000)))
+            throw EscError(ESC_ERR_ARG, "cf_steps out of range");
         hipStream_t s = (hipStream_t)stream;
         const bool js = kind == ESC_DYN_JSJS;
         const std::vector<int> ops = js ? std::vector<int>{ESC_OP_JSX, ESC_OP_JSY, ESC_OP_JSZ}
@@ -1531,7 +1558,7 @@ int esc_dynamics(esc_ctx *ctx, int kind, int method, const void *d_gs, double e0
                 apply_combo(r.coef);
                 int it = 0;
                 double n2 = 0.0;
-                cf_run(ctx, ctx->aux(0), cf_steps, work, al.data(), be.data(), &it, &n2, s);
+                cf_run(ctx, ctx->aux(0), cf_steps, work, al.data(), be.data(), &it, &n2, s, reortho);
                 f.push_back(cf_eval(it, al.data(), be.data(), n2, e0, eta, omega, nw));
             }
             F = cf_assemble(js, nc, runs, f, nw);

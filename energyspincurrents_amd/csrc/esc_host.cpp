@@ -158,7 +158,8 @@ extern "C" int esc_params_load(const char *path, esc_params *p) {
         p->SpectralMode = ESC_SPEC_RITZ;
         if (find_key(lines, "SpectralMode", s)) {
             if (s == "CF") p->SpectralMode = ESC_SPEC_CF;
-            else if (s != "Ritz") throw HostError(ESC_ERR_ARG, "SpectralMode must be Ritz or CF");
+            else if (s == "CFReorth") p->SpectralMode = ESC_SPEC_CF_REORTH;
+            else if (s != "Ritz") throw HostError(ESC_ERR_ARG, "SpectralMode must be Ritz, CF or CFReorth");
         }
         p->StartVector = ESC_START_REF_UNIFORM;
         if (find_key(lines, "StartVector", s)) {

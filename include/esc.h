@@ -185,6 +185,9 @@ int esc_lanczos_cf(esc_ctx *ctx, const void *d_phi, int steps, double *alpha, do
 #define ESC_DYN_JEJE 1   /* DynJeJe: 5 value columns Im xx yy xy yx tt */
 #define ESC_SPEC_RITZ 0  /* Lehmann sum over the Ritz states of the last ESC_LANC_ALL run */
 #define ESC_SPEC_CF   1  /* continued fractions from J|gs> (cross terms by polarisation) */
+#define ESC_SPEC_CF_REORTH 2  /* the same with full reorthogonalisation of each run's Lanczos
+                               * vectors (cf_steps + 1 resident vectors): keeps complex64
+                               * recursions as accurate as complex128 ones */
 
 /* Full dynamical-correlation pipeline for one kind, replacing
  * StartDynamics -> DynJsJs / DynJeJe (src/dynamics.h:43-60,222-460) minus the
@@ -226,7 +229,7 @@ typedef struct {
     double J, Gamma, Eta, OmegaMin, OmegaMax;
     int32_t OmegaPoints;      /* 0 = the reference grid omega = 0.002 k, k <= 1000 */
     int32_t Precision;        /* ESC_C128 / ESC_C64 */
-    int32_t SpectralMode;     /* ESC_SPEC_RITZ / ESC_SPEC_CF */
+    int32_t SpectralMode;     /* ESC_SPEC_RITZ / ESC_SPEC_CF / ESC_SPEC_CF_REORTH */
     int32_t StartVector;      /* ESC_START_* */
     int32_t CFSteps;
 } esc_params;

@@ -125,6 +125,9 @@ def test_cf_spectra_equal_exact_lehmann(kind):
     tab, ex = eng.dynamics(kind, "CF", eng.upload(gs), w, eta, e0=e[0], cf_steps=150)
     scale = np.abs(exact[:, 1:]).max()
     assert np.abs(tab[:, 1:] - exact[:, 1:]).max() < 1e-6 * scale
+    # the reorthogonalised recursions give the same sum
+    tab_r, _ = eng.dynamics(kind, "CFReorth", eng.upload(gs), w, eta, e0=e[0], cf_steps=150)
+    assert np.abs(tab_r[:, 1:] - exact[:, 1:]).max() < 1e-6 * scale
     # Ritz mode with a long reorthogonalised run reaches the same exact sum
     gsd = eng.empty()
     eng.lanczos(eng.start_vector(3, "Uniform"), 200, 0.0, "All", gsd)

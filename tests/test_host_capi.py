@@ -152,3 +152,14 @@ def test_context_rejects_bad_models_before_touching_the_device():
     m.K[0] = float("nan")
     rc = _capi.lib().esc_ctx_create(ctypes.byref(m), 0, ctypes.byref(h))
     assert rc == -5
+
+
+@pytest.mark.parametrize("mode,code", [("Ritz", 0), ("CF", 1), ("CFReorth", 2)])
+def test_params_spectral_modes(tmp_path, mode, code):
+    p = esc.load_params(_write(tmp_path, BASE + f"\nSpectralMode={mode}\n"))
+    assert p["SpectralMode"] == code
+
+
+def test_params_spectral_mode_rejects_unknown(tmp_path):
+    with pytest.raises(esc.EscError, match="SpectralMode"):
+        esc.load_params(_write(tmp_path, BASE + "\nSpectralMode=Pade\n"))
