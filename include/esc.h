@@ -287,6 +287,34 @@ int esc_batch_state(esc_batch *b, int i, void *d_out, void *stream);
 int esc_batch_dynamics(esc_batch *b, int kind, int cf_steps, double eta, const double *omega, int nw,
                        int use_graph, double *tables, double *expect);
 
+/* ----------------------------------------------------------- gradients -- */
+/* Loss L = sum_q (S(w_q) - target_q)^2 of the spectrum
+ *   S(w) = Im <phi|(w + E0 - i eta - H)^-1|phi>,  phi = O|gs>,
+ * evaluated as the continued fraction of cf_steps Lanczos steps from phi,
+ * and its gradient with respect to theta = (K, J, Gamma, h): K moves all three
+ * Kitaev couplings together (direction (1,1,1) in Kx, Ky, Kz), h is the field
+ * strength along [111] (B = h (1,1,1)/sqrt 3).  O is a total current
+ * (ESC_OP_JSX..ESC_OP_JEY, whose couplings follow K) or a single-site
+ * operator (ESC_OP_SX..SZ at `site`).  The ground state is formed in FP64
+ * (gs_steps-step replayed Lanczos with one restart, then Newton residual
+ * corrections).  Gradient: adjoint of the continued fraction, of the Lanczos
+ * recursion (checkpoint > 0: its vectors checkpointed every `checkpoint`
+ * steps and recomputed per segment in the reverse sweep, memory 3 + 2
+ * cf_steps / checkpoint vectors; checkpoint = 0: all cf_steps + 1 vectors kept
+ * and reorthogonalised, which keeps long recursions on their exact-arithmetic
+ * course so that the gradient matches finite differences of the loss),
+ * of O's couplings, of E0 (Hellmann-Feynman) and of |gs> (one conjugate-
+ * gradient solve with H - E0 on the complement of |gs>).  Outputs: loss,
+ * spectrum (nw, may be NULL), grad[4], e0, de0[4] = <gs|dH/dtheta|gs> (may be
+ * NULL).  complex128 throughout.  No reference counterpart. */
+#define ESC_THETA_K     0
+#define ESC_THETA_J     1
+#define ESC_THETA_GAMMA 2
+#define ESC_THETA_H     3
+int esc_spectrum_grad(const esc_model *model, int op, int site, int gs_steps, int cf_steps, int checkpoint,
+                      double eta, const double *omega, int nw, const double *target, double *loss,
+                      double *spectrum, double *grad, double *e0, double *de0, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
