@@ -17,7 +17,7 @@ OPS = {"Sx": 0, "Sy": 1, "Sz": 2, "Jsx": 3, "Jsy": 4, "Jsz": 5, "Jex": 6, "Jey":
 SITE_TOTAL = -1
 LANC_MODES = {"OnlyEgs": 0, "GS": 1, "All": 2, "Plain": 3}
 START_KINDS = {"Uniform": 0, "Gaussian": 1}
-DYN_KINDS = {"JsJs": 0, "JeJe": 1}
+DYN_KINDS = {"JsJs": 0, "JeJe": 1, "LocalSzSz": 2}
 SPEC_METHODS = {"Ritz": 0, "CF": 1, "CFReorth": 2}
 
 # every entry point declared in include/esc.h
@@ -64,7 +64,7 @@ class EscParams(ctypes.Structure):
         ("J", ctypes.c_double), ("Gamma", ctypes.c_double), ("Eta", ctypes.c_double),
         ("OmegaMin", ctypes.c_double), ("OmegaMax", ctypes.c_double),
         ("OmegaPoints", ctypes.c_int32), ("Precision", ctypes.c_int32), ("SpectralMode", ctypes.c_int32),
-        ("StartVector", ctypes.c_int32), ("CFSteps", ctypes.c_int32),
+        ("StartVector", ctypes.c_int32), ("CFSteps", ctypes.c_int32), ("LocalSzSz", ctypes.c_int32),
     ]
 
 

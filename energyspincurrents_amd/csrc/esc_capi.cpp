@@ -1053,6 +1053,134 @@ void write_spectrum_table(bool js, int nc, const std::vector<cplx> &ex,
     }
 }
 
+
+// LocalDynSzSz (src/dynamics.h:64-217): per site i the spin components
+// S^a_i (a = x, y, z) and the rotated S+e3_i, S-e3_i (src/Observables.h:
+// 539-589, the reference's six-digit coefficients) applied to |gs>; columns
+// Im of I_xx, I_yy, I_zz, I_xy, I_xz, I_yz summed over sites, each site with
+// its elastic term conj(<S^a_i>) <S^b_i> / (w - i eta), then the two rotated
+// channels with <S-e3_i><S+e3_i> subtracted as written.  Ritz mode sums over
+// the kept Lanczos basis; CF mode runs 11 continued fractions per site.
+const cplx kSpe3[3] = {cplx(-0.408248, 0.707107), cplx(-0.408248, -0.707107), cplx(0.816497, 0.0)};
+const cplx kSme3[3] = {cplx(-0.408248, -0.707107), cplx(-0.408248, 0.707107), cplx(0.816497, 0.0)};
+
+void local_szsz_spectra(esc_ctx *ctx, int method, bool reortho, const void *d_gs, double e0, int cf_steps,
+                        double eta, const double *omega, int nw, double *table, double *expect, hipStream_t s) {
+    const int N = ctx->n_sites;
+    const bool cf = method == ESC_SPEC_CF;
+    const bool cf_in_vecs = cf && !ctx->basis_kept;
+    if (cf_in_vecs) ctx->ensure_vecs(3);
+    ctx->ensure_aux(cf && !cf_in_vecs ? 4 : 1);
+    ctx->ensure_partials(2 * esc::kRedBlocks);
+    ctx->ensure_result(8);
+    if (!ctx->d_combo && hipMalloc(&ctx->d_combo, sizeof(esc::CurrentTable)) != hipSuccess)
+        throw EscError(ESC_ERR_ALLOC, "table allocation failed");
+    // 8 channels: xx yy zz xy xz yz pp mm
+    std::vector<std::vector<cplx>> acc(8, std::vector<cplx>(nw, cplx(0.0, 0.0)));
+    cplx sum_ex[3] = {0.0, 0.0, 0.0};
+    void *work[3] = {cf_in_vecs ? ctx->vec(0) : ctx->aux(1), cf_in_vecs ? ctx->vec(1) : ctx->aux(2),
+                     cf_in_vecs ? ctx->vec(2) : ctx->aux(3)};
+    std::vector<double> al(cf_steps + 2), be(cf_steps + 3);
+    for (int si = 0; si < N; ++si) {
+        esc::CurrentTable comp[3];
+        for (int c = 0; c < 3; ++c) host_current_table(ctx, ESC_OP_SX + c, si, &comp[c]);
+        auto apply = [&](const cplx *coef) {   // aux(0) = (sum_c coef_c S^c_i)|gs>
+            esc::CurrentTable t;
+            std::memset(&t, 0, sizeof(t));
+            for (int c = 0; c < 3; ++c)
+                if (coef[c] != cplx(0.0, 0.0)) append_scaled(t, comp[c], coef[c].real(), coef[c].imag());
+            hip_check(hipMemcpy(ctx->d_combo, &t, sizeof(t), hipMemcpyHostToDevice), "hipMemcpy");
+            hip_check(ctx->c64() ? esc::launch_current_c64(ctx->d_combo, d_gs, ctx->aux(0), ctx->dim, s)
+                                 : esc::launch_current_c128(ctx->d_combo, d_gs, ctx->aux(0), ctx->dim, s),
+                      "current launch");
+        };
+        const cplx unit[3][3] = {{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
+        cplx ex[3];
+        std::vector<std::vector<cplx>> O(3);
+        for (int a = 0; a < 3; ++a) {
+            apply(unit[a]);
+            double d[2];
+            device_dot(ctx, d_gs, ctx->aux(0), d, s);
+            ex[a] = cplx(d[0], d[1]);
+            sum_ex[a] += ex[a];
+            if (!cf) {   // O(n, a) = sum_m Z(m, n) <v_m|S^a_i gs>
+                const int n = ctx->basis_n;
+                ctx->ensure_partials((size_t)2 * n * 256);
+                ctx->ensure_result((size_t)2 * n + 8);
+                hip_check(esc::launch_multidot(ctx->c64(), ctx->vec(0), n, ctx->aux(0), ctx->dim, ctx->d_partials,
+                                               ctx->d_result, s), "multidot");
+                std::vector<double> tv(2 * (size_t)n);
+                sync_get(s, tv.data(), ctx->d_result, sizeof(double) * 2 * n);
+                O[a].assign(n, cplx(0.0, 0.0));
+                for (int k = 0; k < n; ++k) {
+                    cplx x(0.0, 0.0);
+                    for (int m = 0; m < n; ++m) x += ctx->Z[(size_t)k * n + m] * cplx(tv[2 * m], tv[2 * m + 1]);
+                    O[a][k] = x;
+                }
+            }
+        }
+        cplx exp_ = 0.0, exm = 0.0;
+        for (int c = 0; c < 3; ++c) { exp_ += kSpe3[c] * ex[c]; exm += kSme3[c] * ex[c]; }
+        const int pr[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+        std::vector<std::vector<cplx>> F(8, std::vector<cplx>(nw));
+        if (!cf) {
+            const int n = ctx->basis_n;
+            std::vector<cplx> Op(n), Om(n);
+            for (int k = 0; k < n; ++k) {
+                Op[k] = kSpe3[0] * O[0][k] + kSpe3[1] * O[1][k] + kSpe3[2] * O[2][k];
+                Om[k] = kSme3[0] * O[0][k] + kSme3[1] * O[1][k] + kSme3[2] * O[2][k];
+            }
+            const double Eg = ctx->evals[0];
+            for (int q = 0; q < nw; ++q) {
+                cplx v[8] = {};
+                for (int k = 0; k < n; ++k) {
+                    const cplx den(omega[q] - (ctx->evals[k] - Eg), -eta);
+                    for (int c = 0; c < 6; ++c) v[c] += std::conj(O[pr[c][0]][k]) * O[pr[c][1]][k] / den;
+                    v[6] += std::norm(Op[k]) / den;
+                    v[7] += std::norm(Om[k]) / den;
+                }
+                for (int c = 0; c < 8; ++c) F[c][q] = v[c];
+            }
+        } else {
+            auto run = [&](const cplx *coef) {
+                apply(coef);
+                int it = 0;
+                double n2 = 0.0;
+                cf_run(ctx, ctx->aux(0), cf_steps, work, al.data(), be.data(), &it, &n2, s, reortho);
+                return cf_eval(it, al.data(), be.data(), n2, e0, eta, omega, nw);
+            };
+            for (int a = 0; a < 3; ++a) F[a] = run(unit[a]);
+            F[6] = run(kSpe3);
+            F[7] = run(kSme3);
+            for (int c = 3; c < 6; ++c) {   // cross channels by polarisation
+                const int a = pr[c][0], b = pr[c][1];
+                cplx c1[3] = {0.0, 0.0, 0.0}, ci[3] = {0.0, 0.0, 0.0};
+                c1[a] = 1.0; c1[b] = 1.0;
+                ci[a] = 1.0; ci[b] = cplx(0.0, 1.0);
+                const std::vector<cplx> S1 = run(c1), Si = run(ci);
+                for (int q = 0; q < nw; ++q) {
+                    const cplx u = S1[q] - F[a][q] - F[b][q];
+                    const cplx v = Si[q] - F[a][q] - F[b][q];
+                    F[c][q] = 0.5 * (u - cplx(0.0, 1.0) * v);
+                }
+            }
+        }
+        for (int q = 0; q < nw; ++q) {
+            const cplx dn1(omega[q], -eta);
+            for (int c = 0; c < 6; ++c) acc[c][q] += F[c][q] - std::conj(ex[pr[c][0]]) * ex[pr[c][1]] / dn1;
+            acc[6][q] += F[6][q] - exm * exp_ / dn1;
+            acc[7][q] += F[7][q] - exp_ * exm / dn1;
+        }
+    }
+    for (int q = 0; q < nw; ++q) {
+        double *row = table + (size_t)q * 9;
+        row[0] = omega[q];
+        for (int c = 0; c < 8; ++c) row[1 + c] = acc[c][q].imag();
+    }
+    if (expect)
+        for (int a = 0; a < 3; ++a) { expect[2 * a] = sum_ex[a].real(); expect[2 * a + 1] = sum_ex[a].imag(); }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1452,7 +1580,8 @@ int esc_dynamics(esc_ctx *ctx, int kind, int method, const void *d_gs, double e0
         check_ptr(d_gs, "d_gs");
         check_ptr(omega, "omega");
         check_ptr(table, "table");
-        if (kind != ESC_DYN_JSJS && kind != ESC_DYN_JEJE) throw EscError(ESC_ERR_ARG, "unknown dynamics kind");
+        if (kind != ESC_DYN_JSJS && kind != ESC_DYN_JEJE && kind != ESC_DYN_LOCALSZSZ)
+            throw EscError(ESC_ERR_ARG, "unknown dynamics kind");
         if (method != ESC_SPEC_RITZ && method != ESC_SPEC_CF && method != ESC_SPEC_CF_REORTH)
             throw EscError(ESC_ERR_ARG, "unknown spectral method");
         const bool reortho = method == ESC_SPEC_CF_REORTH;
@@ -1463,6 +1592,10 @@ int esc_dynamics(esc_ctx *ctx, int kind, int method, const void *d_gs, double e0
         if (method == ESC_SPEC_CF && (cf_steps < 1 || cf_steps > (reortho ? 4096 : 100000)))
             throw EscError(ESC_ERR_ARG, "cf_steps out of range");
         hipStream_t s = (hipStream_t)stream;
+        if (kind == ESC_DYN_LOCALSZSZ) {
+            local_szsz_spectra(ctx, method, reortho, d_gs, e0, cf_steps, eta, omega, nw, table, expect, s);
+            return;
+        }
         const bool js = kind == ESC_DYN_JSJS;
         const std::vector<int> ops = js ? std::vector<int>{ESC_OP_JSX, ESC_OP_JSY, ESC_OP_JSZ}
                                         : std::vector<int>{ESC_OP_JEX, ESC_OP_JEY};

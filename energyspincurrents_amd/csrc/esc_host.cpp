@@ -171,9 +171,9 @@ extern "C" int esc_params_load(const char *path, esc_params *p) {
         // (src/ParametersEngine.h:113-144)
         const bool ritz = p->SpectralMode == ESC_SPEC_RITZ;
         const std::string lt = p->LancType;
-        const char *dyn_keys[3] = {"LocalSkw", "SpinCurrDynamics", "EnergyCurrDynamics"};
-        int32_t *dyn_vals[3] = {&p->LocalSkw, &p->SpinCurrDynamics, &p->EnergyCurrDynamics};
-        for (int k = 0; k < 3; ++k) {
+        const char *dyn_keys[4] = {"LocalSkw", "SpinCurrDynamics", "EnergyCurrDynamics", "LocalSzSz"};
+        int32_t *dyn_vals[4] = {&p->LocalSkw, &p->SpinCurrDynamics, &p->EnergyCurrDynamics, &p->LocalSzSz};
+        for (int k = 0; k < 4; ++k) {
             *dyn_vals[k] = find_key(lines, dyn_keys[k], v) ? (v != 0.0) : 0;
             if (*dyn_vals[k] && ritz && lt != "All")
                 throw HostError(ESC_ERR_ARG, "You need to set LancType=All for Dynamical calculations --- you dummy!");

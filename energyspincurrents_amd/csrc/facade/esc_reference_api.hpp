@@ -183,7 +183,7 @@ struct ConstVariables {
         Model = p.Model; Solver = p.Solver; LancType = p.LancType;
         J = p.J; Gamma = p.Gamma; Eta = p.Eta; OmegaMin = p.OmegaMin; OmegaMax = p.OmegaMax;
         OmegaPoints = p.OmegaPoints; Precision = p.Precision; SpectralMode = p.SpectralMode;
-        StartVector = p.StartVector; CFSteps = p.CFSteps;
+        StartVector = p.StartVector; CFSteps = p.CFSteps; LocalSzSz = p.LocalSzSz;
         SpecialSite = NumberofSites / 2;
         generator_.seed((uint64_t)RandomSeed);
         if (LancType == "OnlyEgs") std::cout << "WARNING:: Reortho will be turned off for LancType=OnlyEgs (set by default) " << std::endl;
@@ -197,6 +197,7 @@ struct ConstVariables {
     // extensions
     double J, Gamma, Eta, OmegaMin, OmegaMax;
     int OmegaPoints, Precision, SpectralMode, StartVector, CFSteps;
+    bool LocalSzSz;
 
     // uniform [0,1) draw: double(g()) * 2^-64 (src/ParametersEngine.h:152-154)
     double grnd() { return (double)generator_() * (1.0 / 18446744073709551616.0); }
@@ -460,15 +461,18 @@ private:
 // --------------------------------------------------------------------------
 // StartDynamics (src/dynamics.h:43-60): spectrum table -> *_outSpectrum.real
 // --------------------------------------------------------------------------
-inline void write_spectrum_file(const std::string &path, const std::vector<double> &table, int ncols, int nw) {
+inline void write_spectrum_file(const std::string &path, const std::vector<double> &table, int ncols, int nw,
+                                bool tab_after_last = false) {
     // byte format of src/dynamics.h:325-328,432-435: default ostream precision,
-    // " \t " separators, every row terminated by " \n "
+    // " \t " separators, every row terminated by " \n "; LocalDynSzSz also
+    // writes " \t " after the last value (:187-192)
     std::ofstream f(path);
     for (int q = 0; q < nw; ++q) {
         const double *row = &table[(size_t)q * (1 + ncols)];
         f << row[0] << " \t ";
         for (int c = 1; c < ncols; ++c) f << row[c] << " \t ";
         f << row[ncols];
+        if (tab_after_last) f << " \t ";
         f << " \n ";
     }
 }
@@ -479,13 +483,14 @@ inline void StartDynamics(std::string &dynstr, ConstVariables &variables, Lattic
     std::string file;
     if (dynstr == "Spin Current") { kind = ESC_DYN_JSJS; file = "JsJs_outSpectrum.real"; }
     else if (dynstr == "Energy Current") { kind = ESC_DYN_JEJE; file = "JeJe_outSpectrum.real"; }
+    else if (dynstr == "LocalSzSz") { kind = ESC_DYN_LOCALSZSZ; file = "2SSLocal_outSpectrum.real"; }
     else {
         std::cerr << "dynstr=" << dynstr << "\n";
         throw std::string("Unknown dynstr parameter \n");
     }
     std::cout << " --> Startin Dynamics: " << dynstr << std::endl;
     const std::vector<double> w = variables.omega_grid();
-    const int ncols = kind == ESC_DYN_JSJS ? 14 : 5;
+    const int ncols = kind == ESC_DYN_JSJS ? 14 : (kind == ESC_DYN_JEJE ? 5 : 8);
     std::vector<double> table(w.size() * (1 + ncols));
     const int method = variables.SpectralMode;
     if (method == ESC_SPEC_RITZ && !DLancGS.basis_kept)
@@ -493,7 +498,7 @@ inline void StartDynamics(std::string &dynstr, ConstVariables &variables, Lattic
     const double e0 = DLancGS.TriDeval.empty() ? 0.0 : DLancGS.TriDeval[0];
     check(esc_dynamics(Hamil.context()->get(), kind, method, Psi.data(), e0, variables.CFSteps, variables.Eta,
                        w.data(), (int)w.size(), table.data(), nullptr, nullptr));
-    write_spectrum_file(file, table, ncols, (int)w.size());
+    write_spectrum_file(file, table, ncols, (int)w.size(), kind == ESC_DYN_LOCALSZSZ);
     std::cout << std::setprecision(8);
     for (size_t q = 0; q < w.size(); q += 200) {
         std::cout << table[q * (1 + ncols)];

@@ -3,7 +3,8 @@
 // lattice, basis, Hamiltonian, Lanczos (or ED), local measurements, then the
 // spin-current (LocalSkw=1) and energy-current (EnergyCurrDynamics=1) spectra,
 // written to JsJs_outSpectrum.real / JeJe_outSpectrum.real in the working
-// directory.  Everything below the class API runs on the GPU through libesc.so.
+// directory, and with the extension key LocalSzSz=1 the local spin spectra of
+// LocalDynSzSz in 2SSLocal_outSpectrum.real.  Everything below the class API runs on the GPU through libesc.so.
 #include "esc_reference_api.hpp"
 
 #include <iostream>
@@ -52,6 +53,13 @@ int main(int argc, char *argv[]) {
             std::cout << " ==================================================================== " << std::endl;
             std::string str1 = "Energy Current";
             StartDynamics(str1, Parameters, Lat, Basis, Ham, Lanczos, Psi);
+        }
+        if (Parameters.LocalSzSz) {   // extension key: src/dynamics.h:64-217, unreachable from main.cpp
+            std::cout << " ==================================================================== " << std::endl;
+            std::cout << "                  Local SzSz Dynamical Correlations                    " << std::endl;
+            std::cout << " ==================================================================== " << std::endl;
+            std::string str2 = "LocalSzSz";
+            StartDynamics(str2, Parameters, Lat, Basis, Ham, Lanczos, Psi);
         }
     } catch (const std::exception &e) {
         std::cerr << "LanczosCond: " << e.what() << std::endl;

@@ -257,7 +257,7 @@ class Engine:
         """StartDynamics -> DynJsJs / DynJeJe without the file (src/dynamics.h:43-60,222-460).
         Returns (table, expectations)."""
         omega = np.ascontiguousarray(omega, np.float64)
-        ncols = 14 if kind == "JsJs" else 5
+        ncols = {"JsJs": 14, "JeJe": 5, "LocalSzSz": 8}[kind]
         table = np.zeros((len(omega), 1 + ncols))
         expect = np.zeros(2 * 4)
         check(lib().esc_dynamics(self._h, DYN_KINDS[kind], SPEC_METHODS[method], _dev_ptr(gs), float(e0),

@@ -183,6 +183,8 @@ int esc_lanczos_cf(esc_ctx *ctx, const void *d_phi, int steps, double *alpha, do
 /* ------------------------------------------------------------- dynamics -- */
 #define ESC_DYN_JSJS 0   /* DynJsJs: 14 value columns (Re, Im) xx yy zz xy xz yz tt */
 #define ESC_DYN_JEJE 1   /* DynJeJe: 5 value columns Im xx yy xy yx tt */
+#define ESC_DYN_LOCALSZSZ 2  /* LocalDynSzSz: 8 value columns Im xx yy zz xy xz yz, S+e3 S+e3, S-e3 S-e3
+                              * (src/dynamics.h:64-217); expect = sum over sites of <S^x>, <S^y>, <S^z> */
 #define ESC_SPEC_RITZ 0  /* Lehmann sum over the Ritz states of the last ESC_LANC_ALL run */
 #define ESC_SPEC_CF   1  /* continued fractions from J|gs> (cross terms by polarisation) */
 #define ESC_SPEC_CF_REORTH 2  /* the same with full reorthogonalisation of each run's Lanczos
@@ -218,8 +220,8 @@ int esc_connectors(int n_sites, const int32_t *neigh, const double *K, int32_t *
  * ConstVariables::Initialize / matchstring / matchstring2
  * (src/ParametersEngine.h:53-229) including its defaults and checks.
  * Extension keys (optional): J, Gamma, Eta, OmegaMin, OmegaMax, OmegaPoints,
- * Precision (c128|c64), SpectralMode (Ritz|CF), StartVector (Uniform|Gaussian),
- * CFSteps. */
+ * Precision (c128|c64), SpectralMode (Ritz|CF|CFReorth), StartVector
+ * (Uniform|Gaussian), CFSteps, LocalSzSz (0|1). */
 typedef struct {
     int32_t Lx, Ly, NumberofSites, RandomSeed, LanczosSteps, Threads;
     int32_t IsPeriodicX, IsPeriodicY, LocalSkw, SpinCurrDynamics, EnergyCurrDynamics;
@@ -232,6 +234,7 @@ typedef struct {
     int32_t SpectralMode;     /* ESC_SPEC_RITZ / ESC_SPEC_CF / ESC_SPEC_CF_REORTH */
     int32_t StartVector;      /* ESC_START_* */
     int32_t CFSteps;
+    int32_t LocalSzSz;        /* 1: LocalDynSzSz -> 2SSLocal_outSpectrum.real (src/dynamics.h:64-217) */
 } esc_params;
 int esc_params_load(const char *path, esc_params *out);
 

@@ -467,6 +467,39 @@ def jsjs_ref(m: Model, evals, states, gs, omega=REF_OMEGA, eta=REF_ETA):
     return np.column_stack([np.asarray(omega)] + cols)
 
 
+# rotated spin-flip operators of src/Observables.h:539-589, with the
+# reference's own six-digit coefficients
+SPE3 = (complex(-0.408248, 0.707107), complex(-0.408248, -0.707107), 0.816497)
+SME3 = (complex(-0.408248, -0.707107), complex(-0.408248, 0.707107), 0.816497)
+
+
+def local_szsz_ref(m: Model, evals, states, gs, omega=REF_OMEGA, eta=REF_ETA):
+    """LocalDynSzSz columns: omega, Im of I_xx, I_yy, I_zz, I_xy, I_xz, I_yz
+    summed over sites, then Im of the S+e3 S+e3 and S-e3 S-e3 channels
+    (src/dynamics.h:64-217).  Per site: overlaps <n|S^a_i|gs> with the elastic
+    term conj(<S^a_i>) <S^b_i> / (w - i eta) subtracted; the two rotated
+    channels subtract <S-e3_i><S+e3_i> (as written, without a conjugate)."""
+    N = m.N
+    w = np.asarray(omega, np.float64)
+    E0 = evals[0]
+    den = (w[:, None] - (np.asarray(evals)[None, :] - E0)) - 1j * eta
+    dn1 = w - 1j * eta
+    acc = {k: np.zeros(len(w), complex) for k in ("xx", "yy", "zz", "xy", "xz", "yz", "pp", "mm")}
+    for si in range(N):
+        sx, sy, sz = (apply_site_op(N, n, si, gs) for n in ("Sx", "Sy", "Sz"))
+        sp = SPE3[0] * sx + SPE3[1] * sy + SPE3[2] * sz
+        sm = SME3[0] * sx + SME3[1] * sy + SME3[2] * sz
+        vec = {"x": sx, "y": sy, "z": sz, "p": sp, "m": sm}
+        ex = {a: np.vdot(gs, v) for a, v in vec.items()}
+        O = {a: states.conj() @ v for a, v in vec.items()}
+        for a, b in (("x", "x"), ("y", "y"), ("z", "z"), ("x", "y"), ("x", "z"), ("y", "z")):
+            acc[a + b] += (np.conj(O[a]) * O[b])[None, :].__truediv__(den).sum(axis=1) - np.conj(ex[a]) * ex[b] / dn1
+        acc["pp"] += (np.abs(O["p"]) ** 2)[None, :].__truediv__(den).sum(axis=1) - ex["m"] * ex["p"] / dn1
+        acc["mm"] += (np.abs(O["m"]) ** 2)[None, :].__truediv__(den).sum(axis=1) - ex["p"] * ex["m"] / dn1
+    cols = [acc[k].imag for k in ("xx", "yy", "zz", "xy", "xz", "yz", "pp", "mm")]
+    return np.column_stack([w] + cols)
+
+
 def format_g6(x):
     """C++ ostream default formatting (precision 6, %g-like)."""
     s = "%g" % x

@@ -68,3 +68,23 @@ def test_cli_missing_key_fails_cleanly(tmp_path):
     inp.write_text(text)
     r = subprocess.run([CLI, str(inp)], cwd=tmp_path, capture_output=True, text=True, timeout=60)
     assert r.returncode != 0 and "Kzz= argument is missing" in r.stderr
+
+
+def test_cli_local_szsz_matches_committed_file(tmp_path):
+    """Extension key LocalSzSz=1 runs LocalDynSzSz (src/dynamics.h:64-217) and
+    writes 2SSLocal_outSpectrum.real: the reference's committed file (inputs:
+    Midinput.inp with B=0) in values and byte layout, including the tab after
+    the last column of each row."""
+    text = open(os.path.join(GOLDEN, "Midinput.inp")).read()
+    text = text.replace("Bxx=0.3", "Bxx=0.0").replace("Byy=0.3", "Byy=0.0").replace("Bzz=0.3", "Bzz=0.0")
+    text = text.replace("EnergyCurrDynamics=1", "EnergyCurrDynamics=0") + "\nLocalSzSz=1\n"
+    _run(tmp_path, text)
+    out = tmp_path / "2SSLocal_outSpectrum.real"
+    _check_format(out)
+    raw = open(out, "rb").read()
+    ref_raw = open(os.path.join(GOLDEN, "2SSLocal_outSpectrum.real"), "rb").read()
+    assert raw.split(b" \n ")[0].endswith(b" \t ") and ref_raw.split(b" \n ")[0].endswith(b" \t ")
+    tab = O.read_real(str(out))
+    ref = O.read_real(os.path.join(GOLDEN, "2SSLocal_outSpectrum.real"))
+    assert tab.shape == ref.shape == (1001, 9)
+    assert _l1(tab, ref).max() < 2e-3

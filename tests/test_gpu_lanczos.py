@@ -92,7 +92,8 @@ def _l1(tab, ref):
     return np.abs(tab[:, 1:] - ref[:, 1:]).sum(0) / np.abs(ref[:, 1:]).sum(0)
 
 
-@pytest.mark.parametrize("kind,B,tol", [("JsJs", (0.0, 0.0, 0.0), 5e-3), ("JeJe", (0.3, 0.3, 0.3), 0.06)])
+@pytest.mark.parametrize("kind,B,tol", [("JsJs", (0.0, 0.0, 0.0), 5e-3), ("JeJe", (0.3, 0.3, 0.3), 0.06),
+                                        ("LocalSzSz", (0.0, 0.0, 0.0), 2e-3)])
 def test_ritz_spectra_reproduce_committed_files(kind, B, tol):
     """Reference pipeline on the device: Midinput.inp (B as listed), 100 steps,
     seed 3, reference grid and eta; compared with the reference's own files."""
@@ -102,7 +103,8 @@ def test_ritz_spectra_reproduce_committed_files(kind, B, tol):
     gs = eng.empty()
     eng.lanczos(eng.start_vector(3, "Uniform"), 100, 0.0, "All", gs)
     tab, _ = eng.dynamics(kind, "Ritz", gs, esc.reference_omega(), 0.005)
-    ref = O.read_real(os.path.join(GOLDEN, f"{kind}_outSpectrum.real"))
+    name = "2SSLocal" if kind == "LocalSzSz" else kind
+    ref = O.read_real(os.path.join(GOLDEN, f"{name}_outSpectrum.real"))
     assert _l1(tab, ref).max() < tol
 
 
@@ -112,7 +114,7 @@ def _exact(m):
     return e, U
 
 
-@pytest.mark.parametrize("kind", ["JeJe", "JsJs"])
+@pytest.mark.parametrize("kind", ["JeJe", "JsJs", "LocalSzSz"])
 def test_cf_spectra_equal_exact_lehmann(kind):
     m = O.Model(2, 4, K=(-1.0, -1.0, -1.0), J=0.1, G=0.2, B=O.field111(0.2))
     e, U = _exact(m)
@@ -120,7 +122,8 @@ def test_cf_spectra_equal_exact_lehmann(kind):
     w = np.linspace(0.0, 6.0, 400)
     eta = 0.1
     states = U.T.copy()
-    exact = (O.jeje_ref if kind == "JeJe" else O.jsjs_ref)(m, e, states, gs, omega=w, eta=eta)
+    ref_fn = {"JeJe": O.jeje_ref, "JsJs": O.jsjs_ref, "LocalSzSz": O.local_szsz_ref}[kind]
+    exact = ref_fn(m, e, states, gs, omega=w, eta=eta)
     eng = engine_for(m)
     tab, ex = eng.dynamics(kind, "CF", eng.upload(gs), w, eta, e0=e[0], cf_steps=150)
     scale = np.abs(exact[:, 1:]).max()
@@ -129,6 +132,11 @@ def test_cf_spectra_equal_exact_lehmann(kind):
     tab_r, _ = eng.dynamics(kind, "CFReorth", eng.upload(gs), w, eta, e0=e[0], cf_steps=150)
     assert np.abs(tab_r[:, 1:] - exact[:, 1:]).max() < 1e-6 * scale
     # Ritz mode with a long reorthogonalised run reaches the same exact sum
+    # (not for the local operators: S^a_i|gs> has weight on all 256 states,
+    # more than 200 Ritz states hold; the Ritz path of LocalSzSz is checked
+    # against the reference's file instead)
+    if kind == "LocalSzSz":
+        return
     gsd = eng.empty()
     eng.lanczos(eng.start_vector(3, "Uniform"), 200, 0.0, "All", gsd)
     tab2, _ = eng.dynamics(kind, "Ritz", gsd, w, eta)

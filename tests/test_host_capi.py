@@ -163,3 +163,9 @@ def test_params_spectral_modes(tmp_path, mode, code):
 def test_params_spectral_mode_rejects_unknown(tmp_path):
     with pytest.raises(esc.EscError, match="SpectralMode"):
         esc.load_params(_write(tmp_path, BASE + "\nSpectralMode=Pade\n"))
+
+
+def test_params_local_szsz_key(tmp_path):
+    p = esc.load_params(_write(tmp_path, BASE + "\nLocalSzSz=1\n"))
+    assert p["LocalSzSz"] == 1
+    assert esc.load_params(_write(tmp_path, BASE))["LocalSzSz"] == 0

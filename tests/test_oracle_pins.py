@@ -314,3 +314,43 @@ def test_sparse_ground_pair_equals_dense():
         e0, g = O.ground_pair(m)
         assert abs(e0 - w[0]) < 1e-11
         assert np.linalg.norm(H @ g - e0 * g) < 1e-8
+
+
+def test_golden_local_szsz_reproduced():
+    """2SSLocal_outSpectrum.real (LocalDynSzSz, src/dynamics.h:64-217) =
+    Midinput.inp with B = 0, the same run as JsJs (measured 3e-4 L1 per
+    column); the committed field B = 0.3 does not reproduce it."""
+    ref = O.read_real(os.path.join(GOLDEN, "2SSLocal_outSpectrum.real"))
+    m, res, st = _ref_run((0.0, 0.0, 0.0))
+    tab = O.local_szsz_ref(m, res.evals, st, res.gs)
+    assert [O.format_g6(w) for w in tab[:, 0]] == [O.format_g6(w) for w in ref[:, 0]]
+    assert _l1(tab, ref).max() < 2e-3
+    m3, res3, st3 = _ref_run((0.3, 0.3, 0.3))
+    assert _l1(O.local_szsz_ref(m3, res3.evals, st3, res3.gs), ref).max() > 0.5
+
+
+def test_local_szsz_lehmann_equals_resolvent():
+    """With all eigenstates, the Lehmann sums of local_szsz_ref equal the
+    resolvent forms sum_i <A_i gs|(w + E0 - i eta - H)^-1|B_i gs> minus the
+    elastic terms (dense solves at N = 8), for xx, xy and the rotated channel."""
+    m = O.Model(2, 4, K=(-1.0, -1.0, -1.0), J=0.1, G=0.2, B=O.field111(0.2))
+    H = O.hamiltonian_dense_kron(m)
+    e, U = np.linalg.eigh(H)
+    gs = U[:, 0]
+    w = np.linspace(0.0, 3.0, 7)
+    eta = 0.1
+    tab = O.local_szsz_ref(m, e, U.T.copy(), gs, omega=w, eta=eta)
+    xx, xy, pp = np.zeros(len(w), complex), np.zeros(len(w), complex), np.zeros(len(w), complex)
+    for i in range(m.N):
+        sx, sy, sz = (O.apply_site_op(m.N, n, i, gs) for n in ("Sx", "Sy", "Sz"))
+        sp = O.SPE3[0] * sx + O.SPE3[1] * sy + O.SPE3[2] * sz
+        sm = O.SME3[0] * sx + O.SME3[1] * sy + O.SME3[2] * sz
+        for q, om in enumerate(w):
+            R = np.linalg.inv((om + e[0] - 1j * eta) * np.eye(m.dim) - H)
+            dn1 = om - 1j * eta
+            xx[q] += np.vdot(sx, R @ sx) - abs(np.vdot(gs, sx)) ** 2 / dn1
+            xy[q] += np.vdot(sx, R @ sy) - np.conj(np.vdot(gs, sx)) * np.vdot(gs, sy) / dn1
+            pp[q] += np.vdot(sp, R @ sp) - np.vdot(gs, sm) * np.vdot(gs, sp) / dn1
+    assert np.abs(tab[:, 1] - xx.imag).max() < 1e-10
+    assert np.abs(tab[:, 4] - xy.imag).max() < 1e-10
+    assert np.abs(tab[:, 7] - pp.imag).max() < 1e-10
