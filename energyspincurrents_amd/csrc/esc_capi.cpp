@@ -1517,6 +1517,46 @@ int esc_ground_state(esc_ctx *ctx, const void *d_v0, int steps, double eps, int 
     });
 }
 
+int esc_two_point(esc_ctx *ctx, int op_a, int op_b, const void *d_gs, double *corr, void *stream) {
+    return guarded([&] {
+        check_ctx(ctx);
+        check_ptr(d_gs, "d_gs");
+        check_ptr(corr, "corr");
+        if (op_a < ESC_OP_SX || op_a > ESC_OP_JEY || op_b < ESC_OP_SX || op_b > ESC_OP_JEY)
+            throw EscError(ESC_ERR_ARG, "unknown operator");
+        hipStream_t s = (hipStream_t)stream;
+        const int N = ctx->n_sites;
+        // a_i = A_i|gs> kept for all i (the context's vectors), b_j = B_j|gs>
+        // one at a time; Corr(i, j) = <gs|A_i B_j|gs> = <a_i|b_j> (every
+        // operator here is Hermitian: Pauli strings with real weights)
+        ctx->ensure_vecs((size_t)N);
+        ctx->basis_kept = false;
+        ctx->ensure_aux(1);
+        ctx->ensure_partials((size_t)2 * N * 256 + 2 * esc::kRedBlocks);
+        ctx->ensure_result((size_t)2 * N + 8);
+        auto apply = [&](int op, int site, void *out) {
+            esc::CurrentTable *t = current_table(ctx, op, site);
+            hip_check(ctx->c64() ? esc::launch_current_c64(t, d_gs, out, ctx->dim, s)
+                                 : esc::launch_current_c128(t, d_gs, out, ctx->dim, s), "operator");
+        };
+        for (int i = 0; i < N; ++i) apply(op_a, i, ctx->vec((size_t)i));
+        std::vector<double> tv(2 * (size_t)N);
+        for (int j = 0; j < N; ++j) {
+            apply(op_b, j, ctx->aux(0));
+            hip_check(esc::launch_multidot(ctx->c64(), ctx->vec(0), N, ctx->aux(0), ctx->dim, ctx->d_partials,
+                                           ctx->d_result, s), "multidot");
+            sync_get(s, tv.data(), ctx->d_result, sizeof(double) * 2 * N);
+            // the reference fills si >= sj and mirrors it (src/Observables.h:142-151)
+            for (int i = j; i < N; ++i) {
+                for (int q = 0; q < 2; ++q) {
+                    corr[2 * ((size_t)i * N + j) + q] = tv[2 * i + q];
+                    corr[2 * ((size_t)j * N + i) + q] = tv[2 * i + q];
+                }
+            }
+        }
+    });
+}
+
 int esc_ritz_overlaps(esc_ctx *ctx, const void *d_vec, int n_states, double *out, void *stream) {
     return guarded([&] {
         check_ctx(ctx);

@@ -446,6 +446,25 @@ public:
         }
     }
 
+    // src/Observables.h:132-155: Corr(si, sj) = <GS|A_si B_sj|GS> for si >= sj,
+    // mirrored; returned row-major as N x N
+    std::vector<dcomplex> TwoPointCorr(const std::string OpA, const std::string OpB, VectorXcd &GS_) {
+        std::cout << "Calculating " << OpA << "." << OpB << " correlations " << std::endl;
+        static const char *names[] = {"Sx", "Sy", "Sz", "Jsx", "Jsy", "Jsz", "Jex", "Jey"};
+        int a = -1, b = -1;
+        for (int op = 0; op < 8; ++op) {
+            if (OpA == names[op]) a = op;
+            if (OpB == names[op]) b = op;
+        }
+        if (a < 0 || b < 0) throw std::string("Unknown Operator label \n");
+        const int N = variables_.NumberofSites;
+        std::vector<double> raw(2 * (size_t)N * N);
+        check(esc_two_point(Hamil_.context()->get(), a, b, GS_.data(), raw.data(), nullptr));
+        std::vector<dcomplex> corr((size_t)N * N);
+        for (size_t k = 0; k < corr.size(); ++k) corr[k] = dcomplex(raw[2 * k], raw[2 * k + 1]);
+        return corr;
+    }
+
 private:
     VectorXcd apply(int op, int site, VectorXcd &in) {
         VectorXcd out(Hamil_.context());

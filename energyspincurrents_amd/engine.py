@@ -243,6 +243,14 @@ class Engine:
         check(lib().esc_ritz_vector(self._h, int(k), _dev_ptr(out), _stream(stream)))
         return out
 
+    def two_point(self, op_a: str, op_b: str, gs, stream=None):
+        """Observables::TwoPointCorr (src/Observables.h:132-155): N x N matrix
+        <gs|A_i B_j|gs>, filled for i >= j and mirrored."""
+        N = self.model.lattice.N
+        out = np.zeros(2 * N * N)
+        check(lib().esc_two_point(self._h, OPS[op_a], OPS[op_b], _dev_ptr(gs), _np_ptr(out), _stream(stream)))
+        return (out[0::2] + 1j * out[1::2]).reshape(N, N)
+
     def lanczos_cf(self, phi, steps, stream=None):
         a = np.zeros(steps + 1)
         b = np.zeros(steps + 2)

@@ -174,6 +174,13 @@ int esc_ritz_overlaps(esc_ctx *ctx, const void *d_vec, int n_states, double *out
 /* d_out = PsiAll[k] = sum_m Z(m,k) v_m (src/DynLanczosSolver.h:213-223). */
 int esc_ritz_vector(esc_ctx *ctx, int k, void *d_out, void *stream);
 
+/* Static two-point correlations Corr(i, j) = <gs|A_i B_j|gs> of site
+ * operators op_a, op_b (ESC_OP_SX..ESC_OP_JEY) over all site pairs, filled for
+ * i >= j and mirrored as Observables::TwoPointCorr does
+ * (src/Observables.h:132-155).  corr: N x N interleaved complex, row-major
+ * (corr[2(iN + j)]).  Uses N + 1 device vectors. */
+int esc_two_point(esc_ctx *ctx, int op_a, int op_b, const void *d_gs, double *corr, void *stream);
+
 /* Plain three-vector Lanczos (no reorthogonalisation, no basis) from
  * d_phi / |d_phi| for the continued-fraction spectra; norm2 = |d_phi|^2.
  * No reference counterpart (the reference uses Ritz-Lehmann sums). */

@@ -467,6 +467,26 @@ def jsjs_ref(m: Model, evals, states, gs, omega=REF_OMEGA, eta=REF_ETA):
     return np.column_stack([np.asarray(omega)] + cols)
 
 
+def apply_op(m: Model, name, site, v):
+    """Observables::ApplyOp for one site (src/Observables.h:24-67)."""
+    if name in ("Sx", "Sy", "Sz"):
+        return apply_site_op(m.N, name, site, v)
+    return apply_current_site(m, name, site, v)
+
+
+def two_point_ref(m: Model, op_a, op_b, gs):
+    """Observables::TwoPointCorr (src/Observables.h:132-155): Corr(si, sj) =
+    <gs|A_si B_sj|gs> for si >= sj, copied to (sj, si)."""
+    N = m.N
+    C = np.zeros((N, N), complex)
+    for sj in range(N):
+        out1 = apply_op(m, op_b, sj, gs)
+        for si in range(sj, N):
+            C[si, sj] = np.vdot(gs, apply_op(m, op_a, si, out1))
+            C[sj, si] = C[si, sj]
+    return C
+
+
 # rotated spin-flip operators of src/Observables.h:539-589, with the
 # reference's own six-digit coefficients
 SPE3 = (complex(-0.408248, 0.707107), complex(-0.408248, -0.707107), 0.816497)

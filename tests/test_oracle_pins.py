@@ -354,3 +354,18 @@ def test_local_szsz_lehmann_equals_resolvent():
     assert np.abs(tab[:, 1] - xx.imag).max() < 1e-10
     assert np.abs(tab[:, 4] - xy.imag).max() < 1e-10
     assert np.abs(tab[:, 7] - pp.imag).max() < 1e-10
+
+
+def test_two_point_ref_against_dense_operators():
+    """TwoPointCorr restatement: <gs|S^a_i S^b_j|gs> from dense Kronecker site
+    operators (i >= j, mirrored), and Sz.Sz on the diagonal = 1/4."""
+    m = O.Model(2, 4, K=(-1.0, -1.0, -1.0), J=0.1, G=0.2, B=O.field111(0.2))
+    H = O.hamiltonian_dense_kron(m)
+    gs = np.linalg.eigh(H)[1][:, 0]
+    C = O.two_point_ref(m, "Sx", "Sz", gs)
+    for i in range(m.N):
+        for j in range(i + 1):
+            d = np.vdot(gs, O.site_op_dense(m.N, "Sx", i) @ (O.site_op_dense(m.N, "Sz", j) @ gs))
+            assert abs(C[i, j] - d) < 1e-13 and C[j, i] == C[i, j]
+    Czz = O.two_point_ref(m, "Sz", "Sz", gs)
+    assert np.allclose(np.diag(Czz), 0.25, atol=1e-13)
