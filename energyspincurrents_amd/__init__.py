@@ -5,8 +5,8 @@ include/esc.h); this package is the Python binding of that ABI.
 """
 from ._capi import EscError, lib  # noqa: F401
 from .grad import SpectrumLoss, spectrum_loss  # noqa: F401
-from .engine import (Batch, Engine, Honeycomb, LanczosResult, SpinModel, cf_spectrum, field111,  # noqa: F401
+from .engine import (Batch, Chain, Engine, Honeycomb, LanczosResult, SpinModel, cf_spectrum, field111,  # noqa: F401
                      lehmann_spectrum, load_params, reference_omega, tridiag_eig)
 
-__all__ = ["SpectrumLoss", "spectrum_loss", "Batch", "Engine", "Honeycomb", "SpinModel", "LanczosResult", "EscError", "lib", "cf_spectrum",
+__all__ = ["SpectrumLoss", "spectrum_loss", "Batch", "Chain", "Engine", "Honeycomb", "SpinModel", "LanczosResult", "EscError", "lib", "cf_spectrum",
            "lehmann_spectrum", "tridiag_eig", "load_params", "field111", "reference_omega"]

@@ -28,7 +28,7 @@ EXPORTS = [
     "esc_connectors", "esc_params_load", "esc_tridiag_eig", "esc_cf_spectrum", "esc_lehmann_spectrum",
     "esc_autotune", "esc_set_tile_bits", "esc_kernel_configs", "esc_ground_state",
     "esc_batch_create", "esc_batch_destroy", "esc_batch_ground_state", "esc_batch_state", "esc_batch_dynamics",
-    "esc_batch_autotune", "esc_spectrum_grad", "esc_two_point",
+    "esc_batch_autotune", "esc_spectrum_grad", "esc_two_point", "esc_chain", "esc_chain_connectors",
 ]
 
 
@@ -113,6 +113,8 @@ def lib():
         "esc_batch_autotune": ([P, I, P], I),
         "esc_spectrum_grad": ([P, I, I, I, I, I, D, P, I, P, P, P, P, P, P, P], I),
         "esc_two_point": ([P, I, I, P, P, P], I),
+        "esc_chain": ([I, I, P], I),
+        "esc_chain_connectors": ([I, P, P, P, P, P, P], I),
         "esc_batch_dynamics": ([P, I, I, D, P, I, I, P, P], I),
         "esc_dynamics": ([P, I, I, P, D, I, D, P, I, P, P, P], I),
         "esc_honeycomb": ([I, I, I, I, P], I),

@@ -306,6 +306,17 @@ std::vector<HostTerm> host_terms(const esc_ctx *c) {
     const esc::HTables &h = c->htab;
     std::vector<HostTerm> out;
     auto push = [&](uint32_t mask, int b0, int b1, int b2, const double (*coef)[2]) {
+        // a pair that sits in several families (the chain's XYZ bonds) is one
+        // gather with the summed table
+        for (HostTerm &o : out)
+            if (o.mask == mask && o.b0 == b0 && o.b1 == b1 && o.b2 == b2) {
+                for (int e = 0; e < 8; ++e) {
+                    o.coef[e][0] += coef[e][0];
+                    o.coef[e][1] += coef[e][1];
+                    if (o.coef[e][1] != 0.0) o.cplx = true;
+                }
+                return;
+            }
         HostTerm t;
         t.mask = mask;
         t.b0 = b0; t.b1 = b1; t.b2 = b2;

@@ -1,5 +1,5 @@
 // esc_host.cpp -- host-side geometry and parameter-file handling of the C ABI
-// (esc_honeycomb, esc_connectors, esc_params_load).
+// (esc_honeycomb, esc_connectors, esc_chain, esc_chain_connectors, esc_params_load).
 #include "esc.h"
 #include "esc_internal.h"
 
@@ -109,6 +109,49 @@ extern "C" int esc_connectors(int n_sites, const int32_t *neigh, const double *K
                     }
                 }
             counts[f] = c;
+        }
+    });
+}
+
+extern "C" int esc_chain(int n_sites, int periodic, int32_t *neigh) {
+    return host_guard([&] {
+        if (!neigh) throw HostError(ESC_ERR_ARG, "null argument");
+        if (n_sites < 2 || n_sites > 64) throw HostError(ESC_ERR_ARG, "n_sites must be in [2, 64]");
+        // Lattice::Chain (src/Lattice.h:60-125): column 0 = +x neighbour,
+        // column 1 = -x neighbour, wrapped when periodic
+        for (int i = 0; i < n_sites; ++i) {
+            neigh[2 * i] = (i + 1 < n_sites) ? i + 1 : (periodic ? 0 : -1);
+            neigh[2 * i + 1] = (i - 1 >= 0) ? i - 1 : (periodic ? n_sites - 1 : -1);
+        }
+    });
+}
+
+extern "C" int esc_chain_connectors(int n_sites, const int32_t *neigh, const double *K, int32_t *pairs_x,
+                                    int32_t *pairs_y, int32_t *pairs_z, int32_t *counts) {
+    return host_guard([&] {
+        if (!neigh || !K || !pairs_x || !pairs_y || !pairs_z || !counts) throw HostError(ESC_ERR_ARG, "null argument");
+        if (n_sites < 2) throw HostError(ESC_ERR_ARG, "n_sites must be >= 2");
+        // HeisenbergChain_Connectors (src/Hamiltonian.h:381-475): every family
+        // holds the chain bonds (i < j from both neighbour columns) of a
+        // non-zero coupling, listed in row-major order of the coupling matrix
+        std::vector<char> bond((size_t)n_sites * n_sites, 0);
+        for (int i = 0; i < n_sites; ++i)
+            for (int c = 0; c < 2; ++c) {
+                const int j = neigh[2 * i + c];
+                if (j > i && j < n_sites) bond[(size_t)i * n_sites + j] = 1;
+            }
+        int32_t *out[3] = {pairs_x, pairs_y, pairs_z};
+        for (int f = 0; f < 3; ++f) {
+            int n = 0;
+            if (K[f] != 0.0)
+                for (int i = 0; i < n_sites; ++i)
+                    for (int j = 0; j < n_sites; ++j)
+                        if (bond[(size_t)i * n_sites + j]) {
+                            out[f][2 * n] = i;
+                            out[f][2 * n + 1] = j;
+                            ++n;
+                        }
+            counts[f] = n;
         }
     });
 }

@@ -221,15 +221,26 @@ class Lattice {
 public:
     struct NeighTable {
         std::vector<int32_t> v;
-        int operator()(int i, int c) const { return v[(size_t)i * 3 + c]; }
+        int cols = 3;
+        int operator()(int i, int c) const { return v[(size_t)i * cols + c]; }
     };
+    // src/Lattice.h:37-58: Kitaev -> Honeycomb1, HeisenbergChain -> Chain
     explicit Lattice(ConstVariables &p) : variables_(p) {
-        if (p.Model != "Kitaev")
-            throw std::invalid_argument("Model=" + p.Model + ": only the Kitaev honeycomb is on this engine's path");
-        N1neigh_.v.resize((size_t)p.NumberofSites * 3);
-        check(esc_honeycomb(p.Lx, p.Ly, p.IsPeriodicX, p.IsPeriodicY, N1neigh_.v.data()));
-        std::cout << "creating Honeycomb lattice: right shifted " << std::endl;
+        if (p.Model == "Kitaev") {
+            N1neigh_.v.resize((size_t)p.NumberofSites * 3);
+            check(esc_honeycomb(p.Lx, p.Ly, p.IsPeriodicX, p.IsPeriodicY, N1neigh_.v.data()));
+            std::cout << "creating Honeycomb lattice: right shifted " << std::endl;
+        } else if (p.Model == "HeisenbergChain") {
+            N1neigh_.cols = 2;
+            N1neigh_.v.resize((size_t)p.NumberofSites * 2);
+            check(esc_chain(p.NumberofSites, p.IsPeriodicX, N1neigh_.v.data()));
+            std::cout << "creating Chain lattice " << std::endl;
+        } else {
+            std::cerr << "Model=" << p.Model << "\n";
+            throw std::string("Unknown Model parameter \n");
+        }
     }
+    bool chain() const { return N1neigh_.cols == 2; }
     NeighTable N1neigh_;
 
 private:
@@ -283,9 +294,16 @@ public:
         const double ones[3] = {1.0, 1.0, 1.0};
         for (auto &f : pairs_) f.assign((size_t)2 * N, 0);
         int32_t cnt[3];
-        // families follow the lattice; a zero coupling adds exact zeros
-        check(esc_connectors(N, lat.N1neigh_.v.data(), (p.J != 0.0 || p.Gamma != 0.0) ? ones : K,
-                             pairs_[0].data(), pairs_[1].data(), pairs_[2].data(), cnt));
+        if (lat.chain()) {   // src/Hamiltonian.h:44-53: HeisenbergChain_Connectors
+            if (p.J != 0.0 || p.Gamma != 0.0)
+                throw std::invalid_argument("J and Gamma are honeycomb extensions; use Kxx, Kyy, Kzz on a chain");
+            check(esc_chain_connectors(N, lat.N1neigh_.v.data(), K, pairs_[0].data(), pairs_[1].data(),
+                                       pairs_[2].data(), cnt));
+        } else {
+            // families follow the lattice; a zero coupling adds exact zeros
+            check(esc_connectors(N, lat.N1neigh_.v.data(), (p.J != 0.0 || p.Gamma != 0.0) ? ones : K,
+                                 pairs_[0].data(), pairs_[1].data(), pairs_[2].data(), cnt));
+        }
         for (int f = 0; f < 3; ++f) {
             // a family with exactly one bond is skipped (src/Hamiltonian.h:116,143,347)
             if (cnt[f] == 1) cnt[f] = 0;
@@ -297,7 +315,7 @@ public:
             m.n_bonds[f] = (int32_t)(pairs_[f].size() / 2);
             m.bonds[f] = pairs_[f].data();
         }
-        m.neigh = lat.N1neigh_.v.data();
+        m.neigh = lat.chain() ? nullptr : lat.N1neigh_.v.data();   // current operators are honeycomb-only
         m.K[0] = p.Kxx; m.K[1] = p.Kyy; m.K[2] = p.Kzz;
         m.B[0] = p.Bxx; m.B[1] = p.Byy; m.B[2] = p.Bzz;
         m.J = p.J;

@@ -37,8 +37,10 @@ int main(int argc, char *argv[]) {
         std::cout << std::setprecision(8) << std::fixed;
         if (Parameters.LancType != "OnlyEgs") {
             Observ.measureLocalS(Psi);
-            Observ.measureLocalJe(Psi);
+            if (!Lat.chain()) Observ.measureLocalJe(Psi);   // currents live on the honeycomb only
         }
+        if (Lat.chain() && (Parameters.LocalSkw || Parameters.EnergyCurrDynamics))
+            throw std::invalid_argument("current spectra are defined on the honeycomb only (Model=Kitaev)");
         std::cout.unsetf(std::ios_base::floatfield);
         if (Parameters.LocalSkw) {
             std::cout << " ==================================================================== " << std::endl;

@@ -4,6 +4,7 @@ Classes follow the reference roles (src/Lattice.h, src/Hamiltonian.h,
 src/DynLanczosSolver.h, src/Observables.h, src/dynamics.h):
 
   Honeycomb   Lattice::Honeycomb1 + Hamiltonian::Connectors  (esc_honeycomb / esc_connectors)
+  Chain       Lattice::Chain + HeisenbergChain_Connectors     (esc_chain / esc_chain_connectors)
   SpinModel   the coupling set (Kxx..Kzz, Bxx..Bzz, and the J / Gamma extensions)
   Engine      one device context: H.psi (MatrixVectorMult), ApplyOp / ApplyJ*Total,
               Lanczos_Nirav, overlaps, StartDynamics (esc_dynamics)
@@ -64,6 +65,27 @@ class Honeycomb:
         return [px[: cnt[0]].copy(), py[: cnt[1]].copy(), pz[: cnt[2]].copy()]
 
 
+class Chain:
+    """Chain of N sites for Model=HeisenbergChain (src/Lattice.h:60-125): column
+    0 of the neighbour table is the +x neighbour, column 1 the -x neighbour."""
+
+    def __init__(self, N, periodic=True):
+        self.N = int(N)
+        self.periodic = bool(periodic)
+        self.neigh = np.empty((self.N, 2), np.int32)
+        check(lib().esc_chain(self.N, int(periodic), _np_ptr(self.neigh)))
+
+    def bonds(self, K=(1.0, 1.0, 1.0)):
+        """Bond families x, y, z: every family of a non-zero coupling holds all
+        chain bonds (src/Hamiltonian.h:381-475)."""
+        Kd = np.asarray(K, np.float64)
+        px, py, pz = (np.zeros((self.N, 2), np.int32) for _ in range(3))
+        cnt = np.zeros(3, np.int32)
+        check(lib().esc_chain_connectors(self.N, _np_ptr(self.neigh), _np_ptr(Kd), _np_ptr(px), _np_ptr(py),
+                                         _np_ptr(pz), _np_ptr(cnt)))
+        return [px[: cnt[0]].copy(), py[: cnt[1]].copy(), pz[: cnt[2]].copy()]
+
+
 def field111(h):
     b = h / np.sqrt(3.0)
     return (b, b, b)
@@ -71,7 +93,7 @@ def field111(h):
 
 @dataclass
 class SpinModel:
-    lattice: Honeycomb
+    lattice: "Honeycomb | Chain"
     K: tuple = (1.0, 1.0, 1.0)
     J: float = 0.0
     Gamma: float = 0.0
@@ -80,8 +102,12 @@ class SpinModel:
 
     def struct(self):
         """esc_model for this model; keeps the arrays it points into alive on self."""
-        # families exist for every lattice bond; a zero coupling only zeroes the weight
-        self._bonds = self.lattice.bonds((1.0, 1.0, 1.0))
+        chain = isinstance(self.lattice, Chain)
+        if chain and (self.J != 0.0 or self.Gamma != 0.0):
+            raise ValueError("J and Gamma are honeycomb extensions; set K on a chain")
+        # honeycomb families exist for every lattice bond (a zero coupling only
+        # zeroes the weight); chain families exist for non-zero couplings
+        self._bonds = self.lattice.bonds(self.K if chain else (1.0, 1.0, 1.0))
         if self.skip_single_bond_family:
             self._bonds = [b if len(b) != 1 else b[:0] for b in self._bonds]
         m = EscModel()
@@ -90,7 +116,7 @@ class SpinModel:
             self._bonds[f] = np.ascontiguousarray(self._bonds[f], np.int32)
             m.n_bonds[f] = len(self._bonds[f])
             m.bonds[f] = self._bonds[f].ctypes.data if len(self._bonds[f]) else None
-        m.neigh = self.lattice.neigh.ctypes.data
+        m.neigh = None if chain else self.lattice.neigh.ctypes.data   # current operators are honeycomb-only
         for f in range(3):
             m.K[f] = float(self.K[f])
             m.B[f] = float(self.B[f])

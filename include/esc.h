@@ -223,6 +223,17 @@ int esc_honeycomb(int Lx, int Ly, int periodic_x, int periodic_y, int32_t *neigh
 int esc_connectors(int n_sites, const int32_t *neigh, const double *K, int32_t *pairs_x, int32_t *pairs_y,
                    int32_t *pairs_z, int32_t *counts);
 
+/* Chain lattice for Model=HeisenbergChain, replacing Lattice::Chain
+ * (src/Lattice.h:60-125): neigh is N x 2, column 0 = +x and column 1 = -x
+ * neighbour (-1 when absent; wrapped when periodic). */
+int esc_chain(int n_sites, int periodic, int32_t *neigh);
+/* HeisenbergChain_Connectors (src/Hamiltonian.h:381-475): every family with a
+ * non-zero coupling holds all chain bonds (i < j), in row-major order; with
+ * these families the Kitaev-form terms K_f S^f_i S^f_j give the XYZ chain.
+ * The J and Gamma extensions are honeycomb-only (leave them 0 on a chain). */
+int esc_chain_connectors(int n_sites, const int32_t *neigh, const double *K, int32_t *pairs_x, int32_t *pairs_y,
+                         int32_t *pairs_z, int32_t *counts);
+
 /* Parameter file (key=value, first exact key match wins), replacing
  * ConstVariables::Initialize / matchstring / matchstring2
  * (src/ParametersEngine.h:53-229) including its defaults and checks.

@@ -10,6 +10,8 @@ file:line it restates (paths relative to the fengshi96/EnergySpinCurrents root).
 
 Parity pins (checked in ``tests/test_oracle_*.py``):
   honeycomb / bonds       closed form + hand-derived 12-site table
+  chain / chain bonds     hand-derived 4-site tables; XX chain (free fermions)
+                          and transverse-field Ising closed forms, ring E0s
   hamiltonian (COO, gather) dense Kronecker product of Pauli matrices, Tr H,
                           Tr H^2 closed form, Hermiticity, W_p commutation
   currents                dense Kronecker operators
@@ -117,9 +119,70 @@ class Model:
         self.K = tuple(float(k) for k in self.K)
         self.B = tuple(float(b) for b in self.B)
 
+    def families(self, K):
+        return connectors(self.neigh, K)
+
     @property
     def N(self):
         return self.Lx * self.Ly
+
+    @property
+    def dim(self):
+        return 1 << self.N
+
+
+def chain(N, periodic=True):
+    """Neighbour table (N x 2; col 0 = +x, col 1 = -x, -1 if absent) --
+    src/Lattice.h:60-125."""
+    neigh = np.full((N, 2), -1, np.int32)
+    for i in range(N):
+        if i + 1 < N:
+            neigh[i, 0] = i + 1
+        elif periodic:
+            neigh[i, 0] = 0
+        if i - 1 >= 0:
+            neigh[i, 1] = i - 1
+        elif periodic:
+            neigh[i, 1] = N - 1
+    return neigh
+
+
+def chain_connectors(neigh, K):
+    """Bond families x, y, z of the chain: each family of a non-zero coupling
+    holds every bond i < j, in row-major order of the coupling matrix --
+    src/Hamiltonian.h:381-475."""
+    N = neigh.shape[0]
+    coupled = np.zeros((N, N), bool)
+    for i in range(N):
+        for c in (0, 1):
+            j = int(neigh[i, c])
+            if i < j:
+                coupled[i, j] = True
+    pairs = np.argwhere(coupled).astype(np.int32)      # row-major (i, j) order
+    return [pairs.copy() if K[f] != 0.0 else np.zeros((0, 2), np.int32) for f in range(3)]
+
+
+@dataclass
+class ChainModel:
+    """HeisenbergChain model: H = sum_bonds sum_f K_f S^f_i S^f_j + B.S."""
+
+    N: int
+    K: tuple = (1.0, 1.0, 1.0)
+    B: tuple = (0.0, 0.0, 0.0)
+    periodic: bool = True
+    J: float = 0.0
+    G: float = 0.0
+    neigh: np.ndarray = field(init=False, repr=False)
+    bonds: list = field(init=False, repr=False)
+
+    def __post_init__(self):
+        self.K = tuple(float(k) for k in self.K)
+        self.B = tuple(float(b) for b in self.B)
+        self.neigh = chain(self.N, self.periodic)
+        self.bonds = chain_connectors(self.neigh, self.K)
+
+    def families(self, K):
+        return chain_connectors(self.neigh, K)
 
     @property
     def dim(self):
@@ -193,7 +256,7 @@ class CooHamiltonian:
         assert m.J == 0.0 and m.G == 0.0
         self.m = m
         # the reference builds families only for non-zero couplings (:269-285)
-        self.bonds = connectors(m.neigh, m.K)
+        self.bonds = m.families(m.K)
         px, py, pz = (np.ascontiguousarray(b, np.int32) for b in self.bonds)
         self._keep = (px, py, pz)
         K = np.asarray(m.K, np.float64)
