@@ -29,6 +29,7 @@ EXPORTS = [
     "esc_autotune", "esc_set_tile_bits", "esc_kernel_configs", "esc_ground_state",
     "esc_batch_create", "esc_batch_destroy", "esc_batch_ground_state", "esc_batch_state", "esc_batch_dynamics",
     "esc_batch_autotune", "esc_spectrum_grad", "esc_two_point", "esc_chain", "esc_chain_connectors",
+    "esc_rixs", "esc_szsz",
 ]
 
 
@@ -65,6 +66,7 @@ class EscParams(ctypes.Structure):
         ("OmegaMin", ctypes.c_double), ("OmegaMax", ctypes.c_double),
         ("OmegaPoints", ctypes.c_int32), ("Precision", ctypes.c_int32), ("SpectralMode", ctypes.c_int32),
         ("StartVector", ctypes.c_int32), ("CFSteps", ctypes.c_int32), ("LocalSzSz", ctypes.c_int32),
+        ("SzSz", ctypes.c_int32), ("LocalRIXS", ctypes.c_int32), ("SpecialSite", ctypes.c_int32),
     ]
 
 
@@ -114,6 +116,8 @@ def lib():
         "esc_spectrum_grad": ([P, I, I, I, I, I, D, P, I, P, P, P, P, P, P, P], I),
         "esc_two_point": ([P, I, I, P, P, P], I),
         "esc_chain": ([I, I, P], I),
+        "esc_rixs": ([P, P, I, I, P, D, I, D, D, P, I, P, P], I),
+        "esc_szsz": ([P, P, D, P, I, P, P], I),
         "esc_chain_connectors": ([I, P, P, P, P, P, P], I),
         "esc_batch_dynamics": ([P, I, I, D, P, I, I, P, P], I),
         "esc_dynamics": ([P, I, I, P, D, I, D, P, I, P, P, P], I),

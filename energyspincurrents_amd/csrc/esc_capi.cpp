@@ -905,9 +905,9 @@ void cf_run(esc_ctx *ctx, const void *d_phi, int steps, void *work[3], double *a
 // `steps` iterations; also when the Krylov space is exhausted).  Returns the
 // number n of alphas.  With coef != nullptr the same recursion is replayed
 // (deterministic kernels: the same vectors bit for bit) for exactly n_fixed
-// vectors, accumulating acc += coef[k] v_k.
-int plain_recursion(esc_ctx *c, const void *start, int steps, double eps, const double *coef, int n_fixed, void *acc,
-                    std::vector<double> &ar, std::vector<double> &bt, hipStream_t s) {
+// vectors, accumulating acc += coef[k] v_k (complex coefficients).
+int plain_recursion(esc_ctx *c, const void *start, int steps, double eps, const std::complex<double> *coef,
+                    int n_fixed, void *acc, std::vector<double> &ar, std::vector<double> &bt, hipStream_t s) {
     const bool c64 = c->c64();
     const uint64_t dim = c->dim;
     c->ensure_partials(2 * esc::kRedBlocks);
@@ -919,7 +919,7 @@ int plain_recursion(esc_ctx *c, const void *start, int steps, double eps, const 
     double E0 = 1.0;
     for (int k = 0;; ++k) {
         if (coef) {
-            hip_check(esc::launch_axpby(c64, coef[k], 0.0, v, 1.0, 0.0, acc, dim, s), "accumulate");
+            hip_check(esc::launch_axpby(c64, coef[k].real(), coef[k].imag(), v, 1.0, 0.0, acc, dim, s), "accumulate");
             if (k + 1 == n_fixed) return n_fixed;
         }
         run_hpsi(c, v, w, s);
@@ -1190,6 +1190,142 @@ void local_szsz_spectra(esc_ctx *ctx, int method, bool reortho, const void *d_gs
     }
     if (expect)
         for (int a = 0; a < 3; ++a) { expect[2 * a] = sum_ex[a].real(); expect[2 * a + 1] = sum_ex[a].imag(); }
+}
+
+// <psi_k|vec> for every Ritz state psi_k = sum_m Z(m, k) V_m of the kept basis
+std::vector<cplx> ritz_amplitudes(esc_ctx *ctx, const void *d_vec, hipStream_t s) {
+    const int n = ctx->basis_n;
+    ctx->ensure_partials((size_t)2 * n * 256);
+    ctx->ensure_result((size_t)2 * n + 8);
+    hip_check(esc::launch_multidot(ctx->c64(), ctx->vec(0), n, d_vec, ctx->dim, ctx->d_partials, ctx->d_result, s),
+              "multidot");
+    std::vector<double> t(2 * (size_t)n);
+    sync_get(s, t.data(), ctx->d_result, sizeof(double) * 2 * n);
+    std::vector<cplx> out(n);
+    for (int k = 0; k < n; ++k) {
+        cplx x(0.0, 0.0);
+        for (int m = 0; m < n; ++m) x += ctx->Z[(size_t)k * n + m] * cplx(t[2 * m], t[2 * m + 1]);
+        out[k] = x;
+    }
+    return out;
+}
+
+// invComplex(x, y) = (x + i y)/(x^2 + y^2) = 1/(x - i y)  (src/dynamics.h:34-39)
+cplx inv_complex(double x, double y) {
+    const double d = x * x + y * y;
+    return cplx(x / d, y / d);
+}
+
+// LocalDynRIXS (src/dynamics.h:465-562).  Intermediate states n belong to
+// H_int, the Hamiltonian without the bonds of the core-hole site c
+// (src/Hamiltonian.h:57-91); final states m to H.  With
+// inv_n = invComplex(0 - E_n, Gamma) the reference forms
+//   A_X(m, n) = <n|gs> <m|X_c|n> inv_n,  X = 1, S^x_c, S^y_c, S^z_c,
+//   I_X(w)   = sum_{m >= 1} |sum_n A_X(m, n)|^2 eta / ((w - (E_m - Eg))^2 + eta^2).
+// The n sum is one vector, chi = sum_n |n> <n|gs> inv_n, so that
+// sum_n A_X(m, n) = <m|X_c chi>: one multi-axpy over the intermediate basis and
+// one multi-dot per X over the final basis, in place of |m| x |n| x 4 dot
+// products.  Ritz mode: n and m run over the Ritz states of the two kept bases,
+// Eg = the lowest Ritz value of H.  CF mode: chi from the Krylov space of H_int
+// started at |gs> (replayed recursion of cf_steps steps), I_X from the
+// continued fraction of H started at X_c chi at z = w + e0 - i eta, minus the
+// m = 0 term |<gs|X_c chi>|^2 eta / (w^2 + eta^2).  Table rows: w, Re I_1,
+// Re I_x, Re I_y, Re I_z, then the imaginary parts (zero: each I_X is a sum
+// of |.|^2 times a real Lorentzian).
+void rixs_spectra(esc_ctx *g, esc_ctx *q, int method, int site, const void *d_gs, double e0, int cf_steps,
+                  double gamma, double eta, const double *omega, int nw, double *table, hipStream_t s) {
+    const bool cf = method == ESC_SPEC_CF;
+    const uint64_t dim = g->dim;
+    g->ensure_aux(cf && g->basis_kept ? 5 : 2);
+    g->ensure_partials(2 * esc::kRedBlocks);
+    g->ensure_result(8);
+    void *chi = g->aux(0), *phi = g->aux(1);
+    if (!cf) {
+        const int nq = q->basis_n;
+        const std::vector<cplx> n0 = ritz_amplitudes(q, d_gs, s);   // <n|gs>
+        std::vector<cplx> w(nq);
+        for (int n = 0; n < nq; ++n) w[n] = n0[n] * inv_complex(0.0 - q->evals[n], gamma);
+        std::vector<double> c(2 * (size_t)nq);
+        for (int m = 0; m < nq; ++m) {
+            cplx x(0.0, 0.0);
+            for (int n = 0; n < nq; ++n) x += q->Z[(size_t)n * nq + m] * w[n];
+            c[2 * m] = x.real();
+            c[2 * m + 1] = x.imag();
+        }
+        q->ensure_result((size_t)2 * nq + 8);
+        hip_check(hipMemcpyAsync(q->d_result, c.data(), sizeof(double) * c.size(), hipMemcpyHostToDevice, s), "coef");
+        hip_check(esc::launch_multi_axpy(q->c64(), q->vec(0), nq, q->d_result, 1.0, chi, dim, true, s), "chi");
+        hip_check(hipStreamSynchronize(s), "sync");
+    } else {
+        // chi = (-H_int - i gamma)^-1 |gs> in the Krylov space of H_int from
+        // |gs>: with T = Z diag(theta) Z^T,
+        // chi = |gs| sum_k v_k sum_j Z(k, j) Z(0, j) inv(theta_j)
+        q->ensure_vecs(4);
+        q->basis_kept = false;
+        q->ensure_partials(2 * esc::kRedBlocks);
+        q->ensure_result(8);
+        void *start = q->vec(3);
+        hip_check(hipMemcpyAsync(start, d_gs, dim * q->esz, hipMemcpyDeviceToDevice, s), "copy gs");
+        const double n2 = device_norm2(q, start, s);
+        if (!(n2 > 0.0) || !std::isfinite(n2)) throw EscError(ESC_ERR_NUMERIC, "ground state has zero or non-finite norm");
+        hip_check(esc::launch_scale(q->c64(), start, 1.0 / std::sqrt(n2), dim, s), "scale");
+        std::vector<double> ar, bt;
+        const int n = plain_recursion(q, start, cf_steps, 0.0, nullptr, 0, nullptr, ar, bt, s);
+        std::vector<double> ev(n), Z((size_t)n * n);
+        if (tridiag_ql(n, ar.data(), bt.data(), ev.data(), Z.data()) != 0)
+            throw EscError(ESC_ERR_NUMERIC, "tridiagonal QL did not converge");
+        std::vector<cplx> y(n, cplx(0.0, 0.0));
+        for (int j = 0; j < n; ++j) {
+            const cplx f = std::sqrt(n2) * Z[(size_t)j * n] * inv_complex(0.0 - ev[j], gamma);
+            for (int k = 0; k < n; ++k) y[k] += Z[(size_t)j * n + k] * f;
+        }
+        hip_check(hipMemsetAsync(chi, 0, dim * g->esz, s), "clear");
+        std::vector<double> ar2, bt2;
+        plain_recursion(q, start, cf_steps, 0.0, y.data(), n, chi, ar2, bt2, s);
+    }
+    const int ops[4] = {-1, ESC_OP_SX, ESC_OP_SY, ESC_OP_SZ};
+    std::vector<double> al(cf_steps + 2), be(cf_steps + 3);
+    if (cf && !g->basis_kept) g->ensure_vecs(3);
+    void *work[3] = {nullptr, nullptr, nullptr};
+    if (cf)
+        for (int k = 0; k < 3; ++k) work[k] = g->basis_kept ? g->aux(2 + k) : g->vec(k);
+    for (int x = 0; x < 4; ++x) {
+        const void *src = chi;
+        if (ops[x] >= 0) {
+            esc::CurrentTable *t = current_table(g, ops[x], site);
+            hip_check(g->c64() ? esc::launch_current_c64(t, chi, phi, dim, s) : esc::launch_current_c128(t, chi, phi, dim, s),
+                      "site operator");
+            src = phi;
+        }
+        std::vector<double> I(nw, 0.0);
+        if (!cf) {
+            const std::vector<cplx> F = ritz_amplitudes(g, src, s);   // sum_n A_X(m, n)
+            const double Eg = g->evals[0];
+            for (int qw = 0; qw < nw; ++qw) {
+                double acc = 0.0;
+                for (int m = 1; m < g->basis_n; ++m) {
+                    const double xm = omega[qw] - (g->evals[m] - Eg);
+                    acc += std::norm(F[m]) * (eta / (xm * xm + eta * eta));
+                }
+                I[qw] = acc;
+            }
+        } else {
+            double d[2];
+            device_dot(g, d_gs, src, d, s);
+            const double el = d[0] * d[0] + d[1] * d[1];
+            int it = 0;
+            double n2 = 0.0;
+            cf_run(g, src, cf_steps, work, al.data(), be.data(), &it, &n2, s);
+            const std::vector<cplx> f = cf_eval(it, al.data(), be.data(), n2, e0, eta, omega, nw);
+            for (int qw = 0; qw < nw; ++qw) I[qw] = f[qw].imag() - el * eta / (omega[qw] * omega[qw] + eta * eta);
+        }
+        for (int qw = 0; qw < nw; ++qw) {
+            double *row = table + (size_t)qw * 9;
+            row[0] = omega[qw];
+            row[1 + x] = I[qw];
+            row[5 + x] = 0.0;
+        }
+    }
 }
 
 }  // namespace
@@ -1505,7 +1641,7 @@ int esc_ground_state(esc_ctx *ctx, const void *d_v0, int steps, double eps, int 
             std::vector<double> ev(n), Z((size_t)n * n);
             if (tridiag_ql(n, ar.data(), bt.data(), ev.data(), Z.data()) != 0)
                 throw EscError(ESC_ERR_NUMERIC, "tridiagonal QL did not converge");
-            std::vector<double> coef(Z.begin(), Z.begin() + n);   // column 0: the lowest Ritz pair
+            std::vector<std::complex<double>> coef(Z.begin(), Z.begin() + n);   // column 0: the lowest Ritz pair
             hip_check(hipMemsetAsync(d_gs, 0, dim * ctx->esz, s), "clear");
             std::vector<double> ar2, bt2;
             plain_recursion(ctx, start, steps, eps, coef.data(), n, d_gs, ar2, bt2, s);
@@ -1748,6 +1884,74 @@ int esc_dynamics(esc_ctx *ctx, int kind, int method, const void *d_gs, double e0
             F = cf_assemble(js, nc, runs, f, nw);
         }
         write_spectrum_table(js, nc, ex, F, omega, nw, eta, table);
+    });
+}
+
+int esc_rixs(esc_ctx *gs_ctx, esc_ctx *int_ctx, int method, int site, const void *d_gs, double e0, int cf_steps,
+             double gamma, double eta, const double *omega, int nw, double *table, void *stream) {
+    return guarded([&] {
+        check_ctx(gs_ctx);
+        check_ctx(int_ctx);
+        check_ptr(d_gs, "d_gs");
+        check_ptr(omega, "omega");
+        check_ptr(table, "table");
+        if (gs_ctx->dim != int_ctx->dim || gs_ctx->precision != int_ctx->precision)
+            throw EscError(ESC_ERR_ARG, "the two contexts must have the same size and precision");
+        if (site < 0 || site >= gs_ctx->n_sites) throw EscError(ESC_ERR_ARG, "site out of range");
+        if (method != ESC_SPEC_RITZ && method != ESC_SPEC_CF) throw EscError(ESC_ERR_ARG, "method must be Ritz or CF");
+        if (!(gamma > 0.0) || !std::isfinite(gamma)) throw EscError(ESC_ERR_ARG, "gamma must be positive");
+        if (!(eta > 0.0) || !std::isfinite(eta)) throw EscError(ESC_ERR_ARG, "eta must be positive");
+        if (nw < 1) throw EscError(ESC_ERR_ARG, "nw must be >= 1");
+        if (method == ESC_SPEC_RITZ && (!gs_ctx->basis_kept || !int_ctx->basis_kept))
+            throw EscError(ESC_ERR_ARG, "Ritz RIXS needs the kept bases of ESC_LANC_ALL runs on both contexts");
+        if (method == ESC_SPEC_CF && (cf_steps < 2 || cf_steps > 4096)) throw EscError(ESC_ERR_ARG, "cf_steps out of range");
+        rixs_spectra(gs_ctx, int_ctx, method, site, d_gs, e0, cf_steps, gamma, eta, omega, nw, table,
+                     (hipStream_t)stream);
+    });
+}
+
+int esc_szsz(esc_ctx *ctx, const void *d_gs, double eta, const double *omega, int nw, double *table, void *stream) {
+    return guarded([&] {
+        check_ctx(ctx);
+        check_ptr(d_gs, "d_gs");
+        check_ptr(omega, "omega");
+        check_ptr(table, "table");
+        if (!ctx->basis_kept) throw EscError(ESC_ERR_ARG, "SzSz needs the kept basis of an ESC_LANC_ALL run");
+        if (!(eta > 0.0) || !std::isfinite(eta)) throw EscError(ESC_ERR_ARG, "eta must be positive");
+        if (nw < 1) throw EscError(ESC_ERR_ARG, "nw must be >= 1");
+        hipStream_t s = (hipStream_t)stream;
+        const int N = ctx->n_sites, n = ctx->basis_n;
+        ctx->ensure_aux(1);
+        // O_i(n) = <n|S^z_i gs> over the Ritz states of the kept basis
+        std::vector<std::vector<cplx>> O(N);
+        for (int i = 0; i < N; ++i) {
+            esc::CurrentTable *t = current_table(ctx, ESC_OP_SZ, i);
+            hip_check(ctx->c64() ? esc::launch_current_c64(t, d_gs, ctx->aux(0), ctx->dim, s)
+                                 : esc::launch_current_c128(t, d_gs, ctx->aux(0), ctx->dim, s), "S^z");
+            O[i] = ritz_amplitudes(ctx, ctx->aux(0), s);
+        }
+        // DynSzSz (src/dynamics.h:566-611): Re(<S^z_j gs|n><n|S^z_i gs>) times
+        // x/(x^2 + eta^2) and eta/(x^2 + eta^2), x = w - (E_n - Eg), Eg = the
+        // lowest Ritz value; table[w][i][j] = (real, imaginary) sums
+        const double Eg = ctx->evals[0];
+        std::vector<double> tmp((size_t)N * N * n);
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j)
+                for (int k = 0; k < n; ++k) tmp[((size_t)i * N + j) * n + k] = (std::conj(O[j][k]) * O[i][k]).real();
+        for (int qw = 0; qw < nw; ++qw)
+            for (int i = 0; i < N; ++i)
+                for (int j = 0; j < N; ++j) {
+                    double re = 0.0, im = 0.0;
+                    const double *tp = &tmp[((size_t)i * N + j) * n];
+                    for (int k = 0; k < n; ++k) {
+                        const double x = omega[qw] - (ctx->evals[k] - Eg);
+                        re += tp[k] * x / (x * x + eta * eta);
+                        im += tp[k] * eta / (x * x + eta * eta);
+                    }
+                    double *out = table + (((size_t)qw * N + i) * N + j) * 2;
+                    out[0] = re;
+                    out[1] = im;
+                }
     });
 }
 

@@ -214,12 +214,18 @@ extern "C" int esc_params_load(const char *path, esc_params *p) {
         // (src/ParametersEngine.h:113-144)
         const bool ritz = p->SpectralMode == ESC_SPEC_RITZ;
         const std::string lt = p->LancType;
-        const char *dyn_keys[4] = {"LocalSkw", "SpinCurrDynamics", "EnergyCurrDynamics", "LocalSzSz"};
-        int32_t *dyn_vals[4] = {&p->LocalSkw, &p->SpinCurrDynamics, &p->EnergyCurrDynamics, &p->LocalSzSz};
-        for (int k = 0; k < 4; ++k) {
+        const char *dyn_keys[5] = {"LocalSkw", "SpinCurrDynamics", "EnergyCurrDynamics", "LocalSzSz", "LocalRIXS"};
+        int32_t *dyn_vals[5] = {&p->LocalSkw, &p->SpinCurrDynamics, &p->EnergyCurrDynamics, &p->LocalSzSz,
+                                &p->LocalRIXS};
+        for (int k = 0; k < 5; ++k) {
             *dyn_vals[k] = find_key(lines, dyn_keys[k], v) ? (v != 0.0) : 0;
             if (*dyn_vals[k] && ritz && lt != "All")
                 throw HostError(ESC_ERR_ARG, "You need to set LancType=All for Dynamical calculations --- you dummy!");
         }
+        // DynSzSz runs its own Lanczos (src/dynamics.h:575-576) and needs no kept basis
+        p->SzSz = find_key(lines, "SzSz", v) ? (v != 0.0) : 0;
+        p->SpecialSite = find_key(lines, "SpecialSite", v) ? (int32_t)v : p->NumberofSites / 2;
+        if (p->SpecialSite < 0 || p->SpecialSite >= p->NumberofSites)
+            throw HostError(ESC_ERR_ARG, "SpecialSite must be a site index in [0, NumberofSites)");
     });
 }

@@ -98,6 +98,9 @@ public:
     esc_ctx *get() const { return h_; }
     int64_t dim() const { return dim_; }
     int32_t elem_bytes() const { return esz_; }
+    // one Lanczos basis is kept per context; each run gets a number so that
+    // the Ritz states of an earlier run are not read from a later run's basis
+    int lanczos_runs = 0;
 
 private:
     void info() { check(esc_ctx_info(h_, &dim_, &esz_, nullptr)); }
@@ -184,7 +187,7 @@ struct ConstVariables {
         J = p.J; Gamma = p.Gamma; Eta = p.Eta; OmegaMin = p.OmegaMin; OmegaMax = p.OmegaMax;
         OmegaPoints = p.OmegaPoints; Precision = p.Precision; SpectralMode = p.SpectralMode;
         StartVector = p.StartVector; CFSteps = p.CFSteps; LocalSzSz = p.LocalSzSz;
-        SpecialSite = NumberofSites / 2;
+        SzSz = p.SzSz; LocalRIXS = p.LocalRIXS; SpecialSite = p.SpecialSite;
         generator_.seed((uint64_t)RandomSeed);
         if (LancType == "OnlyEgs") std::cout << "WARNING:: Reortho will be turned off for LancType=OnlyEgs (set by default) " << std::endl;
     }
@@ -197,7 +200,8 @@ struct ConstVariables {
     // extensions
     double J, Gamma, Eta, OmegaMin, OmegaMax;
     int OmegaPoints, Precision, SpectralMode, StartVector, CFSteps;
-    bool LocalSzSz;
+    bool LocalSzSz, SzSz, LocalRIXS;
+    int lanczos_runs = 0;   // Gaussian start vectors: seed RandomSeed + run
 
     // uniform [0,1) draw: double(g()) * 2^-64 (src/ParametersEngine.h:152-154)
     double grnd() { return (double)generator_() * (1.0 / 18446744073709551616.0); }
@@ -309,21 +313,32 @@ public:
             if (cnt[f] == 1) cnt[f] = 0;
             pairs_[f].resize((size_t)2 * cnt[f]);
         }
-        esc_model m{};
-        m.n_sites = N;
-        for (int f = 0; f < 3; ++f) {
-            m.n_bonds[f] = (int32_t)(pairs_[f].size() / 2);
-            m.bonds[f] = pairs_[f].data();
-        }
-        m.neigh = lat.chain() ? nullptr : lat.N1neigh_.v.data();   // current operators are honeycomb-only
-        m.K[0] = p.Kxx; m.K[1] = p.Kyy; m.K[2] = p.Kzz;
-        m.B[0] = p.Bxx; m.B[1] = p.Byy; m.B[2] = p.Bzz;
-        m.J = p.J;
-        m.Gamma = p.Gamma;
-        ctx_ = std::make_shared<Context>(m, p.Precision);
+        make_context();
     }
     void TightB_Ham() { std::cout << "create Tight-Binding/Off-Diag Hamiltonian (matrix-free)" << std::endl; }
     void Diagonal() { std::cout << "create Diagonal matrix elements (on the fly)" << std::endl; }
+    // the bond families are built once in the constructor (src/Hamiltonian.h:44-54)
+    void SetupConnectors() {}
+
+    // Intermediate Hamiltonian of LocalDynRIXS (src/Hamiltonian.h:57-91): every
+    // bond touching `center` loses its coupling.  Like the reference this
+    // changes this Hamiltonian for all later calls; vectors and Lanczos runs
+    // made before keep the previous context alive and stay valid.
+    void RIXS_killcenter(int &center) {
+        std::cout << " ==== RIXS - Intermediate Hamiltonian ==== " << std::endl;
+        const double K[3] = {variables_.Kxx, variables_.Kyy, variables_.Kzz};
+        for (int f = 0; f < 3; ++f) {
+            std::vector<int32_t> keep;
+            for (size_t b = 0; b + 1 < pairs_[f].size(); b += 2) {
+                const int i = pairs_[f][b], j = pairs_[f][b + 1];
+                const bool killed = i == center || j == center;
+                std::cout << i << " \t " << j << " \t " << (killed ? 0.0 : K[f]) << std::endl;
+                if (!killed) { keep.push_back(i); keep.push_back(j); }
+            }
+            pairs_[f] = keep;
+        }
+        make_context();
+    }
 
     // H|vin> (src/Hamiltonian.h:480-557)
     VectorXcd MatrixVectorMult(VectorXcd &Vin) {
@@ -335,7 +350,26 @@ public:
     const std::shared_ptr<Context> &context() const { return ctx_; }
     const std::vector<int32_t> &pairs(int family) const { return pairs_[family]; }
 
+    // a second device context of the same Hamiltonian (its own vectors)
+    std::shared_ptr<Context> fresh_context() const { return build_context(); }
+
 private:
+    void make_context() { ctx_ = build_context(); }
+    std::shared_ptr<Context> build_context() const {
+        const ConstVariables &p = variables_;
+        esc_model m{};
+        m.n_sites = p.NumberofSites;
+        for (int f = 0; f < 3; ++f) {
+            m.n_bonds[f] = (int32_t)(pairs_[f].size() / 2);
+            m.bonds[f] = pairs_[f].data();
+        }
+        m.neigh = Lat_.chain() ? nullptr : Lat_.N1neigh_.v.data();   // current operators are honeycomb-only
+        m.K[0] = p.Kxx; m.K[1] = p.Kyy; m.K[2] = p.Kzz;
+        m.B[0] = p.Bxx; m.B[1] = p.Byy; m.B[2] = p.Bzz;
+        m.J = p.J;
+        m.Gamma = p.Gamma;
+        return std::make_shared<Context>(m, p.Precision);
+    }
     ConstVariables &variables_;
     Lattice &Lat_;
     QBasis &Basis_;
@@ -348,7 +382,11 @@ private:
 // --------------------------------------------------------------------------
 class DynLanczosSolver {
 public:
-    DynLanczosSolver(ConstVariables &p, QBasis &basis, Hamiltonian &H) : variables_(p), Basis_(basis), Hamil_(H) {}
+    // own_basis: run on a fresh context of the same Hamiltonian, so that the
+    // kept basis of another solver on it survives (the reference keeps one
+    // PsiAll per solver object)
+    DynLanczosSolver(ConstVariables &p, QBasis &basis, Hamiltonian &H, bool own_basis = false)
+        : variables_(p), Basis_(basis), Hamil_(H), own_basis_(own_basis) {}
 
     // Ritz states of the last "All" run, formed on demand (PsiAll[k])
     struct RitzStates {
@@ -356,8 +394,9 @@ public:
         int n = 0;
         int size() const { return n; }
         VectorXcd operator[](int k) const {
-            VectorXcd v(s->Hamil_.context());
-            check(esc_ritz_vector(s->Hamil_.context()->get(), k, v.data(), nullptr));
+            s->check_basis();
+            VectorXcd v(s->ctx_);
+            check(esc_ritz_vector(s->ctx_->get(), k, v.data(), nullptr));
             return v;
         }
     };
@@ -374,9 +413,11 @@ public:
         else if (PsiOrAll == "GS") mode = ESC_LANC_GS;
         else if (PsiOrAll == "All") mode = ESC_LANC_ALL;
         else throw std::runtime_error("DynLanczosSolver: Use PsiOrAll==GS or All");
-        auto ctx = Hamil_.context();
+        auto ctx = own_basis_ ? Hamil_.fresh_context() : Hamil_.context();
+        ctx_ = ctx;
+        run_ = ++ctx->lanczos_runs;
         Vorig = VectorXcd(ctx);
-        check(esc_start_vector(ctx->get(), (uint64_t)variables_.RandomSeed, variables_.StartVector, Vorig.data(), nullptr));
+        Randomize(Vorig);
         const int M = variables_.LanczosSteps;
         alpha.assign(M + 1, 0.0);
         beta.assign(M + 2, 0.0);
@@ -398,7 +439,40 @@ public:
         return gs;
     }
 
+    // Start vector (src/DynLanczosSolver.h:58-59,465-474): re then im drawn
+    // from the parameters' generator, which every run continues, then
+    // normalised; the Gaussian extension uses seed RandomSeed + run.
+    void Randomize(VectorXcd &V) {
+        auto ctx = V.context();
+        if (variables_.StartVector == ESC_START_REF_UNIFORM) {
+            if (variables_.NumberofSites > 28)
+                throw std::invalid_argument("the reference start vector is limited to N <= 28; set StartVector=Gaussian");
+            std::vector<double> h(2 * (size_t)ctx->dim());
+            for (double &x : h) x = -1.0 + variables_.grnd() * 2.0;
+            double n2 = 0.0;
+            for (double x : h) n2 += x * x;
+            const double nrm = std::sqrt(n2);
+            for (double &x : h) x /= nrm;
+            check(esc_upload(ctx->get(), h.data(), V.data(), nullptr));
+        } else {
+            check(esc_start_vector(ctx->get(), (uint64_t)(variables_.RandomSeed + variables_.lanczos_runs),
+                                   variables_.StartVector, V.data(), nullptr));
+        }
+        ++variables_.lanczos_runs;
+    }
+
+    // the context and the kept basis of this solver's last run
+    const std::shared_ptr<Context> &context() const { return ctx_; }
+    void check_basis() const {
+        if (!basis_kept || !ctx_ || ctx_->lanczos_runs != run_)
+            throw std::runtime_error("PsiAll: the Lanczos basis of this run is not kept (LancType=All is needed, "
+                                     "and a later run on the same Hamiltonian replaces it)");
+    }
+
 private:
+    std::shared_ptr<Context> ctx_;
+    int run_ = 0;
+    bool own_basis_ = false;
     ConstVariables &variables_;
     QBasis &Basis_;
     Hamiltonian &Hamil_;
@@ -514,8 +588,86 @@ inline void write_spectrum_file(const std::string &path, const std::vector<doubl
     }
 }
 
+// the grid of DynSzSz / LocalDynRIXS: w = 0.1 k, k = 0..120 (src/dynamics.h:532-533,587-588),
+// or the OmegaMin/OmegaMax/OmegaPoints extension
+inline std::vector<double> tenth_grid(const ConstVariables &v) {
+    if (v.OmegaPoints > 0) return v.omega_grid();
+    std::vector<double> w;
+    for (int om = 0; om <= 120; ++om) w.push_back(0.1 * om);
+    return w;
+}
+
+// DynSzSz (src/dynamics.h:566-611): a second Lanczos run ("All") on the same
+// Hamiltonian, then S^z_i S^z_j spectra over its Ritz states, written as
+// "si \t sj \t w \t real \t imag" rows, w outermost, to SzSz_outSpectrum.real.
+inline void DynSzSz(std::string &dynstr, ConstVariables &Parameters, Lattice &, QBasis &Basis, Hamiltonian &Hamil,
+                    DynLanczosSolver &, VectorXcd &Psi) {
+    std::cout << " ================================================ " << std::endl;
+    std::cout << " ------- Startin Dynamics " << dynstr << " ------ " << std::endl;
+    std::cout << " ================================================ " << std::endl;
+    const int N = Parameters.NumberofSites;
+    DynLanczosSolver DLanc(Parameters, Basis, Hamil, true);
+    DLanc.Lanczos_Nirav("All");
+    std::cout << DLanc.PsiAll.size() << " \t " << DLanc.PsiAll.size() << std::endl;
+    const std::vector<double> w = tenth_grid(Parameters);
+    const double eta = 0.1;
+    std::vector<double> table(w.size() * N * N * 2);
+    check(esc_szsz(DLanc.context()->get(), Psi.data(), eta, w.data(), (int)w.size(), table.data(), nullptr));
+    std::ofstream f("SzSz_outSpectrum.real");
+    for (size_t q = 0; q < w.size(); ++q)
+        for (int si = 0; si < N; ++si)
+            for (int sj = 0; sj < N; ++sj) {
+                const double *v = &table[((q * N + si) * N + sj) * 2];
+                f << si << " \t " << sj << " \t " << w[q] << " \t " << v[0] << " \t " << v[1] << std::endl;
+            }
+}
+
+// LocalDynRIXS (src/dynamics.h:465-562): Eg from the ground-state run, the
+// intermediate Hamiltonian without the bonds of SpecialSite, a Lanczos run
+// ("All") on it, then the four channels (1, S^x, S^y, S^z at the site) with
+// Gamma = 100 and eta = 0.1, written as "w \t Re.. \t Im.." rows to
+// RIXS_outSpectrum.real.  SpectralMode=CF computes the same sums from
+// Krylov spaces started at |gs> and at X chi (no second "All" run).
+inline void LocalDynRIXS(std::string &dynstr, ConstVariables &Parameters, Lattice &, QBasis &Basis,
+                         Hamiltonian &Hamiltonian, DynLanczosSolver &DLancGS, VectorXcd &Psi) {
+    std::cout << " ================================================ " << std::endl;
+    std::cout << " ------- Startin RIXS Local Dynamics " << dynstr << " ------ " << std::endl;
+    std::cout << " ================================================ " << std::endl;
+    int centeral_site = Parameters.SpecialSite;
+    const bool ritz = Parameters.SpectralMode == ESC_SPEC_RITZ;
+    if (ritz) DLancGS.check_basis();
+    const double Eg = DLancGS.TriDeval.empty() ? 0.0 : DLancGS.TriDeval[0];
+    auto gs_ctx = ritz ? DLancGS.context() : Psi.context();
+    Hamiltonian.SetupConnectors();
+    Hamiltonian.RIXS_killcenter(centeral_site);
+    Hamiltonian.TightB_Ham();
+    Hamiltonian.Diagonal();
+    DynLanczosSolver DLancInter(Parameters, Basis, Hamiltonian);
+    if (ritz) {
+        DLancInter.Lanczos_Nirav("All");
+        std::cout << DLancInter.PsiAll.size() << " \t " << DLancGS.PsiAll.size() << std::endl;
+    }
+    const std::vector<double> w = tenth_grid(Parameters);
+    const double eta = 0.1, Gamma = 100;
+    std::vector<double> table(w.size() * 9);
+    check(esc_rixs(gs_ctx->get(), Hamiltonian.context()->get(), ritz ? ESC_SPEC_RITZ : ESC_SPEC_CF, centeral_site,
+                   Psi.data(), Eg, Parameters.CFSteps, Gamma, eta, w.data(), (int)w.size(), table.data(), nullptr));
+    std::ofstream f("RIXS_outSpectrum.real");
+    for (size_t q = 0; q < w.size(); ++q) {
+        const double *r = &table[q * 9];
+        for (int c = 0; c < 9; ++c) {
+            std::cout << r[c] << (c < 8 ? " \t " : "");
+            f << r[c] << (c < 8 ? " \t " : "");
+        }
+        std::cout << std::endl;
+        f << std::endl;
+    }
+}
+
 inline void StartDynamics(std::string &dynstr, ConstVariables &variables, Lattice &Lat, QBasis &Basis,
                           Hamiltonian &Hamil, DynLanczosSolver &DLancGS, VectorXcd &Psi) {
+    if (dynstr == "SzSz") return DynSzSz(dynstr, variables, Lat, Basis, Hamil, DLancGS, Psi);
+    if (dynstr == "LocalRIXS") return LocalDynRIXS(dynstr, variables, Lat, Basis, Hamil, DLancGS, Psi);
     int kind;
     std::string file;
     if (dynstr == "Spin Current") { kind = ESC_DYN_JSJS; file = "JsJs_outSpectrum.real"; }
@@ -532,9 +684,11 @@ inline void StartDynamics(std::string &dynstr, ConstVariables &variables, Lattic
     const int method = variables.SpectralMode;
     if (method == ESC_SPEC_RITZ && !DLancGS.basis_kept)
         throw std::invalid_argument("You need to set LancType=All for Dynamical calculations --- you dummy!");
+    if (method == ESC_SPEC_RITZ) DLancGS.check_basis();
     const double e0 = DLancGS.TriDeval.empty() ? 0.0 : DLancGS.TriDeval[0];
-    check(esc_dynamics(Hamil.context()->get(), kind, method, Psi.data(), e0, variables.CFSteps, variables.Eta,
-                       w.data(), (int)w.size(), table.data(), nullptr, nullptr));
+    auto ctx = method == ESC_SPEC_RITZ ? DLancGS.context() : Hamil.context();
+    check(esc_dynamics(ctx->get(), kind, method, Psi.data(), e0, variables.CFSteps, variables.Eta, w.data(),
+                       (int)w.size(), table.data(), nullptr, nullptr));
     write_spectrum_file(file, table, ncols, (int)w.size(), kind == ESC_DYN_LOCALSZSZ);
     std::cout << std::setprecision(8);
     for (size_t q = 0; q < w.size(); q += 200) {

@@ -3,8 +3,11 @@
 // lattice, basis, Hamiltonian, Lanczos (or ED), local measurements, then the
 // spin-current (LocalSkw=1) and energy-current (EnergyCurrDynamics=1) spectra,
 // written to JsJs_outSpectrum.real / JeJe_outSpectrum.real in the working
-// directory, and with the extension key LocalSzSz=1 the local spin spectra of
-// LocalDynSzSz in 2SSLocal_outSpectrum.real.  Everything below the class API runs on the GPU through libesc.so.
+// directory; with the extension keys LocalSzSz=1, SzSz=1 and LocalRIXS=1 the
+// spectra of LocalDynSzSz (2SSLocal_outSpectrum.real), DynSzSz
+// (SzSz_outSpectrum.real) and LocalDynRIXS at SpecialSite
+// (RIXS_outSpectrum.real).  Everything below the class API runs on the GPU
+// through libesc.so.
 #include "esc_reference_api.hpp"
 
 #include <iostream>
@@ -62,6 +65,14 @@ int main(int argc, char *argv[]) {
             std::cout << " ==================================================================== " << std::endl;
             std::string str2 = "LocalSzSz";
             StartDynamics(str2, Parameters, Lat, Basis, Ham, Lanczos, Psi);
+        }
+        if (Parameters.SzSz) {        // extension key: src/dynamics.h:566-611
+            std::string str3 = "SzSz";
+            StartDynamics(str3, Parameters, Lat, Basis, Ham, Lanczos, Psi);
+        }
+        if (Parameters.LocalRIXS) {   // extension key: src/dynamics.h:465-562 (last: it changes the Hamiltonian)
+            std::string str4 = "LocalRIXS";
+            StartDynamics(str4, Parameters, Lat, Basis, Ham, Lanczos, Psi);
         }
     } catch (const std::exception &e) {
         std::cerr << "LanczosCond: " << e.what() << std::endl;

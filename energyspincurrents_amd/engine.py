@@ -99,6 +99,13 @@ class SpinModel:
     Gamma: float = 0.0
     B: tuple = (0.0, 0.0, 0.0)
     skip_single_bond_family: bool = False   # reference rule, src/Hamiltonian.h:116,143,347
+    core_hole: int | None = None            # bonds of this site removed (src/Hamiltonian.h:57-91)
+
+    def intermediate(self, site):
+        """The intermediate Hamiltonian of LocalDynRIXS: this model with every
+        bond of `site` removed (Hamiltonian::RIXS_killcenter)."""
+        import dataclasses
+        return dataclasses.replace(self, core_hole=int(site))
 
     def struct(self):
         """esc_model for this model; keeps the arrays it points into alive on self."""
@@ -110,6 +117,9 @@ class SpinModel:
         self._bonds = self.lattice.bonds(self.K if chain else (1.0, 1.0, 1.0))
         if self.skip_single_bond_family:
             self._bonds = [b if len(b) != 1 else b[:0] for b in self._bonds]
+        if self.core_hole is not None:
+            c = self.core_hole
+            self._bonds = [b[(b[:, 0] != c) & (b[:, 1] != c)] for b in self._bonds]
         m = EscModel()
         m.n_sites = self.lattice.N
         for f in range(3):
@@ -301,6 +311,29 @@ class Engine:
         return table, (expect[0::2] + 1j * expect[1::2])[:nexp]
 
 
+    def rixs(self, inter: "Engine", site, gs, method="Ritz", e0=0.0, cf_steps=100, gamma=100.0, eta=0.1,
+             omega=None, stream=None):
+        """LocalDynRIXS (src/dynamics.h:465-562) at core-hole `site`; `inter` is the
+        engine of model.intermediate(site).  Ritz mode needs "All" Lanczos runs
+        kept on both engines.  Returns the (nw, 9) table of RIXS_outSpectrum.real."""
+        omega = np.ascontiguousarray(tenth_omega() if omega is None else omega, np.float64)
+        table = np.zeros((len(omega), 9))
+        check(lib().esc_rixs(self._h, inter._h, SPEC_METHODS[method], int(site), _dev_ptr(gs), float(e0),
+                             int(cf_steps), float(gamma), float(eta), _np_ptr(omega), len(omega), _np_ptr(table),
+                             _stream(stream)))
+        return table
+
+    def szsz(self, gs, eta=0.1, omega=None, stream=None):
+        """DynSzSz (src/dynamics.h:566-611) over the Ritz states of the kept "All"
+        run.  Returns S[w, i, j] = real + i imag of the reference's sums."""
+        omega = np.ascontiguousarray(tenth_omega() if omega is None else omega, np.float64)
+        N = self.model.lattice.N
+        table = np.zeros((len(omega), N, N, 2))
+        check(lib().esc_szsz(self._h, _dev_ptr(gs), float(eta), _np_ptr(omega), len(omega), _np_ptr(table),
+                             _stream(stream)))
+        return table[..., 0] + 1j * table[..., 1]
+
+
 # -- host FP64 small solves -------------------------------------------------
 class Batch:
     """Many models solved in lockstep (esc_batch_*): each Lanczos step of all
@@ -420,6 +453,11 @@ def load_params(path) -> dict:
         v = getattr(p, name)
         out[name] = v.decode() if isinstance(v, bytes) else v
     return out
+
+
+def tenth_omega():
+    """The grid of DynSzSz and LocalDynRIXS: w = 0.1 k, k = 0..120 (src/dynamics.h:532-533,587-588)."""
+    return 0.1 * np.arange(121)
 
 
 def reference_omega():

@@ -212,6 +212,31 @@ int esc_dynamics(esc_ctx *ctx, int kind, int method, const void *d_gs, double e0
                  double eta, const double *omega, int nw, double *table, double *expect, void *stream);
 
 /* ------------------------------------------------- lattice and parameters -- */
+/* LocalDynRIXS (src/dynamics.h:465-562): RIXS at core-hole site `site`.
+ * gs_ctx holds H (final states), int_ctx the intermediate Hamiltonian H_int
+ * = H without the bonds of `site` (Hamiltonian::RIXS_killcenter,
+ * src/Hamiltonian.h:57-91; build it with esc_ctx_create from bond lists that
+ * leave those bonds out).  With inv_n = 1/(-E_n - i gamma) and
+ * chi = sum_n |n><n|gs> inv_n over the intermediate states,
+ *   I_X(w) = sum_{m >= 1} |<m|X chi>|^2 eta/((w - (E_m - Eg))^2 + eta^2),
+ * X = 1, S^x_site, S^y_site, S^z_site.  ESC_SPEC_RITZ: n and m run over the
+ * Ritz states of ESC_LANC_ALL runs kept on int_ctx and gs_ctx (the
+ * reference's sums), Eg = the lowest Ritz value of gs_ctx, e0 and cf_steps
+ * unused.  ESC_SPEC_CF: chi from cf_steps Lanczos steps of H_int started at
+ * |gs>, I_X from the continued fraction of H at X chi minus the m = 0 term,
+ * Eg = e0.  table: nw rows of 9 doubles (w, Re I_1, Re I_x, Re I_y, Re I_z,
+ * Im I_1..Im I_z), the columns of RIXS_outSpectrum.real; the reference uses
+ * gamma = 100, eta = 0.1, w = 0.1 k, k = 0..120. */
+int esc_rixs(esc_ctx *gs_ctx, esc_ctx *int_ctx, int method, int site, const void *d_gs, double e0, int cf_steps,
+             double gamma, double eta, const double *omega, int nw, double *table, void *stream);
+/* DynSzSz (src/dynamics.h:566-611): for every pair of sites (i, j)
+ *   sum_n Re(<S^z_j gs|n><n|S^z_i gs>) / (w - (E_n - Eg) - i eta)
+ * over the Ritz states n of the ESC_LANC_ALL run kept on ctx (Eg = its lowest
+ * Ritz value); table[w][i][j] = (real, imaginary part), nw x N x N x 2
+ * doubles (SzSz_outSpectrum.real lists i, j, w, real, imaginary with w
+ * outermost; the reference uses eta = 0.1, w = 0.1 k, k = 0..120). */
+int esc_szsz(esc_ctx *ctx, const void *d_gs, double eta, const double *omega, int nw, double *table, void *stream);
+
 /* Honeycomb torus neighbour table (N x 3 row-major, column 0 = z, 1 = y,
  * 2 = x, -1 = none) for Lx unit cells along a1 and Ly/2 along a2, site numbering
  * A = 2 (i Ly/2 + j), B = A + 1.  Replaces Lattice::Honeycomb1
@@ -239,7 +264,10 @@ int esc_chain_connectors(int n_sites, const int32_t *neigh, const double *K, int
  * (src/ParametersEngine.h:53-229) including its defaults and checks.
  * Extension keys (optional): J, Gamma, Eta, OmegaMin, OmegaMax, OmegaPoints,
  * Precision (c128|c64), SpectralMode (Ritz|CF|CFReorth), StartVector
- * (Uniform|Gaussian), CFSteps, LocalSzSz (0|1). */
+ * (Uniform|Gaussian), CFSteps, LocalSzSz (0|1), SzSz (0|1), LocalRIXS (0|1),
+ * SpecialSite.  The reference declares SpecialSite but never reads it
+ * (src/ParametersEngine.h:29,92-97); here it is a key, defaulting to the
+ * value of the reference's commented-out line, NumberofSites / 2. */
 typedef struct {
     int32_t Lx, Ly, NumberofSites, RandomSeed, LanczosSteps, Threads;
     int32_t IsPeriodicX, IsPeriodicY, LocalSkw, SpinCurrDynamics, EnergyCurrDynamics;
@@ -253,6 +281,9 @@ typedef struct {
     int32_t StartVector;      /* ESC_START_* */
     int32_t CFSteps;
     int32_t LocalSzSz;        /* 1: LocalDynSzSz -> 2SSLocal_outSpectrum.real (src/dynamics.h:64-217) */
+    int32_t SzSz;             /* 1: DynSzSz -> SzSz_outSpectrum.real (src/dynamics.h:566-611) */
+    int32_t LocalRIXS;        /* 1: LocalDynRIXS -> RIXS_outSpectrum.real (src/dynamics.h:465-562) */
+    int32_t SpecialSite;      /* core-hole site of LocalDynRIXS; default NumberofSites / 2 */
 } esc_params;
 int esc_params_load(const char *path, esc_params *out);
 

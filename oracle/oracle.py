@@ -583,6 +583,62 @@ def local_szsz_ref(m: Model, evals, states, gs, omega=REF_OMEGA, eta=REF_ETA):
     return np.column_stack([w] + cols)
 
 
+def kill_site(m, site):
+    """Hamiltonian::RIXS_killcenter (src/Hamiltonian.h:57-91): the model with
+    every bond of `site` removed (its couplings set to zero)."""
+    import copy
+    k = copy.copy(m)
+    k.bonds = [b[(b[:, 0] != site) & (b[:, 1] != site)] if len(b) else b for b in m.bonds]
+    return k
+
+
+def inv_complex(x, y):
+    """invComplex (src/dynamics.h:34-39): (x + i y) / (x^2 + y^2)."""
+    d = x * x + y * y
+    return x / d + 1j * y / d
+
+
+TENTH_OMEGA = 0.1 * np.arange(121)    # src/dynamics.h:532-533,587-588
+
+
+def rixs_ref(N, site, evals_gs, states_gs, evals_int, states_int, gs, omega=TENTH_OMEGA, eta=0.1, gamma=100.0):
+    """LocalDynRIXS (src/dynamics.h:497-560) as written: A_X(m, n) = <n|gs>
+    <m|X|n> invComplex(0 - E_n, Gamma) for X = 1, S^x, S^y, S^z at `site`, then
+    I_X(w) = sum_{m >= 1} |sum_n A_X(m, n)|^2 eta / ((w - (E_m - E_0))^2 + eta^2).
+    states_* are rows (Ritz or eigen vectors).  Returns the 9-column table."""
+    n0 = states_int.conj() @ gs                            # nstate.dot(Psi)
+    inv = inv_complex(0.0 - np.asarray(evals_int), gamma)
+    Eg = evals_gs[0]
+    w = np.asarray(omega, np.float64)
+    xm = w[:, None] - (np.asarray(evals_gs)[None, 1:] - Eg)
+    lor = eta / (xm * xm + eta * eta)
+    cols = []
+    for X in (None, "Sx", "Sy", "Sz"):
+        Xn = states_int if X is None else np.array([apply_site_op(N, X, site, v) for v in states_int])
+        mXn = states_gs.conj() @ Xn.T                      # mstate.dot(X nstate)
+        S = (n0[None, :] * mXn * inv[None, :]).sum(axis=1)
+        cols.append(((S * S.conj()).real[None, 1:] * lor).sum(axis=1))
+    return np.column_stack([w] + cols + [np.zeros_like(w)] * 4)
+
+
+def szsz_ref(N, evals, states, gs, omega=TENTH_OMEGA, eta=0.1):
+    """DynSzSz (src/dynamics.h:566-611) as written: for every (si, sj, w) the
+    sums over states n of Re(<Szj|n><n|Szi>) x/(x^2+eta^2) and
+    ... eta/(x^2+eta^2), x = w - (E_n - E_0).  Returns S[w, si, sj] = real + i imag."""
+    Z = np.array([apply_site_op(N, "Sz", i, gs) for i in range(N)])
+    O = Z.conj() @ states.T                                # O[i, n] = <Szi|n>
+    w = np.asarray(omega, np.float64)
+    x = w[:, None] - (np.asarray(evals)[None, :] - evals[0])
+    kre = x / (x * x + eta * eta)
+    kim = eta / (x * x + eta * eta)
+    out = np.zeros((len(w), N, N), complex)
+    for si in range(N):
+        for sj in range(N):
+            tmp = (O[sj] * O[si].conj()).real              # Szj.dot(n) * n.dot(Szi)
+            out[:, si, sj] = kre @ tmp + 1j * (kim @ tmp)
+    return out
+
+
 def format_g6(x):
     """C++ ostream default formatting (precision 6, %g-like)."""
     s = "%g" % x

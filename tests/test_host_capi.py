@@ -169,3 +169,19 @@ def test_params_local_szsz_key(tmp_path):
     p = esc.load_params(_write(tmp_path, BASE + "\nLocalSzSz=1\n"))
     assert p["LocalSzSz"] == 1
     assert esc.load_params(_write(tmp_path, BASE))["LocalSzSz"] == 0
+
+
+def test_params_szsz_rixs_and_special_site(tmp_path):
+    p = esc.load_params(_write(tmp_path, BASE))
+    assert p["SzSz"] == 0 and p["LocalRIXS"] == 0
+    assert p["SpecialSite"] == p["NumberofSites"] // 2      # the reference's commented-out default
+    p = esc.load_params(_write(tmp_path, BASE + "\nSzSz=1\nLocalRIXS=1\nSpecialSite=5\n"))
+    assert p["SzSz"] == 1 and p["LocalRIXS"] == 1 and p["SpecialSite"] == 5
+    with pytest.raises(esc.EscError, match="SpecialSite"):
+        esc.load_params(_write(tmp_path, BASE + "\nSpecialSite=99\n"))
+    # the RIXS sums run over the ground-state run's Ritz states: LancType=All
+    gs_only = BASE.replace("LancType=All", "LancType=GS").replace("EnergyCurrDynamics=1", "EnergyCurrDynamics=0")
+    esc.load_params(_write(tmp_path, gs_only))
+    with pytest.raises(esc.EscError, match="LancType=All"):
+        esc.load_params(_write(tmp_path, gs_only + "\nLocalRIXS=1\n"))
+    esc.load_params(_write(tmp_path, gs_only + "\nSzSz=1\n"))      # DynSzSz runs its own "All" run
