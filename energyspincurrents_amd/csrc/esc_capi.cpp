@@ -399,7 +399,11 @@ uint32_t grow_window(int N, uint32_t T, const std::vector<uint32_t> &masks, int 
 // sites that make the most of its terms "near" (partner inside the LDS tile).
 // Terms that are near in no planned pass are applied as "far" terms (partner
 // read from the L2-resident window) by a pass whose window holds them; terms
-// that fit no window open further passes.
+// that fit no window open further passes.  P = 5 selects the cover-first
+// planner below instead: windows chosen to hold all terms in as few passes as
+// the window width allows, tiles second.
+constexpr int kCoverFirst = 5;   // pass digit of the cover-first planner
+
 std::vector<esc::GTables> build_passes(const esc_ctx *c, int TB, int w, int P, int WPB) {
     const int N = c->n_sites;
     if (w > N) w = N;
@@ -426,6 +430,79 @@ std::vector<esc::GTables> build_passes(const esc_ctx *c, int TB, int w, int P, i
         rem.swap(keep);
     };
     if (w >= N) P = 1;
+    if (P == kCoverFirst) {
+        // Windows first: each holds as many of the terms no earlier window
+        // holds as possible (grown site by site from the lane sites, then
+        // single-site swaps while they gain), until every term lies in some
+        // window.  Then each window's tile makes the most of its terms near
+        // that no earlier tile has, and every term goes to the first pass in
+        // which it is near, else (far) to the window holding it with the
+        // fewest far terms so far.
+        auto held = [&](uint32_t W, const std::vector<size_t> &ids) {
+            long n = 0;
+            for (size_t t : ids) n += (terms[t].mask & ~W) == 0;
+            return n;
+        };
+        std::vector<size_t> left(rem);
+        std::vector<uint32_t> wins;
+        while (!left.empty()) {
+            uint32_t W = 63u;
+            while (__builtin_popcount(W) < w) {
+                int best = -1;
+                long bk = -1;
+                for (int st = 6; st < N; ++st) {
+                    if ((W >> st) & 1u) continue;
+                    long touch = 0;
+                    for (size_t t : left) touch += (terms[t].mask >> st) & 1u;
+                    const long key = held(W | (1u << st), left) * 1024 + touch;
+                    if (key > bk) { bk = key; best = st; }
+                }
+                W |= 1u << best;
+            }
+            long val = held(W, left);
+            for (bool gain = true; gain;) {
+                gain = false;
+                for (int a = 6; a < N && !gain; ++a) {
+                    if (!((W >> a) & 1u)) continue;
+                    for (int b = 6; b < N && !gain; ++b) {
+                        if ((W >> b) & 1u) continue;
+                        const uint32_t W2 = (W & ~(1u << a)) | (1u << b);
+                        const long v = held(W2, left);
+                        if (v > val) { W = W2; val = v; gain = true; }
+                    }
+                }
+            }
+            if (val == 0) throw EscError(ESC_ERR_UNSUPPORTED, "window construction made no progress");
+            wins.push_back(W);
+            std::vector<size_t> keep;
+            for (size_t t : left)
+                if ((terms[t].mask & ~W) != 0) keep.push_back(t);
+            left.swap(keep);
+            if (wins.size() > 8) throw EscError(ESC_ERR_UNSUPPORTED, "too many window passes");
+        }
+        std::vector<char> is_near(terms.size(), 0);
+        for (uint32_t W : wins) {
+            std::vector<uint32_t> cand;
+            for (size_t t = 0; t < terms.size(); ++t)
+                if (!is_near[t] && (terms[t].mask & ~W) == 0) cand.push_back(terms[t].mask);
+            Plan p;
+            p.W = W;
+            p.T = greedy_tile(N, W, cand, TB);
+            for (size_t t = 0; t < terms.size(); ++t)
+                if ((terms[t].mask & ~p.T) == 0) is_near[t] = 1;
+            plan.push_back(std::move(p));
+        }
+        for (size_t t = 0; t < terms.size(); ++t) {
+            Plan *dst = nullptr;
+            for (auto &p : plan)
+                if ((terms[t].mask & ~p.T) == 0) { dst = &p; break; }
+            if (dst) { dst->near.push_back(t); continue; }
+            for (auto &p : plan)
+                if ((terms[t].mask & ~p.W) == 0 && (!dst || p.far.size() < dst->far.size())) dst = &p;
+            dst->far.push_back(t);
+        }
+        rem.clear();
+    }
     while (plan.empty() || (!rem.empty() && (int)plan.size() < P)) {
         Plan p;
         const std::vector<uint32_t> rm = masks_of(rem);
@@ -732,7 +809,8 @@ double device_norm2(esc_ctx *c, const void *x, hipStream_t s) {
 const std::vector<esc_ctx::PassDev> &passes_for(esc_ctx *c, int config) {
     auto it = c->d_pass.find(config);
     if (it != c->d_pass.end()) return it->second;
-    const std::vector<esc::GTables> host = build_passes(c, (config / 100) % 100, config % 100, config / 100000, (config / 10000) % 10);
+    const std::vector<esc::GTables> host =
+        build_passes(c, (config / 100) % 100, config % 100, (config / 100000) % 10, (config / 10000) % 10);
     std::vector<esc_ctx::PassDev> dev;
     for (const auto &g : host) {
         esc::GTables *d = nullptr;
@@ -752,7 +830,8 @@ bool tile_config_fits(const esc_ctx *c, int config) {
     if (it != c->tile_fits.end()) return it->second;
     bool ok = true;
     try {
-        for (const auto &g : build_passes(c, (config / 100) % 100, config % 100, config / 100000, (config / 10000) % 10))
+        for (const auto &g :
+             build_passes(c, (config / 100) % 100, config % 100, (config / 100000) % 10, (config / 10000) % 10))
             if (esc::tile_lds_bytes(c->c64(), (int)g.tile_bits, (int)g.cls_begin[4],
                                     (int)g.cls_begin[esc::kNumClasses]) > esc::kMaxTileLds)
                 ok = false;
@@ -776,7 +855,7 @@ std::vector<int> tile_candidates(const esc_ctx *c) {
             for (int w : windows) {
                 const int ww = w > N ? N : w;
                 if (ww < tb || N - ww > 16 || ww - tb > 16) continue;
-                for (int P : {1, 2, 3}) {
+                for (int P : {1, 2, 3, kCoverFirst}) {
                     if ((ww == N) != (P == 1)) continue;
                     const int cfg = 100000 * P + 10000 * waves + 100 * tb + ww;
                     if (std::find(v.begin(), v.end(), cfg) == v.end() && tile_config_fits(c, cfg)) v.push_back(cfg);
@@ -787,15 +866,15 @@ std::vector<int> tile_candidates(const esc_ctx *c) {
     return v;
 }
 
-// before tuning: the shapes measured fastest at N = 24 (c128: 8 waves,
-// 2^11-row tiles, 2^16-row windows, two planned passes; c64: 8 waves, 2^12-row
-// tiles, 2^19-row windows, three planned passes), else a single pass over
-// 2^11-row tiles; systems below 2^11 rows use one row per lane
+// before tuning: the shapes measured fastest at N = 24 (8 waves, cover-first
+// 2^18-row windows, 2^11-row tiles for c128 and 2^12-row tiles for c64: two
+// passes), else a single pass over 2^11-row tiles; systems below 2^11 rows use
+// one row per lane
 int default_tile_bits(const esc_ctx *c) {
     const int N = c->n_sites;
     if (N < 11) return 0;
     const std::vector<int> cand = tile_candidates(c);
-    for (int cfg : {c->c64() ? 381219 : 281116, 100000 + 80000 + 100 * 11 + N, 100000 + 40000 + 100 * 10 + N})
+    for (int cfg : {c->c64() ? 581218 : 581118, 100000 + 80000 + 100 * 11 + N, 100000 + 40000 + 100 * 10 + N})
         if (std::find(cand.begin(), cand.end(), cfg) != cand.end()) return cfg;
     return 0;
 }

@@ -103,8 +103,9 @@ int esc_ctx_info(const esc_ctx *ctx, int64_t *dim, int32_t *elem_bytes, int32_t 
  * and reports it as tile_bits.  A configuration is 0 (one row per lane) or
  * 100000 * P + 10000 * V + 100 * TB + W: workgroups of V waves sharing 2^TB
  * rows staged in LDS, walking windows of 2^W rows (one XCD per window) in P
- * planned passes (P = 1 when W = N), plus passes for terms no window held.
- * d_out is overwritten. */
+ * planned passes (P = 1 when W = N), plus passes for terms no window held;
+ * P = 5: windows chosen first to hold every term in as few passes as the
+ * width W allows.  d_out is overwritten. */
 int esc_autotune(esc_ctx *ctx, const void *d_in, void *d_out, int reps, int *tile_bits, double *ms, void *stream);
 /* Select the H.psi kernel explicitly (a configuration as reported by
  * esc_autotune or listed by esc_kernel_configs). */
