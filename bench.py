@@ -45,7 +45,8 @@ def parse():
     ap.add_argument("--no-n32", action="store_true", help="skip the 32-site complex64 H.psi timing")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batched scan")
     ap.add_argument("--kernel-config", default=None,
-                    help="C128,C64: use these H.psi kernel configurations instead of autotuning (profiling runs)")
+                    help="C128,C64: H.psi kernel configurations (default: the ones profiled under profiles/)")
+    ap.add_argument("--autotune", action="store_true", help="pick the H.psi kernel configurations by timing them")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run a plain device copy of one vector per step (HBM counter calibration)")
     return ap.parse_args()
@@ -77,9 +78,12 @@ def time_hpsi(eng, x, y, steps, warmup, copy_calib=False):
 # WRITE_SIZE on this workload, FETCH scaled by 2.0 and WRITE by 1.0 (the
 # factors that make a plain one-vector read / write read its own size); the
 # summary with the commands is profiles/hpsi_n24_hbm_traffic.md
+# the configurations the autotuner picks for this workload, profiled under profiles/
+PROFILED_CONFIG = {"c128": 581118, "c64": 581218}
+
 MEASURED_TRAFFIC = {
-    ("c128", 281116): 2716265472,   # 933039 KiB fetched x 2.0 + 786525 KiB written
-    ("c64", 381219): 1656249344,    # 611918 KiB fetched x 2.0 + 393595 KiB written
+    ("c128", 581118): 2090866720,   # 758781 KiB fetched x 2.0 + 524301 KiB written
+    ("c64", 581218): 834079552,     # 276191 KiB fetched x 2.0 + 262150 KiB written
 }
 
 
@@ -120,11 +124,12 @@ def main():
         eng = esc.Engine(model, prec)
         x = eng.start_vector(1234, "Gaussian")
         y = eng.empty()
-        if args.kernel_config:
-            config = int(args.kernel_config.split(",")[0 if prec == "c128" else 1])
-            eng.set_tile_bits(config)
-        else:
+        if args.autotune:
             config, _ = eng.autotune(x, y, reps=5)
+        else:
+            config = int(args.kernel_config.split(",")[0 if prec == "c128" else 1]) if args.kernel_config \
+                else PROFILED_CONFIG[prec]
+            eng.set_tile_bits(config)
         ms, wall = time_hpsi(eng, x, y, args.steps, args.warmup, args.copy_calib and prec == "c128")
         bytes_alg = 2 * eng.dim * eng.elem_bytes
         kname = f"k_hpsi_tile<{'float' if prec == 'c64' else 'double'}, TB, WPB>" if config else \
