@@ -664,6 +664,7 @@ std::vector<esc::GTables> build_passes(const esc_ctx *c, int TB, int w, int P, i
         passes.push_back(g);
         first = false;
     }
+    passes.back().last_pass = 1u;
     return passes;
 }
 
@@ -879,12 +880,12 @@ int default_tile_bits(const esc_ctx *c) {
     return 0;
 }
 
-void launch_hpsi_with(esc_ctx *c, int config, const void *in, void *out, hipStream_t s) {
+void launch_hpsi_with(esc_ctx *c, int config, const void *in, void *out, hipStream_t s, double *dot_part = nullptr) {
     if (config > 0) {
         const int waves = (config / 10000) % 10, tb = (config / 100) % 100;
         for (const auto &p : passes_for(c, config))
             hip_check(esc::launch_hpsi_tile(c->c64(), p.d, c->d_htab, tb, waves, p.n_uniform, p.n_terms, in, out,
-                                            c->dim, s),
+                                            dot_part, c->dim, s),
                       "hpsi tile launch");
     } else {
         hip_check(c->c64() ? esc::launch_hpsi_c64(c->d_htab, in, out, c->dim, s)
@@ -899,6 +900,23 @@ void run_hpsi(esc_ctx *c, const void *in, void *out, hipStream_t s) {
     } else {
         hip_check(c->c64() ? esc::launch_hpsi_c64(c->d_htab, in, out, c->dim, s)
                            : esc::launch_hpsi_c128(c->d_htab, in, out, c->dim, s), "hpsi launch");
+    }
+}
+
+// out = H in and <in|out> into c->d_result[0..1] (FP64): the tiled kernel's
+// last pass forms the per-tile partial sums from its staged input rows, so no
+// separate pass over the two vectors is needed
+void run_hpsi_dot(esc_ctx *c, const void *in, void *out, hipStream_t s) {
+    c->ensure_result(8);
+    if (c->tile_bits > 0) {
+        const uint64_t ntiles = c->dim >> ((c->tile_bits / 100) % 100);
+        c->ensure_partials(2 * ntiles + 2 * esc::kRedBlocks);
+        launch_hpsi_with(c, c->tile_bits, in, out, s, c->d_partials);
+        hip_check(esc::launch_reduce(c->d_partials, (int)ntiles, 2, c->d_result, s), "reduce");
+    } else {
+        run_hpsi(c, in, out, s);
+        c->ensure_partials(2 * esc::kRedBlocks);
+        hip_check(esc::launch_dot(c->c64(), in, out, c->dim, c->d_partials, c->d_result, s), "dot");
     }
 }
 
@@ -935,17 +953,51 @@ void cf_run(esc_ctx *ctx, const void *d_phi, int steps, void *work[3], double *a
         ctx->ensure_partials((size_t)2 * (steps + 2) * 256 + 2 * esc::kRedBlocks);
         ctx->ensure_result((size_t)2 * (steps + 2) + 8);
     }
-    void *vp = reortho ? nullptr : work[0], *v = reortho ? ctx->vec(0) : work[1], *w = reortho ? ctx->vec(1) : work[2];
-    hip_check(hipMemcpyAsync(v, d_phi, dim * ctx->esz, hipMemcpyDeviceToDevice, s), "copy");
-    hip_check(esc::launch_scale(c64, v, 1.0 / std::sqrt(n2), dim, s), "scale");
     beta[0] = 0.0;
     int k = 0;
-    for (; k < steps; ++k) {
-        if (reortho) {
-            vp = k > 0 ? ctx->vec((size_t)k - 1) : nullptr;
-            v = ctx->vec((size_t)k);
-            w = ctx->vec((size_t)k + 1);
+    if (!reortho) {
+        // The recursion keeps unnormalised vectors u_k with v_k = c_k u_k
+        // (c_0 = 1/|phi|, c_k = 1/beta_k): H v_k = c_k H u_k and alpha_k =
+        // c_k^2 <u_k|H u_k> (formed by the H.psi kernel itself), and the
+        // update w = c_k H u_k - alpha_k c_k u_k - beta_k c_{k-1} u_{k-1}
+        // becomes u_{k+1}.  No separate dot or scaling pass per step.
+        void *up = work[0], *u = work[1], *w = work[2];
+        hip_check(hipMemcpyAsync(u, d_phi, dim * ctx->esz, hipMemcpyDeviceToDevice, s), "copy");
+        double c = 1.0 / std::sqrt(n2), cp = 0.0;
+        for (; k < steps; ++k) {
+            run_hpsi_dot(ctx, u, w, s);
+            double d[2];
+            sync_get(s, d, ctx->d_result, 2 * sizeof(double));
+            const double ar = c * c * d[0], ai = c * c * d[1];
+            if (!std::isfinite(ar)) throw EscError(ESC_ERR_NUMERIC, "non-finite alpha");
+            alpha[k] = ar;
+            hip_check(esc::launch_lanczos_update_s(c64, w, u, k > 0 ? up : nullptr, c, -ar * c, -ai * c,
+                                                   -beta[k] * cp, dim, ctx->d_partials, ctx->d_result + 2, s),
+                      "update");
+            double b2;
+            sync_get(s, &b2, ctx->d_result + 2, sizeof(double));
+            const double b = std::sqrt(b2);
+            beta[k + 1] = b;
+            if (!std::isfinite(b)) throw EscError(ESC_ERR_NUMERIC, "non-finite beta");
+            if (b <= 1e-13 * std::max(1.0, std::fabs(ar))) {   // Krylov space exhausted: the fraction terminates
+                beta[k + 1] = 0.0;
+                ++k;
+                break;
+            }
+            cp = c;
+            c = 1.0 / b;
+            void *t = up; up = u; u = w; w = t;
         }
+        *iters = k;
+        return;
+    }
+    void *vp = nullptr, *v = ctx->vec(0), *w = ctx->vec(1);
+    hip_check(hipMemcpyAsync(v, d_phi, dim * ctx->esz, hipMemcpyDeviceToDevice, s), "copy");
+    hip_check(esc::launch_scale(c64, v, 1.0 / std::sqrt(n2), dim, s), "scale");
+    for (; k < steps; ++k) {
+        vp = k > 0 ? ctx->vec((size_t)k - 1) : nullptr;
+        v = ctx->vec((size_t)k);
+        w = ctx->vec((size_t)k + 1);
         run_hpsi(ctx, v, w, s);
         double a[2];
         hip_check(esc::launch_dot(c64, v, w, dim, ctx->d_partials, ctx->d_result, s), "dot");
@@ -965,16 +1017,12 @@ void cf_run(esc_ctx *ctx, const void *d_phi, int steps, void *work[3], double *a
             break;
         }
         hip_check(esc::launch_scale(c64, w, 1.0 / b, dim, s), "scale");
-        if (reortho) {
-            if (k > 0) {
-                cgs_pass(ctx, k + 1, w, s);
-                cgs_pass(ctx, k + 1, w, s);
-                hip_check(esc::launch_norm2(c64, w, dim, ctx->d_partials, ctx->d_result + 4, s), "norm2");
-                hip_check(esc::launch_normalize(c64, w, ctx->d_result + 4, dim, s), "normalize");
-            }
-            continue;
+        if (k > 0) {
+            cgs_pass(ctx, k + 1, w, s);
+            cgs_pass(ctx, k + 1, w, s);
+            hip_check(esc::launch_norm2(c64, w, dim, ctx->d_partials, ctx->d_result + 4, s), "norm2");
+            hip_check(esc::launch_normalize(c64, w, ctx->d_result + 4, dim, s), "normalize");
         }
-        void *t = vp; vp = v; v = w; w = t;
     }
     *iters = k;
 }
@@ -991,24 +1039,30 @@ int plain_recursion(esc_ctx *c, const void *start, int steps, double eps, const 
     const uint64_t dim = c->dim;
     c->ensure_partials(2 * esc::kRedBlocks);
     c->ensure_result(8);
+    // unnormalised vectors u_k with v_k = cs_k u_k (cs_0 = 1, cs_k = 1/beta_k):
+    // alpha_k = cs_k^2 <u_k|H u_k> comes from the H.psi kernel, and the update
+    // w = cs_k H u_k - alpha_k cs_k u_k - beta_k cs_{k-1} u_{k-1} is u_{k+1}
     void *vp = c->vec(0), *v = c->vec(1), *w = c->vec(2);
     hip_check(hipMemcpyAsync(v, start, dim * c->esz, hipMemcpyDeviceToDevice, s), "copy start");
     ar.clear();
     bt.assign(1, 0.0);
-    double E0 = 1.0;
+    double E0 = 1.0, cs = 1.0, csp = 0.0;
     for (int k = 0;; ++k) {
         if (coef) {
-            hip_check(esc::launch_axpby(c64, coef[k].real(), coef[k].imag(), v, 1.0, 0.0, acc, dim, s), "accumulate");
+            hip_check(esc::launch_axpby(c64, coef[k].real() * cs, coef[k].imag() * cs, v, 1.0, 0.0, acc, dim, s),
+                      "accumulate");
             if (k + 1 == n_fixed) return n_fixed;
         }
-        run_hpsi(c, v, w, s);
+        run_hpsi_dot(c, v, w, s);
         double a[2];
-        hip_check(esc::launch_dot(c64, v, w, dim, c->d_partials, c->d_result, s), "dot");
         sync_get(s, a, c->d_result, 2 * sizeof(double));
+        a[0] *= cs * cs;
+        a[1] *= cs * cs;
         if (!std::isfinite(a[0]) || !std::isfinite(a[1])) throw EscError(ESC_ERR_NUMERIC, "non-finite alpha");
         ar.push_back(a[0]);
-        hip_check(esc::launch_lanczos_update(c64, w, v, k > 0 ? vp : nullptr, c->d_result, bt[k], dim, c->d_partials,
-                                             c->d_result + 2, s), "update");
+        hip_check(esc::launch_lanczos_update_s(c64, w, v, k > 0 ? vp : nullptr, cs, -a[0] * cs, -a[1] * cs,
+                                               -bt[k] * csp, dim, c->d_partials, c->d_result + 2, s),
+                  "update");
         double n2;
         sync_get(s, &n2, c->d_result + 2, sizeof(double));
         const double b = std::sqrt(n2);
@@ -1027,7 +1081,8 @@ int plain_recursion(esc_ctx *c, const void *start, int steps, double eps, const 
         } else if (exhausted) {
             throw EscError(ESC_ERR_NUMERIC, "replayed recursion ended early");
         }
-        hip_check(esc::launch_scale(c64, w, 1.0 / b, dim, s), "scale");
+        csp = cs;
+        cs = 1.0 / b;
         void *t = vp; vp = v; v = w; w = t;
     }
 }

@@ -83,6 +83,7 @@ struct GTerm {
 struct GTables {
     uint32_t n_sites, tile_bits, waves;
     uint32_t first_pass;            // 1: acc = diag * psi, else acc = out
+    uint32_t last_pass;             // 1: the output rows are final (the fused <psi|H psi> is formed here)
     uint32_t n_windows, blocks_per_window, xcd_windows;
     uint32_t tscat[128];
     uint32_t bscat[2][256];
@@ -127,15 +128,18 @@ void clear_error();
 // sequential-diagonal table used when g->diag_closed == 0.
 // n_uniform = number of terms in classes 0..3, n_terms = all terms (for the
 // LDS size).
+// dot_part (may be null): the last pass also writes, per tile, the partial
+// sums (re, im) of <in|out> over the tile's rows, at dot_part[2 * tile].
 hipError_t launch_hpsi_tile(bool c64, const GTables *d_g, const HTables *d_h, int tile_bits, int waves,
-                            int n_uniform, int n_terms, const void *in, void *out, uint64_t dim, hipStream_t s);
+                            int n_uniform, int n_terms, const void *in, void *out, double *dot_part, uint64_t dim,
+                            hipStream_t s);
 // tile shapes the tiled kernel is compiled for: 2^10..2^12 rows (2^13 for
 // c64), 4, 8 or 16 waves per workgroup, 2 to 8 row groups of 64 per wave
 bool tile_shape_supported(bool c64, int tile_bits, int waves);
 // dynamic LDS of one tiled workgroup: the tile, the uniform coefficients per
 // row group, the lane-dependent tables and their per-lane offsets
 size_t tile_lds_bytes(bool c64, int tile_bits, int n_uniform, int n_terms);
-constexpr size_t kMaxTileLds = 160 * 1024 - 512;   // 160 KiB per CU minus the static row-group table
+constexpr size_t kMaxTileLds = 160 * 1024 - 1024;  // 160 KiB per CU minus the static row-group and sum tables
 hipError_t launch_hpsi_c128(const HTables *d_tab, const void *in, void *out, uint64_t dim, hipStream_t s);
 hipError_t launch_hpsi_c64(const HTables *d_tab, const void *in, void *out, uint64_t dim, hipStream_t s);
 hipError_t launch_current_c128(const CurrentTable *d_tab, const void *in, void *out, uint64_t dim, hipStream_t s);
@@ -157,6 +161,11 @@ hipError_t launch_lanczos_update(bool c64, void *w, const void *v, const void *u
 hipError_t launch_normalize(bool c64, void *x, const double *d_norm2, uint64_t dim, hipStream_t s);
 // x *= scale (real)
 hipError_t launch_scale(bool c64, void *x, double scale, uint64_t dim, hipStream_t s);
+// result[q] = sum over p of partials[p * nvals + q], q < nvals, in a fixed order
+hipError_t launch_reduce(const double *partials, int npart, int nvals, double *result, hipStream_t s);
+// w = a w + b v + e u (a, e real, b complex; u may be null), |w|^2 into *result
+hipError_t launch_lanczos_update_s(bool c64, void *w, const void *v, const void *u, double a, double br, double bi,
+                                   double e, uint64_t dim, double *partials, double *result, hipStream_t s);
 // result[0] = |x|^2
 hipError_t launch_norm2(bool c64, const void *x, uint64_t dim, double *partials, double *result, hipStream_t s);
 // coef[m] = <V_m | w> for m < k (V_m = base + m * dim), FP64; result has 2k doubles

@@ -208,10 +208,38 @@ __device__ __forceinline__ void hpsi_class(const GTables *__restrict__ G, uint32
     }
 }
 
+// (a, b) summed over the NT threads of a workgroup in a fixed order; thread 0
+// writes the two sums to dst
+template <int NT>
+__device__ __forceinline__ void group_sum2_store(double a, double b, double *dst) {
+    __shared__ double sh[2][NT / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        b += __shfl_xor(b, off);
+    }
+    if ((threadIdx.x & 63u) == 0) {
+        sh[0][threadIdx.x >> 6] = a;
+        sh[1][threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double x = 0.0, y = 0.0;
+#pragma unroll
+        for (int i = 0; i < NT / 64; ++i) {
+            x += sh[0][i];
+            y += sh[1][i];
+        }
+        dst[0] = x;
+        dst[1] = y;
+    }
+}
+
 template <typename R, int TB, int WPB>
 __global__ __launch_bounds__(WPB * 64) void k_hpsi_tile(const GTables *__restrict__ G, const HTables *__restrict__ H,
                                                         const typename Vec2<R>::T *__restrict__ in,
-                                                        typename Vec2<R>::T *__restrict__ out, uint32_t nblk_total,
+                                                        typename Vec2<R>::T *__restrict__ out,
+                                                        double *__restrict__ dot_part, uint32_t nblk_total,
                                                         uint32_t block_base) {
     using V2 = typename Vec2<R>::T;
     constexpr int NGRP = 1 << (TB - 6), NG = NGRP / WPB;
@@ -309,6 +337,19 @@ __global__ __launch_bounds__(WPB * 64) void k_hpsi_tile(const GTables *__restric
         const uint32_t r = s_rhi[wave + (uint32_t)WPB * (uint32_t)k] | lane;
         nvec2 o = {ar[k], ai[k]};
         __builtin_nontemporal_store(o, reinterpret_cast<nvec2 *>(out + r));
+    }
+    // <in|out> over the tile (the Lanczos alpha), from the staged input rows
+    // and the stored values, accumulated in FP64
+    if (dot_part != nullptr && G->last_pass) {
+        double dr = 0.0, di = 0.0;
+#pragma unroll
+        for (int k = 0; k < NG; ++k) {
+            const V2 x = s_in[((wave + (uint32_t)WPB * (uint32_t)k) << 6) | lane];
+            const double xr = x.x, xi = x.y, yr = ar[k], yi = ai[k];
+            dr += xr * yr + xi * yi;
+            di += xr * yi - xi * yr;
+        }
+        group_sum2_store<WPB * 64>(dr, di, dot_part + 2 * (size_t)(blockIdx.x + block_base));
     }
 }
 
@@ -475,6 +516,33 @@ __global__ __launch_bounds__(kThreads) void k_lanczos_update(typename Vec2<R>::T
         o.y = (R)wi;
         w[i] = o;
         acc[0] += (double)o.x * o.x + (double)o.y * o.y;
+    }
+    block_reduce_store<1>(acc, partials + blockIdx.x);
+}
+
+// w = a w + b v + e u (a, e real, b complex; u may be null), partial |w|^2
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_lanczos_update_s(typename Vec2<R>::T *__restrict__ w,
+                                                               const typename Vec2<R>::T *__restrict__ v,
+                                                               const typename Vec2<R>::T *__restrict__ u, double a,
+                                                               double br, double bi, double e, uint64_t dim,
+                                                               double *__restrict__ partials) {
+    double acc[1] = {0.0};
+    const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+    for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < dim; i += stride) {
+        const double vr = v[i].x, vi = v[i].y;
+        double wr = a * (double)w[i].x + (br * vr - bi * vi);
+        double wi = a * (double)w[i].y + (br * vi + bi * vr);
+        if (u != nullptr) {
+            wr += e * (double)u[i].x;
+            wi += e * (double)u[i].y;
+        }
+        typename Vec2<R>::T o;
+        o.x = (R)wr;
+        o.y = (R)wi;
+        w[i] = o;
+        const double sr = o.x, si = o.y;
+        acc[0] += sr * sr + si * si;
     }
     block_reduce_store<1>(acc, partials + blockIdx.x);
 }
@@ -793,8 +861,8 @@ static hipError_t launch_rows_current(const CurrentTable *t, const void *in, voi
 }
 
 template <typename R, int TB, int WPB>
-static hipError_t launch_tile_tb(const GTables *g, const HTables *h, const void *in, void *out, uint32_t ntiles,
-                                 size_t lds, hipStream_t s) {
+static hipError_t launch_tile_tb(const GTables *g, const HTables *h, const void *in, void *out, double *dot_part,
+                                 uint32_t ntiles, size_t lds, hipStream_t s) {
     // tiles above 64 KiB need the raised dynamic-LDS limit; raised once to
     // the full 160 KiB, outside any stream capture (the first launch of a
     // shape is an eager one)
@@ -808,7 +876,7 @@ static hipError_t launch_tile_tb(const GTables *g, const HTables *h, const void 
     for (uint32_t base = 0; base < ntiles; base += kMaxGrid) {
         const uint32_t n = (ntiles - base) < kMaxGrid ? (ntiles - base) : kMaxGrid;
         hipLaunchKernelGGL((k_hpsi_tile<R, TB, WPB>), dim3(n), dim3(WPB * 64), lds, s, g, h, (CV2p<R>)in,
-                           (V2p<R>)out, ntiles, base);
+                           (V2p<R>)out, dot_part, ntiles, base);
     }
     return hipGetLastError();
 }
@@ -830,7 +898,7 @@ bool tile_shape_supported(bool c64, int tile_bits, int waves) {
 }
 
 hipError_t launch_hpsi_tile(bool c64, const GTables *g, const HTables *h, int tile_bits, int waves, int n_uniform,
-                            int n_terms, const void *in, void *out, uint64_t dim, hipStream_t s) {
+                            int n_terms, const void *in, void *out, double *dot_part, uint64_t dim, hipStream_t s) {
     const uint64_t ntiles64 = dim >> tile_bits;
     if (ntiles64 == 0 || ntiles64 > 0x7fffffffull || !tile_shape_supported(c64, tile_bits, waves))
         return hipErrorInvalidValue;
@@ -839,8 +907,8 @@ hipError_t launch_hpsi_tile(bool c64, const GTables *g, const HTables *h, int ti
     if (lds > kMaxTileLds) return hipErrorInvalidValue;
 #define ESC_TILE_CASE(TBV, WV)                                                                          \
     if (tile_bits == TBV && waves == WV)                                                                \
-        return c64 ? launch_tile_tb<float, TBV, WV>(g, h, in, out, ntiles, lds, s)                      \
-                   : launch_tile_tb<double, TBV, WV>(g, h, in, out, ntiles, lds, s);
+        return c64 ? launch_tile_tb<float, TBV, WV>(g, h, in, out, dot_part, ntiles, lds, s)            \
+                   : launch_tile_tb<double, TBV, WV>(g, h, in, out, dot_part, ntiles, lds, s);
     ESC_TILE_CASE(10, 4)
     ESC_TILE_CASE(11, 4)
     ESC_TILE_CASE(10, 8)
@@ -848,7 +916,8 @@ hipError_t launch_hpsi_tile(bool c64, const GTables *g, const HTables *h, int ti
     ESC_TILE_CASE(12, 8)
     ESC_TILE_CASE(11, 16)
     ESC_TILE_CASE(12, 16)
-    if (tile_bits == 13 && waves == 16 && c64) return launch_tile_tb<float, 13, 16>(g, h, in, out, ntiles, lds, s);
+    if (tile_bits == 13 && waves == 16 && c64)
+        return launch_tile_tb<float, 13, 16>(g, h, in, out, dot_part, ntiles, lds, s);
 #undef ESC_TILE_CASE
     return hipErrorInvalidValue;
 }
@@ -904,6 +973,23 @@ hipError_t launch_axpby(bool c64, double ar, double ai, const void *x, double br
     else
         hipLaunchKernelGGL(k_axpby<double>, dim3(stream_blocks(dim)), dim3(kThreads), 0, s, ar, ai, (CV2p<double>)x,
                            br, bi, (V2p<double>)y, dim);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce(const double *partials, int npart, int nvals, double *result, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_final, dim3(nvals), dim3(kThreads), 0, s, partials, npart, nvals, result);
+    return hipGetLastError();
+}
+
+hipError_t launch_lanczos_update_s(bool c64, void *w, const void *v, const void *u, double a, double br, double bi,
+                                   double e, uint64_t dim, double *partials, double *result, hipStream_t s) {
+    if (c64)
+        hipLaunchKernelGGL(k_lanczos_update_s<float>, dim3(kRedBlocks), dim3(kThreads), 0, s, (V2p<float>)w,
+                           (CV2p<float>)v, (CV2p<float>)u, a, br, bi, e, dim, partials);
+    else
+        hipLaunchKernelGGL(k_lanczos_update_s<double>, dim3(kRedBlocks), dim3(kThreads), 0, s, (V2p<double>)w,
+                           (CV2p<double>)v, (CV2p<double>)u, a, br, bi, e, dim, partials);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kThreads), 0, s, partials, kRedBlocks, 1, result);
     return hipGetLastError();
 }
 
