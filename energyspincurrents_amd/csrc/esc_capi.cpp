@@ -920,12 +920,32 @@ void run_hpsi_dot(esc_ctx *c, const void *in, void *out, hipStream_t s) {
     }
 }
 
-// classical Gram-Schmidt pass against V_0..V_{k-1} (applied twice by callers)
+// classical Gram-Schmidt pass against V_0..V_{k-1}
 void cgs_pass(esc_ctx *c, int k, void *w, hipStream_t s) {
     c->ensure_partials((size_t)2 * k * 256);
-    c->ensure_result((size_t)2 * k + 4);
+    c->ensure_result((size_t)2 * k + 8);
     hip_check(esc::launch_multidot(c->c64(), c->vec(0), k, w, c->dim, c->d_partials, c->d_result, s), "multidot");
     hip_check(esc::launch_multi_axpy(c->c64(), c->vec(0), k, c->d_result, -1.0, w, c->dim, false, s), "multi_axpy");
+}
+
+// Reorthogonalise the unit vector w against V_0..V_{k-1} and normalise it
+// (continued fractions with reorthogonalisation): one classical Gram-Schmidt
+// pass, and a second one only when the first removed more than half of w's
+// squared norm (|w| < 1/sqrt 2 afterwards: the "twice is enough" test of
+// Daniel, Gragg, Kaufman and Stewart).  With reorthogonalisation at every
+// step the first pass removes components at the rounding level and the second
+// is skipped; after a breakdown (w mostly inside the basis) both run.
+void reorthogonalize(esc_ctx *c, int k, void *w, hipStream_t s) {
+    const bool c64 = c->c64();
+    cgs_pass(c, k, w, s);
+    hip_check(esc::launch_norm2(c64, w, c->dim, c->d_partials, c->d_result + 4, s), "norm2");
+    double n2;
+    sync_get(s, &n2, c->d_result + 4, sizeof(double));
+    if (!(n2 >= 0.5)) {
+        cgs_pass(c, k, w, s);
+        hip_check(esc::launch_norm2(c64, w, c->dim, c->d_partials, c->d_result + 4, s), "norm2");
+    }
+    hip_check(esc::launch_normalize(c64, w, c->d_result + 4, c->dim, s), "normalize");
 }
 
 // Three-vector Lanczos without reorthogonalisation from phi/|phi| (work holds
@@ -1017,12 +1037,7 @@ void cf_run(esc_ctx *ctx, const void *d_phi, int steps, void *work[3], double *a
             break;
         }
         hip_check(esc::launch_scale(c64, w, 1.0 / b, dim, s), "scale");
-        if (k > 0) {
-            cgs_pass(ctx, k + 1, w, s);
-            cgs_pass(ctx, k + 1, w, s);
-            hip_check(esc::launch_norm2(c64, w, dim, ctx->d_partials, ctx->d_result + 4, s), "norm2");
-            hip_check(esc::launch_normalize(c64, w, ctx->d_result + 4, dim, s), "normalize");
-        }
+        if (k > 0) reorthogonalize(ctx, k + 1, w, s);
     }
     *iters = k;
 }
@@ -1706,6 +1721,9 @@ int esc_lanczos(esc_ctx *ctx, const void *d_v0, int steps, double eps, int mode,
             bt.push_back(b);
             hip_check(esc::launch_scale(c64, w, 1.0 / b, dim, s), "scale");
             if (reortho && k > 0) {   // (src/DynLanczosSolver.h:105-119), no reorthogonalisation at k = 0
+                // two classical passes always: the orthogonality of the
+                // reference's modified Gram-Schmidt, on which its Ritz spectra
+                // (and their reproduction) depend
                 cgs_pass(ctx, k + 1, w, s);
                 cgs_pass(ctx, k + 1, w, s);
                 hip_check(esc::launch_norm2(c64, w, dim, ctx->d_partials, ctx->d_result + 4, s), "norm2");

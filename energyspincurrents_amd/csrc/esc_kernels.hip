@@ -571,19 +571,24 @@ __global__ __launch_bounds__(kThreads) void k_scale(typename Vec2<R>::T *__restr
     }
 }
 
-// coef[m] = <V_m|w> partials: grid (nchunk, ceil(k/4)); 4 basis vectors per block
+// coef[m] = <V_m|w> partials: grid (nchunk, ceil(k/kMultiVecs)); kMultiVecs
+// basis vectors per block, so w is read ceil(k/kMultiVecs) times
 constexpr int kMultiChunks = 256;
+constexpr int kMultiVecs = 4;
 template <typename R>
 __global__ __launch_bounds__(kThreads) void k_multidot_partial(const typename Vec2<R>::T *__restrict__ base, int k,
                                                                const typename Vec2<R>::T *__restrict__ w,
                                                                uint64_t dim, double *__restrict__ partials) {
-    const int m0 = blockIdx.y * 4;
-    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    constexpr int MV = kMultiVecs;
+    const int m0 = blockIdx.y * MV;
+    double v[2 * MV];
+#pragma unroll
+    for (int q = 0; q < 2 * MV; ++q) v[q] = 0.0;
     const uint64_t stride = (uint64_t)gridDim.x * kThreads;
     for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < dim; i += stride) {
         const double wr = w[i].x, wi = w[i].y;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < MV; ++q) {
             if (m0 + q < k) {
                 const typename Vec2<R>::T x = base[(uint64_t)(m0 + q) * dim + i];
                 v[2 * q] += (double)x.x * wr + (double)x.y * wi;
@@ -592,18 +597,18 @@ __global__ __launch_bounds__(kThreads) void k_multidot_partial(const typename Ve
         }
     }
     // partial layout: partials[chunk * (2k) + 2m + {0,1}]
-    __shared__ double sh[8][kThreads];
+    __shared__ double sh[2 * MV][kThreads];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) sh[q][threadIdx.x] = v[q];
+    for (int q = 0; q < 2 * MV; ++q) sh[q][threadIdx.x] = v[q];
     __syncthreads();
     for (int off = kThreads / 2; off > 0; off >>= 1) {
         if ((int)threadIdx.x < off) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + off];
+            for (int q = 0; q < 2 * MV; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + off];
         }
         __syncthreads();
     }
-    if (threadIdx.x < 8) {
+    if (threadIdx.x < 2 * MV) {
         const int q = threadIdx.x / 2;
         if (m0 + q < k) partials[(size_t)blockIdx.x * (2 * k) + 2 * (m0 + q) + (threadIdx.x & 1)] = sh[threadIdx.x][0];
     }
@@ -1026,7 +1031,7 @@ hipError_t launch_scale(bool c64, void *x, double sc, uint64_t dim, hipStream_t 
 hipError_t launch_multidot(bool c64, const void *base, int k, const void *w, uint64_t dim, double *partials,
                            double *result, hipStream_t s) {
     if (k <= 0) return hipSuccess;
-    dim3 grid(kMultiChunks, (k + 3) / 4);
+    dim3 grid(kMultiChunks, (k + kMultiVecs - 1) / kMultiVecs);
     if (c64)
         hipLaunchKernelGGL(k_multidot_partial<float>, grid, dim3(kThreads), 0, s, (CV2p<float>)base, k,
                            (CV2p<float>)w, dim, partials);
