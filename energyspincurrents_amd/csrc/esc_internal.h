@@ -185,9 +185,10 @@ struct BatchLane {
     uint64_t dim;
     uint32_t blk0, nblk;
 };
-constexpr int kBatchDot = 0;      // alpha[i][k] = <v|w>
-constexpr int kBatchUpdate = 1;   // w -= alpha v + beta_k vp; beta[i][k+1] = |w|; w /= beta; k += 1
-constexpr int kBatchAccum = 2;    // acc += coef[i][k] v
+// The batched recursion keeps unnormalised vectors u_k (v_k = u_k / beta_k):
+constexpr int kBatchDot = 0;      // alpha[i][k] = <v_k|H v_k>
+constexpr int kBatchUpdate = 1;   // w = H v_k - alpha v_k - beta_k v_{k-1} (unscaled); beta[i][k+1] = |w|; k += 1
+constexpr int kBatchAccum = 2;    // acc += coef[i][k] v_k
 hipError_t launch_batch_step(bool c64, int piece, const BatchLane *d_lanes, int n, uint32_t total_blocks,
                              double *partials, double *alpha, int lda, double *beta, int ldb, const double *coef,
                              int ldc, uint32_t *step, hipStream_t s);

@@ -719,20 +719,40 @@ __global__ __launch_bounds__(kThreads) void k_b_dot(const BatchLane *__restrict_
     block_reduce_store<2>(v, partials + 2 * blockIdx.x);
 }
 
-// one block per instance: alpha[i][k] = sum of its partials (fixed order)
+// The batched recursion keeps unnormalised vectors u_k, v_k = c_k u_k with
+// c_0 = 1 (unit start vectors) and c_k = 1/beta_k (0 once the Krylov space is
+// exhausted), so no pass rescales w after its norm is known.
+__device__ __forceinline__ double batch_scale(const double *__restrict__ beta_row, uint32_t k) {
+    if (k == 0) return 1.0;
+    const double b = beta_row[k];
+    return b > 0.0 ? 1.0 / b : 0.0;
+}
+
+// one block per instance: alpha[i][k] = c_k^2 <u_k|H u_k>, the sum of its
+// partials (fixed order)
 __global__ __launch_bounds__(kThreads) void k_b_alpha(const BatchLane *__restrict__ L, const double *__restrict__ partials,
                                                       double *__restrict__ alpha, int lda,
+                                                      const double *__restrict__ beta, int ldb,
                                                       const uint32_t *__restrict__ step) {
     const BatchLane &l = L[blockIdx.x];
+    __shared__ double tot[2];
     double v[2] = {0.0, 0.0};
     for (uint32_t p = threadIdx.x; p < l.nblk; p += kThreads) {
         v[0] += partials[2 * (size_t)(l.blk0 + p)];
         v[1] += partials[2 * (size_t)(l.blk0 + p) + 1];
     }
-    block_reduce_store<2>(v, alpha + 2 * ((size_t)blockIdx.x * lda + *step));
+    block_reduce_store<2>(v, tot);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t k = *step;
+        const double c = batch_scale(beta + (size_t)blockIdx.x * ldb, k);
+        double *a = alpha + 2 * ((size_t)blockIdx.x * lda + k);
+        a[0] = c * c * tot[0];
+        a[1] = c * c * tot[1];
+    }
 }
 
-// w -= alpha_k v + beta_k vp ; partial |w|^2
+// w = c_k w - alpha_k c_k v - beta_k c_{k-1} vp (w holds H u_k); partial |w|^2
 template <typename R>
 __global__ __launch_bounds__(kThreads) void k_b_update(const BatchLane *__restrict__ L, int n,
                                                        const double *__restrict__ alpha, int lda,
@@ -742,16 +762,18 @@ __global__ __launch_bounds__(kThreads) void k_b_update(const BatchLane *__restri
     const int inst = batch_instance(L, n, blockIdx.x);
     const BatchLane &l = L[inst];
     const uint32_t k = *step;
-    const double ar = alpha[2 * ((size_t)inst * lda + k)], ai = alpha[2 * ((size_t)inst * lda + k) + 1];
-    const double b = beta[(size_t)inst * ldb + k];
+    const double *brow = beta + (size_t)inst * ldb;
+    const double c = batch_scale(brow, k), cp = k > 0 ? batch_scale(brow, k - 1) : 0.0;
+    const double ar = alpha[2 * ((size_t)inst * lda + k)] * c, ai = alpha[2 * ((size_t)inst * lda + k) + 1] * c;
+    const double b = brow[k] * cp;
     const V2 *v = (const V2 *)l.v, *u = (const V2 *)l.vp;
     V2 *w = (V2 *)l.w;
     double acc[1] = {0.0};
     const uint64_t stride = (uint64_t)l.nblk * kThreads;
     for (uint64_t i = (uint64_t)(blockIdx.x - l.blk0) * kThreads + threadIdx.x; i < l.dim; i += stride) {
         const double vr = v[i].x, vi = v[i].y;
-        double wr = (double)w[i].x - (ar * vr - ai * vi) - b * (double)u[i].x;
-        double wi = (double)w[i].y - (ar * vi + ai * vr) - b * (double)u[i].y;
+        double wr = c * (double)w[i].x - (ar * vr - ai * vi) - b * (double)u[i].x;
+        double wi = c * (double)w[i].y - (ar * vi + ai * vr) - b * (double)u[i].y;
         V2 o;
         o.x = (R)wr;
         o.y = (R)wi;
@@ -784,35 +806,17 @@ __global__ __launch_bounds__(kThreads) void k_b_beta(const BatchLane *__restrict
     }
 }
 
-// w /= beta_{k+1} (zero when beta_{k+1} = 0)
+// acc += coef[i][k] v_k = coef[i][k] c_k u_k  (replay of a recursion forming
+// a Ritz vector)
 template <typename R>
-__global__ __launch_bounds__(kThreads) void k_b_scale(const BatchLane *__restrict__ L, int n,
+__global__ __launch_bounds__(kThreads) void k_b_accum(const BatchLane *__restrict__ L, int n,
+                                                      const double *__restrict__ coef, int ldc,
                                                       const double *__restrict__ beta, int ldb,
                                                       const uint32_t *__restrict__ step) {
     using V2 = typename Vec2<R>::T;
     const int inst = batch_instance(L, n, blockIdx.x);
     const BatchLane &l = L[inst];
-    const double b = beta[(size_t)inst * ldb + *step + 1];
-    const double sc = b > 0.0 ? 1.0 / b : 0.0;
-    V2 *w = (V2 *)l.w;
-    const uint64_t stride = (uint64_t)l.nblk * kThreads;
-    for (uint64_t i = (uint64_t)(blockIdx.x - l.blk0) * kThreads + threadIdx.x; i < l.dim; i += stride) {
-        V2 o = w[i];
-        o.x = (R)((double)o.x * sc);
-        o.y = (R)((double)o.y * sc);
-        w[i] = o;
-    }
-}
-
-// acc += coef[i][k] v  (replay of a recursion forming a Ritz vector)
-template <typename R>
-__global__ __launch_bounds__(kThreads) void k_b_accum(const BatchLane *__restrict__ L, int n,
-                                                      const double *__restrict__ coef, int ldc,
-                                                      const uint32_t *__restrict__ step) {
-    using V2 = typename Vec2<R>::T;
-    const int inst = batch_instance(L, n, blockIdx.x);
-    const BatchLane &l = L[inst];
-    const double c = coef[(size_t)inst * ldc + *step];
+    const double c = coef[(size_t)inst * ldc + *step] * batch_scale(beta + (size_t)inst * ldb, *step);
     const V2 *v = (const V2 *)l.v;
     V2 *a = (V2 *)l.acc;
     const uint64_t stride = (uint64_t)l.nblk * kThreads;
@@ -1092,7 +1096,8 @@ hipError_t launch_batch_step(bool c64, int piece, const BatchLane *d_lanes, int 
     case kBatchDot:
         if (c64) hipLaunchKernelGGL(k_b_dot<float>, g, t, 0, s, d_lanes, n, partials);
         else hipLaunchKernelGGL(k_b_dot<double>, g, t, 0, s, d_lanes, n, partials);
-        hipLaunchKernelGGL(k_b_alpha, gi, t, 0, s, d_lanes, (const double *)partials, alpha, lda, (const uint32_t *)step);
+        hipLaunchKernelGGL(k_b_alpha, gi, t, 0, s, d_lanes, (const double *)partials, alpha, lda,
+                           (const double *)beta, ldb, (const uint32_t *)step);
         break;
     case kBatchUpdate:
         if (c64)
@@ -1103,13 +1108,15 @@ hipError_t launch_batch_step(bool c64, int piece, const BatchLane *d_lanes, int 
                                (const double *)beta, ldb, (const uint32_t *)step, partials);
         hipLaunchKernelGGL(k_b_beta, gi, t, 0, s, d_lanes, (const double *)partials, (const double *)alpha, lda, beta,
                            ldb, (const uint32_t *)step);
-        if (c64) hipLaunchKernelGGL(k_b_scale<float>, g, t, 0, s, d_lanes, n, (const double *)beta, ldb, (const uint32_t *)step);
-        else hipLaunchKernelGGL(k_b_scale<double>, g, t, 0, s, d_lanes, n, (const double *)beta, ldb, (const uint32_t *)step);
         hipLaunchKernelGGL(k_b_next_step, dim3(1), dim3(64), 0, s, step);
         break;
     case kBatchAccum:
-        if (c64) hipLaunchKernelGGL(k_b_accum<float>, g, t, 0, s, d_lanes, n, coef, ldc, (const uint32_t *)step);
-        else hipLaunchKernelGGL(k_b_accum<double>, g, t, 0, s, d_lanes, n, coef, ldc, (const uint32_t *)step);
+        if (c64)
+            hipLaunchKernelGGL(k_b_accum<float>, g, t, 0, s, d_lanes, n, coef, ldc, (const double *)beta, ldb,
+                               (const uint32_t *)step);
+        else
+            hipLaunchKernelGGL(k_b_accum<double>, g, t, 0, s, d_lanes, n, coef, ldc, (const double *)beta, ldb,
+                               (const uint32_t *)step);
         break;
     default:
         return hipErrorInvalidValue;
