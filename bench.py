@@ -82,8 +82,8 @@ def time_hpsi(eng, x, y, steps, warmup, copy_calib=False):
 PROFILED_CONFIG = {"c128": 581118, "c64": 581218}
 
 MEASURED_TRAFFIC = {
-    ("c128", 581118): 2090866720,   # 758781 KiB fetched x 2.0 + 524301 KiB written
-    ("c64", 581218): 834079552,     # 276191 KiB fetched x 2.0 + 262150 KiB written
+    ("c128", 581118): 2095688704,   # 761128 KiB fetched x 2.0 + 524315 KiB written
+    ("c64", 581218): 832004096,     # 275178 KiB fetched x 2.0 + 262148 KiB written
 }
 
 
