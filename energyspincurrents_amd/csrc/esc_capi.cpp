@@ -404,6 +404,19 @@ uint32_t grow_window(int N, uint32_t T, const std::vector<uint32_t> &masks, int 
 // the window width allows, tiles second.
 constexpr int kCoverFirst = 5;   // pass digit of the cover-first planner
 
+// A kernel configuration is 100000 * P + 10000 * V + 100 * TB + W: P planned
+// passes (kCoverFirst: the cover-first planner), 2^V waves per workgroup,
+// 2^TB-row tiles, 2^W-row windows; 0 = one row per lane.
+int cfg_passes(int config) { return (config / 100000) % 10; }
+int cfg_waves(int config) { return 1 << ((config / 10000) % 10); }
+int cfg_tile_bits(int config) { return (config / 100) % 100; }
+int cfg_window(int config) { return config % 100; }
+int make_config(int P, int waves, int tb, int w) {
+    int v = 0;
+    while ((1 << v) < waves) ++v;
+    return 100000 * P + 10000 * v + 100 * tb + w;
+}
+
 std::vector<esc::GTables> build_passes(const esc_ctx *c, int TB, int w, int P, int WPB) {
     const int N = c->n_sites;
     if (w > N) w = N;
@@ -811,7 +824,7 @@ const std::vector<esc_ctx::PassDev> &passes_for(esc_ctx *c, int config) {
     auto it = c->d_pass.find(config);
     if (it != c->d_pass.end()) return it->second;
     const std::vector<esc::GTables> host =
-        build_passes(c, (config / 100) % 100, config % 100, (config / 100000) % 10, (config / 10000) % 10);
+        build_passes(c, cfg_tile_bits(config), cfg_window(config), cfg_passes(config), cfg_waves(config));
     std::vector<esc_ctx::PassDev> dev;
     for (const auto &g : host) {
         esc::GTables *d = nullptr;
@@ -831,8 +844,8 @@ bool tile_config_fits(const esc_ctx *c, int config) {
     if (it != c->tile_fits.end()) return it->second;
     bool ok = true;
     try {
-        for (const auto &g :
-             build_passes(c, (config / 100) % 100, config % 100, (config / 100000) % 10, (config / 10000) % 10))
+        for (const auto &g : build_passes(c, cfg_tile_bits(config), cfg_window(config), cfg_passes(config),
+                                          cfg_waves(config)))
             if (esc::tile_lds_bytes(c->c64(), (int)g.tile_bits, (int)g.cls_begin[4],
                                     (int)g.cls_begin[esc::kNumClasses]) > esc::kMaxTileLds)
                 ok = false;
@@ -843,9 +856,8 @@ bool tile_config_fits(const esc_ctx *c, int config) {
     return ok;
 }
 
-// kernel configurations: 100000 * planned passes + 10000 * waves per
-// workgroup + 100 * tile bits + window bits (window >= N: one pass);
-// 0 = one row per lane
+// kernel configurations (make_config; window >= N: one pass), 0 = one row per
+// lane
 std::vector<int> tile_candidates(const esc_ctx *c) {
     std::vector<int> v;
     const int N = c->n_sites;
@@ -858,7 +870,7 @@ std::vector<int> tile_candidates(const esc_ctx *c) {
                 if (ww < tb || N - ww > 16 || ww - tb > 16) continue;
                 for (int P : {1, 2, 3, kCoverFirst}) {
                     if ((ww == N) != (P == 1)) continue;
-                    const int cfg = 100000 * P + 10000 * waves + 100 * tb + ww;
+                    const int cfg = make_config(P, waves, tb, ww);
                     if (std::find(v.begin(), v.end(), cfg) == v.end() && tile_config_fits(c, cfg)) v.push_back(cfg);
                 }
             }
@@ -875,14 +887,15 @@ int default_tile_bits(const esc_ctx *c) {
     const int N = c->n_sites;
     if (N < 11) return 0;
     const std::vector<int> cand = tile_candidates(c);
-    for (int cfg : {c->c64() ? 581218 : 581118, 100000 + 80000 + 100 * 11 + N, 100000 + 40000 + 100 * 10 + N})
+    for (int cfg : {c->c64() ? make_config(kCoverFirst, 8, 12, 18) : make_config(kCoverFirst, 8, 11, 18),
+                    make_config(1, 8, 11, N), make_config(1, 4, 10, N)})
         if (std::find(cand.begin(), cand.end(), cfg) != cand.end()) return cfg;
     return 0;
 }
 
 void launch_hpsi_with(esc_ctx *c, int config, const void *in, void *out, hipStream_t s, double *dot_part = nullptr) {
     if (config > 0) {
-        const int waves = (config / 10000) % 10, tb = (config / 100) % 100;
+        const int waves = cfg_waves(config), tb = cfg_tile_bits(config);
         for (const auto &p : passes_for(c, config))
             hip_check(esc::launch_hpsi_tile(c->c64(), p.d, c->d_htab, tb, waves, p.n_uniform, p.n_terms, in, out,
                                             dot_part, c->dim, s),
@@ -909,7 +922,7 @@ void run_hpsi(esc_ctx *c, const void *in, void *out, hipStream_t s) {
 void run_hpsi_dot(esc_ctx *c, const void *in, void *out, hipStream_t s) {
     c->ensure_result(8);
     if (c->tile_bits > 0) {
-        const uint64_t ntiles = c->dim >> ((c->tile_bits / 100) % 100);
+        const uint64_t ntiles = c->dim >> cfg_tile_bits(c->tile_bits);
         c->ensure_partials(2 * ntiles + 2 * esc::kRedBlocks);
         launch_hpsi_with(c, c->tile_bits, in, out, s, c->d_partials);
         hip_check(esc::launch_reduce(c->d_partials, (int)ntiles, 2, c->d_result, s), "reduce");

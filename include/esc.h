@@ -101,7 +101,7 @@ int esc_ctx_info(const esc_ctx *ctx, int64_t *dim, int32_t *elem_bytes, int32_t 
  * kernel configuration the context supports over `reps` launches on
  * d_in -> d_out, keeps the fastest for all later H.psi calls of the context
  * and reports it as tile_bits.  A configuration is 0 (one row per lane) or
- * 100000 * P + 10000 * V + 100 * TB + W: workgroups of V waves sharing 2^TB
+ * 100000 * P + 10000 * V + 100 * TB + W: workgroups of 2^V waves sharing 2^TB
  * rows staged in LDS, walking windows of 2^W rows (one XCD per window) in P
  * planned passes (P = 1 when W = N), plus passes for terms no window held;
  * P = 5: windows chosen first to hold every term in as few passes as the

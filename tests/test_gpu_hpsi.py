@@ -133,8 +133,8 @@ def test_error_paths():
 def test_tiled_kernels_matrix_elements_bit_exact_n18(kw):
     """Every H.psi kernel configuration (one row per lane; tiles of 2^10 to
     2^12 rows with 4, 8 or 16 waves; one pass, or two and three planned passes
-    over 2^16- and 2^17-row windows) gives the oracle's matrix elements bit for
-    bit at N=18."""
+    and the cover-first plan over 2^16- and 2^17-row windows) gives the
+    oracle's matrix elements bit for bit at N=18."""
     m = O.Model(3, 6, **kw)
     eng = engine_for(m)
     rng = np.random.default_rng(16)
