@@ -79,11 +79,11 @@ def time_hpsi(eng, x, y, steps, warmup, copy_calib=False):
 # factors that make a plain one-vector read / write read its own size); the
 # summary with the commands is profiles/hpsi_n24_hbm_traffic.md
 # the configurations the autotuner picks for this workload, profiled under profiles/
-PROFILED_CONFIG = {"c128": 531118, "c64": 531218}
+PROFILED_CONFIG = {"c128": 541218, "c64": 531218}
 
 MEASURED_TRAFFIC = {
-    ("c128", 531118): 2107759616,   # 767028 KiB fetched x 2.0 + 524303 KiB written
-    ("c64", 531218): 827220992,     # 272838 KiB fetched x 2.0 + 262157 KiB written
+    ("c128", 541218): 1845611520,   # 638890 KiB fetched x 2.0 + 524575 KiB written
+    ("c64", 531218): 743146496,     # 231755 KiB fetched x 2.0 + 262219 KiB written
 }
 
 

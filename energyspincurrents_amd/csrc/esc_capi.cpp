@@ -609,6 +609,8 @@ std::vector<esc::GTables> build_passes(const esc_ctx *c, int TB, int w, int P, i
             gt.mlo = ht.mask & 63u;
             gt.bits = (uint32_t)ht.b0 | ((uint32_t)ht.b1 << 8) | ((uint32_t)ht.b2 << 16);
             std::memcpy(gt.coef, ht.coef, sizeof(gt.coef));
+            for (int e = 0; e < 8; ++e)
+                for (int q = 0; q < 2; ++q) gt.coef32[e][q] = (float)ht.coef[e][q];
             const bool far = (ht.mask & ~(63u | gsites)) != 0;
             gt.mhib = ht.mask & ~(63u | gsites);
             uint32_t mg = 0;     // mask on the group index
@@ -879,15 +881,16 @@ std::vector<int> tile_candidates(const esc_ctx *c) {
     return v;
 }
 
-// before tuning: the shapes measured fastest at N = 24 (8 waves, cover-first
-// 2^18-row windows, 2^11-row tiles for c128 and 2^12-row tiles for c64: two
-// passes), else a single pass over 2^11-row tiles; systems below 2^11 rows use
-// one row per lane
+// before tuning: the shapes measured fastest at N = 24 (cover-first 2^18-row
+// windows, two passes, 2^12-row tiles; 16 waves for c128, which moves the
+// least HBM traffic of the equally fast shapes, and 8 waves for c64), else a
+// single pass over 2^11-row tiles; systems below 2^11 rows use one row per
+// lane
 int default_tile_bits(const esc_ctx *c) {
     const int N = c->n_sites;
     if (N < 11) return 0;
     const std::vector<int> cand = tile_candidates(c);
-    for (int cfg : {c->c64() ? make_config(kCoverFirst, 8, 12, 18) : make_config(kCoverFirst, 8, 11, 18),
+    for (int cfg : {c->c64() ? make_config(kCoverFirst, 8, 12, 18) : make_config(kCoverFirst, 16, 12, 18),
                     make_config(1, 8, 11, N), make_config(1, 4, 10, N)})
         if (std::find(cand.begin(), cand.end(), cfg) != cand.end()) return cfg;
     return 0;

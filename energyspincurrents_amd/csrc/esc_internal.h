@@ -64,6 +64,7 @@ struct GTerm {
     uint32_t bits;          // b0 | b1 << 8 | b2 << 16
     uint32_t pad[2];
     double coef[8][2];
+    float coef32[8][2];     // the same table rounded to float (complex64 vectors)
     // per row-group slot j = wave * (groups per wave) + k of the workgroup:
     // near terms: tile element index of the partner group's first row;
     // far terms: partner row bits on the tile-group sites

@@ -159,14 +159,31 @@ __device__ __forceinline__ double hpsi_diag(const GTables *__restrict__ G, const
     return d;
 }
 
+// coefficient table of a term in the vector precision
+template <typename R> __device__ __forceinline__ const unsigned char *coef_table(const GTerm &T);
+template <> __device__ __forceinline__ const unsigned char *coef_table<double>(const GTerm &T) {
+    return reinterpret_cast<const unsigned char *>(T.coef);
+}
+template <> __device__ __forceinline__ const unsigned char *coef_table<float>(const GTerm &T) {
+    return reinterpret_cast<const unsigned char *>(T.coef32);
+}
+
+// selector bits of a term that sit on tile-base or window-id sites
+__device__ __forceinline__ uint32_t base_selector(const GTerm &T, uint32_t base) {
+    uint32_t sel = 0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) sel |= ((base >> (T.bsel[q] & 0xffffu)) & (T.bsel[q] >> 16)) << q;
+    return sel;
+}
+
 // Terms of one class for all NG row groups of a wave.  Row group k of wave w
 // is g = w + WPB * k, rows r = base | tscat[g] | lane; its per-term constants
 // sit in slot w * NG + k of the term record.
 template <typename R, int TB, int WPB, int CLS>
 __device__ __forceinline__ void hpsi_class(const GTables *__restrict__ G, uint32_t base, uint32_t wave,
                                            uint32_t lane, const typename Vec2<R>::T *__restrict__ in,
-                                           const typename Vec2<R>::T *s_in, const typename Vec2<R>::T *s_cu,
-                                           const typename Vec2<R>::T *s_ct, const uint32_t *s_lp, R *ar, R *ai) {
+                                           const typename Vec2<R>::T *s_in, const typename Vec2<R>::T *s_ct,
+                                           const uint32_t *s_lp, R *ar, R *ai) {
     using V2 = typename Vec2<R>::T;
     constexpr int NGRP = 1 << (TB - 6), NG = NGRP / WPB;
     constexpr bool CPLX = (CLS & 1) != 0, FAR = (CLS & 2) != 0, LANE = (CLS & 4) != 0;
@@ -182,8 +199,11 @@ __device__ __forceinline__ void hpsi_class(const GTables *__restrict__ G, uint32
         const uint32_t loff = lane ^ T.mlo;
         const V2 *src = FAR ? in + (size_t)(base ^ T.mhib) : s_in;
         // lane-dependent coefficients: byte offset of this lane's entry in the
-        // term's table for row group 0 of the tile (set up per workgroup)
+        // term's LDS table for row group 0 of the tile (set up per workgroup);
+        // wave-uniform ones: the tile's entry of the term's table in memory,
+        // read through the scalar unit
         const uint32_t lpc = LANE ? s_lp[(t - nU) * 64u + lane] : 0u;
+        const unsigned char *utab = LANE ? nullptr : coef_table<R>(T) + base_selector(T, base) * (uint32_t)sizeof(V2);
 #pragma unroll
         for (int k = 0; k < NG; ++k) {
             const uint32_t j = slot0 + (uint32_t)k;
@@ -192,7 +212,7 @@ __device__ __forceinline__ void hpsi_class(const GTables *__restrict__ G, uint32
             if (LANE) {
                 c = *reinterpret_cast<const V2 *>(reinterpret_cast<const unsigned char *>(s_ct) + lpc + T.ugrp[j]);
             } else {
-                c = s_cu[t * NGRP + wave + (uint32_t)WPB * (uint32_t)k];
+                c = *reinterpret_cast<const V2 *>(utab + T.ugrp[j]);
             }
             // accumulate with fused multiply-adds (4 FMAs per complex term)
             if (CPLX) {
@@ -247,9 +267,8 @@ __global__ __launch_bounds__(WPB * 64) void k_hpsi_tile(const GTables *__restric
     constexpr uint32_t NROW = 1u << TB;
     extern __shared__ unsigned char esc_lds[];
     V2 *s_in = reinterpret_cast<V2 *>(esc_lds);                 // the tile
-    V2 *s_cu = s_in + NROW;                                     // uniform-term coefficients [nU][NGRP]
     const uint32_t nU = G->cls_begin[4], nT = G->cls_begin[kNumClasses];
-    V2 *s_ct = s_cu + nU * NGRP;                                // lane-term tables [nT - nU][8]
+    V2 *s_ct = s_in + NROW;                                     // lane-term tables [nT - nU][8]
     uint32_t *s_lp = reinterpret_cast<uint32_t *>(s_ct + (nT - nU) * 8u);   // [nT - nU][64] byte offsets
     __shared__ uint32_t s_rhi[NGRP];                            // first row of each row group
     // block -> (window, tile within window); with xcd_windows every window
@@ -270,16 +289,6 @@ __global__ __launch_bounds__(WPB * 64) void k_hpsi_tile(const GTables *__restric
     for (uint32_t g = threadIdx.x; g < (uint32_t)NGRP; g += NT) s_rhi[g] = base | G->tscat[g];
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < NROW; i += NT) s_in[i] = in[s_rhi[i >> 6] | (i & 63u)];
-    for (uint32_t e = threadIdx.x; e < nU * NGRP; e += NT) {
-        const uint32_t t = e / NGRP, g = e % NGRP;
-        const uint32_t r_hi = s_rhi[g], bits = G->term[t].bits;
-        const uint32_t idx = ((r_hi >> (bits & 0xffu)) & 1u) | (((r_hi >> ((bits >> 8) & 0xffu)) & 1u) << 1) |
-                             (((r_hi >> (bits >> 16)) & 1u) << 2);
-        V2 c;
-        c.x = (R)G->term[t].coef[idx][0];
-        c.y = (R)G->term[t].coef[idx][1];
-        s_cu[e] = c;
-    }
     for (uint32_t e = threadIdx.x; e < (nT - nU) * 8u; e += NT) {
         V2 c;
         c.x = (R)G->term[nU + e / 8u].coef[e % 8u][0];
@@ -316,19 +325,22 @@ __global__ __launch_bounds__(WPB * 64) void k_hpsi_tile(const GTables *__restric
     } else {
 #pragma unroll
         for (int k = 0; k < NG; ++k) {
-            const V2 o = out[s_rhi[wave + (uint32_t)WPB * (uint32_t)k] | lane];
+            // streamed like the store below: keeps the window rows in L2
+            typedef R nvec2 __attribute__((ext_vector_type(2)));
+            const nvec2 o = __builtin_nontemporal_load(
+                reinterpret_cast<const nvec2 *>(out + (s_rhi[wave + (uint32_t)WPB * (uint32_t)k] | lane)));
             ar[k] = o.x;
             ai[k] = o.y;
         }
     }
-    hpsi_class<R, TB, WPB, 0>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_lp, ar, ai);
-    hpsi_class<R, TB, WPB, 1>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_lp, ar, ai);
-    hpsi_class<R, TB, WPB, 2>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_lp, ar, ai);
-    hpsi_class<R, TB, WPB, 3>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_lp, ar, ai);
-    hpsi_class<R, TB, WPB, 4>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_lp, ar, ai);
-    hpsi_class<R, TB, WPB, 5>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_lp, ar, ai);
-    hpsi_class<R, TB, WPB, 6>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_lp, ar, ai);
-    hpsi_class<R, TB, WPB, 7>(G, base, wave, lane, in, s_in, s_cu, s_ct, s_lp, ar, ai);
+    hpsi_class<R, TB, WPB, 0>(G, base, wave, lane, in, s_in, s_ct, s_lp, ar, ai);
+    hpsi_class<R, TB, WPB, 1>(G, base, wave, lane, in, s_in, s_ct, s_lp, ar, ai);
+    hpsi_class<R, TB, WPB, 2>(G, base, wave, lane, in, s_in, s_ct, s_lp, ar, ai);
+    hpsi_class<R, TB, WPB, 3>(G, base, wave, lane, in, s_in, s_ct, s_lp, ar, ai);
+    hpsi_class<R, TB, WPB, 4>(G, base, wave, lane, in, s_in, s_ct, s_lp, ar, ai);
+    hpsi_class<R, TB, WPB, 5>(G, base, wave, lane, in, s_in, s_ct, s_lp, ar, ai);
+    hpsi_class<R, TB, WPB, 6>(G, base, wave, lane, in, s_in, s_ct, s_lp, ar, ai);
+    hpsi_class<R, TB, WPB, 7>(G, base, wave, lane, in, s_in, s_ct, s_lp, ar, ai);
     // streamed result: a non-temporal store keeps it from displacing the
     // input rows that other tiles still gather from
     typedef R nvec2 __attribute__((ext_vector_type(2)));
@@ -892,8 +904,7 @@ static hipError_t launch_tile_tb(const GTables *g, const HTables *h, const void 
 
 size_t tile_lds_bytes(bool c64, int tile_bits, int n_uniform, int n_terms) {
     const size_t esz = c64 ? 8 : 16;
-    const size_t ngrp = (size_t)1 << (tile_bits - 6);
-    return esz * (((size_t)1 << tile_bits) + (size_t)n_uniform * ngrp + (size_t)(n_terms - n_uniform) * 8) +
+    return esz * (((size_t)1 << tile_bits) + (size_t)(n_terms - n_uniform) * 8) +
            (size_t)(n_terms - n_uniform) * 64 * sizeof(uint32_t);
 }
 
@@ -901,7 +912,7 @@ size_t tile_lds_bytes(bool c64, int tile_bits, int n_uniform, int n_terms) {
 // unrolled term loop spills
 bool tile_shape_supported(bool c64, int tile_bits, int waves) {
     if (waves != 4 && waves != 8 && waves != 16) return false;
-    if (tile_bits < 10 || tile_bits > (c64 ? 13 : 12)) return false;
+    if (tile_bits < 10 || tile_bits > 13) return false;
     const int groups = (1 << (tile_bits - 6)) / waves;
     return groups >= 2 && groups <= 8;
 }
@@ -925,8 +936,7 @@ hipError_t launch_hpsi_tile(bool c64, const GTables *g, const HTables *h, int ti
     ESC_TILE_CASE(12, 8)
     ESC_TILE_CASE(11, 16)
     ESC_TILE_CASE(12, 16)
-    if (tile_bits == 13 && waves == 16 && c64)
-        return launch_tile_tb<float, 13, 16>(g, h, in, out, dot_part, ntiles, lds, s);
+    ESC_TILE_CASE(13, 16)
 #undef ESC_TILE_CASE
     return hipErrorInvalidValue;
 }
