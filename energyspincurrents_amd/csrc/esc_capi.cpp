@@ -132,11 +132,48 @@ int tridiag_ql(int n, const double *a, const double *b, double *evals, double *Z
     return 0;
 }
 
-// lowest eigenvalue only (used for the per-iteration convergence test)
+// Lowest eigenvalue only (the per-iteration convergence test), by bisection
+// on the Sturm count: the number of eigenvalues below x is the number of
+// negative pivots q_i = (a_i - x) - b_i^2 / q_{i-1} of T - x I.  O(n) per
+// halving instead of the O(n^2) of a QL solve; backward stable like QL, the
+// result is within a few ulp of |lambda| (or 2^-128 of the Gershgorin width).
 double tridiag_lowest(int n, const double *a, const double *b) {
-    std::vector<double> ev(n);
-    if (tridiag_ql(n, a, b, ev.data(), nullptr) != 0) throw EscError(ESC_ERR_NUMERIC, "tridiagonal QL did not converge");
-    return ev[0];
+    if (n == 1) return a[0];
+    double lo = std::numeric_limits<double>::infinity(), hi = -lo, bmax = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double r = (i > 0 ? std::fabs(b[i]) : 0.0) + (i + 1 < n ? std::fabs(b[i + 1]) : 0.0);
+        lo = std::min(lo, a[i] - r);
+        hi = std::max(hi, a[i] + r);
+        if (i > 0) bmax = std::max(bmax, b[i] * b[i]);
+    }
+    if (!std::isfinite(lo) || !std::isfinite(hi)) throw EscError(ESC_ERR_NUMERIC, "non-finite tridiagonal matrix");
+    // pivots below pivmin in magnitude are replaced by -pivmin
+    const double pivmin = std::numeric_limits<double>::min() * std::max(1.0, bmax);
+    auto below = [&](double x) {   // eigenvalues < x
+        int c = 0;
+        double q = a[0] - x;
+        for (int i = 0;;) {
+            if (std::fabs(q) < pivmin) q = -pivmin;
+            if (q < 0.0) ++c;
+            if (++i == n) break;
+            q = (a[i] - x) - (b[i] * b[i]) / q;
+        }
+        return c;
+    };
+    // eigenvalues lie in [lo, hi]: widen by a margin so below(lo) == 0
+    const double margin = 2.0 * std::numeric_limits<double>::epsilon() * std::max(std::fabs(lo), std::fabs(hi)) +
+                          2.0 * pivmin;
+    lo -= margin;
+    hi += margin;
+    for (int it = 0; it < 256; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (!(mid > lo && mid < hi)) break;   // interval at machine resolution
+        if (hi - lo <= 4.0 * std::numeric_limits<double>::epsilon() * std::max(std::fabs(lo), std::fabs(hi)))
+            break;
+        if (below(mid) >= 1) hi = mid;
+        else lo = mid;
+    }
+    return 0.5 * (lo + hi);
 }
 
 }  // namespace
@@ -966,6 +1003,35 @@ void reorthogonalize(esc_ctx *c, int k, void *w, hipStream_t s) {
 
 // Three-vector Lanczos without reorthogonalisation from phi/|phi| (work holds
 // three device vectors distinct from phi).  alpha[0..it-1], beta[0..it].
+// Steps of the plain recursion between two host checks of its scalars: the
+// steps are enqueued without a host round trip (scalars on the device, see
+// k_rec_step), and the exit rules are applied afterwards to every step in
+// order, so the result is that of a check after each step; at most this many
+// minus one steps past the exit are computed and dropped.
+constexpr int kRecChunk = 16;
+
+// device layout of the recursion scalars inside the context's result buffer:
+// [0..1] the H.psi dot, [2] |w|^2, [4..5] rec, [8 ..] 3 doubles per step
+constexpr size_t kRecOff = 4, kHistOff = 8;
+
+// one step k of the recursion from u (u_prev = up, unused at k = 0) into w
+void rec_step(esc_ctx *c, const void *u, void *w, const void *up, int k, hipStream_t s) {
+    double *rec = c->d_result + kRecOff, *hist = c->d_result + kHistOff;
+    run_hpsi_dot(c, u, w, s);
+    hip_check(esc::launch_lanczos_update_rec(c->c64(), w, u, k > 0 ? up : nullptr, rec, c->d_result,
+                                             k > 0 ? hist + 3 * ((size_t)k - 1) + 2 : nullptr, c->dim,
+                                             c->d_partials, c->d_result + 2, s),
+              "update");
+    hip_check(esc::launch_rec_step(rec, c->d_result, hist + 3 * (size_t)k, s), "recursion scalars");
+}
+
+// hist entries [from, to) of the recursion, copied to the host
+std::vector<double> rec_history(esc_ctx *c, int from, int to, hipStream_t s) {
+    std::vector<double> h(3 * (size_t)(to - from));
+    if (to > from) sync_get(s, h.data(), c->d_result + kHistOff + 3 * (size_t)from, h.size() * sizeof(double));
+    return h;
+}
+
 void cf_run(esc_ctx *ctx, const void *d_phi, int steps, void *work[3], double *alpha, double *beta, int *iters,
             double *norm2, hipStream_t s, bool reortho = false) {
     check_ptr(alpha, "alpha");
@@ -996,33 +1062,31 @@ void cf_run(esc_ctx *ctx, const void *d_phi, int steps, void *work[3], double *a
         // (c_0 = 1/|phi|, c_k = 1/beta_k): H v_k = c_k H u_k and alpha_k =
         // c_k^2 <u_k|H u_k> (formed by the H.psi kernel itself), and the
         // update w = c_k H u_k - alpha_k c_k u_k - beta_k c_{k-1} u_{k-1}
-        // becomes u_{k+1}.  No separate dot or scaling pass per step.
+        // becomes u_{k+1}.  No separate dot or scaling pass per step, and the
+        // scalars stay on the device between the checks.
+        ctx->ensure_result(kHistOff + 3 * ((size_t)steps + 1));
         void *up = work[0], *u = work[1], *w = work[2];
         hip_check(hipMemcpyAsync(u, d_phi, dim * ctx->esz, hipMemcpyDeviceToDevice, s), "copy");
-        double c = 1.0 / std::sqrt(n2), cp = 0.0;
+        hip_check(esc::launch_rec_init(ctx->d_result + kRecOff, 1.0 / std::sqrt(n2), s), "recursion start");
+        int checked = 0;
         for (; k < steps; ++k) {
-            run_hpsi_dot(ctx, u, w, s);
-            double d[2];
-            sync_get(s, d, ctx->d_result, 2 * sizeof(double));
-            const double ar = c * c * d[0], ai = c * c * d[1];
-            if (!std::isfinite(ar)) throw EscError(ESC_ERR_NUMERIC, "non-finite alpha");
-            alpha[k] = ar;
-            hip_check(esc::launch_lanczos_update_s(c64, w, u, k > 0 ? up : nullptr, c, -ar * c, -ai * c,
-                                                   -beta[k] * cp, dim, ctx->d_partials, ctx->d_result + 2, s),
-                      "update");
-            double b2;
-            sync_get(s, &b2, ctx->d_result + 2, sizeof(double));
-            const double b = std::sqrt(b2);
-            beta[k + 1] = b;
-            if (!std::isfinite(b)) throw EscError(ESC_ERR_NUMERIC, "non-finite beta");
-            if (b <= 1e-13 * std::max(1.0, std::fabs(ar))) {   // Krylov space exhausted: the fraction terminates
-                beta[k + 1] = 0.0;
-                ++k;
-                break;
-            }
-            cp = c;
-            c = 1.0 / b;
+            rec_step(ctx, u, w, up, k, s);
             void *t = up; up = u; u = w; w = t;
+            if ((k + 1) % kRecChunk != 0 && k + 1 != steps) continue;
+            const std::vector<double> h = rec_history(ctx, checked, k + 1, s);
+            for (int j = checked; j <= k; ++j) {
+                const double ar = h[3 * (size_t)(j - checked)], b = h[3 * (size_t)(j - checked) + 2];
+                if (!std::isfinite(ar)) throw EscError(ESC_ERR_NUMERIC, "non-finite alpha");
+                alpha[j] = ar;
+                beta[j + 1] = b;
+                if (!std::isfinite(b)) throw EscError(ESC_ERR_NUMERIC, "non-finite beta");
+                if (b <= 1e-13 * std::max(1.0, std::fabs(ar))) {   // Krylov space exhausted: the fraction terminates
+                    beta[j + 1] = 0.0;
+                    *iters = j + 1;
+                    return;
+                }
+            }
+            checked = k + 1;
         }
         *iters = k;
         return;
@@ -1069,52 +1133,60 @@ int plain_recursion(esc_ctx *c, const void *start, int steps, double eps, const 
     const bool c64 = c->c64();
     const uint64_t dim = c->dim;
     c->ensure_partials(2 * esc::kRedBlocks);
-    c->ensure_result(8);
+    c->ensure_result(kHistOff + 3 * ((size_t)std::max(steps, n_fixed) + 1));
     // unnormalised vectors u_k with v_k = cs_k u_k (cs_0 = 1, cs_k = 1/beta_k):
     // alpha_k = cs_k^2 <u_k|H u_k> comes from the H.psi kernel, and the update
-    // w = cs_k H u_k - alpha_k cs_k u_k - beta_k cs_{k-1} u_{k-1} is u_{k+1}
+    // w = cs_k H u_k - alpha_k cs_k u_k - beta_k cs_{k-1} u_{k-1} is u_{k+1};
+    // the scalars stay on the device (rec_step)
     void *vp = c->vec(0), *v = c->vec(1), *w = c->vec(2);
     hip_check(hipMemcpyAsync(v, start, dim * c->esz, hipMemcpyDeviceToDevice, s), "copy start");
+    hip_check(esc::launch_rec_init(c->d_result + kRecOff, 1.0, s), "recursion start");
     ar.clear();
     bt.assign(1, 0.0);
-    double E0 = 1.0, cs = 1.0, csp = 0.0;
-    for (int k = 0;; ++k) {
-        if (coef) {
-            hip_check(esc::launch_axpby(c64, coef[k].real() * cs, coef[k].imag() * cs, v, 1.0, 0.0, acc, dim, s),
+    if (coef) {
+        for (int k = 0;; ++k) {
+            hip_check(esc::launch_axpy_rec(c64, coef[k].real(), coef[k].imag(), c->d_result + kRecOff, v, acc, dim, s),
                       "accumulate");
-            if (k + 1 == n_fixed) return n_fixed;
+            if (k + 1 == n_fixed) break;
+            rec_step(c, v, w, vp, k, s);
+            void *t = vp; vp = v; v = w; w = t;
         }
-        run_hpsi_dot(c, v, w, s);
-        double a[2];
-        sync_get(s, a, c->d_result, 2 * sizeof(double));
-        a[0] *= cs * cs;
-        a[1] *= cs * cs;
-        if (!std::isfinite(a[0]) || !std::isfinite(a[1])) throw EscError(ESC_ERR_NUMERIC, "non-finite alpha");
-        ar.push_back(a[0]);
-        hip_check(esc::launch_lanczos_update_s(c64, w, v, k > 0 ? vp : nullptr, cs, -a[0] * cs, -a[1] * cs,
-                                               -bt[k] * csp, dim, c->d_partials, c->d_result + 2, s),
-                  "update");
-        double n2;
-        sync_get(s, &n2, c->d_result + 2, sizeof(double));
-        const double b = std::sqrt(n2);
-        if (!std::isfinite(b)) throw EscError(ESC_ERR_NUMERIC, "non-finite beta");
-        bt.push_back(b);
-        const bool exhausted = b <= 1e-13 * std::max(1.0, std::fabs(a[0]));
-        if (!coef) {
-            const int n = k + 1;
+        // the replay runs as long as the recorded recursion did
+        const std::vector<double> h = rec_history(c, 0, n_fixed - 1, s);
+        for (int j = 0; j + 1 < n_fixed; ++j) {
+            const double a0 = h[3 * (size_t)j], b = h[3 * (size_t)j + 2];
+            ar.push_back(a0);
+            bt.push_back(b);
+            if (!std::isfinite(b) || b <= 1e-13 * std::max(1.0, std::fabs(a0)))
+                throw EscError(ESC_ERR_NUMERIC, "replayed recursion ended early");
+        }
+        return n_fixed;
+    }
+    double E0 = 1.0;
+    int checked = 0;
+    for (int k = 0;; ++k) {
+        rec_step(c, v, w, vp, k, s);
+        void *t = vp; vp = v; v = w; w = t;
+        if ((k + 1) % kRecChunk != 0 && k + 1 != steps) continue;
+        const std::vector<double> h = rec_history(c, checked, k + 1, s);
+        for (int j = checked; j <= k; ++j) {
+            const double a0 = h[3 * (size_t)(j - checked)], a1 = h[3 * (size_t)(j - checked) + 1];
+            const double b = h[3 * (size_t)(j - checked) + 2];
+            if (!std::isfinite(a0) || !std::isfinite(a1)) throw EscError(ESC_ERR_NUMERIC, "non-finite alpha");
+            ar.push_back(a0);
+            if (!std::isfinite(b)) throw EscError(ESC_ERR_NUMERIC, "non-finite beta");
+            bt.push_back(b);
+            const bool exhausted = b <= 1e-13 * std::max(1.0, std::fabs(a0));
+            const int n = j + 1;
             bool done = exhausted || n == steps;
-            if (n >= 2) {
+            if (n >= 2 && eps > 0.0) {   // with eps <= 0 the rule cannot fire
                 const double D0 = tridiag_lowest(n, ar.data(), bt.data());
                 if (std::fabs(E0 - D0) < eps) done = true;
                 E0 = D0;
             }
             if (done) return n;
-        } else if (exhausted) {
-            throw EscError(ESC_ERR_NUMERIC, "replayed recursion ended early");
         }
-        csp = cs;
-        cs = 1.0 / b;
-        void *t = vp; vp = v; v = w; w = t;
+        checked = k + 1;
     }
 }
 

@@ -164,9 +164,19 @@ hipError_t launch_normalize(bool c64, void *x, const double *d_norm2, uint64_t d
 hipError_t launch_scale(bool c64, void *x, double scale, uint64_t dim, hipStream_t s);
 // result[q] = sum over p of partials[p * nvals + q], q < nvals, in a fixed order
 hipError_t launch_reduce(const double *partials, int npart, int nvals, double *result, hipStream_t s);
-// w = a w + b v + e u (a, e real, b complex; u may be null), |w|^2 into *result
-hipError_t launch_lanczos_update_s(bool c64, void *w, const void *v, const void *u, double a, double br, double bi,
-                                   double e, uint64_t dim, double *partials, double *result, hipStream_t s);
+// The plain recursion with device-side scalars (rec = {c_k, c_{k-1}}, dres =
+// the H.psi dot and |w|^2, hist = 3 doubles per step); see k_rec_step.
+hipError_t launch_rec_init(double *rec, double c0, hipStream_t s);
+// w = c H u - alpha c u - beta_k c' u_prev from rec, dres[0..1], *beta_k;
+// |w|^2 into *result
+hipError_t launch_lanczos_update_rec(bool c64, void *w, const void *v, const void *u, const double *rec,
+                                     const double *dres, const double *beta_k, uint64_t dim, double *partials,
+                                     double *result, hipStream_t s);
+// hist[0..2] = (alpha re, alpha im, beta), then rec advances one step
+hipError_t launch_rec_step(double *rec, const double *dres, double *hist, hipStream_t s);
+// y += (cr + i ci) rec[0] x
+hipError_t launch_axpy_rec(bool c64, double cr, double ci, const double *rec, const void *x, void *y, uint64_t dim,
+                           hipStream_t s);
 // result[0] = |x|^2
 hipError_t launch_norm2(bool c64, const void *x, uint64_t dim, double *partials, double *result, hipStream_t s);
 // coef[m] = <V_m | w> for m < k (V_m = base + m * dim), FP64; result has 2k doubles
