@@ -182,6 +182,35 @@ def test_ground_state_two_pass_matches_exact_diagonalisation(kw):
     assert it > 0
 
 
+def test_ground_state_exit_rule_matches_oracle():
+    """The plain recursion stops at the first step n with |E0(n) - E0(n-1)| <
+    eps (E0(1) = 1), E0(n) the lowest eigenvalue of the n x n tridiagonal
+    matrix (src/DynLanczosSolver.h:139-162).  The device evaluates E0 by Sturm
+    bisection and applies the rule after the fact every 16 steps; the oracle
+    runs the same recursion in numpy with a dense eigvalsh per step.  eps is
+    placed between the differences so that the exit step is unambiguous; the
+    reported H.psi count of a run without restarts is 2n (n steps, a replay of
+    n - 1, and the Rayleigh quotient)."""
+    m = O.Model(2, 6, K=(-1.0, -0.8, -1.2), J=0.1, G=0.2, B=O.field111(0.2))
+    eng = engine_for(m)
+    v0 = eng.start_vector(9, "Gaussian")
+    ref = O.lanczos_ref(lambda v: O.hpsi_gather(m, v), eng.download(v0), 80, eps=0.0, reortho=False)
+    E = [1.0] + list(ref.energies)            # E0 before n = 2, then after n = 2, 3, ...
+    diff = np.abs(np.diff(E))                 # diff[n - 2]: the test after n steps
+    # exit steps with a clear margin: the difference there is below half of
+    # every earlier one and far above rounding; prefer steps past the first
+    # 16-step host check
+    clear = [n for n in range(3, 80) if 1e-10 < diff[n - 2] < 0.5 * diff[: n - 2].min()]
+    assert clear
+    picks = ([n for n in clear if n > 16] or clear)[:3]
+    for n_exit in picks:
+        eps = float(np.sqrt(diff[n_exit - 2] * diff[: n_exit - 2].min()))
+        _, e0, _, it = eng.ground_state(v0, steps=200, eps=eps, restarts=0)
+        print(f"[exit rule] n = {n_exit} eps = {eps:.3e} H.psi = {it}", flush=True)
+        assert it == 2 * n_exit
+        assert abs(e0 - ref.energies[n_exit - 2]) < 1e-5 * max(1.0, abs(e0))
+
+
 def test_ground_state_replay_equals_reorthogonalised_ritz_vector():
     """For a short run (no loss of orthogonality yet) the replayed three-vector
     ground state equals the reorthogonalised one."""
