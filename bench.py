@@ -9,6 +9,10 @@ the JeJe spectrum pipeline in both precisions, the 32-site complex64 H.psi
 oracle's OpenMP gather H.psi on the host cores, a bounded sample).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-spectrum] [--no-n32] [--no-batch]
+                  [--pipelines]
+
+--pipelines adds the whole 12-site and 32-site runs (ground state, JsJs and
+JeJe spectra), timed stage by stage; it is not part of the default run.
 
 The spectrum block times the whole JeJe pipeline at the same size in both
 precisions: ground state (200 Lanczos steps with one restart from the
@@ -47,6 +51,9 @@ def parse():
     ap.add_argument("--kernel-config", default=None,
                     help="C128,C64: H.psi kernel configurations (default: the ones profiled under profiles/)")
     ap.add_argument("--autotune", action="store_true", help="pick the H.psi kernel configurations by timing them")
+    ap.add_argument("--pipelines", action="store_true",
+                    help="also time the full 12-site and 32-site runs (ground state + JsJs and JeJe spectra; "
+                         "about 6 minutes, progress on stderr)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run a plain device copy of one vector per step (HBM counter calibration)")
     return ap.parse_args()
@@ -113,6 +120,8 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding "
               "of the 2^N vector is a design note, not a measured path", file=sys.stderr)
         sys.exit(2)
+    if args.pipelines:
+        os.environ.setdefault("ESC_VERBOSE", "1")   # one stderr line per finished stage
     import torch
     import energyspincurrents_amd as esc
 
@@ -176,6 +185,8 @@ def main():
         out["n32"] = n32_run(esc)
     if not args.no_batch:
         out["batch"] = batch_run(esc)
+    if args.pipelines:
+        out["pipelines"] = pipelines_run(esc)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_reps)
     else:
@@ -201,6 +212,57 @@ def n32_run(esc):
             "hpsi_per_s": 1e3 / ms, "ms_per_hpsi": ms, "achieved_GBs": bytes_alg / (ms * 1e-3) / 1e9,
             "frac": bytes_alg / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, "bytes_per_launch": bytes_alg,
             "kernel_config": config}
+
+
+def pipelines_run(esc):
+    """Two whole runs, timed stage by stage (seconds of wall time, the device
+    synchronised after each stage), omega = linspace(0, 6, 2000), eta = 0.05,
+    spectra by continued fractions, seeded complex-Gaussian start vectors:
+    * 12 sites (2x3 cells, periodic), K = -1, h = 0, complex128: ground state
+      300 steps, then JsJs and JeJe with 200 steps per fraction;
+    * 32 sites (4x4 cells, periodic), K = -1, h[111] = 0.3, complex64 vectors
+      with FP64 scalars: ground state 250 steps, then JeJe and JsJs with 150
+      steps per fraction."""
+    import torch
+    w = np.linspace(0.0, 6.0, 2000)
+    out = {}
+    for name, (lx, ly), field, prec, gs_steps, cf_steps in (
+            ("n12", (2, 6), 0.0, "c128", 300, 200), ("n32", (4, 8), 0.3, "c64", 250, 150)):
+        lat = esc.Honeycomb(lx, ly)
+        eng = esc.Engine(esc.SpinModel(lat, K=(-1.0, -1.0, -1.0), B=esc.field111(field)), prec)
+        config = None
+        if name == "n32":   # the 12-site run keeps the default kernel configuration
+            config, _ = eng.autotune(eng.start_vector(1234, "Gaussian"), reps=1)
+        torch.cuda.empty_cache()
+        r = {"workload": f"{lx * ly} sites ({lx}x{ly // 2} cells, periodic), K=-1, h[111]={field}, {prec}: "
+                         f"ground state {gs_steps} steps, JsJs and JeJe continued fractions {cf_steps} steps, "
+                         "eta 0.05, 2000 omegas"}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        v0 = eng.start_vector(7, "Gaussian")
+        gs, e0, resid, it = eng.ground_state(v0, steps=gs_steps, eps=0.0, restarts=0)
+        del v0
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        print(f"[pipelines] {name} ground state {t1 - t0:.2f} s, E0 {e0:.10f}", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+        tab_je, _ = eng.dynamics("JeJe", "CF", gs, w, 0.05, e0=e0, cf_steps=cf_steps)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        print(f"[pipelines] {name} JeJe {t2 - t1:.2f} s", file=sys.stderr, flush=True)
+        tab_js, _ = eng.dynamics("JsJs", "CF", gs, w, 0.05, e0=e0, cf_steps=cf_steps)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        print(f"[pipelines] {name} JsJs {t3 - t2:.2f} s", file=sys.stderr, flush=True)
+        r.update({"E0": e0, "residual": resid, "hpsi_ground_state": it, "ground_state_s": t1 - t0,
+                  "jeje_s": t2 - t1, "jsjs_s": t3 - t2, "total_s": t3 - t0,
+                  "finite": bool(np.all(np.isfinite(tab_je)) and np.all(np.isfinite(tab_js)))})
+        if name == "n32":
+            r["kernel_config"] = config
+        out[name] = r
+        del gs, eng
+        torch.cuda.empty_cache()
+    return out
 
 
 def batch_run(esc):

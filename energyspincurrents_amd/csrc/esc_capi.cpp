@@ -11,6 +11,9 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <map>
@@ -177,6 +180,24 @@ double tridiag_lowest(int n, const double *a, const double *b) {
 }
 
 }  // namespace
+
+// One line on stderr per finished stage of a long call (a continued fraction,
+// a ground-state cycle) when the environment sets ESC_VERBOSE to anything but
+// "0"; silent otherwise.
+void progress(const char *fmt, ...) {
+    static const bool on = [] {
+        const char *v = std::getenv("ESC_VERBOSE");
+        return v != nullptr && v[0] != '\0' && std::strcmp(v, "0") != 0;
+    }();
+    if (!on) return;
+    char buf[256];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    std::fprintf(stderr, "esc: %s\n", buf);
+    std::fflush(stderr);
+}
 
 // ------------------------------------------------------------------------
 // Context
@@ -1878,6 +1899,7 @@ int esc_ground_state(esc_ctx *ctx, const void *d_v0, int steps, double eps, int 
             std::vector<double> ar, bt;
             const int n = plain_recursion(ctx, start, steps, eps, nullptr, 0, nullptr, ar, bt, s);
             total += n;
+            progress("ground state cycle %d: recursion of %d steps", cyc, n);
             std::vector<double> ev(n), Z((size_t)n * n);
             if (tridiag_ql(n, ar.data(), bt.data(), ev.data(), Z.data()) != 0)
                 throw EscError(ESC_ERR_NUMERIC, "tridiagonal QL did not converge");
@@ -1886,6 +1908,7 @@ int esc_ground_state(esc_ctx *ctx, const void *d_v0, int steps, double eps, int 
             std::vector<double> ar2, bt2;
             plain_recursion(ctx, start, steps, eps, coef.data(), n, d_gs, ar2, bt2, s);
             total += n - 1;
+            progress("ground state cycle %d: replayed", cyc);
             const double ng = device_norm2(ctx, d_gs, s);
             hip_check(esc::launch_scale(c64, d_gs, 1.0 / std::sqrt(ng), dim, s), "scale");
             if (cyc < restarts) hip_check(hipMemcpyAsync(start, d_gs, dim * ctx->esz, hipMemcpyDeviceToDevice, s), "copy");
@@ -2120,6 +2143,7 @@ int esc_dynamics(esc_ctx *ctx, int kind, int method, const void *d_gs, double e0
                 double n2 = 0.0;
                 cf_run(ctx, ctx->aux(0), cf_steps, work, al.data(), be.data(), &it, &n2, s, reortho);
                 f.push_back(cf_eval(it, al.data(), be.data(), n2, e0, eta, omega, nw));
+                progress("continued fraction %zu of %zu: %d steps", f.size(), runs.size(), it);
             }
             F = cf_assemble(js, nc, runs, f, nw);
         }
