@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstdarg>
@@ -190,12 +191,14 @@ void progress(const char *fmt, ...) {
         return v != nullptr && v[0] != '\0' && std::strcmp(v, "0") != 0;
     }();
     if (!on) return;
+    static const auto t0 = std::chrono::steady_clock::now();
     char buf[256];
     va_list ap;
     va_start(ap, fmt);
     std::vsnprintf(buf, sizeof(buf), fmt, ap);
     va_end(ap);
-    std::fprintf(stderr, "esc: %s\n", buf);
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    std::fprintf(stderr, "esc: [%.1f ms] %s\n", ms, buf);
     std::fflush(stderr);
 }
 
@@ -2840,6 +2843,7 @@ int esc_spectrum_grad(const esc_model *model, int op, int site, int gs_steps, in
         int it = 0;
         if (esc_ground_state(c, v0, gs_steps, 1e-14, 1, &e0, &resid, &it, gs, stream) != ESC_OK)
             throw EscError(ESC_ERR_NUMERIC, std::string("ground state: ") + esc_last_error());
+        progress("gradient: ground state, %d H.psi", it);
         void *t0 = V.get(), *t1 = V.get();
         void *cg_r = V.get(), *cg_p = V.get(), *cg_ap = V.get();
         // x = (H - e)^+ b on the complement of gs (b orthogonal to gs), by
@@ -2851,7 +2855,8 @@ int esc_spectrum_grad(const esc_model *model, int op, int site, int gs_steps, in
             double rr = device_norm2(c, cg_r, s);
             const double bn = std::sqrt(rr);
             if (!(bn > 0.0)) return 0.0;
-            for (int iter = 0; iter < 5000 && std::sqrt(rr) > tol * bn; ++iter) {
+            int iter = 0;
+            for (; iter < 5000 && std::sqrt(rr) > tol * bn; ++iter) {
                 run_hpsi(c, cg_p, cg_ap, s);
                 axpby(-e, cg_p, 1.0, cg_ap);
                 axpby(-dot(gs, cg_ap), gs, 1.0, cg_ap);
@@ -2864,6 +2869,7 @@ int esc_spectrum_grad(const esc_model *model, int op, int site, int gs_steps, in
                 axpby(1.0, cg_r, rr2 / rr, cg_p);
                 rr = rr2;
             }
+            progress("conjugate gradients: %d iterations, relative residual %.2e", iter, std::sqrt(rr) / bn);
             return std::sqrt(rr) / bn;
         };
         // residual correction of the ground state (Newton steps for the
@@ -2873,6 +2879,7 @@ int esc_spectrum_grad(const esc_model *model, int op, int site, int gs_steps, in
             e0 = dot(gs, t0).real();
             axpby(-e0, gs, 1.0, t0);
             resid = std::sqrt(device_norm2(c, t0, s));
+            progress("gradient: Newton pass %d, E0 %.15f, residual %.3e", pass, e0, resid);
             if (resid < 1e-13 * std::max(1.0, std::fabs(e0))) break;
             cg(e0, t0, t1, 1e-12);
             axpby(-1.0, t1, 1.0, gs);
@@ -2958,6 +2965,7 @@ int esc_spectrum_grad(const esc_model *model, int op, int site, int gs_steps, in
             M = k + 1;
             if (!(bk1 > 1e-13 * std::max(1.0, std::fabs(ak)))) break;   // Krylov space exhausted
         }
+        progress("gradient: forward recursion, %d steps", M);
         // loss and its adjoints with respect to alpha, beta, n2, e0
         std::vector<double> abar, bbar;
         double n2bar = 0.0, e0bar = 0.0;
@@ -3025,6 +3033,7 @@ int esc_spectrum_grad(const esc_model *model, int op, int site, int gs_steps, in
                 std::swap(vbar_k, vbar_prev);
             }
         }
+        progress("gradient: reverse sweep");
         // vbar_next holds v_bar_0: phi_bar = (v_bar_0 - v_0 Re<v_0|v_bar_0>) / |phi| + 2 n2_bar phi
         void *v0u = seg[1];   // the last segment restored was the first: v_0
         void *phibar = V.get();
@@ -3056,6 +3065,7 @@ int esc_spectrum_grad(const esc_model *model, int op, int site, int gs_steps, in
             g[p] -= dot(mu, t0).real();
         }
         hip_check(hipStreamSynchronize(s), "sync");
+        progress("gradient: done");
         for (int p = 0; p < 4; ++p) grad[p] = g[p];
     });
 }
