@@ -1027,27 +1027,64 @@ void reorthogonalize(esc_ctx *c, int k, void *w, hipStream_t s) {
 
 // Three-vector Lanczos without reorthogonalisation from phi/|phi| (work holds
 // three device vectors distinct from phi).  alpha[0..it-1], beta[0..it].
-// Steps of the plain recursion between two host checks of its scalars: the
-// steps are enqueued without a host round trip (scalars on the device, see
-// k_rec_step), and the exit rules are applied afterwards to every step in
-// order, so the result is that of a check after each step; at most this many
-// minus one steps past the exit are computed and dropped.
+// The plain recursion steps are enqueued without a host round trip (scalars
+// and the step index on the device, see k_rec_step); the host reads the
+// scalars back every kRecChunk steps and applies the exit rules to every step
+// in order, so the result is that of a check after each step.  At most
+// kRecChunk - 1 steps past the exit are computed and dropped.
+//
+// Device layout inside the context's result buffer: [0..1] the H.psi dot,
+// [2] |w|^2, [4..5] rec, [6] the step index (uint32), [8 ..] 3 doubles per
+// step (history), then 2 doubles per step (replay coefficients).
+constexpr size_t kRecOff = 4, kStepOff = 6, kHistOff = 8;
 constexpr int kRecChunk = 16;
 
-// device layout of the recursion scalars inside the context's result buffer:
-// [0..1] the H.psi dot, [2] |w|^2, [4..5] rec, [8 ..] 3 doubles per step
-constexpr size_t kRecOff = 4, kHistOff = 8;
+// Runs the recursion in the three rotating vectors buf: step k maps u_{k-1},
+// u_k, u_{k+1} to buf[k % 3], buf[(k + 1) % 3], buf[(k + 2) % 3].  With coef
+// (device, 2 doubles per step) every step first accumulates acc += coef_k
+// c_k u_k.
+struct RecRunner {
+    esc_ctx *c;
+    hipStream_t s;
+    void *buf[3];
+    const double *coef;   // device replay coefficients or null
+    void *acc;
+    uint32_t cap;         // history capacity in steps
+    int k = 0;            // next step
 
-// one step k of the recursion from u (u_prev = up, unused at k = 0) into w
-void rec_step(esc_ctx *c, const void *u, void *w, const void *up, int k, hipStream_t s) {
-    double *rec = c->d_result + kRecOff, *hist = c->d_result + kHistOff;
-    run_hpsi_dot(c, u, w, s);
-    hip_check(esc::launch_lanczos_update_rec(c->c64(), w, u, k > 0 ? up : nullptr, rec, c->d_result,
-                                             k > 0 ? hist + 3 * ((size_t)k - 1) + 2 : nullptr, c->dim,
-                                             c->d_partials, c->d_result + 2, s),
-              "update");
-    hip_check(esc::launch_rec_step(rec, c->d_result, hist + 3 * (size_t)k, s), "recursion scalars");
-}
+    RecRunner(esc_ctx *cx, hipStream_t st, void *b0, void *b1, void *b2, uint32_t capacity,
+              const double *d_coef = nullptr, void *d_acc = nullptr)
+        : c(cx), s(st), buf{b0, b1, b2}, coef(d_coef), acc(d_acc), cap(capacity) {}
+    double *rec() const { return c->d_result + kRecOff; }
+    uint32_t *step() const { return reinterpret_cast<uint32_t *>(c->d_result + kStepOff); }
+    double *hist() const { return c->d_result + kHistOff; }
+
+    void start(double c0) {
+        hip_check(esc::launch_rec_init(rec(), step(), c0, s), "recursion start");
+        k = 0;
+    }
+    void accumulate(int kk) {
+        hip_check(esc::launch_axpy_rec_step(c->c64(), coef, step(), rec(), buf[(kk + 1) % 3], acc, c->dim, s),
+                  "accumulate");
+    }
+    void enqueue(int kk) {
+        if (coef) accumulate(kk);
+        const void *u = buf[(kk + 1) % 3];
+        void *w = buf[(kk + 2) % 3];
+        run_hpsi_dot(c, u, w, s);
+        hip_check(esc::launch_lanczos_update_rec(c->c64(), w, u, kk > 0 ? buf[kk % 3] : nullptr, rec(), c->d_result,
+                                                 hist(), step(), c->dim, c->d_partials, c->d_result + 2, s),
+                  "update");
+        hip_check(esc::launch_rec_step(rec(), c->d_result, hist(), step(), cap, s), "recursion scalars");
+    }
+    // up to kRecChunk more steps, stopping at `target`; returns the number of
+    // steps done so far
+    int advance(int target) {
+        const int stop = std::min(target, k + kRecChunk);
+        for (; k < stop; ++k) enqueue(k);
+        return k;
+    }
+};
 
 // hist entries [from, to) of the recursion, copied to the host
 std::vector<double> rec_history(esc_ctx *c, int from, int to, hipStream_t s) {
@@ -1089,14 +1126,12 @@ void cf_run(esc_ctx *ctx, const void *d_phi, int steps, void *work[3], double *a
         // becomes u_{k+1}.  No separate dot or scaling pass per step, and the
         // scalars stay on the device between the checks.
         ctx->ensure_result(kHistOff + 3 * ((size_t)steps + 1));
-        void *up = work[0], *u = work[1], *w = work[2];
-        hip_check(hipMemcpyAsync(u, d_phi, dim * ctx->esz, hipMemcpyDeviceToDevice, s), "copy");
-        hip_check(esc::launch_rec_init(ctx->d_result + kRecOff, 1.0 / std::sqrt(n2), s), "recursion start");
+        hip_check(hipMemcpyAsync(work[1], d_phi, dim * ctx->esz, hipMemcpyDeviceToDevice, s), "copy");
+        RecRunner run(ctx, s, work[0], work[1], work[2], (uint32_t)steps);
+        run.start(1.0 / std::sqrt(n2));
         int checked = 0;
-        for (; k < steps; ++k) {
-            rec_step(ctx, u, w, up, k, s);
-            void *t = up; up = u; u = w; w = t;
-            if ((k + 1) % kRecChunk != 0 && k + 1 != steps) continue;
+        while (checked < steps) {
+            k = run.advance(steps) - 1;
             const std::vector<double> h = rec_history(ctx, checked, k + 1, s);
             for (int j = checked; j <= k; ++j) {
                 const double ar = h[3 * (size_t)(j - checked)], b = h[3 * (size_t)(j - checked) + 2];
@@ -1112,7 +1147,7 @@ void cf_run(esc_ctx *ctx, const void *d_phi, int steps, void *work[3], double *a
             }
             checked = k + 1;
         }
-        *iters = k;
+        *iters = steps;
         return;
     }
     void *vp = nullptr, *v = ctx->vec(0), *w = ctx->vec(1);
@@ -1154,31 +1189,33 @@ void cf_run(esc_ctx *ctx, const void *d_phi, int steps, void *work[3], double *a
 // vectors, accumulating acc += coef[k] v_k (complex coefficients).
 int plain_recursion(esc_ctx *c, const void *start, int steps, double eps, const std::complex<double> *coef,
                     int n_fixed, void *acc, std::vector<double> &ar, std::vector<double> &bt, hipStream_t s) {
-    const bool c64 = c->c64();
     const uint64_t dim = c->dim;
     c->ensure_partials(2 * esc::kRedBlocks);
-    c->ensure_result(kHistOff + 3 * ((size_t)std::max(steps, n_fixed) + 1));
+    const int cap = coef ? n_fixed : steps;
+    c->ensure_result(kHistOff + 3 * ((size_t)cap + 1) + 2 * (size_t)n_fixed);
     // unnormalised vectors u_k with v_k = cs_k u_k (cs_0 = 1, cs_k = 1/beta_k):
     // alpha_k = cs_k^2 <u_k|H u_k> comes from the H.psi kernel, and the update
     // w = cs_k H u_k - alpha_k cs_k u_k - beta_k cs_{k-1} u_{k-1} is u_{k+1};
-    // the scalars stay on the device (rec_step)
-    void *vp = c->vec(0), *v = c->vec(1), *w = c->vec(2);
-    hip_check(hipMemcpyAsync(v, start, dim * c->esz, hipMemcpyDeviceToDevice, s), "copy start");
-    hip_check(esc::launch_rec_init(c->d_result + kRecOff, 1.0, s), "recursion start");
+    // the scalars stay on the device (RecRunner)
+    hip_check(hipMemcpyAsync(c->vec(1), start, dim * c->esz, hipMemcpyDeviceToDevice, s), "copy start");
     ar.clear();
     bt.assign(1, 0.0);
     if (coef) {
-        for (int k = 0;; ++k) {
-            hip_check(esc::launch_axpy_rec(c64, coef[k].real(), coef[k].imag(), c->d_result + kRecOff, v, acc, dim, s),
-                      "accumulate");
-            if (k + 1 == n_fixed) break;
-            rec_step(c, v, w, vp, k, s);
-            void *t = vp; vp = v; v = w; w = t;
+        double *d_coef = c->d_result + kHistOff + 3 * ((size_t)cap + 1);
+        std::vector<double> h(2 * (size_t)n_fixed);
+        for (int j = 0; j < n_fixed; ++j) {
+            h[2 * (size_t)j] = coef[j].real();
+            h[2 * (size_t)j + 1] = coef[j].imag();
         }
-        // the replay runs as long as the recorded recursion did
-        const std::vector<double> h = rec_history(c, 0, n_fixed - 1, s);
+        hip_check(hipMemcpyAsync(d_coef, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s), "coef");
+        RecRunner run(c, s, c->vec(0), c->vec(1), c->vec(2), (uint32_t)cap, d_coef, acc);
+        run.start(1.0);
+        while (run.k < n_fixed - 1) run.advance(n_fixed - 1);
+        run.accumulate(n_fixed - 1);   // the last vector: accumulate only
+        // the replay ran as long as the recorded recursion did
+        const std::vector<double> hh = rec_history(c, 0, n_fixed - 1, s);   // also orders the upload above
         for (int j = 0; j + 1 < n_fixed; ++j) {
-            const double a0 = h[3 * (size_t)j], b = h[3 * (size_t)j + 2];
+            const double a0 = hh[3 * (size_t)j], b = hh[3 * (size_t)j + 2];
             ar.push_back(a0);
             bt.push_back(b);
             if (!std::isfinite(b) || b <= 1e-13 * std::max(1.0, std::fabs(a0)))
@@ -1186,12 +1223,12 @@ int plain_recursion(esc_ctx *c, const void *start, int steps, double eps, const 
         }
         return n_fixed;
     }
+    RecRunner run(c, s, c->vec(0), c->vec(1), c->vec(2), (uint32_t)cap);
+    run.start(1.0);
     double E0 = 1.0;
     int checked = 0;
-    for (int k = 0;; ++k) {
-        rec_step(c, v, w, vp, k, s);
-        void *t = vp; vp = v; v = w; w = t;
-        if ((k + 1) % kRecChunk != 0 && k + 1 != steps) continue;
+    for (;;) {
+        const int k = run.advance(steps) - 1;
         const std::vector<double> h = rec_history(c, checked, k + 1, s);
         for (int j = checked; j <= k; ++j) {
             const double a0 = h[3 * (size_t)(j - checked)], a1 = h[3 * (size_t)(j - checked) + 1];

@@ -164,19 +164,22 @@ hipError_t launch_normalize(bool c64, void *x, const double *d_norm2, uint64_t d
 hipError_t launch_scale(bool c64, void *x, double scale, uint64_t dim, hipStream_t s);
 // result[q] = sum over p of partials[p * nvals + q], q < nvals, in a fixed order
 hipError_t launch_reduce(const double *partials, int npart, int nvals, double *result, hipStream_t s);
-// The plain recursion with device-side scalars (rec = {c_k, c_{k-1}}, dres =
-// the H.psi dot and |w|^2, hist = 3 doubles per step); see k_rec_step.
-hipError_t launch_rec_init(double *rec, double c0, hipStream_t s);
-// w = c H u - alpha c u - beta_k c' u_prev from rec, dres[0..1], *beta_k;
-// |w|^2 into *result
+// The plain recursion with device-side scalars (rec = {c_k, c_{k-1}}, the
+// step index k in *step, dres = the H.psi dot and |w|^2, hist = 3 doubles per
+// step); see k_rec_step.  Nothing in a step depends on host values.
+hipError_t launch_rec_init(double *rec, uint32_t *step, double c0, hipStream_t s);
+// w = c H u - alpha c u - beta_k c' u_prev from rec, dres[0..1] and
+// hist[3 (k - 1) + 2] (u null at k = 0); |w|^2 into *result
 hipError_t launch_lanczos_update_rec(bool c64, void *w, const void *v, const void *u, const double *rec,
-                                     const double *dres, const double *beta_k, uint64_t dim, double *partials,
-                                     double *result, hipStream_t s);
-// hist[0..2] = (alpha re, alpha im, beta), then rec advances one step
-hipError_t launch_rec_step(double *rec, const double *dres, double *hist, hipStream_t s);
-// y += (cr + i ci) rec[0] x
-hipError_t launch_axpy_rec(bool c64, double cr, double ci, const double *rec, const void *x, void *y, uint64_t dim,
+                                     const double *dres, const double *hist, const uint32_t *step, uint64_t dim,
+                                     double *partials, double *result, hipStream_t s);
+// hist[3k .. 3k+2] = (alpha re, alpha im, beta) if k < cap, then rec and
+// *step advance one step
+hipError_t launch_rec_step(double *rec, const double *dres, double *hist, uint32_t *step, uint32_t cap,
                            hipStream_t s);
+// y += (coef[2k] + i coef[2k+1]) rec[0] x, k = *step
+hipError_t launch_axpy_rec_step(bool c64, const double *coef, const uint32_t *step, const double *rec, const void *x,
+                                void *y, uint64_t dim, hipStream_t s);
 // result[0] = |x|^2
 hipError_t launch_norm2(bool c64, const void *x, uint64_t dim, double *partials, double *result, hipStream_t s);
 // coef[m] = <V_m | w> for m < k (V_m = base + m * dim), FP64; result has 2k doubles

@@ -510,14 +510,6 @@ __global__ __launch_bounds__(kThreads) void k_axpby(double ar, double ai, const 
     axpby_body<R>(ar, ai, x, br, bi, y, dim);
 }
 
-// y += (cr + i ci) c x with the recursion scale c = rec[0] read on the device
-template <typename R>
-__global__ __launch_bounds__(kThreads) void k_axpy_rec(double cr, double ci, const double *__restrict__ rec,
-                                                       const typename Vec2<R>::T *__restrict__ x,
-                                                       typename Vec2<R>::T *__restrict__ y, uint64_t dim) {
-    const double c = rec[0];
-    axpby_body<R>(cr * c, ci * c, x, 1.0, 0.0, y, dim);
-}
 
 // w = w - alpha v - beta u  (as V2 - alpha*V1 - beta*V0, src/DynLanczosSolver.h:96),
 // partial |w|^2 of the updated w
@@ -575,16 +567,19 @@ __device__ __forceinline__ void lanczos_update_body(typename Vec2<R>::T *__restr
 
 // ---------------------------------------------------------------------------
 // The plain Lanczos recursion with its scalars kept on the device, so a step
-// needs no host round trip.  The kept vector is u_k with v_k = c_k u_k;
-// rec[0] = c_k, rec[1] = c_{k-1}; dres[0..1] = <u_k|H u_k> (the fused dot of
-// the H.psi kernel), dres[2] = |u_{k+1}|^2; hist[3k .. 3k+2] = (Re alpha_k,
-// Im alpha_k, beta_{k+1}).  Same arithmetic as the host-driven step: alpha_k =
-// c_k^2 <u_k|H u_k>, u_{k+1} = c_k H u_k - alpha_k c_k u_k - beta_k c_{k-1}
-// u_{k-1}, beta_{k+1} = |u_{k+1}|, c_{k+1} = 1 / beta_{k+1}.
+// needs no host round trip.
+// The kept vector is u_k with v_k = c_k u_k; rec[0] = c_k, rec[1] = c_{k-1},
+// and the step index k sits in device memory (rec_step); dres[0..1] =
+// <u_k|H u_k> (the fused dot of the H.psi kernel), dres[2] = |u_{k+1}|^2;
+// hist[3k .. 3k+2] = (Re alpha_k, Im alpha_k, beta_{k+1}).  Same arithmetic
+// as a host-driven step: alpha_k = c_k^2 <u_k|H u_k>, u_{k+1} = c_k H u_k -
+// alpha_k c_k u_k - beta_k c_{k-1} u_{k-1}, beta_{k+1} = |u_{k+1}|, c_{k+1} =
+// 1 / beta_{k+1}.
 // ---------------------------------------------------------------------------
-__global__ void k_rec_init(double *__restrict__ rec, double c0) {
+__global__ void k_rec_init(double *__restrict__ rec, uint32_t *__restrict__ step, double c0) {
     rec[0] = c0;
     rec[1] = 0.0;
+    step[0] = 0;
 }
 
 template <typename R>
@@ -593,22 +588,41 @@ __global__ __launch_bounds__(kThreads) void k_lanczos_update_rec(typename Vec2<R
                                                                  const typename Vec2<R>::T *__restrict__ u,
                                                                  const double *__restrict__ rec,
                                                                  const double *__restrict__ dres,
-                                                                 const double *__restrict__ beta_k, uint64_t dim,
+                                                                 const double *__restrict__ hist,
+                                                                 const uint32_t *__restrict__ step, uint64_t dim,
                                                                  double *__restrict__ partials) {
     const double c = rec[0], cp = rec[1];
     const double ar = c * c * dres[0], ai = c * c * dres[1];
-    const double e = (u != nullptr) ? -beta_k[0] * cp : 0.0;
+    // beta_k of the previous step (u is null at k = 0)
+    const double e = (u != nullptr) ? -hist[3 * (size_t)(step[0] - 1) + 2] * cp : 0.0;
     lanczos_update_body<R>(w, v, u, c, -ar * c, -ai * c, e, dim, partials);
 }
 
-__global__ void k_rec_step(double *__restrict__ rec, const double *__restrict__ dres, double *__restrict__ hist) {
+__global__ void k_rec_step(double *__restrict__ rec, const double *__restrict__ dres, double *__restrict__ hist,
+                           uint32_t *__restrict__ step, uint32_t cap) {
+    const uint32_t k = step[0];
     const double c = rec[0];
     const double b = sqrt(dres[2]);
-    hist[0] = c * c * dres[0];
-    hist[1] = c * c * dres[1];
-    hist[2] = b;
+    if (k < cap) {   // steps past the recorded capacity are never read
+        hist[3 * (size_t)k] = c * c * dres[0];
+        hist[3 * (size_t)k + 1] = c * c * dres[1];
+        hist[3 * (size_t)k + 2] = b;
+    }
     rec[1] = c;
     rec[0] = 1.0 / b;
+    step[0] = k + 1;
+}
+
+// y += (coef[2k] + i coef[2k+1]) c_k x for the step k held on the device
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_axpy_rec_step(const double *__restrict__ coef,
+                                                            const uint32_t *__restrict__ step,
+                                                            const double *__restrict__ rec,
+                                                            const typename Vec2<R>::T *__restrict__ x,
+                                                            typename Vec2<R>::T *__restrict__ y, uint64_t dim) {
+    const uint32_t k = step[0];
+    const double c = rec[0];
+    axpby_body<R>(coef[2 * (size_t)k] * c, coef[2 * (size_t)k + 1] * c, x, 1.0, 0.0, y, dim);
 }
 
 template <typename R>
@@ -1047,36 +1061,37 @@ hipError_t launch_axpby(bool c64, double ar, double ai, const void *x, double br
     return hipGetLastError();
 }
 
-hipError_t launch_rec_init(double *rec, double c0, hipStream_t s) {
-    hipLaunchKernelGGL(k_rec_init, dim3(1), dim3(1), 0, s, rec, c0);
+hipError_t launch_rec_init(double *rec, uint32_t *step, double c0, hipStream_t s) {
+    hipLaunchKernelGGL(k_rec_init, dim3(1), dim3(1), 0, s, rec, step, c0);
     return hipGetLastError();
 }
 
 hipError_t launch_lanczos_update_rec(bool c64, void *w, const void *v, const void *u, const double *rec,
-                                     const double *dres, const double *beta_k, uint64_t dim, double *partials,
-                                     double *result, hipStream_t s) {
+                                     const double *dres, const double *hist, const uint32_t *step, uint64_t dim,
+                                     double *partials, double *result, hipStream_t s) {
     if (c64)
         hipLaunchKernelGGL(k_lanczos_update_rec<float>, dim3(kRedBlocks), dim3(kThreads), 0, s, (V2p<float>)w,
-                           (CV2p<float>)v, (CV2p<float>)u, rec, dres, beta_k, dim, partials);
+                           (CV2p<float>)v, (CV2p<float>)u, rec, dres, hist, step, dim, partials);
     else
         hipLaunchKernelGGL(k_lanczos_update_rec<double>, dim3(kRedBlocks), dim3(kThreads), 0, s, (V2p<double>)w,
-                           (CV2p<double>)v, (CV2p<double>)u, rec, dres, beta_k, dim, partials);
+                           (CV2p<double>)v, (CV2p<double>)u, rec, dres, hist, step, dim, partials);
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kThreads), 0, s, partials, kRedBlocks, 1, result);
     return hipGetLastError();
 }
 
-hipError_t launch_rec_step(double *rec, const double *dres, double *hist, hipStream_t s) {
-    hipLaunchKernelGGL(k_rec_step, dim3(1), dim3(1), 0, s, rec, dres, hist);
+hipError_t launch_rec_step(double *rec, const double *dres, double *hist, uint32_t *step, uint32_t cap,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(k_rec_step, dim3(1), dim3(1), 0, s, rec, dres, hist, step, cap);
     return hipGetLastError();
 }
 
-hipError_t launch_axpy_rec(bool c64, double cr, double ci, const double *rec, const void *x, void *y, uint64_t dim,
-                           hipStream_t s) {
+hipError_t launch_axpy_rec_step(bool c64, const double *coef, const uint32_t *step, const double *rec, const void *x,
+                                void *y, uint64_t dim, hipStream_t s) {
     if (c64)
-        hipLaunchKernelGGL(k_axpy_rec<float>, dim3(stream_blocks(dim)), dim3(kThreads), 0, s, cr, ci, rec,
+        hipLaunchKernelGGL(k_axpy_rec_step<float>, dim3(stream_blocks(dim)), dim3(kThreads), 0, s, coef, step, rec,
                            (CV2p<float>)x, (V2p<float>)y, dim);
     else
-        hipLaunchKernelGGL(k_axpy_rec<double>, dim3(stream_blocks(dim)), dim3(kThreads), 0, s, cr, ci, rec,
+        hipLaunchKernelGGL(k_axpy_rec_step<double>, dim3(stream_blocks(dim)), dim3(kThreads), 0, s, coef, step, rec,
                            (CV2p<double>)x, (V2p<double>)y, dim);
     return hipGetLastError();
 }
