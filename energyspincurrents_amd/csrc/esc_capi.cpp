@@ -1073,9 +1073,9 @@ struct RecRunner {
         void *w = buf[(kk + 2) % 3];
         run_hpsi_dot(c, u, w, s);
         hip_check(esc::launch_lanczos_update_rec(c->c64(), w, u, kk > 0 ? buf[kk % 3] : nullptr, rec(), c->d_result,
-                                                 hist(), step(), c->dim, c->d_partials, c->d_result + 2, s),
+                                                 hist(), step(), c->dim, c->d_partials, s),
                   "update");
-        hip_check(esc::launch_rec_step(rec(), c->d_result, hist(), step(), cap, s), "recursion scalars");
+        hip_check(esc::launch_rec_step(rec(), c->d_result, hist(), step(), cap, c->d_partials, s), "recursion scalars");
     }
     // up to kRecChunk more steps, stopping at `target`; returns the number of
     // steps done so far

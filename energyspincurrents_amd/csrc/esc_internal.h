@@ -169,14 +169,14 @@ hipError_t launch_reduce(const double *partials, int npart, int nvals, double *r
 // step); see k_rec_step.  Nothing in a step depends on host values.
 hipError_t launch_rec_init(double *rec, uint32_t *step, double c0, hipStream_t s);
 // w = c H u - alpha c u - beta_k c' u_prev from rec, dres[0..1] and
-// hist[3 (k - 1) + 2] (u null at k = 0); |w|^2 into *result
+// hist[3 (k - 1) + 2] (u null at k = 0); kRedBlocks partials of |w|^2
 hipError_t launch_lanczos_update_rec(bool c64, void *w, const void *v, const void *u, const double *rec,
                                      const double *dres, const double *hist, const uint32_t *step, uint64_t dim,
-                                     double *partials, double *result, hipStream_t s);
-// hist[3k .. 3k+2] = (alpha re, alpha im, beta) if k < cap, then rec and
-// *step advance one step
-hipError_t launch_rec_step(double *rec, const double *dres, double *hist, uint32_t *step, uint32_t cap,
-                           hipStream_t s);
+                                     double *partials, hipStream_t s);
+// dres[2] = |w|^2 from those partials, hist[3k .. 3k+2] = (alpha re,
+// alpha im, beta) if k < cap, then rec and *step advance one step
+hipError_t launch_rec_step(double *rec, double *dres, double *hist, uint32_t *step, uint32_t cap,
+                           const double *partials, hipStream_t s);
 // y += (coef[2k] + i coef[2k+1]) rec[0] x, k = *step
 hipError_t launch_axpy_rec_step(bool c64, const double *coef, const uint32_t *step, const double *rec, const void *x,
                                 void *y, uint64_t dim, hipStream_t s);

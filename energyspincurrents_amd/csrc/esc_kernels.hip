@@ -598,8 +598,16 @@ __global__ __launch_bounds__(kThreads) void k_lanczos_update_rec(typename Vec2<R
     lanczos_update_body<R>(w, v, u, c, -ar * c, -ai * c, e, dim, partials);
 }
 
-__global__ void k_rec_step(double *__restrict__ rec, const double *__restrict__ dres, double *__restrict__ hist,
-                           uint32_t *__restrict__ step, uint32_t cap) {
+// one block: dres[2] = |u_{k+1}|^2 from the update's partials (as
+// k_reduce_final sums them), then the scalars of the step
+__global__ __launch_bounds__(kThreads) void k_rec_step(double *__restrict__ rec, double *__restrict__ dres,
+                                                       double *__restrict__ hist, uint32_t *__restrict__ step,
+                                                       uint32_t cap, const double *__restrict__ partials,
+                                                       int npart) {
+    double v[1] = {0.0};
+    for (int p = threadIdx.x; p < npart; p += kThreads) v[0] += partials[p];
+    block_reduce_store<1>(v, dres + 2);
+    if (threadIdx.x != 0) return;
     const uint32_t k = step[0];
     const double c = rec[0];
     const double b = sqrt(dres[2]);
@@ -1068,20 +1076,19 @@ hipError_t launch_rec_init(double *rec, uint32_t *step, double c0, hipStream_t s
 
 hipError_t launch_lanczos_update_rec(bool c64, void *w, const void *v, const void *u, const double *rec,
                                      const double *dres, const double *hist, const uint32_t *step, uint64_t dim,
-                                     double *partials, double *result, hipStream_t s) {
+                                     double *partials, hipStream_t s) {
     if (c64)
         hipLaunchKernelGGL(k_lanczos_update_rec<float>, dim3(kRedBlocks), dim3(kThreads), 0, s, (V2p<float>)w,
                            (CV2p<float>)v, (CV2p<float>)u, rec, dres, hist, step, dim, partials);
     else
         hipLaunchKernelGGL(k_lanczos_update_rec<double>, dim3(kRedBlocks), dim3(kThreads), 0, s, (V2p<double>)w,
                            (CV2p<double>)v, (CV2p<double>)u, rec, dres, hist, step, dim, partials);
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kThreads), 0, s, partials, kRedBlocks, 1, result);
     return hipGetLastError();
 }
 
-hipError_t launch_rec_step(double *rec, const double *dres, double *hist, uint32_t *step, uint32_t cap,
-                           hipStream_t s) {
-    hipLaunchKernelGGL(k_rec_step, dim3(1), dim3(1), 0, s, rec, dres, hist, step, cap);
+hipError_t launch_rec_step(double *rec, double *dres, double *hist, uint32_t *step, uint32_t cap,
+                           const double *partials, hipStream_t s) {
+    hipLaunchKernelGGL(k_rec_step, dim3(1), dim3(kThreads), 0, s, rec, dres, hist, step, cap, partials, kRedBlocks);
     return hipGetLastError();
 }
 
