@@ -462,8 +462,12 @@ uint32_t grow_window(int N, uint32_t T, const std::vector<uint32_t> &masks, int 
 // read from the L2-resident window) by a pass whose window holds them; terms
 // that fit no window open further passes.  P = 5 selects the cover-first
 // planner below instead: windows chosen to hold all terms in as few passes as
-// the window width allows, tiles second.
+// the window width allows, tiles second.  P = 6 is the same plan with the
+// windows run in reverse order: the first window (the low sites, one
+// contiguous block per window) runs last, where the output is re-read, and
+// the first tile is chosen for the window that holds the fewest terms.
 constexpr int kCoverFirst = 5;   // pass digit of the cover-first planner
+constexpr int kCoverLast = 6;    // cover-first, windows in reverse order
 
 // A kernel configuration is 100000 * P + 10000 * V + 100 * TB + W: P planned
 // passes (kCoverFirst: the cover-first planner), 2^V waves per workgroup,
@@ -504,7 +508,7 @@ std::vector<esc::GTables> build_passes(const esc_ctx *c, int TB, int w, int P, i
         rem.swap(keep);
     };
     if (w >= N) P = 1;
-    if (P == kCoverFirst) {
+    if (P == kCoverFirst || P == kCoverLast) {
         // Windows first: each holds as many of the terms no earlier window
         // holds as possible (grown site by site from the lane sites, then
         // single-site swaps while they gain), until every term lies in some
@@ -554,6 +558,7 @@ std::vector<esc::GTables> build_passes(const esc_ctx *c, int TB, int w, int P, i
             left.swap(keep);
             if (wins.size() > 8) throw EscError(ESC_ERR_UNSUPPORTED, "too many window passes");
         }
+        if (P == kCoverLast) std::reverse(wins.begin(), wins.end());
         std::vector<char> is_near(terms.size(), 0);
         for (uint32_t W : wins) {
             std::vector<uint32_t> cand;
@@ -931,7 +936,7 @@ std::vector<int> tile_candidates(const esc_ctx *c) {
             for (int w : windows) {
                 const int ww = w > N ? N : w;
                 if (ww < tb || N - ww > 16 || ww - tb > 16) continue;
-                for (int P : {1, 2, 3, kCoverFirst}) {
+                for (int P : {1, 2, 3, kCoverFirst, kCoverLast}) {
                     if ((ww == N) != (P == 1)) continue;
                     const int cfg = make_config(P, waves, tb, ww);
                     if (std::find(v.begin(), v.end(), cfg) == v.end() && tile_config_fits(c, cfg)) v.push_back(cfg);

@@ -211,7 +211,8 @@ class Engine:
 
     def kernel_configs(self):
         """H.psi kernel configurations: 0 (one row per lane) or
-        100000 * planned passes (5: cover-first planner) + 10000 * log2(waves
+        100000 * planned passes (5: cover-first planner, 6: its windows in
+        reverse order) + 10000 * log2(waves
         per workgroup) + 100 * tile bits + window bits."""
         n = ctypes.c_int32()
         check(lib().esc_kernel_configs(self._h, None, 0, ctypes.byref(n)))

@@ -105,7 +105,8 @@ int esc_ctx_info(const esc_ctx *ctx, int64_t *dim, int32_t *elem_bytes, int32_t 
  * rows staged in LDS, walking windows of 2^W rows (one XCD per window) in P
  * planned passes (P = 1 when W = N), plus passes for terms no window held;
  * P = 5: windows chosen first to hold every term in as few passes as the
- * width W allows.  d_out is overwritten. */
+ * width W allows; P = 6: the same windows run in reverse order.  d_out is
+ * overwritten. */
 int esc_autotune(esc_ctx *ctx, const void *d_in, void *d_out, int reps, int *tile_bits, double *ms, void *stream);
 /* Select the H.psi kernel explicitly (a configuration as reported by
  * esc_autotune or listed by esc_kernel_configs). */
