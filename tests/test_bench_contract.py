@@ -1,0 +1,51 @@
+"""bench.py's command-line contract and the agreement of its constants with
+the committed profile summaries (CPU only: nothing here touches a GPU)."""
+import importlib.util
+import os
+import subprocess
+import sys
+
+from conftest import ROOT
+
+
+def load_bench():
+    spec = importlib.util.spec_from_file_location("bench_module", os.path.join(ROOT, "bench.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_more_than_one_gpu_is_refused_before_any_gpu_use():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_documented_flags_parse():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--help"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0
+    for flag in ("--gpus", "--steps", "--warmup", "--no-cpu", "--no-spectrum", "--no-n32", "--no-batch",
+                 "--kernel-config", "--autotune", "--pipelines"):
+        assert flag in r.stdout, flag
+
+
+def test_profiled_configurations_and_traffic_match_the_profiles():
+    b = load_bench()
+    cfg = b.PROFILED_CONFIG
+    assert set(b.MEASURED_TRAFFIC) == {("c128", cfg["c128"]), ("c64", cfg["c64"])}
+    pair = f"--kernel-config {cfg['c128']},{cfg['c64']}"
+    trace = open(os.path.join(ROOT, "profiles", "hpsi_n24_kernel_trace.md")).read()
+    traffic = open(os.path.join(ROOT, "profiles", "hpsi_n24_hbm_traffic.md")).read()
+    assert pair in trace and traffic.count(pair) == 2
+    # per-H.psi totals: FETCH KiB x 2.0 + WRITE KiB, from the per-pass table
+    lines = traffic.splitlines()
+    for prec, key in (("c128", "double"), ("c64", "float")):
+        i = next(n for n, ln in enumerate(lines) if ln.startswith(f"| k_hpsi_tile<{key}"))
+        assert f"(config {cfg[prec]})" in lines[i]
+        cells = [[c.strip() for c in ln.strip().strip("|").split("|")] for ln in lines[i:i + 2]]
+        assert [c[1] for c in cells] == ["1", "2"]
+        fetch = sum(int(c[2]) for c in cells)
+        write = sum(int(c[4]) for c in cells)
+        assert b.MEASURED_TRAFFIC[(prec, cfg[prec])] == (2 * fetch + write) * 1024
