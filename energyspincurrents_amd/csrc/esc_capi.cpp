@@ -947,17 +947,27 @@ std::vector<int> tile_candidates(const esc_ctx *c) {
     return v;
 }
 
-// before tuning: the shapes measured fastest at N = 24 (cover-first 2^18-row
-// windows, two passes, 2^12-row tiles; 16 waves for c128, which moves the
-// least HBM traffic of the equally fast shapes, and 8 waves for c64), else a
-// single pass over 2^11-row tiles; systems below 2^11 rows use one row per
-// lane
+// Before tuning: the shapes measured fastest on the K-J-Gamma-h honeycomb.
+// Up to 16 sites (c128 up to 18 on the tiled side) the vector stays in L2 and
+// one row per lane wins (N = 16: 8.8 against 15.9 us for 2^11-row tiles);
+// 18 sites c128: a single pass over 2^10-row tiles; 19-22 sites: cover-first
+// over 2^17-row windows, 16 waves (c128 2^11-row tiles; c64 2^12-row tiles,
+// windows in reverse order); from 23 sites: cover-first over 2^18-row windows
+// with 2^12-row tiles (16 waves for c128, which moves the least HBM traffic of
+// the equally fast shapes at N = 24, 8 waves for c64).  A shape the context
+// cannot run falls back to a single pass over 2^11- or 2^10-row tiles.
 int default_tile_bits(const esc_ctx *c) {
     const int N = c->n_sites;
-    if (N < 11) return 0;
+    const bool c64 = c->c64();
+    if (N <= 16 || (N <= 18 && c64)) return 0;
+    std::vector<int> pref;
+    if (N <= 18) pref.push_back(make_config(1, 8, 10, N));
+    else if (N <= 22) pref.push_back(c64 ? make_config(kCoverLast, 16, 12, 17) : make_config(kCoverFirst, 16, 11, 17));
+    else pref.push_back(c64 ? make_config(kCoverFirst, 8, 12, 18) : make_config(kCoverFirst, 16, 12, 18));
+    pref.push_back(make_config(1, 8, 11, N));
+    pref.push_back(make_config(1, 4, 10, N));
     const std::vector<int> cand = tile_candidates(c);
-    for (int cfg : {c->c64() ? make_config(kCoverFirst, 8, 12, 18) : make_config(kCoverFirst, 16, 12, 18),
-                    make_config(1, 8, 11, N), make_config(1, 4, 10, N)})
+    for (int cfg : pref)
         if (std::find(cand.begin(), cand.end(), cfg) != cand.end()) return cfg;
     return 0;
 }
