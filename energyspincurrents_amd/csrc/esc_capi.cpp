@@ -3025,6 +3025,12 @@ int esc_spectrum_grad(const esc_model *model, int op, int site, int gs_steps, in
                                          n2bar, e0bar);
         if (loss) *loss = L;
         double g[4] = {0.0, 0.0, 0.0, 0.0};
+        // the sweep's Re<u_bar|H_p v_k> go to device slots without a host
+        // round trip and are added to g afterwards, in the sweep's order
+        DevVecs GD(sizeof(double) * 8 * (size_t)std::max(M, 1));
+        double *gdot = static_cast<double *>(GD.get());
+        size_t ngdot = 0;
+        c->ensure_partials(2 * esc::kRedBlocks);
 
         // reverse sweep over the recursion, segment by segment
         std::vector<void *> seg(C + 2);
@@ -3079,11 +3085,16 @@ int esc_spectrum_grad(const esc_model *model, int op, int site, int gs_steps, in
                 // theta_bar_p += Re<u_bar|H_p v_k>
                 for (int p = 0; p < 4; ++p) {
                     run_hpsi(P[p].c, vk, t0, s);
-                    g[p] += dot(ubar, t0).real();
+                    hip_check(esc::launch_dot(false, ubar, t0, dim, c->d_partials, gdot + 2 * ngdot++, s), "dot");
                 }
                 std::swap(vbar_next, vbar_k);   // v_bar_k is complete
                 std::swap(vbar_k, vbar_prev);
             }
+        }
+        {
+            std::vector<double> h(2 * ngdot);
+            if (ngdot) sync_get(s, h.data(), gdot, h.size() * sizeof(double));
+            for (size_t q = 0; q < ngdot; ++q) g[q % 4] += h[2 * q];
         }
         progress("gradient: reverse sweep");
         // vbar_next holds v_bar_0: phi_bar = (v_bar_0 - v_0 Re<v_0|v_bar_0>) / |phi| + 2 n2_bar phi
