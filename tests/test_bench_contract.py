@@ -33,19 +33,19 @@ def test_documented_flags_parse():
 
 def test_profiled_configurations_and_traffic_match_the_profiles():
     b = load_bench()
-    cfg = b.PROFILED_CONFIG
-    assert set(b.MEASURED_TRAFFIC) == {("c128", cfg["c128"]), ("c64", cfg["c64"])}
-    pair = f"--kernel-config {cfg['c128']},{cfg['c64']}"
+    cfgs = b.PROFILED_CONFIGS
+    assert set(b.MEASURED_TRAFFIC) == {(p, c) for p in cfgs for c in cfgs[p]}
     trace = open(os.path.join(ROOT, "profiles", "hpsi_n24_kernel_trace.md")).read()
     traffic = open(os.path.join(ROOT, "profiles", "hpsi_n24_hbm_traffic.md")).read()
-    assert pair in trace and traffic.count(pair) == 2
+    for c128 in cfgs["c128"]:
+        assert f"{c128},{cfgs['c64'][0]}" in trace and f"{c128},{cfgs['c64'][0]}" in traffic
     # per-H.psi totals: FETCH KiB x 2.0 + WRITE KiB, from the per-pass table
     lines = traffic.splitlines()
-    for prec, key in (("c128", "double"), ("c64", "float")):
-        i = next(n for n, ln in enumerate(lines) if ln.startswith(f"| k_hpsi_tile<{key}"))
-        assert f"(config {cfg[prec]})" in lines[i]
+    for (prec, cfg), total in b.MEASURED_TRAFFIC.items():
+        i = next(n for n, ln in enumerate(lines) if ln.startswith("| k_hpsi_tile<") and f"(config {cfg})" in ln)
+        assert ("<double" in lines[i]) == (prec == "c128")
         cells = [[c.strip() for c in ln.strip().strip("|").split("|")] for ln in lines[i:i + 2]]
         assert [c[1] for c in cells] == ["1", "2"]
         fetch = sum(int(c[2]) for c in cells)
         write = sum(int(c[4]) for c in cells)
-        assert b.MEASURED_TRAFFIC[(prec, cfg[prec])] == (2 * fetch + write) * 1024
+        assert total == (2 * fetch + write) * 1024
