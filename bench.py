@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--no-cpu-coo", action="store_true",
+                    help="skip the reference-algorithm (COO lists) CPU H.psi, which holds 29 GB of host memory")
     ap.add_argument("--no-spectrum", action="store_true", help="skip the JeJe spectrum timing")
     ap.add_argument("--no-n32", action="store_true", help="skip the 32-site complex64 H.psi timing")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batched scan")
@@ -115,6 +117,39 @@ def cpu_baseline(reps):
     return {"value": 1.0 / med, "unit": "H.psi/s", "cores": cores, "kind": "port",
             "sample": f"{reps} FP64 gather H.psi at N=24 after 1 warm-up (median {med:.3f} s each), "
                       "oracle/esc_oracle.c orc_hpsi_gather, OpenMP"}
+
+
+def cpu_coo_baseline(reps):
+    """The reference's own H.psi algorithm on the host cores: explicit COO lists
+    (TightB_Ham + Diagonal, src/Hamiltonian.h:94-206,331-377) applied with
+    per-thread private outputs and a final reduction (MatrixVectorMult,
+    src/Hamiltonian.h:480-557), restated in oracle/esc_oracle.c.  Same 24-site
+    lattice; the reference model has no J or Gamma, so K=-1 and h[111]=0.2 only.
+    The list build is timed apart from the applications."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    m = O.Model(LX, LY, K=(K, K, K), B=O.field111(H111))
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    h = O.CooHamiltonian(m)
+    build = time.perf_counter() - t0
+    rng = np.random.default_rng(0)
+    v = rng.standard_normal(m.dim) + 1j * rng.standard_normal(m.dim)
+    h.apply(v, cores)                          # warm-up (page-in of the private outputs)
+    t = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        h.apply(v, cores)
+        t.append(time.perf_counter() - t0)
+    del h
+    med = float(np.median(t))
+    fx, fy, _ = m.families(m.K)
+    entries = (len(fx) + len(fy) + 2 * m.N) * m.dim
+    return {"value": 1.0 / med, "unit": "H.psi/s", "cores": cores, "kind": "port",
+            "list_build_s": build, "list_entries": entries, "list_bytes": 24 * entries,
+            "sample": f"{reps} FP64 COO H.psi at N={m.N} (K=-1, h[111]=0.2) after 1 warm-up "
+                      f"(median {med:.3f} s each; list build {build:.1f} s, not included), "
+                      "oracle/esc_oracle.c orc_coo_build + orc_coo_apply, OpenMP"}
 
 
 def main():
@@ -200,6 +235,8 @@ def main():
         out["pipelines"] = pipelines_run(esc)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_reps)
+        if not args.no_cpu_coo:
+            out["cpu_baseline"]["ref_algorithm"] = cpu_coo_baseline(2)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -139,26 +139,34 @@ void *orc_coo_build(int N, const int *pairs_x, int nbx, const int *pairs_y, int 
     h->yi = (uint32_t *)malloc(sizeof(uint32_t) * h->ny); h->yf = (uint32_t *)malloc(sizeof(uint32_t) * h->ny);
     h->xv = (double *)malloc(sizeof(double) * 2 * h->nx); h->yv = (double *)malloc(sizeof(double) * 2 * h->ny);
     h->diag = (double *)calloc(2 * dim, sizeof(double));
-    int64_t e = 0;
+    /* Entries sit in the reference's push_back order: bond b, ket at
+     * b*dim + ket, then the field entries at base + ket*N + s.  Each entry's
+     * index is fixed, so the loops over ket run in parallel. */
+    const int64_t fx = use_x ? (int64_t)nbx * dim : 0, fy = use_y ? (int64_t)nby * dim : 0;
     if (use_x)                                                     /* :113-132 */
         for (int b = 0; b < nbx; b++) {
             int si = pairs_x[2 * b], sj = pairs_x[2 * b + 1];
-            for (int64_t ket = 0; ket < dim; ket++, e++) {
+#pragma omp parallel for schedule(static)
+            for (int64_t ket = 0; ket < dim; ket++) {
+                int64_t e = (int64_t)b * dim + ket;
                 h->xi[e] = (uint32_t)ket;
                 h->xf[e] = (uint32_t)(ket ^ ((uint64_t)1 << si) ^ ((uint64_t)1 << sj));
                 h->xv[2 * e] = 0.25 * K[0]; h->xv[2 * e + 1] = 0.0;   /* Sx*Sx*Kxx */
             }
         }
+#pragma omp parallel for schedule(static)
     for (int64_t ket = 0; ket < dim; ket++)                         /* :167-179 */
-        for (int s = 0; s < N; s++, e++) {
+        for (int s = 0; s < N; s++) {
+            int64_t e = fx + ket * N + s;
             h->xi[e] = (uint32_t)(ket ^ ((uint64_t)1 << s)); h->xf[e] = (uint32_t)ket;
             h->xv[2 * e] = 0.5 * B[0]; h->xv[2 * e + 1] = 0.0;
         }
-    e = 0;
     if (use_y)                                                     /* :140-158 */
         for (int b = 0; b < nby; b++) {
             int si = pairs_y[2 * b], sj = pairs_y[2 * b + 1];
-            for (int64_t ket = 0; ket < dim; ket++, e++) {
+#pragma omp parallel for schedule(static)
+            for (int64_t ket = 0; ket < dim; ket++) {
+                int64_t e = (int64_t)b * dim + ket;
                 double ci = bitof(ket, si) == 0 ? 0.5 : -0.5;       /* ApplySy imag part */
                 double cj = bitof(ket, sj) == 0 ? 0.5 : -0.5;
                 h->yi[e] = (uint32_t)ket;
@@ -166,8 +174,10 @@ void *orc_coo_build(int N, const int *pairs_x, int nbx, const int *pairs_y, int 
                 h->yv[2 * e] = -(ci * cj) * K[1]; h->yv[2 * e + 1] = 0.0;   /* (i ci)(i cj) Kyy */
             }
         }
+#pragma omp parallel for schedule(static)
     for (int64_t ket = 0; ket < dim; ket++)                         /* :186-198 */
-        for (int s = 0; s < N; s++, e++) {
+        for (int s = 0; s < N; s++) {
+            int64_t e = fy + ket * N + s;
             double c = bitof(ket, s) == 0 ? 0.5 : -0.5;
             h->yi[e] = (uint32_t)(ket ^ ((uint64_t)1 << s)); h->yf[e] = (uint32_t)ket;
             h->yv[2 * e] = 0.0; h->yv[2 * e + 1] = B[1] * c;

@@ -27,7 +27,7 @@ def test_documented_flags_parse():
                        timeout=120)
     assert r.returncode == 0
     for flag in ("--gpus", "--steps", "--warmup", "--no-cpu", "--no-spectrum", "--no-n32", "--no-batch",
-                 "--kernel-config", "--autotune", "--pipelines"):
+                 "--kernel-config", "--autotune", "--pipelines", "--no-cpu-coo"):
         assert flag in r.stdout, flag
 
 
