@@ -56,6 +56,9 @@ def parse():
     ap.add_argument("--pipelines", action="store_true",
                     help="also time the full 12-site and 32-site runs (ground state + JsJs and JeJe spectra; "
                          "about 6 minutes, progress on stderr)")
+    ap.add_argument("--jeje-e2e", action="store_true",
+                    help="time the JeJe pipeline end to end at 12 and 20 sites on the device and, unless --no-cpu, "
+                         "the same algorithm in FP64 on the host cores (included in --pipelines)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run a plain device copy of one vector per step (HBM counter calibration)")
     return ap.parse_args()
@@ -97,6 +100,18 @@ MEASURED_TRAFFIC = {
     ("c128", 631118): 1980452864,   # 704819 KiB fetched x 2.0 + 524398 KiB written
     ("c64", 531218): 743146496,     # 231755 KiB fetched x 2.0 + 262219 KiB written
 }
+
+
+def time_lanczos_step(eng, phi, steps):
+    """ms per step of one plain Lanczos recursion from phi (esc_lanczos_cf),
+    after a 10-step run that warms the path up."""
+    import torch
+    eng.lanczos_cf(phi, 10)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.lanczos_cf(phi, steps)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
 
 
 def cpu_baseline(reps):
@@ -192,6 +207,7 @@ def main():
                      "hpsi_per_s": 1e3 / ms, "config": config, "kernel": kname,
                      "traffic": MEASURED_TRAFFIC.get((prec, config)),
                      "probe_ms": {str(k): v for k, v in probe.items()}}
+        res[prec]["step_ms"] = time_lanczos_step(eng, x, 200)
         del x, y
         if not args.no_spectrum:
             spec[prec] = spectrum_run(eng, None if prec == "c128" else spec["c128"]["gs"])
@@ -224,6 +240,16 @@ def main():
                 "bytes_per_launch": res["c64"]["bytes"], "kernel": res["c64"]["kernel"],
                 "kernel_config": res["c64"]["config"], "traffic": res["c64"]["traffic"]},
     }
+    # one Lanczos step priced at the 3 dim s bytes of a fully fused step (read v
+    # and v_prev, write w)
+    out["lanczos_step"] = {
+        p: {"ms_per_step": res[p]["step_ms"], "hpsi_ms": res[p]["ms"], "bytes_fused_step": 3 * res[p]["bytes"] // 2,
+            "achieved_GBs": 1.5 * res[p]["bytes"] / (res[p]["step_ms"] * 1e-3) / 1e9,
+            "frac": 1.5 * res[p]["bytes"] / (res[p]["step_ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS,
+            "workload": "continued-fraction recursion of 200 steps from a seeded vector at N=24 (H.psi with "
+                        "alpha formed in the kernel, the update with |w|^2, the device-side scalar step), "
+                        "wall time per step"}
+        for p in ("c128", "c64")}
     if spec:
         spec["c128"]["gs"] = None
         out["spectrum"] = spectrum_summary(spec)
@@ -233,10 +259,15 @@ def main():
         out["batch"] = batch_run(esc)
     if args.pipelines:
         out["pipelines"] = pipelines_run(esc)
+    e2e_inputs = None
+    if args.pipelines or args.jeje_e2e:
+        out["jeje_end_to_end"], e2e_inputs = jeje_e2e_run(esc)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_reps)
         if not args.no_cpu_coo:
             out["cpu_baseline"]["ref_algorithm"] = cpu_coo_baseline(2)
+        if e2e_inputs is not None:
+            out["cpu_baseline"]["jeje_end_to_end"] = cpu_jeje_e2e(e2e_inputs)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
@@ -310,6 +341,147 @@ def pipelines_run(esc):
         out[name] = r
         del gs, eng
         torch.cuda.empty_cache()
+    return out
+
+
+# The JeJe pipeline timed end to end on the device and on the host cores:
+# (name, Lx, Ly, ground-state steps); K = -1 on every bond, h[111] = 0.2,
+# complex128, JeJe by continued fractions of 200 steps, eta = 0.05,
+# omega = linspace(0, 6, 2000).
+JEJE_E2E = (("n12", 2, 6, 300), ("n20", 2, 10, 200))
+JEJE_E2E_CF, JEJE_E2E_ETA, JEJE_E2E_FIELD = 200, 0.05, 0.2
+
+
+def jeje_e2e_run(esc):
+    """Device leg: seeded complex-Gaussian start vector, ground state (plain
+    Lanczos, then the replay that forms |gs>), JeJe table by continued
+    fractions; seconds of wall time with the device synchronised.  Returns
+    the record and, for the host leg, each run's start vector and results."""
+    import torch
+    w = np.linspace(0.0, 6.0, 2000)
+    out, keep = {}, {}
+    for name, lx, ly, gs_steps in JEJE_E2E:
+        eng = esc.Engine(esc.SpinModel(esc.Honeycomb(lx, ly), K=(K, K, K), B=esc.field111(JEJE_E2E_FIELD)), "c128")
+        v0 = eng.start_vector(7, "Gaussian")
+        v0h = eng.download(v0)
+        eng.hpsi(v0)                                 # first launch outside the timed region
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        gs, e0, resid, it = eng.ground_state(v0, steps=gs_steps, eps=0.0, restarts=0)
+        tab, _ = eng.dynamics("JeJe", "CF", gs, w, JEJE_E2E_ETA, e0=e0, cf_steps=JEJE_E2E_CF)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        print(f"[jeje e2e] {name} device {t1 - t0:.3f} s, E0 {e0:.12f}", file=sys.stderr, flush=True)
+        out[name] = {"workload": f"{lx * ly} sites ({lx}x{ly // 2} cells, periodic), K=-1, "
+                                 f"h[111]={JEJE_E2E_FIELD}, c128: ground state {gs_steps} steps + replay, "
+                                 f"JeJe xx, yy, xy, yx, tt by 4 continued fractions of {JEJE_E2E_CF} steps, "
+                                 f"eta {JEJE_E2E_ETA}, 2000 omegas",
+                     "seconds": t1 - t0, "E0": e0, "gs_residual": resid, "hpsi_ground_state": it}
+        keep[name] = (lx, ly, gs_steps, v0h, tab, e0)
+        del gs, v0, eng
+        torch.cuda.empty_cache()
+    return out, keep
+
+
+def cpu_jeje_pipeline(m, v0, gs_steps, cf_steps, w, eta):
+    """The device pipeline's algorithm in FP64 on the host cores (the oracle's
+    gather H.psi, OpenMP): plain Lanczos from v0 for the ground state and a
+    replay pass forming |gs> = sum_k Z[k,0] v_k; then continued fractions
+    <phi|(z - H)^-1|phi>, z = omega + E0 - i eta, from phi = J_x|gs>, J_y|gs>,
+    (J_x + J_y)|gs> and (J_x + i J_y)|gs>, the cross terms by polarisation,
+    and the elastic terms of DynJeJe (src/dynamics.h:389-425; the yx column
+    has none, as the reference's Jez is never set, :368-369,424).  Returns
+    (E0, table with columns omega, Im xx, Im yy, Im xy, Im yx, Im tt)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    t0 = time.perf_counter()
+
+    def note(what):
+        print(f"[jeje e2e] host N={m.N}: {what} ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+
+    def hp(v):
+        return O.hpsi_gather(m, v)
+
+    def recursion(phi, steps, coef=None):
+        """Three-vector Lanczos from phi/|phi| with in-place updates (the
+        order of operations of lanczos_plain).  With coef, a replay that
+        returns sum_k coef[k] v_k instead of alpha and beta."""
+        nrm = np.linalg.norm(phi)
+        v = phi / nrm
+        vp = np.zeros_like(v)
+        acc = coef[0] * v if coef is not None else None
+        alpha, beta = [], [0.0]
+        for k in range(steps):
+            if coef is not None and k + 1 == len(coef):
+                break                                 # v_0 .. v_{n-1} summed: n - 1 H.psi
+            x = hp(v)
+            vp *= beta[-1]
+            x -= vp
+            a = np.vdot(v, x).real
+            np.multiply(v, a, out=vp)
+            x -= vp
+            alpha.append(a)
+            bn = np.linalg.norm(x)
+            beta.append(bn)
+            if bn == 0.0:
+                break
+            x /= bn
+            vp, v = v, x
+            if coef is not None:
+                acc += coef[k + 1] * v
+        return (acc if coef is not None else None), np.array(alpha), np.array(beta), nrm ** 2
+
+    _, a, b, _ = recursion(v0, gs_steps)
+    evals, Z = O.tridiag_eig(a, b)
+    note(f"ground-state recursion, {len(a)} steps")
+    gs, _, _, _ = recursion(v0, len(a), Z[:, 0])
+    gs = gs / np.linalg.norm(gs)
+    e0 = float(evals[0])
+    note("replay")
+    jx = O.apply_current_total(m, "Jex", gs)
+    jy = O.apply_current_total(m, "Jey", gs)
+    z = w + e0 - 1j * eta
+
+    def res(phi, i):
+        _, aa, bb, nn = recursion(phi, cf_steps)
+        note(f"continued fraction {i} of 4")
+        return O.cf_resolvent(aa, bb, nn, z)
+
+    gx, gy, gt, gi = res(jx, 1), res(jy, 2), res(jx + jy, 3), res(jx + 1j * jy, 4)
+    ex, ey = np.vdot(gs, jx), np.vdot(gs, jy)
+    el = 1.0 / (w - 1j * eta)
+    s_re = 0.5 * (gt - gx - gy)          # sum_n Re(conj(O_x) O_y) / d_n
+    s_im = -0.5 * (gi - gx - gy)         # sum_n Im(conj(O_x) O_y) / d_n
+    ixx = gx - abs(ex) ** 2 * el
+    iyy = gy - abs(ey) ** 2 * el
+    ixy = s_re + 1j * s_im - np.conj(ex) * ey * el
+    iyx = s_re - 1j * s_im
+    itt = gt - abs(ex + ey) ** 2 * el
+    return e0, np.column_stack([w, ixx.imag, iyy.imag, ixy.imag, iyx.imag, itt.imag])
+
+
+def cpu_jeje_e2e(inputs):
+    """Host leg of the end-to-end JeJe timing: the same start vectors, the
+    algorithm of cpu_jeje_pipeline, and its agreement with the device tables
+    (largest difference relative to the largest entry of each table)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    w = np.linspace(0.0, 6.0, 2000)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    out = {"cores": cores, "kind": "port",
+           "sample": "the whole pipeline of each size once, FP64, oracle/esc_oracle.c orc_hpsi_gather and "
+                     "orc_current_total (OpenMP) with numpy vector updates"}
+    for name, (lx, ly, gs_steps, v0h, tab, e0_dev) in inputs.items():
+        m = O.Model(lx, ly, K=(K, K, K), B=O.field111(JEJE_E2E_FIELD))
+        t0 = time.perf_counter()
+        e0, ref = cpu_jeje_pipeline(m, v0h, gs_steps, JEJE_E2E_CF, w, JEJE_E2E_ETA)
+        dt = time.perf_counter() - t0
+        print(f"[jeje e2e] {name} host {dt:.2f} s, E0 {e0:.12f}", file=sys.stderr, flush=True)
+        d = np.abs(tab[:, 1:] - ref[:, 1:])
+        out[name] = {"seconds": dt, "E0": e0, "E0_diff_device": abs(e0 - e0_dev),
+                     "table_max_rel_diff_device": float(d.max() / np.abs(ref[:, 1:]).max()),
+                     "table_l1_rel_diff_device": float((d.sum(0) / np.abs(ref[:, 1:]).sum(0)).max())}
     return out
 
 

@@ -27,7 +27,7 @@ def test_documented_flags_parse():
                        timeout=120)
     assert r.returncode == 0
     for flag in ("--gpus", "--steps", "--warmup", "--no-cpu", "--no-spectrum", "--no-n32", "--no-batch",
-                 "--kernel-config", "--autotune", "--pipelines", "--no-cpu-coo"):
+                 "--kernel-config", "--autotune", "--pipelines", "--no-cpu-coo", "--jeje-e2e"):
         assert flag in r.stdout, flag
 
 
@@ -49,3 +49,22 @@ def test_profiled_configurations_and_traffic_match_the_profiles():
         fetch = sum(int(c[2]) for c in cells)
         write = sum(int(c[4]) for c in cells)
         assert total == (2 * fetch + write) * 1024
+
+
+def test_host_jeje_pipeline_equals_exact_lehmann_sum():
+    """The host leg of the end-to-end JeJe timing (plain Lanczos ground state
+    with replay, four continued fractions, cross terms by polarisation) against
+    DynJeJe over the exact eigenstates of a dense 8-site Hamiltonian."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    b = load_bench()
+    m = O.Model(2, 4, K=(-1.0, -1.0, -1.0), B=O.field111(0.2))
+    e, U = np.linalg.eigh(O.hamiltonian_dense_kron(m))
+    w = np.linspace(0.0, 6.0, 400)
+    exact = O.jeje_ref(m, e, U.T.copy(), U[:, 0], omega=w, eta=0.1)
+    rng = np.random.default_rng(3)
+    v0 = rng.standard_normal(m.dim) + 1j * rng.standard_normal(m.dim)
+    e0, tab = b.cpu_jeje_pipeline(m, v0, 60, 150, w, 0.1)
+    assert abs(e0 - e[0]) < 1e-12
+    assert np.abs(tab[:, 1:] - exact[:, 1:]).max() < 1e-10 * np.abs(exact[:, 1:]).max()
