@@ -1046,13 +1046,14 @@ void reorthogonalize(esc_ctx *c, int k, void *w, hipStream_t s) {
 // and the step index on the device, see k_rec_step); the host reads the
 // scalars back every kRecChunk steps and applies the exit rules to every step
 // in order, so the result is that of a check after each step.  At most
-// kRecChunk - 1 steps past the exit are computed and dropped.
+// kRecChunk - 1 steps past the exit are computed and dropped (small systems,
+// one launch per chunk: kRecChunkSmall - 1).
 //
 // Device layout inside the context's result buffer: [0..1] the H.psi dot,
 // [2] |w|^2, [4..5] rec, [6] the step index (uint32), [8 ..] 3 doubles per
 // step (history), then 2 doubles per step (replay coefficients).
 constexpr size_t kRecOff = 4, kStepOff = 6, kHistOff = 8;
-constexpr int kRecChunk = 16;
+constexpr int kRecChunk = 16, kRecChunkSmall = 64;
 
 // Runs the recursion in the three rotating vectors buf: step k maps u_{k-1},
 // u_k, u_{k+1} to buf[k % 3], buf[(k + 1) % 3], buf[(k + 2) % 3].  With coef
@@ -1092,10 +1093,23 @@ struct RecRunner {
                   "update");
         hip_check(esc::launch_rec_step(rec(), c->d_result, hist(), step(), cap, c->d_partials, s), "recursion scalars");
     }
-    // up to kRecChunk more steps, stopping at `target`; returns the number of
-    // steps done so far
+    // small systems with the one-row-per-lane H.psi run whole chunks of steps
+    // in one single-workgroup launch
+    bool small() const { return c->tile_bits == 0 && c->dim <= esc::kSmallRecMaxDim; }
+    // up to kRecChunk (small systems: kRecChunkSmall) more steps, stopping at
+    // `target`; returns the number of steps done so far.  A single-workgroup
+    // chunk ends early after an exhausted step, which the caller's exit rule
+    // meets first.
     int advance(int target) {
-        const int stop = std::min(target, k + kRecChunk);
+        const int stop = std::min(target, k + (small() ? kRecChunkSmall : kRecChunk));
+        if (small()) {
+            if (stop > k)
+                hip_check(esc::launch_rec_small(c->c64(), c->d_htab, buf[0], buf[1], buf[2], rec(), step(), hist(), cap,
+                                                stop - k, coef, acc, c->dim, s),
+                          "small recursion");
+            k = std::max(k, stop);
+            return k;
+        }
         for (; k < stop; ++k) enqueue(k);
         return k;
     }

@@ -177,6 +177,15 @@ hipError_t launch_lanczos_update_rec(bool c64, void *w, const void *v, const voi
 // alpha im, beta) if k < cap, then rec and *step advance one step
 hipError_t launch_rec_step(double *rec, double *dres, double *hist, uint32_t *step, uint32_t cap,
                            const double *partials, hipStream_t s);
+// Up to nsteps steps of the same recursion in one workgroup for dim <=
+// kSmallRecMaxDim (one-row-per-lane H.psi from t; buffers b0..b2 rotate as u_k
+// = b[(k+1) % 3], w = b[(k+2) % 3], u_{k-1} = b[k % 3]; with coef, acc +=
+// coef_k c_k u_k before step k).  Ends with rec, *step and hist as after as
+// many k_rec_step launches; stops after an exhausted step.
+constexpr uint64_t kSmallRecMaxDim = 1u << 11;   // at most 2 rows per lane
+hipError_t launch_rec_small(bool c64, const HTables *t, void *b0, void *b1, void *b2, double *rec, uint32_t *step,
+                            double *hist, uint32_t cap, int nsteps, const double *coef, void *acc, uint64_t dim,
+                            hipStream_t s);
 // y += (coef[2k] + i coef[2k+1]) rec[0] x, k = *step
 hipError_t launch_axpy_rec_step(bool c64, const double *coef, const uint32_t *step, const double *rec, const void *x,
                                 void *y, uint64_t dim, hipStream_t s);

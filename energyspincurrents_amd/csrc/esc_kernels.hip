@@ -36,78 +36,122 @@ __device__ __forceinline__ double sgn(uint32_t r, uint32_t k) { return ((r >> k)
 // reference order (z bonds, then Heisenberg bonds, then Bzz * Sz_total), so for
 // c128 every matrix element equals the reference's bit for bit.
 // ---------------------------------------------------------------------------
-template <typename R>
-__global__ __launch_bounds__(kThreads) void k_hpsi(const HTables *__restrict__ T,
-                                                   const typename Vec2<R>::T *__restrict__ in,
-                                                   typename Vec2<R>::T *__restrict__ out, uint64_t dim,
-                                                   uint32_t nblk_total, uint32_t block_base) {
+// NR rows at a time, the term loop outermost, so every term has NR
+// independent gathers in flight; per row the arithmetic is that of one row.
+template <typename R, int NR>
+__device__ __forceinline__ void hpsi_rows(const HTables *__restrict__ T, const typename Vec2<R>::T *__restrict__ in,
+                                          const uint32_t (&r)[NR], typename Vec2<R>::T (&o)[NR]) {
     using V2 = typename Vec2<R>::T;
-    const uint32_t g = xcd_block(blockIdx.x + block_base, nblk_total);
-    const uint64_t r64 = (uint64_t)g * kThreads + threadIdx.x;
-    if (r64 >= dim) return;
-    const uint32_t r = (uint32_t)r64;
-
-    double d = 0.0;
+    // terms unrolled by 4: 4 NR gathers in flight (NR <= 2 fits the registers
+    // of a 1024-lane workgroup without spills)
+    constexpr int kTermUnroll = 4;
+    double d[NR];
     {
         // no FMA contraction: Bzz * Sz_total is rounded before it is added, as
         // the reference's separate complex operators do (src/Hamiltonian.h:374)
 #pragma clang fp contract(off)
+#pragma unroll
+        for (int q = 0; q < NR; ++q) d[q] = 0.0;
         const uint32_t nz = T->n_zdiag;
         const double kz4 = T->kz4;
         for (uint32_t b = 0; b < nz; ++b) {
             const uint32_t p = T->zdiag[b];
-            const uint32_t x = ((r >> (p & 0xffu)) ^ (r >> ((p >> 8) & 0xffu))) & 1u;
-            d += x ? -kz4 : kz4;
+#pragma unroll
+            for (int q = 0; q < NR; ++q) {
+                const uint32_t x = ((r[q] >> (p & 0xffu)) ^ (r[q] >> ((p >> 8) & 0xffu))) & 1u;
+                d[q] += x ? -kz4 : kz4;
+            }
         }
         const uint32_t nj = T->n_jdiag;
         const double j4 = T->j4;
         for (uint32_t b = 0; b < nj; ++b) {
             const uint32_t p = T->jdiag[b];
-            const uint32_t x = ((r >> (p & 0xffu)) ^ (r >> ((p >> 8) & 0xffu))) & 1u;
-            d += x ? -j4 : j4;
+#pragma unroll
+            for (int q = 0; q < NR; ++q) {
+                const uint32_t x = ((r[q] >> (p & 0xffu)) ^ (r[q] >> ((p >> 8) & 0xffu))) & 1u;
+                d[q] += x ? -j4 : j4;
+            }
         }
         // __popc returns unsigned: keep the difference signed
-        d += T->bz * (0.5 * (double)((int)T->n_sites - 2 * (int)__popc(r)));
+#pragma unroll
+        for (int q = 0; q < NR; ++q) d[q] += T->bz * (0.5 * (double)((int)T->n_sites - 2 * (int)__popc(r[q])));
     }
-    const V2 v0 = in[r];
-    const R dd = (R)d;
-    R ar = dd * v0.x, ai = dd * v0.y;
+    R ar[NR], ai[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+        const V2 v0 = in[r[q]];
+        const R dd = (R)d[q];
+        ar[q] = dd * v0.x;
+        ai[q] = dd * v0.y;
+    }
 
     const uint32_t nb = T->n_bonds;
-#pragma unroll 4
+#pragma unroll(kTermUnroll)
     for (uint32_t b = 0; b < nb; ++b) {
         const uint32_t p = T->bond[b];
         const uint32_t i = p & 0xffu, j = (p >> 8) & 0xffu, f = p >> 16;
-        const R si = ((r >> i) & 1u) ? R(-1) : R(1);
-        const R sj = ((r >> j) & 1u) ? R(-1) : R(1);
-        const R cr = (R)T->p0[f] + (R)T->p1[f] * (si * sj);
-        const R ci = (R)T->q[f] * (si + sj);
-        const V2 x = in[r ^ ((1u << i) | (1u << j))];
-        ar += cr * x.x - ci * x.y;
-        ai += cr * x.y + ci * x.x;
+        const uint32_t m = (1u << i) | (1u << j);
+        V2 x[NR];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) x[q] = in[r[q] ^ m];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            const R si = ((r[q] >> i) & 1u) ? R(-1) : R(1);
+            const R sj = ((r[q] >> j) & 1u) ? R(-1) : R(1);
+            const R cr = (R)T->p0[f] + (R)T->p1[f] * (si * sj);
+            const R ci = (R)T->q[f] * (si + sj);
+            ar[q] += cr * x[q].x - ci * x[q].y;
+            ai[q] += cr * x[q].y + ci * x[q].x;
+        }
     }
 
     const uint32_t nf = T->n_flip;
     const R bx2 = (R)T->bx2, by2 = (R)T->by2, g4 = (R)T->g4;
     const bool gam = T->has_gamma_flip != 0;
-#pragma unroll 4
+#pragma unroll(kTermUnroll)
     for (uint32_t k = 0; k < nf; ++k) {
-        const R sk = ((r >> k) & 1u) ? R(-1) : R(1);
-        R re = bx2, im = by2;
-        if (gam) {
-            const int32_t xp = T->xpart[k], yp = T->ypart[k];
-            if (xp >= 0) im += g4 * (((r >> xp) & 1u) ? R(-1) : R(1));
-            if (yp >= 0) re += g4 * (((r >> yp) & 1u) ? R(-1) : R(1));
+        V2 x[NR];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) x[q] = in[r[q] ^ (1u << k)];
+        const int32_t xp = gam ? T->xpart[k] : -1, yp = gam ? T->ypart[k] : -1;
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            const R sk = ((r[q] >> k) & 1u) ? R(-1) : R(1);
+            R re = bx2, im = by2;
+            if (gam) {
+                if (xp >= 0) im += g4 * (((r[q] >> xp) & 1u) ? R(-1) : R(1));
+                if (yp >= 0) re += g4 * (((r[q] >> yp) & 1u) ? R(-1) : R(1));
+            }
+            im = -sk * im;
+            ar[q] += re * x[q].x - im * x[q].y;
+            ai[q] += re * x[q].y + im * x[q].x;
         }
-        im = -sk * im;
-        const V2 x = in[r ^ (1u << k)];
-        ar += re * x.x - im * x.y;
-        ai += re * x.y + im * x.x;
     }
-    V2 o;
-    o.x = ar;
-    o.y = ai;
-    out[r] = o;
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+        o[q].x = ar[q];
+        o[q].y = ai[q];
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ typename Vec2<R>::T hpsi_row(const HTables *__restrict__ T,
+                                                        const typename Vec2<R>::T *__restrict__ in, uint32_t r) {
+    const uint32_t rr[1] = {r};
+    typename Vec2<R>::T o[1];
+    hpsi_rows<R, 1>(T, in, rr, o);
+    return o[0];
+}
+
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_hpsi(const HTables *__restrict__ T,
+                                                   const typename Vec2<R>::T *__restrict__ in,
+                                                   typename Vec2<R>::T *__restrict__ out, uint64_t dim,
+                                                   uint32_t nblk_total, uint32_t block_base) {
+    const uint32_t g = xcd_block(blockIdx.x + block_base, nblk_total);
+    const uint64_t r64 = (uint64_t)g * kThreads + threadIdx.x;
+    if (r64 >= dim) return;
+    out[r64] = hpsi_row<R>(T, in, (uint32_t)r64);
 }
 
 // ---------------------------------------------------------------------------
@@ -633,6 +677,129 @@ __global__ __launch_bounds__(kThreads) void k_axpy_rec_step(const double *__rest
     axpby_body<R>(coef[2 * (size_t)k] * c, coef[2 * (size_t)k + 1] * c, x, 1.0, 0.0, y, dim);
 }
 
+// ---------------------------------------------------------------------------
+// The same recursion for small systems, up to `nsteps` steps in one
+// workgroup.  In the multi-kernel form a step costs five launches (H.psi, the
+// dot and its reduction, the update, the scalars), and below a few thousand
+// rows those launches, not the rows, set the time.  Here one workgroup of
+// kSmallThreads lanes walks the rows of every pass and keeps the scalars in
+// registers; rec, *step and hist end with the values k_rec_step would leave,
+// so the two forms can follow each other.  After the update pass every wave
+// waits for its stores and invalidates the CU's L1 (agent-scope fence) before
+// the barrier, so the next step's gathers read the new rows.  The loop stops
+// after a step whose beta is not above 1e-13 max(1, |alpha|) (the host's
+// exhaustion rule; a non-finite beta stops it too).
+// ---------------------------------------------------------------------------
+constexpr int kSmallThreads = 1024;
+constexpr int kSmallWaves = kSmallThreads / 64;
+
+// sum over the workgroup in a fixed order (lanes by xor shuffle, then the
+// waves' sums in wave order); every lane returns the total
+template <int NV>
+__device__ __forceinline__ void small_block_sum(double (&v)[NV], double (&sh)[NV][kSmallWaves]) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+        for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_xor(v[q], off);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < NV; ++q) sh[q][threadIdx.x >> 6] = v[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        double t = 0.0;
+        for (int wv = 0; wv < kSmallWaves; ++wv) t += sh[q][wv];
+        v[q] = t;
+    }
+    __syncthreads();   // sh is written again by the next sum
+}
+
+template <typename R, int NR>
+__global__ __launch_bounds__(kSmallThreads) void k_rec_small(const HTables *__restrict__ T, typename Vec2<R>::T *b0,
+                                                             typename Vec2<R>::T *b1, typename Vec2<R>::T *b2,
+                                                             double *__restrict__ rec, uint32_t *__restrict__ step,
+                                                             double *__restrict__ hist, uint32_t cap, int nsteps,
+                                                             const double *__restrict__ coef,
+                                                             typename Vec2<R>::T *acc, uint32_t dim) {
+    using V2 = typename Vec2<R>::T;
+    __shared__ double sh_dot[2][kSmallWaves];
+    __shared__ double sh_n2[1][kSmallWaves];
+    V2 *const buf[3] = {b0, b1, b2};
+    uint32_t k = step[0];
+    double c = rec[0], cp = rec[1];
+    double beta = k > 0 ? hist[3 * (size_t)(k - 1) + 2] : 0.0;
+    for (int it = 0; it < nsteps; ++it) {
+        V2 *const u = buf[(k + 1) % 3];
+        V2 *const w = buf[(k + 2) % 3];
+        const V2 *const up = k > 0 ? buf[k % 3] : nullptr;
+        if (coef != nullptr) {   // acc += (coef[2k] + i coef[2k+1]) c_k u_k, as k_axpy_rec_step
+            const double ar = coef[2 * (size_t)k] * c, ai = coef[2 * (size_t)k + 1] * c;
+            for (uint32_t i = threadIdx.x; i < dim; i += kSmallThreads) {
+                const double xr = u[i].x, xi = u[i].y, yr = acc[i].x, yi = acc[i].y;
+                V2 o;
+                o.x = (R)((ar * xr - ai * xi) + yr);
+                o.y = (R)((ar * xi + ai * xr) + yi);
+                acc[i] = o;
+            }
+        }
+        // w = H u_k and <u_k|H u_k>: lane t owns rows t + q kSmallThreads,
+        // q < NR (dim = NR kSmallThreads when NR > 1), and reads back only
+        // its own rows of w
+        double d[2] = {0.0, 0.0};
+        if (NR > 1 || threadIdx.x < dim) {
+            uint32_t rr[NR];
+#pragma unroll
+            for (int q = 0; q < NR; ++q) rr[q] = threadIdx.x + (uint32_t)q * kSmallThreads;
+            V2 o[NR];
+            hpsi_rows<R, NR>(T, u, rr, o);
+#pragma unroll
+            for (int q = 0; q < NR; ++q) {
+                w[rr[q]] = o[q];
+                const double xr = u[rr[q]].x, xi = u[rr[q]].y, yr = o[q].x, yi = o[q].y;
+                d[0] += xr * yr + xi * yi;
+                d[1] += xr * yi - xi * yr;
+            }
+        }
+        small_block_sum<2>(d, sh_dot);
+        // u_{k+1} = c_k H u_k - alpha_k c_k u_k - beta_k c_{k-1} u_{k-1}
+        const double ar = c * c * d[0], ai = c * c * d[1];
+        const double br = -ar * c, bi = -ai * c, e = up != nullptr ? -beta * cp : 0.0;
+        double n2[1] = {0.0};
+        for (uint32_t i = threadIdx.x; i < dim; i += kSmallThreads) {
+            const double vr = u[i].x, vi = u[i].y;
+            double wr = c * (double)w[i].x + (br * vr - bi * vi);
+            double wi = c * (double)w[i].y + (br * vi + bi * vr);
+            if (up != nullptr) {
+                wr += e * (double)up[i].x;
+                wi += e * (double)up[i].y;
+            }
+            V2 o;
+            o.x = (R)wr;
+            o.y = (R)wi;
+            w[i] = o;
+            n2[0] += (double)o.x * o.x + (double)o.y * o.y;
+        }
+        small_block_sum<1>(n2, sh_n2);
+        const double b = sqrt(n2[0]);
+        if (threadIdx.x == 0 && k < cap) {
+            hist[3 * (size_t)k] = ar;
+            hist[3 * (size_t)k + 1] = ai;
+            hist[3 * (size_t)k + 2] = b;
+        }
+        cp = c;
+        c = 1.0 / b;
+        beta = b;
+        ++k;
+        __threadfence();
+        __syncthreads();
+        if (!(b > 1e-13 * fmax(1.0, fabs(ar)))) break;   // the same value in every lane
+    }
+    if (threadIdx.x == 0) {
+        rec[0] = c;
+        rec[1] = cp;
+        step[0] = k;
+    }
+}
+
 template <typename R>
 __global__ __launch_bounds__(kThreads) void k_normalize(typename Vec2<R>::T *__restrict__ x,
                                                         const double *__restrict__ d_norm2, uint64_t dim) {
@@ -1100,6 +1267,32 @@ hipError_t launch_axpy_rec_step(bool c64, const double *coef, const uint32_t *st
     else
         hipLaunchKernelGGL(k_axpy_rec_step<double>, dim3(stream_blocks(dim)), dim3(kThreads), 0, s, coef, step, rec,
                            (CV2p<double>)x, (V2p<double>)y, dim);
+    return hipGetLastError();
+}
+
+hipError_t launch_rec_small(bool c64, const HTables *t, void *b0, void *b1, void *b2, double *rec, uint32_t *step,
+                            double *hist, uint32_t cap, int nsteps, const double *coef, void *acc, uint64_t dim,
+                            hipStream_t s) {
+    if (dim > kSmallRecMaxDim || nsteps < 1 || (coef != nullptr && acc == nullptr)) return hipErrorInvalidValue;
+    // rows per lane: 1 up to kSmallThreads rows, else dim / kSmallThreads
+    // (dim is a power of two)
+    const int nr = dim <= (uint64_t)kSmallThreads ? 1 : (int)(dim / kSmallThreads);
+#define ESC_REC_SMALL(R, NRV)                                                                                     \
+    hipLaunchKernelGGL((k_rec_small<R, NRV>), dim3(1), dim3(kSmallThreads), 0, s, t, (V2p<R>)b0, (V2p<R>)b1,      \
+                       (V2p<R>)b2, rec, step, hist, cap, nsteps, coef, (V2p<R>)acc, (uint32_t)dim)
+#define ESC_REC_SMALL_NR(R)                                   \
+    switch (nr) {                                             \
+        case 1: ESC_REC_SMALL(R, 1); break;                   \
+        case 2: ESC_REC_SMALL(R, 2); break;                   \
+        default: return hipErrorInvalidValue;                 \
+    }
+    if (c64) {
+        ESC_REC_SMALL_NR(float)
+    } else {
+        ESC_REC_SMALL_NR(double)
+    }
+#undef ESC_REC_SMALL_NR
+#undef ESC_REC_SMALL
     return hipGetLastError();
 }
 

@@ -240,3 +240,59 @@ def test_ground_state_c64_with_restarts_matches_c128():
     assert abs(E0b - E128) < 1e-6 * abs(E128)
     assert rb <= ra * 1.01
     assert r128 < 1e-4
+
+
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+@pytest.mark.parametrize("geom", [(2, 4), 10, 11, (2, 6), (2, 8)])
+def test_plain_recursion_matches_oracle(geom, precision):
+    """The plain recursion behind the continued fractions: single-workgroup
+    chunks up to 2^11 rows (8-site honeycomb, 10- and 11-site chains: one and
+    two rows per lane), the multi-kernel form at 12 and 16 sites.  Leading
+    alpha/beta against the oracle's FP64 three-vector Lanczos from the same
+    start vector: 1e-10 relative for c128 (30 steps), 1e-5 for c64 (10 steps)."""
+    if isinstance(geom, int):
+        m = O.ChainModel(N=geom, K=(0.7, -1.3, 0.4), B=(0.2, -0.5, 0.3))
+        eng = esc.Engine(esc.SpinModel(esc.Chain(m.N, m.periodic), K=m.K, B=m.B), precision)
+    else:
+        m = O.Model(*geom, K=(-1.0, -1.0, -1.0), J=0.1, G=0.2, B=O.field111(0.2))
+        eng = engine_for(m, precision)
+    print(f"[plain recursion] N={m.N} {precision}", flush=True)
+    phi = eng.start_vector(5, "Gaussian")
+    a, b, n2 = eng.lanczos_cf(phi, 60)
+    assert len(a) == 60
+    ph = eng.download(phi)
+    ra, rb, rn2 = O.lanczos_plain(lambda v: O.hpsi_gather(m, v), ph, 60)
+    n, tol = (30, 1e-10) if precision == "c128" else (10, 1e-5)
+    scale = np.abs(ra[:n]).max()
+    assert abs(n2 - rn2) < 1e-12 * rn2
+    assert np.abs(a[:n] - ra[:n]).max() < tol * scale
+    assert np.abs(b[1:n + 1] - rb[1:n + 1]).max() < tol * scale
+
+
+@pytest.mark.parametrize("geom", [(2, 4), (2, 8)])
+def test_plain_recursion_stops_when_the_krylov_space_is_exhausted(geom):
+    """Started inside an invariant subspace, the recursion asked for 50 steps
+    ends at the exhausted step and its continued fraction is the exact
+    resolvent of the poles there.  8 sites (single-workgroup chunks): 2 |e_0>
+    + |e_5> from the dense Hamiltonian, two steps.  16 sites (multi-kernel
+    form): the ground state of a converged reorthogonalised oracle run, one
+    step."""
+    m = O.Model(*geom, K=(-1.0, -1.0, -1.0), B=O.field111(0.3))
+    print(f"[plain recursion exhausted] N={m.N}", flush=True)
+    eng = engine_for(m)
+    if m.N <= 8:
+        e, U = np.linalg.eigh(O.hamiltonian_dense_kron(m))
+        poles = [(e[0], 2.0, U[:, 0]), (e[5], 1.0, U[:, 5])]
+    else:
+        res = O.lanczos_ref(lambda v: O.hpsi_gather(m, v), O.start_vector_ref(3, m.dim), 150)
+        gs = res.gs / np.linalg.norm(res.gs)
+        poles = [(res.evals[0], 1.0, gs)]
+    for ev, _, x in poles:   # eigenvectors to rounding
+        assert np.linalg.norm(O.hpsi_gather(m, x) - ev * x) < 1e-9
+    ph = sum(c * x for _, c, x in poles)
+    a, b, n2 = eng.lanczos_cf(eng.upload(ph), 50)
+    assert len(a) == len(poles) and b[len(poles)] == 0.0
+    z = np.linspace(-6.0, 2.0, 300) - 0.1j
+    exact = sum(c * c / (z - ev) for ev, c, _ in poles)
+    cf = O.cf_resolvent(a, b, n2, z)
+    assert np.abs(cf - exact).max() < 1e-9 * np.abs(exact).max()
