@@ -26,7 +26,7 @@ ARCH = "gfx950"
 # Packed FP32 instructions are disabled for every device pass.
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", f"-I{INC}", f"-I{CSRC}",
-          "-Wno-unused-command-line-argument"]
+          "-Wno-unused-command-line-argument", "-pthread"]
 
 LIB_SOURCES = ["esc_kernels.hip", "esc_capi.cpp", "esc_host.cpp"]
 CLI_SOURCES = ["facade/lanczoscond_main.cpp"]

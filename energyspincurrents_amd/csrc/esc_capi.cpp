@@ -23,6 +23,7 @@
 #include <set>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -1322,18 +1323,57 @@ std::vector<CfRun> cf_runs(bool js, int nc) {
 }
 
 // n2 <phi|(z - H)^-1|phi>/n2 from the recursion coefficients, z = w + e0 - i eta
+// fn(begin, end) over contiguous chunks of [0, n) on up to 16 host threads
+// (the caller takes the first chunk), at least `grain` items per chunk
+template <typename F>
+void host_parallel_for(int n, int grain, const F &fn) {
+    const unsigned hw = std::thread::hardware_concurrency();
+    int nt = (int)std::min(hw ? hw : 1u, 16u);
+    nt = std::max(1, std::min(nt, n / std::max(1, grain)));
+    if (nt <= 1) {
+        fn(0, n);
+        return;
+    }
+    const int chunk = (n + nt - 1) / nt;
+    std::vector<std::thread> th;
+    try {
+        for (int t = 1; t < nt; ++t) {
+            const int b = t * chunk, e = std::min(n, b + chunk);
+            if (b < e) th.emplace_back([&fn, b, e] { fn(b, e); });
+        }
+    } catch (...) {
+        for (auto &x : th) x.join();
+        throw;
+    }
+    fn(0, std::min(n, chunk));
+    for (auto &x : th) x.join();
+}
+
+// norm2 <phi|(z - H)^-1|phi>, z = omega + e0 - i eta, from the recursion
+// coefficients, on every omega.  Each level is 1 / (x + i y) with y <= -eta
+// (the tail's imaginary part is never negative), so the plain formula
+// (x - i y) / (x^2 + y^2) stays in range; the omegas run on the host threads.
 std::vector<cplx> cf_eval(int it, const double *al, const double *be, double n2, double e0, double eta,
                           const double *omega, int nw) {
     std::vector<cplx> f(nw);
-    for (int q = 0; q < nw; ++q) {
-        const cplx z(omega[q] + e0, -eta);
-        cplx g(0.0, 0.0);
-        for (int k = it - 1; k >= 0; --k) {
-            const cplx tail = (k + 1 < it) ? be[k + 1] * be[k + 1] * g : cplx(0.0, 0.0);
-            g = 1.0 / (z - al[k] - tail);
+    host_parallel_for(nw, 128, [&](int q0, int q1) {
+        for (int q = q0; q < q1; ++q) {
+            const double zr = omega[q] + e0, zi = -eta;
+            double gr = 0.0, gi = 0.0;
+            for (int k = it - 1; k >= 0; --k) {
+                double xr = zr - al[k], xi = zi;
+                if (k + 1 < it) {
+                    const double b2 = be[k + 1] * be[k + 1];
+                    xr -= b2 * gr;
+                    xi -= b2 * gi;
+                }
+                const double d = xr * xr + xi * xi;
+                gr = xr / d;
+                gi = -xi / d;
+            }
+            f[q] = cplx(n2 * gr, n2 * gi);
         }
-        f[q] = n2 * g;
-    }
+    });
     return f;
 }
 

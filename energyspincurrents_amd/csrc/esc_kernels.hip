@@ -36,15 +36,13 @@ __device__ __forceinline__ double sgn(uint32_t r, uint32_t k) { return ((r >> k)
 // reference order (z bonds, then Heisenberg bonds, then Bzz * Sz_total), so for
 // c128 every matrix element equals the reference's bit for bit.
 // ---------------------------------------------------------------------------
-// NR rows at a time, the term loop outermost, so every term has NR
-// independent gathers in flight; per row the arithmetic is that of one row.
+// NR rows at a time, the term loop outermost (unrolled by 4), so 4 NR
+// independent gathers are in flight; per row the arithmetic is that of one
+// row.  NR <= 2 fits the registers of a 1024-lane workgroup without spills.
 template <typename R, int NR>
 __device__ __forceinline__ void hpsi_rows(const HTables *__restrict__ T, const typename Vec2<R>::T *__restrict__ in,
                                           const uint32_t (&r)[NR], typename Vec2<R>::T (&o)[NR]) {
     using V2 = typename Vec2<R>::T;
-    // terms unrolled by 4: 4 NR gathers in flight (NR <= 2 fits the registers
-    // of a 1024-lane workgroup without spills)
-    constexpr int kTermUnroll = 4;
     double d[NR];
     {
         // no FMA contraction: Bzz * Sz_total is rounded before it is added, as
@@ -86,7 +84,7 @@ __device__ __forceinline__ void hpsi_rows(const HTables *__restrict__ T, const t
     }
 
     const uint32_t nb = T->n_bonds;
-#pragma unroll(kTermUnroll)
+#pragma unroll 4
     for (uint32_t b = 0; b < nb; ++b) {
         const uint32_t p = T->bond[b];
         const uint32_t i = p & 0xffu, j = (p >> 8) & 0xffu, f = p >> 16;
@@ -108,7 +106,7 @@ __device__ __forceinline__ void hpsi_rows(const HTables *__restrict__ T, const t
     const uint32_t nf = T->n_flip;
     const R bx2 = (R)T->bx2, by2 = (R)T->by2, g4 = (R)T->g4;
     const bool gam = T->has_gamma_flip != 0;
-#pragma unroll(kTermUnroll)
+#pragma unroll 4
     for (uint32_t k = 0; k < nf; ++k) {
         V2 x[NR];
 #pragma unroll
