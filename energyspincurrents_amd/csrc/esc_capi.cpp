@@ -2458,6 +2458,7 @@ struct esc_batch {
     size_t esz = 16;
     std::vector<esc_ctx *> ctx;
     std::vector<void *> mem;              // per instance: slots 0..2, start, gs, phi
+    std::vector<double *> dotbuf;         // per instance: tiled H.psi dot partials, or null
     std::vector<esc::BatchLane> lanes[3]; // per rotation phase
     esc::BatchLane *d_lanes[3] = {nullptr, nullptr, nullptr};
     uint32_t total_blocks = 0;
@@ -2484,6 +2485,7 @@ struct esc_batch {
         if (s) (void)hipStreamSynchronize(s);
         drop_graphs();
         for (void *m : mem) if (m) (void)hipFree(m);
+        for (double *d : dotbuf) if (d) (void)hipFree(d);
         for (auto *d : d_lanes) if (d) (void)hipFree(d);
         for (void *p : {(void *)d_partials, (void *)d_alpha, (void *)d_beta, (void *)d_coef, (void *)d_step})
             if (p) (void)hipFree(p);
@@ -2513,6 +2515,32 @@ void batch_capacity(esc_batch *b, int steps) {
     b->cap = steps;
 }
 
+// Instances with a tiled H.psi get their dot <v|w> from the kernel's per-tile
+// partials (one buffer each, sized for the current kernel choice), so the
+// batched dot pass skips them.  Called when the kernel choices are set
+// (creation, autotuning), with no graph holding the old lanes.
+void batch_setup_dots(esc_batch *b) {
+    b->dotbuf.resize(b->n, nullptr);
+    for (int i = 0; i < b->n; ++i) {
+        esc_ctx *c = b->ctx[i];
+        if (b->dotbuf[i]) hip_check(hipFree(b->dotbuf[i]), "hipFree");
+        b->dotbuf[i] = nullptr;
+        uint32_t ndot = 0;
+        if (c->tile_bits > 0) {
+            ndot = (uint32_t)(c->dim >> cfg_tile_bits(c->tile_bits));
+            if (hipMalloc(&b->dotbuf[i], 2 * (size_t)ndot * sizeof(double)) != hipSuccess)
+                throw EscError(ESC_ERR_ALLOC, "batch dot buffer allocation failed");
+        }
+        for (int p = 0; p < 3; ++p) {
+            b->lanes[p][i].dotp = b->dotbuf[i];
+            b->lanes[p][i].ndot = ndot;
+        }
+    }
+    for (int p = 0; p < 3; ++p)
+        hip_check(hipMemcpy(b->d_lanes[p], b->lanes[p].data(), sizeof(esc::BatchLane) * b->n, hipMemcpyHostToDevice),
+                  "hipMemcpy");
+}
+
 // the launches of step k (phase k % 3) for all instances
 void batch_step_launches(esc_batch *b, int phase, bool accum) {
     const int lda = b->cap, ldb = b->cap + 1, ldc = b->cap;
@@ -2520,7 +2548,11 @@ void batch_step_launches(esc_batch *b, int phase, bool accum) {
         hip_check(esc::launch_batch_step(b->c64, esc::kBatchAccum, b->d_lanes[phase], b->n, b->total_blocks,
                                          b->d_partials, b->d_alpha, lda, b->d_beta, ldb, b->d_coef, ldc, b->d_step,
                                          b->s), "batch accumulate");
-    for (int i = 0; i < b->n; ++i) run_hpsi(b->ctx[i], b->lanes[phase][i].v, b->lanes[phase][i].w, b->s);
+    for (int i = 0; i < b->n; ++i) {
+        const esc::BatchLane &l = b->lanes[phase][i];
+        if (l.ndot > 0) launch_hpsi_with(b->ctx[i], b->ctx[i]->tile_bits, l.v, l.w, b->s, b->dotbuf[i]);
+        else run_hpsi(b->ctx[i], l.v, l.w, b->s);
+    }
     for (int piece : {esc::kBatchDot, esc::kBatchUpdate})
         hip_check(esc::launch_batch_step(b->c64, piece, b->d_lanes[phase], b->n, b->total_blocks, b->d_partials,
                                          b->d_alpha, lda, b->d_beta, ldb, b->d_coef, ldc, b->d_step, b->s),
@@ -2637,6 +2669,8 @@ int esc_batch_create(int n, const esc_model *models, int precision, esc_batch **
                 l.dim = dim;
                 l.blk0 = blk;
                 l.nblk = (uint32_t)nb;
+                l.dotp = nullptr;
+                l.ndot = 0;
             }
             blk += (uint32_t)nb;
         }
@@ -2650,6 +2684,7 @@ int esc_batch_create(int n, const esc_model *models, int precision, esc_batch **
         if (hipMalloc(&b->d_partials, (size_t)blk * 2 * sizeof(double)) != hipSuccess ||
             hipMalloc(&b->d_step, sizeof(uint32_t)) != hipSuccess)
             throw EscError(ESC_ERR_ALLOC, "batch workspace allocation failed");
+        batch_setup_dots(b.get());
         b->e0.assign(n, 0.0);
         *out = b.release();
     });
@@ -2719,6 +2754,7 @@ int esc_batch_autotune(esc_batch *b, int reps, int *configs) {
             if (configs) configs[i] = cfg;
         }
         hip_check(hipStreamSynchronize(b->s), "sync");
+        batch_setup_dots(b);
     });
 }
 

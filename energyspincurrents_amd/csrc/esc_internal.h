@@ -207,6 +207,11 @@ struct BatchLane {
     void *vp, *v, *w, *acc;
     uint64_t dim;
     uint32_t blk0, nblk;
+    // tiled H.psi: the dot <v|w> comes from the kernel's ndot per-tile
+    // partials at dotp (2 doubles each), and the batched dot pass skips the
+    // lane; ndot = 0 otherwise
+    const double *dotp;
+    uint32_t ndot;
 };
 // The batched recursion keeps unnormalised vectors u_k (v_k = u_k / beta_k):
 constexpr int kBatchDot = 0;      // alpha[i][k] = <v_k|H v_k>

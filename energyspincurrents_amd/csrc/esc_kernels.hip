@@ -959,6 +959,7 @@ template <typename R>
 __global__ __launch_bounds__(kThreads) void k_b_dot(const BatchLane *__restrict__ L, int n, double *__restrict__ partials) {
     using V2 = typename Vec2<R>::T;
     const BatchLane &l = L[batch_instance(L, n, blockIdx.x)];
+    if (l.ndot > 0) return;   // the whole block: the H.psi kernel formed this lane's dot
     const V2 *x = (const V2 *)l.v, *y = (const V2 *)l.w;
     double v[2] = {0.0, 0.0};
     const uint64_t stride = (uint64_t)l.nblk * kThreads;
@@ -988,9 +989,16 @@ __global__ __launch_bounds__(kThreads) void k_b_alpha(const BatchLane *__restric
     const BatchLane &l = L[blockIdx.x];
     __shared__ double tot[2];
     double v[2] = {0.0, 0.0};
-    for (uint32_t p = threadIdx.x; p < l.nblk; p += kThreads) {
-        v[0] += partials[2 * (size_t)(l.blk0 + p)];
-        v[1] += partials[2 * (size_t)(l.blk0 + p) + 1];
+    if (l.ndot > 0) {
+        for (uint32_t p = threadIdx.x; p < l.ndot; p += kThreads) {
+            v[0] += l.dotp[2 * (size_t)p];
+            v[1] += l.dotp[2 * (size_t)p + 1];
+        }
+    } else {
+        for (uint32_t p = threadIdx.x; p < l.nblk; p += kThreads) {
+            v[0] += partials[2 * (size_t)(l.blk0 + p)];
+            v[1] += partials[2 * (size_t)(l.blk0 + p) + 1];
+        }
     }
     block_reduce_store<2>(v, tot);
     __syncthreads();

@@ -29,11 +29,11 @@ def kjh(phi, h):
 
 MIXED = [O.Model(2, 4, **kjh(0.3, 0.2)), O.Model(3, 4, K=(1.0, 1.0, 1.0), B=(0.3, 0.3, 0.3)),
          O.Model(2, 6, K=(-1.0, -1.0, -1.0), J=0.1, G=0.2, B=O.field111(0.2)), O.Model(2, 8, **kjh(2.0, 0.4)),
-         O.Model(2, 4, **kjh(4.0, 0.1))]
+         O.Model(2, 4, **kjh(4.0, 0.1)), O.Model(3, 6, **kjh(1.0, 0.3))]
 
 
 def test_batch_ground_states_graph_equals_eager_and_exact():
-    print("[batch gs] mixed N=8,12,12,16,8", flush=True)
+    print("[batch gs] mixed N=8,12,12,16,8,18", flush=True)
     b = esc.Batch([spin_model(m) for m in MIXED], "c128")
     e_eager, it_e, al_e, be_e = b.ground_state(seed=21, steps=200, use_graph=False)
     gs_eager = [b.state(i).cpu().numpy() for i in range(b.n)]
