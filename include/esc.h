@@ -9,7 +9,9 @@
  * Every call returns ESC_OK (0) or a negative status and then sets a
  * thread-local message readable with esc_last_error().  No C++ exception
  * crosses this boundary.  Calls on one context must not run concurrently;
- * distinct contexts are independent.
+ * distinct contexts are independent.  Entry points that evaluate continued
+ * fractions on an omega grid (spectra, batches, gradients) spread that host
+ * loop over up to 16 threads that they create and join before returning.
  *
  * Basis convention (as the reference): index r has bit k = site k and bit value
  * 0 = spin up (S^z = +1/2), S = sigma/2 (src/QuantumBasis.h:34-74).
