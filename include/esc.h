@@ -10,8 +10,9 @@
  * thread-local message readable with esc_last_error().  No C++ exception
  * crosses this boundary.  Calls on one context must not run concurrently;
  * distinct contexts are independent.  Entry points that evaluate continued
- * fractions on an omega grid (spectra, batches, gradients) spread that host
- * loop over up to 16 threads that they create and join before returning.
+ * fractions on an omega grid (esc_dynamics, esc_rixs, esc_batch_dynamics)
+ * spread that host loop over up to 16 threads that they create and join
+ * before returning.
  *
  * Basis convention (as the reference): index r has bit k = site k and bit value
  * 0 = spin up (S^z = +1/2), S = sigma/2 (src/QuantumBasis.h:34-74).
