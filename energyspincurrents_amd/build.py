@@ -46,7 +46,7 @@ def build_lib():
         _run([HIPCC, *COMMON, *DEVICE_FLAGS, *lang, "-c", path, "-o", obj])
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([HIPCC, *COMMON, "-shared", *objs, "-o", tmp])
+    _run([HIPCC, *COMMON, "-shared", *objs, "-o", tmp, "-lrocsolver", "-lrocblas"])
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)

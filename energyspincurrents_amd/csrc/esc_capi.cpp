@@ -7,6 +7,8 @@
 #include "esc_internal.h"
 
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <chrono>
@@ -206,6 +208,9 @@ void progress(const char *fmt, ...) {
 // ------------------------------------------------------------------------
 // Context
 // ------------------------------------------------------------------------
+// dense diagonalisation (Solver=ED) up to 2^13 states: 1 GiB of matrix
+constexpr int kMaxEdSites = 13;
+
 struct esc_ctx {
     int n_sites = 0;
     int precision = ESC_C128;
@@ -1980,6 +1985,82 @@ int esc_lanczos(esc_ctx *ctx, const void *d_v0, int steps, double eps, int mode,
             hip_check(hipStreamSynchronize(s), "sync");
             ctx->basis_kept = mode == ESC_LANC_ALL;
             ctx->basis_n = n;
+        }
+    });
+}
+
+// Solver=ED: H as a dense column-major matrix in the context's basis
+// storage, column k = H e_k from the context's H.psi kernel (every element is
+// the kernel's own, so bit-exact), then all eigenpairs by rocSOLVER's
+// divide-and-conquer zheevd on the device.  The eigenvectors stay in place as
+// the kept basis with an identity rotation, so everything that reads the
+// Ritz states of an ESC_LANC_ALL run reads the eigenstates instead
+// (src/Exactdiag.h:36-81, src/DynLanczosSolver.h:269-283).  The reference's
+// matrix is never zero-initialised before the += (src/Exactdiag.h:42); here it
+// is.
+int esc_exact_diag(esc_ctx *ctx, double *evals, void *d_gs, void *stream) {
+    return guarded([&] {
+        check_ctx(ctx);
+        check_ptr(evals, "evals");
+        if (ctx->c64()) throw EscError(ESC_ERR_ARG, "Solver=ED runs in complex128 (Precision=c128)");
+        if (ctx->n_sites > kMaxEdSites)
+            throw EscError(ESC_ERR_ARG, "Solver=ED needs NumberofSites <= " + std::to_string(kMaxEdSites) +
+                                            " (dense 2^N x 2^N matrix); use the Lanczos solver");
+        hipStream_t s = (hipStream_t)stream;
+        const uint64_t dim = ctx->dim;
+        const size_t vb = dim * ctx->esz;
+        ctx->ensure_vecs(dim);
+        ctx->basis_kept = false;
+        ctx->ensure_aux(1);
+        const std::complex<double> one(1.0, 0.0);
+        for (uint64_t k = 0; k < dim; ++k) {
+            hip_check(hipMemsetAsync(ctx->aux(0), 0, vb, s), "hipMemsetAsync");
+            hip_check(hipMemcpyAsync((char *)ctx->aux(0) + k * ctx->esz, &one, sizeof(one), hipMemcpyHostToDevice, s),
+                      "unit vector");
+            run_hpsi(ctx, ctx->aux(0), ctx->vec(k), s);
+        }
+        double *d_work = nullptr;
+        rocblas_int *d_info = nullptr;
+        rocblas_handle h = nullptr;
+        auto release = [&] {
+            if (h) (void)rocblas_destroy_handle(h);
+            if (d_work) (void)hipFree(d_work);
+            if (d_info) (void)hipFree(d_info);
+        };
+        try {
+            if (hipMalloc(&d_work, 2 * dim * sizeof(double)) != hipSuccess ||
+                hipMalloc(&d_info, sizeof(rocblas_int)) != hipSuccess)
+                throw EscError(ESC_ERR_ALLOC, "ED workspace allocation failed");
+            if (rocblas_create_handle(&h) != rocblas_status_success || rocblas_set_stream(h, s) != rocblas_status_success)
+                throw EscError(ESC_ERR_HIP, "rocBLAS handle creation failed");
+            const rocblas_status st =
+                rocsolver_zheevd(h, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)dim,
+                                 reinterpret_cast<rocblas_double_complex *>(ctx->vec(0)), (rocblas_int)dim, d_work,
+                                 d_work + dim, d_info);
+            if (st != rocblas_status_success)
+                throw EscError(ESC_ERR_NUMERIC, "rocsolver_zheevd failed (status " + std::to_string((int)st) + ")");
+            rocblas_int info = 0;
+            std::vector<double> ev(dim);
+            sync_get(s, &info, d_info, sizeof(info));
+            if (info != 0) throw EscError(ESC_ERR_NUMERIC, "zheevd: " + std::to_string(info) + " off-diagonal elements did not converge");
+            sync_get(s, ev.data(), d_work, dim * sizeof(double));
+            release();
+            h = nullptr; d_work = nullptr; d_info = nullptr;
+            for (double e : ev)
+                if (!std::isfinite(e)) throw EscError(ESC_ERR_NUMERIC, "non-finite eigenvalue");
+            std::memcpy(evals, ev.data(), dim * sizeof(double));
+            const int n = (int)dim;
+            ctx->Z.assign((size_t)n * n, 0.0);
+            for (int k = 0; k < n; ++k) ctx->Z[(size_t)k * n + k] = 1.0;
+            ctx->z_n = n;
+            ctx->evals = ev;
+            ctx->basis_n = n;
+            ctx->basis_kept = true;
+            if (d_gs) hip_check(hipMemcpyAsync(d_gs, ctx->vec(0), vb, hipMemcpyDeviceToDevice, s), "copy gs");
+            hip_check(hipStreamSynchronize(s), "sync");
+        } catch (...) {
+            release();
+            throw;
         }
     });
 }

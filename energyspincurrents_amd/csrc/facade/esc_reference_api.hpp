@@ -378,6 +378,33 @@ private:
 };
 
 // --------------------------------------------------------------------------
+// Eigenvectors of Solver=ED (the reference's Eigen::MatrixXcd Evec): column k
+// is eigenstate k, resident on the device as the context's kept basis.
+// --------------------------------------------------------------------------
+using VectorXd = std::vector<double>;
+
+class MatrixXcd {
+public:
+    MatrixXcd() = default;
+    MatrixXcd(std::shared_ptr<Context> ctx, int run) : ctx_(std::move(ctx)), run_(run) {}
+    int64_t rows() const { return ctx_ ? ctx_->dim() : 0; }
+    int64_t cols() const { return rows(); }
+    VectorXcd col(int k) const {
+        if (!ctx_ || ctx_->lanczos_runs != run_)
+            throw std::runtime_error("Evec: the eigenvectors were replaced by a later solve on this Hamiltonian");
+        VectorXcd v(ctx_);
+        check(esc_ritz_vector(ctx_->get(), k, v.data(), nullptr));
+        return v;
+    }
+    const std::shared_ptr<Context> &context() const { return ctx_; }
+    int run() const { return run_; }
+
+private:
+    std::shared_ptr<Context> ctx_;
+    int run_ = 0;
+};
+
+// --------------------------------------------------------------------------
 // DynLanczosSolver (src/DynLanczosSolver.h:24-239)
 // --------------------------------------------------------------------------
 class DynLanczosSolver {
@@ -437,6 +464,20 @@ public:
         PsiAll = RitzStates{this, basis_kept ? iters : (mode == ESC_LANC_GS ? 1 : 0)};
         if (mode == ESC_LANC_ONLYEGS) gs.setZero();
         return gs;
+    }
+
+    // Solver=ED results as the solver's Ritz data: eigenvalues as TriDeval,
+    // eigenvectors as PsiAll (src/DynLanczosSolver.h:269-283)
+    void fillFromED(MatrixXcd Evec, VectorXd eval) {
+        if (!Evec.context() || (int64_t)eval.size() != Evec.rows())
+            throw std::invalid_argument("fillFromED: eigenvalues and eigenvectors do not match");
+        ctx_ = Evec.context();
+        run_ = Evec.run();
+        TriDeval = eval;
+        alpha.clear();
+        beta.clear();
+        basis_kept = true;
+        PsiAll = RitzStates{this, (int)eval.size()};
     }
 
     // Start vector (src/DynLanczosSolver.h:58-59,465-474): re then im drawn
@@ -662,6 +703,22 @@ inline void LocalDynRIXS(std::string &dynstr, ConstVariables &Parameters, Lattic
         std::cout << std::endl;
         f << std::endl;
     }
+}
+
+// Solver=ED (src/Exactdiag.h:36-81): every eigenpair of the dense H on the
+// device; D receives the eigenvalues (ascending), the returned matrix the
+// eigenvectors
+inline MatrixXcd ExactDiag(ConstVariables &variables_, QBasis &Basis_, Hamiltonian &Hamil_, Observables &Observ_,
+                           VectorXd &D) {
+    (void)variables_; (void)Basis_; (void)Observ_;
+    std::cout << "Begin Exact Diagonalization! --- " << std::endl;
+    auto ctx = Hamil_.context();
+    D.assign((size_t)ctx->dim(), 0.0);
+    check(esc_exact_diag(ctx->get(), D.data(), nullptr, nullptr));
+    const int run = ++ctx->lanczos_runs;
+    for (double d : D) std::cout << d << ", ";
+    std::cout << " Ground-state energy:= " << std::setprecision(16) << D[0] << std::endl;
+    return MatrixXcd(ctx, run);
 }
 
 inline void StartDynamics(std::string &dynstr, ConstVariables &variables, Lattice &Lat, QBasis &Basis,

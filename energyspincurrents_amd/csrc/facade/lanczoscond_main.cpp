@@ -29,16 +29,22 @@ int main(int argc, char *argv[]) {
         Ham.Diagonal();
         Observables Observ(Parameters, Lat, Basis, Ham);
         DynLanczosSolver Lanczos(Parameters, Basis, Ham);
-        if (Parameters.Solver == "ED")
-            throw std::invalid_argument("Solver=ED (dense diagonalisation) is not provided by this engine; use the "
-                                        "Lanczos solver (Solver=none)");
-        VectorXcd Psi = Lanczos.Lanczos_Nirav(Parameters.LancType);
-        std::cout << std::setprecision(6) << std::fixed;
-        std::cout << " Ritz values: ";
-        for (double e : Lanczos.TriDeval) std::cout << e << " ";
-        std::cout << std::endl;
+        VectorXcd Psi;
+        const bool ed = Parameters.Solver == "ED";
+        if (ed) {
+            VectorXd eval;
+            MatrixXcd Evec = ExactDiag(Parameters, Basis, Ham, Observ, eval);
+            Psi = Evec.col(0);
+            Lanczos.fillFromED(Evec, eval);
+        } else {
+            Psi = Lanczos.Lanczos_Nirav(Parameters.LancType);
+            std::cout << std::setprecision(6) << std::fixed;
+            std::cout << " Ritz values: ";
+            for (double e : Lanczos.TriDeval) std::cout << e << " ";
+            std::cout << std::endl;
+        }
         std::cout << std::setprecision(8) << std::fixed;
-        if (Parameters.LancType != "OnlyEgs") {
+        if (ed || Parameters.LancType != "OnlyEgs") {
             Observ.measureLocalS(Psi);
             if (!Lat.chain()) Observ.measureLocalJe(Psi);   // currents live on the honeycomb only
         }

@@ -28,11 +28,27 @@ def _np_ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
-def _dev_ptr(x):
+def _dev_ptr(x, dtype, n, name, optional=False):
+    """Device pointer of a vector argument, checked before anything launches.
+
+    The kernels index n elements of the context's precision from the pointer,
+    so a wrong dtype, a short tensor, a strided view or host memory would be an
+    out-of-bounds device access: each raises ValueError/TypeError here."""
     if x is None:
-        return None
-    if isinstance(x, int):
-        return ctypes.c_void_p(x)
+        if optional:
+            return None
+        raise ValueError(f"{name}: a device vector is required")
+    import torch
+    if not isinstance(x, torch.Tensor):
+        raise TypeError(f"{name}: expected a torch tensor on the GPU, got {type(x).__name__}")
+    if x.device.type != "cuda":
+        raise ValueError(f"{name}: tensor is on {x.device}, the engine needs device memory")
+    if x.dtype != dtype:
+        raise ValueError(f"{name}: dtype {x.dtype} does not match the engine precision ({dtype})")
+    if x.numel() != n:
+        raise ValueError(f"{name}: {x.numel()} elements, the basis has {n}")
+    if not x.is_contiguous():
+        raise ValueError(f"{name}: tensor must be contiguous")
     return ctypes.c_void_p(x.data_ptr())
 
 
@@ -180,29 +196,34 @@ class Engine:
         import torch
         return torch.empty(self.dim, dtype=self.torch_dtype, device="cuda")
 
+    def _v(self, x, name, optional=False):
+        return _dev_ptr(x, self.torch_dtype, self.dim, name, optional)
+
     def upload(self, host: np.ndarray, out=None):
         out = self.empty() if out is None else out
-        h = np.ascontiguousarray(host, np.complex128)
-        check(lib().esc_upload(self._h, _np_ptr(h), _dev_ptr(out), _stream(None)))
+        h = np.ascontiguousarray(host, np.complex128).reshape(-1)
+        if h.size != self.dim:
+            raise ValueError(f"host vector has {h.size} elements, the basis has {self.dim}")
+        check(lib().esc_upload(self._h, _np_ptr(h), self._v(out, "out"), _stream(None)))
         return out
 
     def download(self, x) -> np.ndarray:
         h = np.empty(self.dim, np.complex128)
-        check(lib().esc_download(self._h, _dev_ptr(x), _np_ptr(h), _stream(None)))
+        check(lib().esc_download(self._h, self._v(x, "x"), _np_ptr(h), _stream(None)))
         return h
 
     # -- operators --------------------------------------------------------
     def hpsi(self, x, out=None, stream=None):
         """out = H x  (Hamiltonian::MatrixVectorMult, src/Hamiltonian.h:480-557)."""
         out = self.empty() if out is None else out
-        check(lib().esc_hpsi(self._h, _dev_ptr(x), _dev_ptr(out), _stream(stream)))
+        check(lib().esc_hpsi(self._h, self._v(x, "x"), self._v(out, "out"), _stream(stream)))
         return out
 
     def autotune(self, x, out=None, reps=5, stream=None):
         """Time every H.psi tile size on x and keep the fastest (esc_autotune)."""
         out = self.empty() if out is None else out
         tb, ms = ctypes.c_int(), ctypes.c_double()
-        check(lib().esc_autotune(self._h, _dev_ptr(x), _dev_ptr(out), int(reps), ctypes.byref(tb),
+        check(lib().esc_autotune(self._h, self._v(x, "x"), self._v(out, "out"), int(reps), ctypes.byref(tb),
                                  ctypes.byref(ms), _stream(stream)))
         return tb.value, ms.value
 
@@ -223,7 +244,7 @@ class Engine:
     def apply_op(self, op: str, site: int, x, out=None, stream=None):
         """Observables::ApplyOp / ApplyJ*Total (site = -1) (src/Observables.h:24-67,448-470,664-699)."""
         out = self.empty() if out is None else out
-        check(lib().esc_apply_op(self._h, OPS[op], int(site), _dev_ptr(x), _dev_ptr(out), _stream(stream)))
+        check(lib().esc_apply_op(self._h, OPS[op], int(site), self._v(x, "x"), self._v(out, "out"), _stream(stream)))
         return out
 
     def apply_total(self, op: str, x, out=None, stream=None):
@@ -231,19 +252,19 @@ class Engine:
 
     def start_vector(self, seed: int, kind="Uniform", out=None, stream=None):
         out = self.empty() if out is None else out
-        check(lib().esc_start_vector(self._h, ctypes.c_uint64(seed), START_KINDS[kind], _dev_ptr(out),
+        check(lib().esc_start_vector(self._h, ctypes.c_uint64(seed), START_KINDS[kind], self._v(out, "out"),
                                      _stream(stream)))
         return out
 
     def dot(self, x, y, stream=None) -> complex:
         r = np.zeros(2)
-        check(lib().esc_dot(self._h, _dev_ptr(x), _dev_ptr(y), _np_ptr(r), _stream(stream)))
+        check(lib().esc_dot(self._h, self._v(x, "x"), self._v(y, "y"), _np_ptr(r), _stream(stream)))
         return complex(r[0], r[1])
 
     def axpby(self, a: complex, x, b: complex, y, stream=None):
         av = np.array([a.real, a.imag]) if isinstance(a, complex) else np.array([float(a), 0.0])
         bv = np.array([b.real, b.imag]) if isinstance(b, complex) else np.array([float(b), 0.0])
-        check(lib().esc_axpby(self._h, _np_ptr(av), _dev_ptr(x), _np_ptr(bv), _dev_ptr(y), _stream(stream)))
+        check(lib().esc_axpby(self._h, _np_ptr(av), self._v(x, "x"), _np_ptr(bv), self._v(y, "y"), _stream(stream)))
         return y
 
     # -- Lanczos ----------------------------------------------------------
@@ -254,12 +275,21 @@ class Engine:
         ev = np.zeros(steps + 1)
         Z = np.zeros((steps + 1) * (steps + 1))
         it = ctypes.c_int()
-        check(lib().esc_lanczos(self._h, _dev_ptr(v0), int(steps), float(eps), LANC_MODES[mode], _np_ptr(a),
-                                _np_ptr(b), ctypes.byref(it), _np_ptr(ev), _np_ptr(Z), _dev_ptr(gs),
+        check(lib().esc_lanczos(self._h, self._v(v0, "v0"), int(steps), float(eps), LANC_MODES[mode], _np_ptr(a),
+                                _np_ptr(b), ctypes.byref(it), _np_ptr(ev), _np_ptr(Z), self._v(gs, "gs", optional=True),
                                 _stream(stream)))
         n = it.value
         return LanczosResult(a[:n].copy(), b[: n + 1].copy(), n, ev[:n].copy(),
                              Z[: n * n].reshape(n, n).T.copy())
+
+    def exact_diag(self, gs=None, stream=None):
+        """Solver=ED: every eigenvalue (ascending) of the dense H, diagonalised
+        on the device (ExactDiag + fillFromED, src/Exactdiag.h:36-110,
+        src/DynLanczosSolver.h:269-283).  The eigenvectors are kept as the
+        Ritz states for Ritz spectra; gs (optional) receives the lowest."""
+        ev = np.zeros(self.dim)
+        check(lib().esc_exact_diag(self._h, _np_ptr(ev), self._v(gs, "gs", optional=True), _stream(stream)))
+        return ev
 
     def ground_state(self, v0, steps=200, eps=0.0, restarts=0, out=None, stream=None):
         """Ground state with 5 resident vectors (esc_ground_state): two-pass
@@ -267,18 +297,18 @@ class Engine:
         previous state.  Returns (out, e0, residual norm, H.psi count)."""
         out = self.empty() if out is None else out
         e0, res, it = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
-        check(lib().esc_ground_state(self._h, _dev_ptr(v0), int(steps), float(eps), int(restarts), ctypes.byref(e0),
-                                     ctypes.byref(res), ctypes.byref(it), _dev_ptr(out), _stream(stream)))
+        check(lib().esc_ground_state(self._h, self._v(v0, "v0"), int(steps), float(eps), int(restarts), ctypes.byref(e0),
+                                     ctypes.byref(res), ctypes.byref(it), self._v(out, "out"), _stream(stream)))
         return out, e0.value, res.value, it.value
 
     def ritz_overlaps(self, vec, n_states, stream=None) -> np.ndarray:
         out = np.zeros(2 * n_states)
-        check(lib().esc_ritz_overlaps(self._h, _dev_ptr(vec), int(n_states), _np_ptr(out), _stream(stream)))
+        check(lib().esc_ritz_overlaps(self._h, self._v(vec, "vec"), int(n_states), _np_ptr(out), _stream(stream)))
         return out[0::2] + 1j * out[1::2]
 
     def ritz_vector(self, k, out=None, stream=None):
         out = self.empty() if out is None else out
-        check(lib().esc_ritz_vector(self._h, int(k), _dev_ptr(out), _stream(stream)))
+        check(lib().esc_ritz_vector(self._h, int(k), self._v(out, "out"), _stream(stream)))
         return out
 
     def two_point(self, op_a: str, op_b: str, gs, stream=None):
@@ -286,7 +316,7 @@ class Engine:
         <gs|A_i B_j|gs>, filled for i >= j and mirrored."""
         N = self.model.lattice.N
         out = np.zeros(2 * N * N)
-        check(lib().esc_two_point(self._h, OPS[op_a], OPS[op_b], _dev_ptr(gs), _np_ptr(out), _stream(stream)))
+        check(lib().esc_two_point(self._h, OPS[op_a], OPS[op_b], self._v(gs, "gs"), _np_ptr(out), _stream(stream)))
         return (out[0::2] + 1j * out[1::2]).reshape(N, N)
 
     def lanczos_cf(self, phi, steps, stream=None):
@@ -294,7 +324,7 @@ class Engine:
         b = np.zeros(steps + 2)
         it = ctypes.c_int()
         n2 = ctypes.c_double()
-        check(lib().esc_lanczos_cf(self._h, _dev_ptr(phi), int(steps), _np_ptr(a), _np_ptr(b), ctypes.byref(it),
+        check(lib().esc_lanczos_cf(self._h, self._v(phi, "phi"), int(steps), _np_ptr(a), _np_ptr(b), ctypes.byref(it),
                                    ctypes.byref(n2), _stream(stream)))
         n = it.value
         return a[:n].copy(), b[: n + 1].copy(), n2.value
@@ -306,7 +336,7 @@ class Engine:
         ncols = {"JsJs": 14, "JeJe": 5, "LocalSzSz": 8}[kind]
         table = np.zeros((len(omega), 1 + ncols))
         expect = np.zeros(2 * 4)
-        check(lib().esc_dynamics(self._h, DYN_KINDS[kind], SPEC_METHODS[method], _dev_ptr(gs), float(e0),
+        check(lib().esc_dynamics(self._h, DYN_KINDS[kind], SPEC_METHODS[method], self._v(gs, "gs"), float(e0),
                                  int(cf_steps), float(eta), _np_ptr(omega), len(omega), _np_ptr(table),
                                  _np_ptr(expect), _stream(stream)))
         nexp = 4 if kind == "JsJs" else 3
@@ -320,7 +350,7 @@ class Engine:
         kept on both engines.  Returns the (nw, 9) table of RIXS_outSpectrum.real."""
         omega = np.ascontiguousarray(tenth_omega() if omega is None else omega, np.float64)
         table = np.zeros((len(omega), 9))
-        check(lib().esc_rixs(self._h, inter._h, SPEC_METHODS[method], int(site), _dev_ptr(gs), float(e0),
+        check(lib().esc_rixs(self._h, inter._h, SPEC_METHODS[method], int(site), self._v(gs, "gs"), float(e0),
                              int(cf_steps), float(gamma), float(eta), _np_ptr(omega), len(omega), _np_ptr(table),
                              _stream(stream)))
         return table
@@ -331,7 +361,7 @@ class Engine:
         omega = np.ascontiguousarray(tenth_omega() if omega is None else omega, np.float64)
         N = self.model.lattice.N
         table = np.zeros((len(omega), N, N, 2))
-        check(lib().esc_szsz(self._h, _dev_ptr(gs), float(eta), _np_ptr(omega), len(omega), _np_ptr(table),
+        check(lib().esc_szsz(self._h, self._v(gs, "gs"), float(eta), _np_ptr(omega), len(omega), _np_ptr(table),
                              _stream(stream)))
         return table[..., 0] + 1j * table[..., 1]
 
@@ -390,11 +420,13 @@ class Batch:
     def state(self, i, out=None, stream=None):
         """Instance i's ground state as a device tensor."""
         import torch
+        if not 0 <= int(i) < self.n:
+            raise ValueError(f"instance {i} out of range [0, {self.n})")
         m = self.models[i]
+        dt = torch.complex64 if self.precision == "c64" else torch.complex128
         if out is None:
-            out = torch.empty(1 << m.lattice.N, dtype=torch.complex64 if self.precision == "c64" else torch.complex128,
-                              device="cuda")
-        check(lib().esc_batch_state(self._h, int(i), _dev_ptr(out), _stream(stream)))
+            out = torch.empty(1 << m.lattice.N, dtype=dt, device="cuda")
+        check(lib().esc_batch_state(self._h, int(i), _dev_ptr(out, dt, 1 << m.lattice.N, "out"), _stream(stream)))
         if stream is None:
             torch.cuda.current_stream().synchronize()
         return out

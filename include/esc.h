@@ -156,6 +156,17 @@ int esc_lanczos(esc_ctx *ctx, const void *d_v0, int steps, double eps, int mode,
                 double *alpha, double *beta, int *iters, double *evals, double *Z,
                 void *d_gs, void *stream);
 
+/* Solver=ED: every eigenpair of H.  The dense 2^N x 2^N matrix is formed on
+ * the device from the context's own H.psi (column k = H e_k, so every element
+ * is bit-exact) and diagonalised by rocSOLVER zheevd (divide and conquer,
+ * FP64).  evals: 2^N doubles, ascending.  The eigenvectors become the kept
+ * basis, exactly as after an ESC_LANC_ALL run with an identity rotation, so
+ * esc_ritz_vector / esc_ritz_overlaps / ESC_SPEC_RITZ spectra read them as
+ * PsiAll; d_gs (may be NULL) receives eigenvector 0.  complex128 contexts,
+ * N <= 13.  Replaces ExactDiag + Diagonalize (src/Exactdiag.h:36-110) and
+ * DynLanczosSolver::fillFromED (src/DynLanczosSolver.h:269-283). */
+int esc_exact_diag(esc_ctx *ctx, double *evals, void *d_gs, void *stream);
+
 /* Ground state with three resident Lanczos vectors plus the start and the
  * result (the memory plan for N = 32: 5 vectors of 2^32 complex64 = 172 GB).
  * Each cycle runs the recursion of esc_lanczos without reorthogonalisation

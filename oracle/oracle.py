@@ -695,3 +695,52 @@ def lanczos_plain(apply_h, phi, steps):
             break
         vp, v = v, w / b
     return np.array(alpha), np.array(beta), nrm ** 2
+
+
+def cf_dynamics_table(m: Model, gs, e0, omega, eta, steps, kind="JeJe", apply_h=None):
+    """JsJs / JeJe table by continued fractions from J|gs> (north-star spectral
+    mode): the columns, channel set and elastic subtraction of DynJsJs / DynJeJe
+    (src/dynamics.h:244-328, 365-435), each I_ab(w) = <J_a gs|(w + e0 - i eta -
+    H)^-1|J_b gs>.  Diagonal channels are one plain Lanczos run each; a cross
+    channel (a, b) comes from the runs started at J_a + J_b and J_a + i J_b by
+    polarisation.  The JeJe yx column has no elastic term (Jez is never set in
+    the reference, src/dynamics.h:368-369,424).  Pinned by the exact Lehmann
+    sum from full diagonalisation (tests/test_oracle_pins.py)."""
+    if apply_h is None:
+        apply_h = lambda v: hpsi_gather(m, v)  # noqa: E731
+    names = ("Jex", "Jey") if kind == "JeJe" else ("Jsx", "Jsy", "Jsz")
+    J = [apply_current_total(m, n, gs) for n in names]
+    nc = len(J)
+    J.append(sum(J))
+    ex = [np.vdot(gs, x) for x in J]
+    z = np.asarray(omega, np.float64) + e0 - 1j * eta
+
+    def res(phi):
+        a, b, n2 = lanczos_plain(apply_h, phi, steps)
+        return cf_resolvent(a, b, n2, z)
+
+    R = {(a, a): res(J[a]) for a in range(nc + 1)}
+    if kind == "JeJe":
+        pairs = [(0, 0), (1, 1), (0, 1), (1, 0), (nc, nc)]
+    else:
+        pairs = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2), (nc, nc)]
+    for (a, b) in pairs:
+        if a == b or (a, b) in R:
+            continue
+        s1 = res(J[a] + J[b])
+        si = res(J[a] + 1j * J[b])
+        u = s1 - R[(a, a)] - R[(b, b)]
+        v = si - R[(a, a)] - R[(b, b)]
+        R[(a, b)] = 0.5 * (u - 1j * v)
+        R[(b, a)] = 0.5 * (u + 1j * v)
+    dn1 = np.asarray(omega, np.float64) - 1j * eta
+    cols = [np.asarray(omega, np.float64)]
+    for (a, b) in pairs:
+        val = R[(a, b)]
+        if not (kind == "JeJe" and (a, b) == (1, 0)):
+            val = val - np.conj(ex[a]) * ex[b] / dn1
+        if kind == "JeJe":
+            cols.append(val.imag)
+        else:
+            cols += [val.real, val.imag]
+    return np.column_stack(cols)

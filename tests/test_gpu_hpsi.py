@@ -126,6 +126,38 @@ def test_error_paths():
         eng.apply_op("Jex", 99, x)         # site out of range
 
 
+def test_vector_arguments_checked_before_launch():
+    """Every vector argument is checked for device, dtype, length and
+    contiguity before a kernel is launched: a complex64 tensor given to a
+    complex128 engine, a short slice, a strided view or host memory raises."""
+    m = O.Model(3, 4)
+    eng = engine_for(m)
+    x = eng.upload(np.ones(m.dim, complex))
+    out = eng.empty()
+    bad = {
+        "dtype": x.to(torch.complex64),
+        "short": x[: m.dim - 1],
+        "strided": torch.zeros(2 * m.dim, dtype=torch.complex128, device="cuda")[::2],
+        "host": torch.zeros(m.dim, dtype=torch.complex128),
+        "numpy": np.zeros(m.dim, complex),
+    }
+    for name, b in bad.items():
+        with pytest.raises((ValueError, TypeError)):
+            eng.hpsi(b, out=out)
+        with pytest.raises((ValueError, TypeError)):
+            eng.hpsi(x, out=b)
+        with pytest.raises((ValueError, TypeError)):
+            eng.apply_total("Jex", b, out=out)
+        with pytest.raises((ValueError, TypeError)):
+            eng.dot(x, b)
+        with pytest.raises((ValueError, TypeError)):
+            eng.lanczos(b, 5, mode="OnlyEgs")
+    with pytest.raises(ValueError):
+        eng.upload(np.zeros(m.dim - 1, complex))
+    # the good call still works after the rejected ones
+    assert np.isfinite(eng.download(eng.hpsi(x, out=out))).all()
+
+
 
 
 @pytest.mark.parametrize("kw", [dict(K=(1.0, 1.0, 1.0), B=(0.3, 0.3, 0.3)),          # popcount diagonal

@@ -233,6 +233,29 @@ def test_cf_equals_exact_lehmann_sum():
     assert np.abs(cf - exact).max() < 1e-8 * np.abs(exact).max()
 
 
+@pytest.mark.parametrize("kind", ["JeJe", "JsJs"])
+def test_cf_dynamics_table_equals_exact_lehmann_tables(kind):
+    """The oracle's continued-fraction tables (polarised cross channels,
+    elastic terms) equal the Lehmann tables of full diagonalisation, and a
+    planted sign error in the polarisation is caught."""
+    m = O.Model(2, 4, K=(-1.0, -1.0, -1.0), J=0.1, G=0.2, B=O.field111(0.2))
+    H = O.hamiltonian_dense_kron(m)
+    e, U = np.linalg.eigh(H)
+    gs = U[:, 0]
+    w = np.linspace(0, 6, 300)
+    exact = (O.jeje_ref if kind == "JeJe" else O.jsjs_ref)(m, e, U.T.copy(), gs, omega=w, eta=0.1)
+    got = O.cf_dynamics_table(m, gs, e[0], w, 0.1, 150, kind, apply_h=lambda v: H @ v)
+    scale = np.abs(exact[:, 1:]).max()
+    assert np.abs(got - exact).max() < 1e-8 * scale
+    # a transposed cross channel (xy <-> yx) differs at the percent level
+    bad = got.copy()
+    if kind == "JeJe":
+        bad[:, [3, 4]] = bad[:, [4, 3]]
+    else:
+        bad[:, 8] = -bad[:, 8]          # Im xy with the wrong sign of i in the polarisation
+    assert np.abs(bad - exact).max() > 1e-3 * scale
+
+
 def _ref_run(B, steps=100, K=1.0):
     m = O.Model(3, 4, K=(K, K, K), B=B)
     coo = O.CooHamiltonian(m)
