@@ -122,18 +122,23 @@ def test_c3_batch_with_20_and_24_site_instances():
     w = np.linspace(0.0, 6.0, 500)
     print("[C3 batch] JsJs CF 150", flush=True)
     tabs, _ = b.dynamics("JsJs", w, 0.05, cf_steps=150)
+    import torch
     for i, m in enumerate(ms):
         eng = esc.Engine(spin_model(m), "c128")
         _, e1, _, _ = eng.ground_state(eng.start_vector(11 + i, "Gaussian"), steps=200, restarts=1)
+        # the batch's JsJs table against the complex128 single-system
+        # continued fraction from the same state (converted to complex128)
+        st = b.state(i).to(torch.complex128)
+        ref, _ = eng.dynamics("JsJs", "CF", st, w, 0.05, e0=float(e0[i]), cf_steps=150)
+        wb, wr = spectral_weight(tabs[i]), spectral_weight(ref)
+        werr = np.abs(wb - wr).max() / np.abs(wr).max()
+        l1 = np.abs(tabs[i][:, 1:] - ref[:, 1:]).sum() / np.abs(ref[:, 1:]).sum()
+        print(f"[C3 batch] instance {i} N={m.N}: E0 {e0[i]:.10f} vs {e1:.10f}, JsJs weight err {werr:.2e}, "
+              f"L1 {l1:.2e}", flush=True)
         # complex64 vectors with FP64 scalars: E0 within 1e-6 relative
         assert abs(e0[i] - e1) < 1e-6 * abs(e1), (i, m.N, e0[i], e1)
-        # the batch's JsJs table equals a single-system continued fraction of
-        # the same complex64 state (same recursion, different kernels: the
-        # tiled H.psi sums each row in another order, ~1e-7 per step)
-        e64 = esc.Engine(spin_model(m), "c64")
-        st = b.state(i)
-        ref, _ = e64.dynamics("JsJs", "CF", st, w, 0.05, e0=float(e0[i]), cf_steps=150)
-        scale = np.abs(ref[:, 1:]).max()
-        err = np.abs(tabs[i][:, 1:] - ref[:, 1:]).max() / scale
-        print(f"[C3 batch] instance {i} N={m.N}: E0 {e0[i]:.10f} vs {e1:.10f}, JsJs max err {err:.2e}", flush=True)
-        assert err < 2e-2
+        # integrated spectral weight per column within 1e-4 of the largest;
+        # pointwise, 150-step plain recursions in complex64 lose orthogonality
+        # earlier than in complex128 (a few percent at eta = 0.05)
+        assert werr < 1e-4
+        assert l1 < 5e-2

@@ -38,7 +38,9 @@ def dense_eigh(m: O.Model):
     """(H, e, U) of the oracle's dense H, cached per model within the module."""
     key = (m.Lx, m.Ly, m.K, m.J, m.G, m.B)
     if key not in _EIGH:
-        H = O.CooHamiltonian(m).dense()
+        # the COO restatement covers the reference model; J and Gamma models
+        # use the Kronecker-product H (small N only)
+        H = O.CooHamiltonian(m).dense() if m.J == 0.0 and m.G == 0.0 else O.hamiltonian_dense_kron(m)
         e, U = np.linalg.eigh(H)
         _EIGH[key] = (H, e, U)
     return _EIGH[key]

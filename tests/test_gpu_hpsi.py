@@ -130,6 +130,7 @@ def test_vector_arguments_checked_before_launch():
     """Every vector argument is checked for device, dtype, length and
     contiguity before a kernel is launched: a complex64 tensor given to a
     complex128 engine, a short slice, a strided view or host memory raises."""
+    import torch
     m = O.Model(3, 4)
     eng = engine_for(m)
     x = eng.upload(np.ones(m.dim, complex))
