@@ -28,7 +28,7 @@ DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", f"-I{INC}", f"-I{CSRC}",
           "-Wno-unused-command-line-argument", "-pthread"]
 
-LIB_SOURCES = ["esc_kernels.hip", "esc_capi.cpp", "esc_host.cpp"]
+LIB_SOURCES = ["esc_kernels.hip", "esc_capi.cpp", "esc_host.cpp", "esc_cube.cpp"]
 CLI_SOURCES = ["facade/lanczoscond_main.cpp"]
 
 
@@ -46,7 +46,7 @@ def build_lib():
         _run([HIPCC, *COMMON, *DEVICE_FLAGS, *lang, "-c", path, "-o", obj])
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([HIPCC, *COMMON, "-shared", *objs, "-o", tmp, "-lrocsolver", "-lrocblas"])
+    _run([HIPCC, *COMMON, "-shared", *objs, "-o", tmp, "-lrocsolver", "-lrocblas", "-lhiprtc", "-ldl"])
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
@@ -68,12 +68,27 @@ def build_oracle():
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
 
 
+def warm_jit_cache():
+    """Compile the generated H.psi kernels of the benchmark lattices into
+    jit_cache/ beside libesc.so, so that contexts of these models load them
+    instead of compiling (the couplings are not part of the cache key)."""
+    sys.path.insert(0, ROOT)
+    import energyspincurrents_amd as esc
+    kjgh = dict(K=(-1.0, -1.0, -1.0), J=0.1, Gamma=0.2, B=esc.field111(0.2))
+    jobs = [((3, 8), kjgh, "c128"), ((3, 8), kjgh, "c64"), ((2, 10), kjgh, "c128"), ((2, 10), kjgh, "c64"),
+            ((4, 8), dict(K=(-1.0, -1.0, -1.0), B=esc.field111(0.3)), "c64")]
+    for (lx, ly), kw, prec in jobs:
+        esc.precompile_kernel(esc.SpinModel(esc.Honeycomb(lx, ly), **kw), prec)
+        print(f"+ generated H.psi kernels: {lx * ly} sites, {prec}", flush=True)
+
+
 def build(cli=True, oracle=True):
     build_lib()
     if cli:
         build_cli()
     if oracle:
         build_oracle()
+    warm_jit_cache()
 
 
 if __name__ == "__main__":

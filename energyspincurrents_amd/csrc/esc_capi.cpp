@@ -4,6 +4,7 @@
 // FP64 small solves.  All vector work runs in the HIP kernels of
 // esc_kernels.hip; nothing here falls back to a CPU path.
 #include "esc.h"
+#include "esc_cube.h"
 #include "esc_internal.h"
 
 #include <hip/hip_runtime.h>
@@ -245,6 +246,9 @@ struct esc_ctx {
     mutable std::map<int, bool> tile_fits;           // configuration fits the LDS
     int tile_bits = 0;                              // kernel configuration: 0 = one row per lane,
                                                     // else 100 * tile bits + window bits
+    double *d_cube_cf = nullptr;                    // coupling scalars of the generated kernels
+    std::map<int, std::shared_ptr<esc::CubeProgram>> cube;   // generated kernels per configuration
+    mutable std::map<int, std::string> cube_fail;   // configurations whose generation failed, and why
 
     bool c64() const { return precision == ESC_C64; }
 
@@ -256,6 +260,7 @@ struct esc_ctx {
         if (d_vecs) (void)hipFree(d_vecs);
         if (d_aux) (void)hipFree(d_aux);
         if (d_combo) (void)hipFree(d_combo);
+        if (d_cube_cf) (void)hipFree(d_cube_cf);
         for (auto &kv : d_pass)
             for (auto &p : kv.second) (void)hipFree(p.d);
     }
@@ -478,28 +483,36 @@ constexpr int kCoverLast = 6;    // cover-first, windows in reverse order
 // A kernel configuration is 100000 * P + 10000 * V + 100 * TB + W: P planned
 // passes (kCoverFirst: the cover-first planner), 2^V waves per workgroup,
 // 2^TB-row tiles, 2^W-row windows; 0 = one row per lane.
+// Configurations 7 and 8 in the pass digit run the cover-first plan (5) or
+// its reverse (6) with the generated sub-cube kernels of esc_cube.cpp; the
+// millions digit is then k (2^k rows per lane).
+constexpr int kCubeFirst = 7;
+constexpr int kCubeLast = 8;
 int cfg_passes(int config) { return (config / 100000) % 10; }
+int cfg_regs(int config) { return config / 1000000; }
+bool cfg_cube(int config) { return cfg_passes(config) == kCubeFirst || cfg_passes(config) == kCubeLast; }
 int cfg_waves(int config) { return 1 << ((config / 10000) % 10); }
 int cfg_tile_bits(int config) { return (config / 100) % 100; }
 int cfg_window(int config) { return config % 100; }
-int make_config(int P, int waves, int tb, int w) {
+int make_config(int P, int waves, int tb, int w, int k = 0) {
     int v = 0;
     while ((1 << v) < waves) ++v;
-    return 100000 * P + 10000 * v + 100 * tb + w;
+    return 1000000 * k + 100000 * P + 10000 * v + 100 * tb + w;
 }
 
-std::vector<esc::GTables> build_passes(const esc_ctx *c, int TB, int w, int P, int WPB) {
+// One planned pass: window sites W, tile sites T (lane sites 0..5 included)
+// and the terms (indices into host_terms) it applies from the tile (near) and
+// from the rest of the window (far).
+struct Plan {
+    uint32_t W, T;
+    std::vector<size_t> near, far;
+};
+
+std::vector<Plan> plan_windows(const esc_ctx *c, const std::vector<HostTerm> &terms, int TB, int w, int P) {
     const int N = c->n_sites;
     if (w > N) w = N;
     if (TB > w || TB < 7 || N - w > 16 || w - TB > 16) throw EscError(ESC_ERR_UNSUPPORTED, "tile/window shape");
-    if (TB - 6 > 7 || WPB < 1 || ((1 << (TB - 6)) % WPB) != 0) throw EscError(ESC_ERR_UNSUPPORTED, "tile/wave shape");
-    const esc::HTables &h = c->htab;
-    const std::vector<HostTerm> terms = host_terms(c);
     const uint32_t all = (N >= 32) ? 0xffffffffu : ((1u << N) - 1u);
-    struct Plan {
-        uint32_t W, T;
-        std::vector<size_t> near, far;
-    };
     std::vector<Plan> plan;
     std::vector<size_t> rem(terms.size());
     for (size_t t = 0; t < terms.size(); ++t) rem[t] = t;
@@ -624,20 +637,36 @@ std::vector<esc::GTables> build_passes(const esc_ctx *c, int TB, int w, int P, i
         plan.push_back(std::move(p));
         if (plan.size() > 8) throw EscError(ESC_ERR_UNSUPPORTED, "too many window passes");
     }
+    return plan;
+}
+
+// window bit order of a pass: lane sites, the other tile sites, then the rest
+// of the window (wb); the complement (window id bits) ascending (cb)
+void window_order(int N, const Plan &pl, std::vector<int> &wb, std::vector<int> &cb) {
+    wb.clear();
+    cb.clear();
+    for (int b = 0; b < 6; ++b) wb.push_back(b);
+    for (int b = 6; b < N; ++b)
+        if ((pl.T >> b) & 1u) wb.push_back(b);
+    for (int b = 6; b < N; ++b)
+        if (((pl.W & ~pl.T) >> b) & 1u) wb.push_back(b);
+    for (int b = 0; b < N; ++b)
+        if (!((pl.W >> b) & 1u)) cb.push_back(b);
+}
+
+std::vector<esc::GTables> build_passes(const esc_ctx *c, int TB, int w, int P, int WPB) {
+    const int N = c->n_sites;
+    if (w > N) w = N;
+    if (TB - 6 > 7 || WPB < 1 || ((1 << (TB - 6)) % WPB) != 0) throw EscError(ESC_ERR_UNSUPPORTED, "tile/wave shape");
+    const esc::HTables &h = c->htab;
+    const std::vector<HostTerm> terms = host_terms(c);
+    const std::vector<Plan> plan = plan_windows(c, terms, TB, w, P);
 
     std::vector<esc::GTables> passes;
     bool first = true;
     for (const Plan &pl : plan) {
-        // window order: lane sites, the other tile sites, then the rest of
-        // the window; the complement (window id) ascending
         std::vector<int> wb, cb;
-        for (int b = 0; b < 6; ++b) wb.push_back(b);
-        for (int b = 6; b < N; ++b)
-            if ((pl.T >> b) & 1u) wb.push_back(b);
-        for (int b = 6; b < N; ++b)
-            if (((pl.W & ~pl.T) >> b) & 1u) wb.push_back(b);
-        for (int b = 0; b < N; ++b)
-            if (!((pl.W >> b) & 1u)) cb.push_back(b);
+        window_order(N, pl, wb, cb);
         const int ww = (int)wb.size();
         std::vector<size_t> mine(pl.near);
         mine.insert(mine.end(), pl.far.begin(), pl.far.end());
@@ -753,6 +782,99 @@ std::vector<esc::GTables> build_passes(const esc_ctx *c, int TB, int w, int P, i
     }
     passes.back().last_pass = 1u;
     return passes;
+}
+
+// The generated sub-cube kernels (esc_cube.cpp) for configuration cfg: the
+// cover-first plan of the generic tiled kernel, its terms in bond / flip form.
+// Returns false (with the reason) for models they do not cover: a pair in
+// several bond families (the chain's XYZ bonds) or a Kitaev-z diagonal whose
+// partial sums are not exact (the closed form would not equal the reference
+// order).
+bool cube_spec(const esc_ctx *c, int cfg, esc::CubeModel &m, std::vector<esc::CubePass> &passes, std::string *why) {
+    const esc::HTables &h = c->htab;
+    const int N = c->n_sites;
+    const std::vector<HostTerm> terms = host_terms(c);
+    std::vector<esc::CubeTerm> ct;
+    for (uint32_t b = 0; b < h.n_bonds; ++b) {
+        const uint32_t p = h.bond[b];
+        esc::CubeTerm t{};
+        t.kind = 0;
+        t.i = (int)(p & 0xff);
+        t.j = (int)((p >> 8) & 0xff);
+        t.f = (int)(p >> 16);
+        t.mask = (1u << t.i) | (1u << t.j);
+        t.k = t.xp = t.yp = -1;
+        ct.push_back(t);
+    }
+    for (uint32_t k = 0; k < h.n_flip; ++k) {
+        esc::CubeTerm t{};
+        t.kind = 1;
+        t.k = (int)k;
+        t.xp = h.has_gamma_flip ? h.xpart[k] : -1;
+        t.yp = h.has_gamma_flip ? h.ypart[k] : -1;
+        t.mask = 1u << k;
+        t.f = t.i = t.j = -1;
+        ct.push_back(t);
+    }
+    if (ct.size() != terms.size()) {
+        if (why) *why = "a pair sits in several bond families";
+        return false;
+    }
+    const int TB = cfg_tile_bits(cfg), w = cfg_window(cfg), P = cfg_passes(cfg) - 2, WPB = cfg_waves(cfg);
+    const std::vector<Plan> plan = plan_windows(c, terms, TB, w, P);
+    const std::vector<esc::GTables> g = build_passes(c, TB, w, P, 4);
+    if (!g[0].diag_closed) {
+        if (why) *why = "the Kitaev-z diagonal needs the bond-by-bond sum";
+        return false;
+    }
+    std::memset(&m, 0, sizeof(m));
+    m.n_sites = N;
+    m.c64 = c->c64();
+    for (int f = 0; f < 3; ++f) m.bond_cplx[f] = h.q[f] != 0.0;
+    m.flip_cplx = h.by2 != 0.0 || h.g4 != 0.0;
+    m.flip_partners = h.has_gamma_flip != 0;
+    m.nz = g[0].nz;
+    m.nj = g[0].nj;
+    m.n_zgrp = g[0].n_zgrp;
+    m.n_jgrp = g[0].n_jgrp;
+    for (int q = 0; q < 32; ++q) {
+        m.zgrp_d[q] = g[0].zgrp_d[q];
+        m.zgrp_m[q] = g[0].zgrp_m[q];
+        m.jgrp_d[q] = g[0].jgrp_d[q];
+        m.jgrp_m[q] = g[0].jgrp_m[q];
+    }
+    m.reg_sites = cfg_regs(cfg);
+    m.waves = WPB;
+    passes.clear();
+    const std::vector<Plan> &pl = plan;
+    for (size_t i = 0; i < pl.size(); ++i) {
+        esc::CubePass cp;
+        window_order(N, pl[i], cp.wb, cp.cb);
+        cp.tile_bits = TB;
+        for (size_t t : pl[i].near) cp.terms.push_back(ct[t]);
+        for (size_t t : pl[i].far) cp.terms.push_back(ct[t]);
+        cp.first = i == 0;
+        cp.last = i + 1 == pl.size();
+        cp.diag = cp.first;
+        passes.push_back(std::move(cp));
+    }
+    return true;
+}
+
+// coupling scalars of the generated kernels (esc::CubeScalar order)
+void cube_scalars(const esc::HTables &h, double *cf) {
+    for (int f = 0; f < 3; ++f) {
+        cf[3 * f] = h.p0[f] + h.p1[f];
+        cf[3 * f + 1] = h.p0[f] + h.p1[f] * -1.0;
+        cf[3 * f + 2] = h.q[f] * 2.0;
+    }
+    cf[esc::kCubeRp] = h.bx2 + h.g4;
+    cf[esc::kCubeRm] = h.bx2 + h.g4 * -1.0;
+    cf[esc::kCubeIp] = h.by2 + h.g4;
+    cf[esc::kCubeIm] = h.by2 + h.g4 * -1.0;
+    cf[esc::kCubeKz4] = h.kz4;
+    cf[esc::kCubeJ4] = h.j4;
+    cf[esc::kCubeBz] = h.bz;
 }
 
 // Pauli-string table for one operator (src/Observables.h:161-389,429-521)
@@ -912,10 +1034,51 @@ const std::vector<esc_ctx::PassDev> &passes_for(esc_ctx *c, int config) {
     return c->d_pass[config] = dev;
 }
 
+// the generated kernels of a cube configuration, compiled (or taken from the
+// caches) on first use; throws with the reason when they cannot be built
+esc::CubeProgram &cube_for(esc_ctx *c, int config) {
+    auto it = c->cube.find(config);
+    if (it != c->cube.end()) return *it->second;
+    auto f = c->cube_fail.find(config);
+    if (f != c->cube_fail.end()) throw EscError(ESC_ERR_UNSUPPORTED, f->second);
+    esc::CubeModel m;
+    std::vector<esc::CubePass> passes;
+    std::string why;
+    std::shared_ptr<esc::CubeProgram> prog;
+    if (cube_spec(c, config, m, passes, &why)) prog = esc::cube_program(m, passes, &why);
+    if (!prog) {
+        c->cube_fail[config] = "generated H.psi kernels unavailable: " + why;
+        throw EscError(ESC_ERR_UNSUPPORTED, c->cube_fail[config]);
+    }
+    if (!c->d_cube_cf) {
+        double cf[esc::kCubeScalars];
+        cube_scalars(c->htab, cf);
+        if (hipMalloc(&c->d_cube_cf, sizeof(cf)) != hipSuccess) throw EscError(ESC_ERR_ALLOC, "table allocation failed");
+        hip_check(hipMemcpy(c->d_cube_cf, cf, sizeof(cf), hipMemcpyHostToDevice), "hipMemcpy");
+    }
+    c->cube[config] = prog;
+    return *prog;
+}
+
 // whether every pass of a configuration fits the LDS of one CU (cached)
 bool tile_config_fits(const esc_ctx *c, int config) {
     auto it = c->tile_fits.find(config);
     if (it != c->tile_fits.end()) return it->second;
+    if (cfg_cube(config)) {
+        esc::CubeModel m;
+        std::vector<esc::CubePass> passes;
+        bool ok = false;
+        try {
+            ok = cube_spec(c, config, m, passes, nullptr);
+        } catch (const EscError &) {
+            ok = false;
+        }
+        const int TB = cfg_tile_bits(config), k = cfg_regs(config), waves = cfg_waves(config);
+        ok = ok && k >= 1 && TB - 6 - k >= 0 && ((1 << (TB - 6 - k)) % waves) == 0 &&
+             ((size_t)1 << TB) * c->esz <= 131072;
+        c->tile_fits[config] = ok;
+        return ok;
+    }
     bool ok = true;
     try {
         for (const auto &g : build_passes(c, cfg_tile_bits(config), cfg_window(config), cfg_passes(config),
@@ -928,6 +1091,29 @@ bool tile_config_fits(const esc_ctx *c, int config) {
     }
     c->tile_fits[config] = ok;
     return ok;
+}
+
+// generated sub-cube configurations: 2^12-row tiles (2^13 for complex64),
+// 2^3 or 2^4 rows per lane, 4 or 8 waves, the cover-first windows of the
+// generic kernel in both orders.  ESC_NO_JIT=1 leaves them out.
+std::vector<int> cube_candidates(const esc_ctx *c) {
+    std::vector<int> v;
+    const char *nj = std::getenv("ESC_NO_JIT");
+    if (nj && nj[0] && std::strcmp(nj, "0") != 0) return v;
+    const int N = c->n_sites;
+    if (N < 13) return v;
+    const int w = N <= 18 ? N : (N <= 22 ? 17 : 18);
+    for (int tb : {12, 13}) {
+        if (tb > w) continue;
+        for (int k : {3, 4})
+            for (int waves : {4, 8})
+                for (int P : {kCubeFirst, kCubeLast}) {
+                    if (w == N && P == kCubeLast) continue;
+                    const int cfg = make_config(P, waves, tb, w, k);
+                    if (tile_config_fits(c, cfg)) v.push_back(cfg);
+                }
+    }
+    return v;
 }
 
 // kernel configurations (make_config; window >= N: one pass), 0 = one row per
@@ -950,6 +1136,8 @@ std::vector<int> tile_candidates(const esc_ctx *c) {
             }
         }
     }
+    for (int cfg : cube_candidates(c))
+        if (std::find(v.begin(), v.end(), cfg) == v.end()) v.push_back(cfg);
     return v;
 }
 
@@ -962,24 +1150,46 @@ std::vector<int> tile_candidates(const esc_ctx *c) {
 // with 2^12-row tiles (16 waves for c128, which moves the least HBM traffic of
 // the equally fast shapes at N = 24, 8 waves for c64).  A shape the context
 // cannot run falls back to a single pass over 2^11- or 2^10-row tiles.
-int default_tile_bits(const esc_ctx *c) {
+// From 19 sites the generated sub-cube kernels come first (N = 24: 0.36 ms
+// against 0.52 ms for the best generic shape in complex128, 0.21 against
+// 0.32 ms in complex64): complex128 2^3 rows per lane in 8-wave workgroups
+// on 2^12-row tiles, complex64 2^4 rows per lane on 2^13-row tiles; a model
+// they do not cover, or a missing run-time compiler, falls through to the
+// generic shapes.
+int default_tile_bits(esc_ctx *c) {
     const int N = c->n_sites;
     const bool c64 = c->c64();
     if (N <= 16 || (N <= 18 && c64)) return 0;
     std::vector<int> pref;
+    if (N >= 19) {
+        const int w = N <= 22 ? 17 : 18;
+        pref.push_back(c64 ? make_config(kCubeFirst, 8, 13, w, 4) : make_config(kCubeFirst, 8, 12, w, 3));
+    }
     if (N <= 18) pref.push_back(make_config(1, 8, 10, N));
     else if (N <= 22) pref.push_back(c64 ? make_config(kCoverLast, 16, 12, 17) : make_config(kCoverFirst, 16, 11, 17));
     else pref.push_back(c64 ? make_config(kCoverFirst, 8, 12, 18) : make_config(kCoverFirst, 16, 12, 18));
     pref.push_back(make_config(1, 8, 11, N));
     pref.push_back(make_config(1, 4, 10, N));
     const std::vector<int> cand = tile_candidates(c);
-    for (int cfg : pref)
-        if (std::find(cand.begin(), cand.end(), cfg) != cand.end()) return cfg;
+    for (int cfg : pref) {
+        if (std::find(cand.begin(), cand.end(), cfg) == cand.end()) continue;
+        if (cfg_cube(cfg)) {
+            try {
+                cube_for(c, cfg);
+            } catch (const EscError &) {
+                continue;
+            }
+        }
+        return cfg;
+    }
     return 0;
 }
 
 void launch_hpsi_with(esc_ctx *c, int config, const void *in, void *out, hipStream_t s, double *dot_part = nullptr) {
-    if (config > 0) {
+    if (config > 0 && cfg_cube(config)) {
+        esc::CubeProgram &p = cube_for(c, config);
+        hip_check(esc::cube_launch(p, c->d_cube_cf, in, out, dot_part, c->dim, s), "generated hpsi launch");
+    } else if (config > 0) {
         const int waves = cfg_waves(config), tb = cfg_tile_bits(config);
         for (const auto &p : passes_for(c, config))
             hip_check(esc::launch_hpsi_tile(c->c64(), p.d, c->d_htab, tb, waves, p.n_uniform, p.n_terms, in, out,
@@ -1707,17 +1917,15 @@ extern "C" {
 const char *esc_last_error(void) { return g_err.c_str(); }
 int esc_version(void) { return 100; }
 
-int esc_ctx_create(const esc_model *model, int precision, esc_ctx **out) {
-    return guarded([&] {
+namespace {
+// validated copy of the model and its host tables; no device work
+void ctx_init_host(esc_ctx *c, const esc_model *model, int precision) {
         check_ptr(model, "model");
-        check_ptr(out, "out");
-        *out = nullptr;
         const int N = model->n_sites;
         if (N < 2 || N > esc::kMaxSites) throw EscError(ESC_ERR_ARG, "n_sites must be in [2, 32]");
         if (precision != ESC_C128 && precision != ESC_C64) throw EscError(ESC_ERR_ARG, "precision must be ESC_C128 or ESC_C64");
         for (double v : {model->K[0], model->K[1], model->K[2], model->J, model->Gamma, model->B[0], model->B[1], model->B[2]})
             if (!std::isfinite(v)) throw EscError(ESC_ERR_NUMERIC, "non-finite coupling");
-        auto c = std::make_unique<esc_ctx>();
         c->n_sites = N;
         c->precision = precision;
         c->dim = (uint64_t)1 << N;
@@ -1736,12 +1944,44 @@ int esc_ctx_create(const esc_model *model, int precision, esc_ctx **out) {
         }
         if (model->neigh) c->neigh.assign(model->neigh, model->neigh + 3 * N);
         c->model.neigh = nullptr;
+        build_tables(c);
+}
+}  // namespace
+
+int esc_ctx_create(const esc_model *model, int precision, esc_ctx **out) {
+    return guarded([&] {
+        check_ptr(out, "out");
+        *out = nullptr;
+        auto c = std::make_unique<esc_ctx>();
+        ctx_init_host(c.get(), model, precision);
         hip_check(hipGetDevice(&c->device), "hipGetDevice");
-        build_tables(c.get());
         if (hipMalloc(&c->d_htab, sizeof(esc::HTables)) != hipSuccess) throw EscError(ESC_ERR_ALLOC, "table allocation failed");
         hip_check(hipMemcpy(c->d_htab, &c->htab, sizeof(esc::HTables), hipMemcpyHostToDevice), "hipMemcpy");
         c->tile_bits = default_tile_bits(c.get());
         *out = c.release();
+    });
+}
+
+int esc_kernel_precompile(const esc_model *model, int precision, int config, char *source, size_t cap,
+                          size_t *len) {
+    return guarded([&] {
+        esc_ctx c;
+        ctx_init_host(&c, model, precision);
+        if (!cfg_cube(config)) throw EscError(ESC_ERR_ARG, "not a generated-kernel configuration");
+        esc::CubeModel m;
+        std::vector<esc::CubePass> passes;
+        std::string why;
+        if (!tile_config_fits(&c, config) || !cube_spec(&c, config, m, passes, &why))
+            throw EscError(ESC_ERR_UNSUPPORTED, "configuration does not apply to this model" +
+                                                    (why.empty() ? std::string() : ": " + why));
+        const std::string src = esc::cube_source(m, passes);
+        if (len) *len = src.size();
+        if (source && cap) {
+            const size_t n = std::min(cap - 1, src.size());
+            std::memcpy(source, src.data(), n);
+            source[n] = '\0';
+        }
+        if (!esc::cube_precompile(m, passes, &why)) throw EscError(ESC_ERR_UNSUPPORTED, why);
     });
 }
 

@@ -1,0 +1,644 @@
+// esc_cube.cpp -- run-time specialised H.psi passes (see esc_cube.h).
+//
+// Source generation, compilation with hiprtc for gfx950 (packed FP32 off, as
+// for every other kernel of the library), a process cache and a disk cache of
+// code objects, module loading and launches.
+#include "esc_cube.h"
+
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <dlfcn.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <utility>
+
+namespace esc {
+
+namespace {
+
+constexpr uint32_t kCubeMaxGrid = 1u << 22;
+constexpr int kFarAhead = 2;   // far sub-cubes in flight ahead of their terms
+
+// ---------------------------------------------------------------------------
+// Layout of one pass
+// ---------------------------------------------------------------------------
+// Row bits: lane sites 0..5; tile-index bit 6 + q holds tile site wb[6 + q];
+// the register sites are k of those positions (slot bit b of a lane's
+// sub-cube = position regpos[b]), the others are group positions (bit q of
+// the group index = position grppos[q]); far sites wb[TB..w-1] and window-id
+// sites cb are fixed per workgroup.
+enum Role { kLane, kReg, kGrp, kBase };
+
+struct Layout {
+    int N = 0, TB = 0, w = 0, k = 0;
+    std::vector<int> wb, cb, regpos, grppos;
+    int role[64];
+    int idx[64];     // slot bit (kReg) or group bit (kGrp) of a site
+
+    void finish() {
+        for (int s = 0; s < 64; ++s) { role[s] = kBase; idx[s] = -1; }
+        for (int s = 0; s < 6; ++s) role[s] = kLane;
+        for (int b = 0; b < (int)regpos.size(); ++b) { role[wb[regpos[b]]] = kReg; idx[wb[regpos[b]]] = b; }
+        for (int q = 0; q < (int)grppos.size(); ++q) { role[wb[grppos[q]]] = kGrp; idx[wb[grppos[q]]] = q; }
+    }
+    // slot x -> tile-index offset and row bits
+    uint32_t slot_off(uint32_t x) const {
+        uint32_t o = 0;
+        for (int b = 0; b < k; ++b)
+            if ((x >> b) & 1u) o |= 1u << regpos[b];
+        return o;
+    }
+    uint32_t slot_row(uint32_t x) const {
+        uint32_t o = 0;
+        for (int b = 0; b < k; ++b)
+            if ((x >> b) & 1u) o |= 1u << wb[regpos[b]];
+        return o;
+    }
+};
+
+// The part of a mask that selects the partner sub-cube: lane bits, group
+// index bits and far-site row bits.  rho = the register part (slot bits).
+struct Outer {
+    uint32_t lane = 0, grp = 0, far = 0;
+    bool operator<(const Outer &o) const {
+        return std::tie(lane, grp, far) < std::tie(o.lane, o.grp, o.far);
+    }
+    bool operator==(const Outer &o) const { return lane == o.lane && grp == o.grp && far == o.far; }
+    bool own() const { return lane == 0 && grp == 0 && far == 0; }
+};
+
+void split_mask(const Layout &L, uint32_t mask, Outer &o, uint32_t &rho) {
+    o = Outer();
+    rho = 0;
+    for (int s = 0; s < L.N; ++s) {
+        if (!((mask >> s) & 1u)) continue;
+        switch (L.role[s]) {
+        case kLane: o.lane |= 1u << s; break;
+        case kReg: rho |= 1u << L.idx[s]; break;
+        case kGrp: o.grp |= 1u << L.idx[s]; break;
+        default: o.far |= 1u << s; break;
+        }
+    }
+}
+
+// register sites of a pass: the k tile positions (outside the lane sites)
+// that leave the fewest distinct outer parts, then the most terms whose mask
+// lies inside them, then the lowest positions
+void choose_registers(Layout &L, const std::vector<CubeTerm> &terms) {
+    const int npos = L.TB - 6;
+    long best = -1;
+    uint32_t best_sel = 0;
+    for (uint32_t sel = 0; sel < (1u << npos); ++sel) {
+        if (__builtin_popcount(sel) != L.k) continue;
+        Layout t = L;
+        t.regpos.clear();
+        t.grppos.clear();
+        for (int q = 0; q < npos; ++q) ((sel >> q) & 1u ? t.regpos : t.grppos).push_back(6 + q);
+        t.finish();
+        std::vector<Outer> parts;
+        long inside = 0, cost = 0;
+        for (const CubeTerm &ct : terms) {
+            Outer o;
+            uint32_t rho;
+            split_mask(t, ct.mask, o, rho);
+            if (o.own()) ++inside;
+            if (std::find(parts.begin(), parts.end(), o) == parts.end()) {
+                parts.push_back(o);
+                cost += o.far ? 3 : 1;   // a far sub-cube goes through L2 and the texture path
+            }
+        }
+        const long key = (100000 - cost) * 1000 + inside;
+        if (key > best) { best = key; best_sel = sel; }
+    }
+    L.regpos.clear();
+    L.grppos.clear();
+    for (int q = 0; q < npos; ++q) ((best_sel >> q) & 1u ? L.regpos : L.grppos).push_back(6 + q);
+    L.finish();
+}
+
+// ---------------------------------------------------------------------------
+// Boolean expressions of row bits
+// ---------------------------------------------------------------------------
+struct BE {
+    bool is_const = false;
+    int c = 0;
+    std::string s;
+};
+
+BE be_const(int v) { BE b; b.is_const = true; b.c = v ? 1 : 0; b.s = v ? "true" : "false"; return b; }
+BE be_var(const std::string &s) { BE b; b.s = s; return b; }
+
+// bit of `site` in the row of slot x
+BE site_bit(const Layout &L, int site, uint32_t x) {
+    switch (L.role[site]) {
+    case kLane: return be_var("L" + std::to_string(site));
+    case kReg: return be_const((int)((x >> L.idx[site]) & 1u));
+    case kGrp: return be_var("G" + std::to_string(L.idx[site]));
+    default: return be_var("U" + std::to_string(site));
+    }
+}
+
+BE be_eq(const BE &a, const BE &b) {
+    if (a.is_const && b.is_const) return be_const(a.c == b.c);
+    if (a.is_const) return a.c ? b : be_var("(!" + b.s + ")");
+    if (b.is_const) return b.c ? a : be_var("(!" + a.s + ")");
+    return be_var("(" + a.s + " == " + b.s + ")");
+}
+
+// a value: either the literal zero or an expression
+struct Val {
+    bool zero = false;
+    std::string s;
+};
+Val val(const std::string &s) { Val v; v.s = s; return v; }
+Val val_zero() { Val v; v.zero = true; v.s = "0"; return v; }
+
+Val sel(const BE &c, const Val &a, const Val &b) {
+    if (c.is_const) return c.c ? a : b;
+    if (a.zero && b.zero) return val_zero();
+    return val("(" + c.s + " ? " + a.s + " : " + b.s + ")");
+}
+Val neg(const Val &a) { return a.zero ? a : val("(-" + a.s + ")"); }
+
+// coefficient of a term at slot x (real, imaginary)
+void coefficient(const CubeModel &m, const Layout &L, const CubeTerm &t, uint32_t x, Val &cr, Val &ci) {
+    if (t.kind == 0) {
+        const BE bi = site_bit(L, t.i, x), bj = site_bit(L, t.j, x);
+        const BE eq = be_eq(bi, bj);
+        const std::string f = std::to_string(t.f);
+        cr = sel(eq, val("A" + f), val("B" + f));
+        if (m.bond_cplx[t.f]) {
+            const Val q = val("Q" + f);
+            ci = sel(eq, sel(bi, neg(q), q), val_zero());
+        } else {
+            ci = val_zero();
+        }
+    } else {
+        cr = (m.flip_partners && t.yp >= 0) ? sel(site_bit(L, t.yp, x), val("Rm"), val("Rp")) : val("Rp");
+        if (m.flip_cplx) {
+            const Val mag = (m.flip_partners && t.xp >= 0) ? sel(site_bit(L, t.xp, x), val("Im"), val("Ip")) : val("Ip");
+            ci = sel(site_bit(L, t.k, x), mag, neg(mag));
+        } else {
+            ci = val_zero();
+        }
+    }
+}
+
+std::string hex32(uint32_t v) {
+    char b[16];
+    std::snprintf(b, sizeof(b), "0x%xu", v);
+    return b;
+}
+
+// scatter of the low bits of `var` onto the given row bits
+std::string scatter(const std::string &var, const std::vector<int> &bits, int from, int count) {
+    std::ostringstream o;
+    o << "0u";
+    for (int q = 0; q < count; ++q) o << " | (((" << var << " >> " << q << ") & 1u) << " << bits[from + q] << ")";
+    return o.str();
+}
+
+void emit_pass(std::ostringstream &o, const CubeModel &m, const CubePass &p, int index) {
+    Layout L;
+    L.N = m.n_sites;
+    L.TB = p.tile_bits;
+    L.w = (int)p.wb.size();
+    L.k = m.reg_sites;
+    L.wb = p.wb;
+    L.cb = p.cb;
+    choose_registers(L, p.terms);
+    const int TB = L.TB, k = L.k, S = 1 << k, WPB = m.waves;
+    const int ngb = TB - 6 - k;                 // group index bits
+    const int ngrp_stage = 1 << (TB - 6);       // 64-row groups of the tile
+    const int nsub = 1 << ngb;                  // sub-cubes per tile
+    const int NG = nsub / WPB;                  // sub-cubes per wave
+    const int nwin = 1 << (int)L.cb.size();
+    const int bpw = 1 << (L.w - TB);
+    const bool xcdw = nwin % 8 == 0;
+
+    // group the terms by outer part: the own sub-cube first
+    std::map<Outer, std::vector<std::pair<CubeTerm, uint32_t>>> groups;
+    for (const CubeTerm &t : p.terms) {
+        Outer ou;
+        uint32_t rho;
+        split_mask(L, t.mask, ou, rho);
+        groups[ou].push_back({t, rho});
+    }
+
+    o << "// pass " << index << ": window";
+    for (int s : L.wb) o << " " << s;
+    o << "; register sites";
+    for (int b = 0; b < k; ++b) o << " " << L.wb[L.regpos[b]];
+    int nfar = 0, nown = 0;
+    for (const auto &g : groups) {
+        if (g.first.far) ++nfar;
+        if (g.first.own()) nown = (int)g.second.size();
+    }
+    o << "; " << p.terms.size() << " terms (" << nown << " in registers) in " << groups.size()
+      << " partner sub-cubes (" << nfar << " far)\n";
+    for (const auto &g : groups) {
+        o << "//   sub-cube lane " << hex32(g.first.lane) << " group " << hex32(g.first.grp) << " far "
+          << hex32(g.first.far) << ":";
+        for (const auto &pr : g.second)
+            o << (pr.first.kind == 0 ? " b" + std::to_string(pr.first.i) + "-" + std::to_string(pr.first.j)
+                                     : " f" + std::to_string(pr.first.k));
+        o << "\n";
+    }
+    o << "extern \"C\" __global__ void __launch_bounds__(" << WPB * 64 << ") esc_cube_p" << index
+      << "(const double *__restrict__ cf, const V2 *__restrict__ in, V2 *__restrict__ out, "
+         "double *__restrict__ dot_part, unsigned block_base) {\n";
+    o << "  __shared__ V2 tile[" << (1 << TB) << "];\n";
+    o << "  __shared__ double red[2][" << WPB << "];\n";
+    o << "  const unsigned lane = threadIdx.x & 63u;\n";
+    o << "  const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);\n";
+    o << "  const unsigned b = blockIdx.x + block_base;\n";
+    if (xcdw)
+        o << "  const unsigned seq = b / 8u, wi = (seq / " << bpw << "u) * 8u + b % 8u, tl = seq % " << bpw << "u;\n";
+    else
+        o << "  const unsigned wi = b / " << bpw << "u, tl = b % " << bpw << "u;\n";
+    o << "  const unsigned base = (" << scatter("wi", L.cb, 0, (int)L.cb.size()) << ") | ("
+      << scatter("tl", L.wb, TB, L.w - TB) << ");\n";
+    // staging: 64-row groups g = wave + WPB * i
+    o << "#pragma unroll\n";
+    o << "  for (unsigned i = 0; i < " << ngrp_stage / WPB << "u; ++i) {\n";
+    o << "    const unsigned g = wave + " << WPB << "u * i;\n";
+    o << "    tile[(g << 6) | lane] = in[base | (" << scatter("g", L.wb, 6, TB - 6) << ") | lane];\n";
+    o << "  }\n";
+    o << "  __syncthreads();\n";
+    o << "  const R A0 = (R)cf[0], B0 = (R)cf[1], Q0 = (R)cf[2], A1 = (R)cf[3], B1 = (R)cf[4], Q1 = (R)cf[5];\n";
+    o << "  const R A2 = (R)cf[6], B2 = (R)cf[7], Q2 = (R)cf[8], Rp = (R)cf[9], Rm = (R)cf[10], Ip = (R)cf[11], "
+         "Im = (R)cf[12];\n";
+    o << "  (void)A0; (void)B0; (void)Q0; (void)A1; (void)B1; (void)Q1; (void)A2; (void)B2; (void)Q2;\n";
+    o << "  (void)Rp; (void)Rm; (void)Ip; (void)Im;\n";
+    for (int s = 0; s < 6; ++s) o << "  const bool L" << s << " = (lane >> " << s << ") & 1u; (void)L" << s << ";\n";
+    for (int s = 0; s < L.N; ++s)
+        if (L.role[s] == kBase)
+            o << "  const bool U" << s << " = (base >> " << s << ") & 1u; (void)U" << s << ";\n";
+    o << "  double dr = 0.0, di = 0.0;\n";
+    o << "#pragma unroll 1\n";
+    o << "  for (unsigned it = 0; it < " << NG << "u; ++it) {\n";
+    o << "    const unsigned gi = wave + " << WPB << "u * it;\n";
+    {
+        // group index -> tile-index offset and row bits
+        std::vector<int> gp(L.grppos), gr;
+        for (int q : L.grppos) gr.push_back(L.wb[q]);
+        o << "    const unsigned gix = " << scatter("gi", gp, 0, ngb) << ";\n";
+        o << "    const unsigned grw = " << scatter("gi", gr, 0, ngb) << ";\n";
+        for (int q = 0; q < ngb; ++q) o << "    const bool G" << q << " = (gi >> " << q << ") & 1u; (void)G" << q << ";\n";
+    }
+    o << "    R ar[" << S << "], ai[" << S << "];\n";
+    o << "    V2 p[" << S << "];\n";
+    o << "    {\n      const unsigned va = lane | gix;\n";
+    for (int x = 0; x < S; ++x) o << "      p[" << x << "] = tile[va + " << hex32(L.slot_off(x)) << "];\n";
+    o << "    }\n";
+    // accumulator: the diagonal (first pass), else zero; a later pass reads
+    // the previous passes' output here and adds it after its own terms, so
+    // the loads are in flight during the term work
+    for (int x = 0; x < S; ++x) {
+        if (p.diag && p.first)
+            o << "    { const R d = (R)esc_diag(base | grw | lane | " << hex32(L.slot_row(x))
+              << ", cf[13], cf[14], cf[15]); ar[" << x << "] = d * p[" << x << "].x; ai[" << x << "] = d * p[" << x
+              << "].y; }\n";
+        else
+            o << "    ar[" << x << "] = 0; ai[" << x << "] = 0;\n";
+    }
+    if (!p.first) {
+        o << "    NV2 prev[" << S << "];\n";
+        for (int x = 0; x < S; ++x)
+            o << "    prev[" << x << "] = __builtin_nontemporal_load(reinterpret_cast<const NV2 *>(out + (base | grw | "
+              << hex32(L.slot_row(x)) << ") + lane));\n";
+    }
+    auto emit_terms = [&](const std::vector<std::pair<CubeTerm, uint32_t>> &ts, const std::string &src) {
+        for (const auto &pr : ts) {
+            const CubeTerm &t = pr.first;
+            const uint32_t rho = pr.second;
+            o << "    // " << (t.kind == 0 ? "bond " + std::to_string(t.i) + "-" + std::to_string(t.j) + " family " +
+                                                 std::to_string(t.f)
+                                           : "flip " + std::to_string(t.k))
+              << "\n";
+            for (int x = 0; x < S; ++x) {
+                Val cr, ci;
+                coefficient(m, L, t, (uint32_t)x, cr, ci);
+                const std::string q = src + "[" + std::to_string(x ^ (int)rho) + "]";
+                const std::string X = std::to_string(x);
+                o << "    ar[" << X << "] = fma(" << cr.s << ", " << q << ".x, ar[" << X << "]);";
+                if (!ci.zero) o << " ar[" << X << "] = fma(-" << ci.s << ", " << q << ".y, ar[" << X << "]);";
+                o << " ai[" << X << "] = fma(" << cr.s << ", " << q << ".y, ai[" << X << "]);";
+                if (!ci.zero) o << " ai[" << X << "] = fma(" << ci.s << ", " << q << ".x, ai[" << X << "]);";
+                o << "\n";
+            }
+        }
+    };
+    // partner sub-cube loads: near ones from the tile, far ones from the
+    // window (L2), each into its own registers
+    auto emit_load = [&](const Outer &ou, const std::string &name) {
+        o << "    V2 " << name << "[" << S << "];\n";
+        uint32_t gx = 0, grr = 0;
+        for (int qq = 0; qq < ngb; ++qq)
+            if ((ou.grp >> qq) & 1u) { gx |= 1u << L.grppos[qq]; grr |= 1u << L.wb[L.grppos[qq]]; }
+        if (ou.far == 0) {
+            o << "    { const unsigned va = (lane ^ " << hex32(ou.lane) << ") | (gix ^ " << hex32(gx) << ");\n";
+            for (int y = 0; y < S; ++y) o << "      " << name << "[" << y << "] = tile[va + " << hex32(L.slot_off(y)) << "];\n";
+            o << "    }\n";
+        } else {
+            o << "    { const unsigned lo = lane ^ " << hex32(ou.lane) << ";\n";
+            o << "      const V2 *src = in + ((base ^ " << hex32(ou.far) << ") | (grw ^ " << hex32(grr) << "));\n";
+            for (int y = 0; y < S; ++y)
+                o << "      " << name << "[" << y << "] = (src + " << hex32(L.slot_row(y)) << ")[lo];\n";
+            o << "    }\n";
+        }
+    };
+    // order: far sub-cubes are issued kFarAhead near sub-cubes before their
+    // terms run, so their L2 latency overlaps the LDS work
+    std::vector<const std::pair<const Outer, std::vector<std::pair<CubeTerm, uint32_t>>> *> nearg, farg;
+    for (const auto &g : groups) (g.first.far ? farg : nearg).push_back(&g);
+    const int ahead = std::min<int>(kFarAhead, (int)farg.size());
+    size_t next_far = 0, done_far = 0;
+    std::vector<size_t> pending;
+    auto issue_far = [&] {
+        if (next_far < farg.size()) {
+            emit_load(farg[next_far]->first, "fq" + std::to_string(next_far));
+            pending.push_back(next_far++);
+        }
+    };
+    auto retire_far = [&] {
+        const size_t j = pending.front();
+        pending.erase(pending.begin());
+        emit_terms(farg[j]->second, "fq" + std::to_string(j));
+        ++done_far;
+        issue_far();
+    };
+    for (int j = 0; j < ahead; ++j) issue_far();
+    for (size_t i = 0; i < nearg.size(); ++i) {
+        const auto &g = *nearg[i];
+        if (g.first.own()) {
+            emit_terms(g.second, "p");
+        } else {
+            const std::string name = "nq" + std::to_string(i);
+            emit_load(g.first, name);
+            emit_terms(g.second, name);
+        }
+        // spread the far sub-cubes evenly over the near ones
+        while (!pending.empty() && (done_far + 1) * nearg.size() <= (i + 1) * farg.size()) retire_far();
+    }
+    while (!pending.empty()) retire_far();
+    if (!p.first)
+        for (int x = 0; x < S; ++x)
+            o << "    ar[" << x << "] += prev[" << x << "].x; ai[" << x << "] += prev[" << x << "].y;\n";
+    // store (streamed) and the tile's share of <in|out> from the staged rows
+    for (int x = 0; x < S; ++x) {
+        o << "    { NV2 v = {ar[" << x << "], ai[" << x << "]}; __builtin_nontemporal_store(v, reinterpret_cast<NV2 *>(out + "
+          << "(base | grw | " << hex32(L.slot_row(x)) << ") + lane)); }\n";
+    }
+    if (p.last) {
+        o << "    if (dot_part) {\n      const unsigned va = lane | gix;\n";
+        for (int x = 0; x < S; ++x)
+            o << "      { const V2 v = tile[va + " << hex32(L.slot_off(x)) << "]; const double xr = v.x, xi = v.y, yr = ar["
+              << x << "], yi = ai[" << x << "]; dr += xr * yr + xi * yi; di += xr * yi - xi * yr; }\n";
+        o << "    }\n";
+    }
+    o << "  }\n";
+    if (p.last) {
+        o << "  if (dot_part) {\n";
+        o << "#pragma unroll\n    for (int off = 32; off > 0; off >>= 1) { dr += __shfl_xor(dr, off); di += __shfl_xor(di, off); }\n";
+        o << "    if (lane == 0) { red[0][wave] = dr; red[1][wave] = di; }\n";
+        o << "    __syncthreads();\n";
+        o << "    if (threadIdx.x == 0) {\n      double x = 0.0, y = 0.0;\n";
+        o << "      for (int i = 0; i < " << WPB << "; ++i) { x += red[0][i]; y += red[1][i]; }\n";
+        o << "      dot_part[2 * (size_t)b] = x; dot_part[2 * (size_t)b + 1] = y;\n    }\n  }\n";
+    }
+    o << "}\n\n";
+}
+
+}  // namespace
+
+std::string cube_source(const CubeModel &m, const std::vector<CubePass> &passes) {
+    std::ostringstream o;
+    o << "// generated H.psi passes: " << m.n_sites << " sites, " << (m.c64 ? "complex64" : "complex128")
+      << ", 2^" << m.reg_sites << " rows per lane, " << m.waves << " waves per workgroup\n";
+    o << "typedef " << (m.c64 ? "float" : "double") << " R;\n";
+    o << "typedef " << (m.c64 ? "float2" : "double2") << " V2;\n";
+    o << "typedef R NV2 __attribute__((ext_vector_type(2)));\n";
+    // diagonal (closed form; used only when every partial sum of the Kitaev-z
+    // part is exact, so it equals the reference's bond-by-bond sum)
+    o << "__device__ __forceinline__ double esc_diag(unsigned r, double kz4, double j4, double bz) {\n";
+    o << "#pragma clang fp contract(off)\n";
+    o << "  unsigned cz = 0;\n";
+    for (uint32_t q = 0; q < m.n_zgrp; ++q)
+        o << "  cz += __popc((r ^ (r >> " << m.zgrp_d[q] << ")) & " << hex32(m.zgrp_m[q]) << ");\n";
+    o << "  double d = kz4 * (double)(" << (int)m.nz << " - 2 * (int)cz);\n";
+    if (m.nj) {
+        o << "  unsigned cj = 0;\n";
+        for (uint32_t q = 0; q < m.n_jgrp; ++q)
+            o << "  cj += __popc((r ^ (r >> " << m.jgrp_d[q] << ")) & " << hex32(m.jgrp_m[q]) << ");\n";
+        o << "  d = d + j4 * (double)(" << (int)m.nj << " - 2 * (int)cj);\n";
+    }
+    o << "  d = d + bz * (0.5 * (double)(" << m.n_sites << " - 2 * (int)__popc(r)));\n";
+    o << "  (void)j4;\n  return d;\n}\n\n";
+    for (size_t i = 0; i < passes.size(); ++i) emit_pass(o, m, passes[i], (int)i);
+    return o.str();
+}
+
+// ---------------------------------------------------------------------------
+// Compilation and caches
+// ---------------------------------------------------------------------------
+struct CubeProgram {
+    hipModule_t module = nullptr;
+    std::vector<hipFunction_t> fn;
+    int tile_bits = 0, threads = 0;
+    ~CubeProgram() {
+        if (module) (void)hipModuleUnload(module);
+    }
+};
+
+namespace {
+
+std::mutex g_mu;
+std::map<std::string, std::vector<char>> g_code;                        // hash -> code object
+std::map<std::pair<std::string, int>, std::weak_ptr<CubeProgram>> g_prog;  // (hash, device) -> program
+
+const char *const kOpts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+                             "-packed-fp32-ops"};
+
+std::string key_of(const std::string &src) {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](const std::string &s) {
+        for (unsigned char c : s) { h ^= c; h *= 1099511628211ull; }
+    };
+    mix(src);
+    for (const char *opt : kOpts) mix(opt);
+    int maj = 0, min = 0;
+    if (hiprtcVersion(&maj, &min) == HIPRTC_SUCCESS) mix(std::to_string(maj) + "." + std::to_string(min));
+    char b[32];
+    std::snprintf(b, sizeof(b), "%016llx", (unsigned long long)h);
+    return b;
+}
+
+// cache directory: ESC_JIT_CACHE, else jit_cache/ beside the library
+std::string cache_dir() {
+    if (const char *e = std::getenv("ESC_JIT_CACHE")) return e;
+    Dl_info info;
+    if (dladdr((const void *)&cube_source, &info) && info.dli_fname) {
+        std::string p = info.dli_fname;
+        const size_t sl = p.rfind('/');
+        return (sl == std::string::npos ? std::string(".") : p.substr(0, sl)) + "/jit_cache";
+    }
+    return "";
+}
+
+bool read_file(const std::string &path, std::vector<char> &out) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return false;
+    out.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
+    return !out.empty();
+}
+
+void write_file(const std::string &dir, const std::string &name, const std::vector<char> &data) {
+    if (dir.empty()) return;
+    (void)mkdir(dir.c_str(), 0755);
+    const std::string tmp = dir + "/" + name + ".tmp" + std::to_string((long)getpid());
+    {
+        std::ofstream f(tmp, std::ios::binary);
+        if (!f) return;
+        f.write(data.data(), (std::streamsize)data.size());
+        if (!f) { std::remove(tmp.c_str()); return; }
+    }
+    if (std::rename(tmp.c_str(), (dir + "/" + name).c_str()) != 0) std::remove(tmp.c_str());
+}
+
+// code object of a source: process cache, disk cache, then hiprtc
+bool code_for(const std::string &src, const std::string &key, std::vector<char> &code, std::string *err) {
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_code.find(key);
+        if (it != g_code.end()) { code = it->second; return true; }
+    }
+    const std::string dir = cache_dir();
+    if (!dir.empty() && read_file(dir + "/" + key + ".co", code)) {
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_code[key] = code;
+        return true;
+    }
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "esc_cube.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
+        if (err) *err = "hiprtcCreateProgram failed";
+        return false;
+    }
+    const hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof(kOpts) / sizeof(kOpts[0])), kOpts);
+    if (rc != HIPRTC_SUCCESS) {
+        size_t n = 0;
+        hiprtcGetProgramLogSize(prog, &n);
+        std::string log(n, '\0');
+        if (n) hiprtcGetProgramLog(prog, &log[0]);
+        if (err) *err = std::string("hiprtc: ") + hiprtcGetErrorString(rc) + "\n" + log.substr(0, 4000);
+        hiprtcDestroyProgram(&prog);
+        return false;
+    }
+    size_t n = 0;
+    hiprtcGetCodeSize(prog, &n);
+    code.assign(n, 0);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    write_file(dir, key + ".co", code);
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_code[key] = code;
+    return true;
+}
+
+bool check_model(const CubeModel &m, const std::vector<CubePass> &passes, std::string *err) {
+    for (const CubePass &p : passes) {
+        const size_t esz = m.c64 ? 8 : 16;
+        const int ngb = p.tile_bits - 6 - m.reg_sites;
+        if (ngb < 0 || (1 << ngb) % m.waves != 0 || ((size_t)1 << p.tile_bits) * esz > 131072 ||
+            (1 << (p.tile_bits - 6)) % m.waves != 0 || (int)p.wb.size() < p.tile_bits) {
+            if (err) *err = "tile shape does not fit the sub-cube kernel";
+            return false;
+        }
+    }
+    return !passes.empty();
+}
+
+}  // namespace
+
+bool cube_precompile(const CubeModel &m, const std::vector<CubePass> &passes, std::string *err) {
+    if (!check_model(m, passes, err)) return false;
+    const std::string src = cube_source(m, passes);
+    std::vector<char> code;
+    return code_for(src, key_of(src), code, err);
+}
+
+std::shared_ptr<CubeProgram> cube_program(const CubeModel &m, const std::vector<CubePass> &passes,
+                                          std::string *err) {
+    if (!check_model(m, passes, err)) return nullptr;
+    const std::string src = cube_source(m, passes);
+    const std::string key = key_of(src);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        if (err) *err = "no HIP device";
+        return nullptr;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_prog.find({key, dev});
+        if (it != g_prog.end())
+            if (auto sp = it->second.lock()) return sp;
+    }
+    std::vector<char> code;
+    if (!code_for(src, key, code, err)) return nullptr;
+    auto prog = std::make_shared<CubeProgram>();
+    if (hipModuleLoadData(&prog->module, code.data()) != hipSuccess) {
+        prog->module = nullptr;
+        if (err) *err = "hipModuleLoadData failed for the generated H.psi kernels";
+        return nullptr;
+    }
+    for (size_t i = 0; i < passes.size(); ++i) {
+        hipFunction_t f;
+        const std::string name = "esc_cube_p" + std::to_string(i);
+        if (hipModuleGetFunction(&f, prog->module, name.c_str()) != hipSuccess) {
+            if (err) *err = "generated kernel " + name + " missing";
+            return nullptr;
+        }
+        prog->fn.push_back(f);
+    }
+    prog->tile_bits = passes[0].tile_bits;
+    prog->threads = m.waves * 64;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_prog[{key, dev}] = prog;
+    return prog;
+}
+
+int cube_tile_bits(const CubeProgram &p) { return p.tile_bits; }
+
+hipError_t cube_launch(const CubeProgram &p, const double *d_scalars, const void *in, void *out, double *dot_part,
+                       uint64_t dim, hipStream_t s) {
+    const uint64_t ntiles64 = dim >> p.tile_bits;
+    if (ntiles64 == 0 || ntiles64 > 0x7fffffffull) return hipErrorInvalidValue;
+    const uint32_t ntiles = (uint32_t)ntiles64;
+    for (size_t i = 0; i < p.fn.size(); ++i) {
+        double *dp = (i + 1 == p.fn.size()) ? dot_part : nullptr;
+        for (uint32_t base = 0; base < ntiles; base += kCubeMaxGrid) {
+            const uint32_t n = (ntiles - base) < kCubeMaxGrid ? (ntiles - base) : kCubeMaxGrid;
+            const double *a0 = d_scalars;
+            const void *a1 = in;
+            void *a2 = out;
+            double *a3 = dp;
+            uint32_t a4 = base;
+            void *args[] = {&a0, &a1, &a2, &a3, &a4};
+            const hipError_t e = hipModuleLaunchKernel(p.fn[i], n, 1, 1, (unsigned)p.threads, 1, 1, 0, s, args, nullptr);
+            if (e != hipSuccess) return e;
+        }
+    }
+    return hipSuccess;
+}
+
+}  // namespace esc

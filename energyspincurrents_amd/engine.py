@@ -444,6 +444,27 @@ class Batch:
         return tables, (expect[:, 0::2] + 1j * expect[:, 1::2])[:, :nexp]
 
 
+def precompile_kernel(model: SpinModel, precision="c128", config=None):
+    """Build a generated H.psi kernel configuration for this lattice into the
+    code-object cache without a device (esc_kernel_precompile) and return its
+    HIP source.  config None: the configuration a context of this size uses
+    by default (complex128: 2^3 rows per lane, 8 waves, 2^12-row tiles;
+    complex64: 2^4 rows per lane, 2^13-row tiles; windows of 2^17 rows up to
+    22 sites, 2^18 beyond)."""
+    N = model.lattice.N
+    if config is None:
+        w = N if N <= 18 else (17 if N <= 22 else 18)
+        config = 4731300 + w if precision == "c64" else 3731200 + w
+    st = model.struct()
+    n = ctypes.c_size_t()
+    check(lib().esc_kernel_precompile(ctypes.byref(st), ESC_C64 if precision == "c64" else ESC_C128, int(config),
+                                      None, 0, ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(n.value + 1)
+    check(lib().esc_kernel_precompile(ctypes.byref(st), ESC_C64 if precision == "c64" else ESC_C128, int(config),
+                                      buf, len(buf), ctypes.byref(n)))
+    return buf.value.decode()
+
+
 def tridiag_eig(alpha, beta, vectors=True):
     a = np.ascontiguousarray(np.real(alpha), np.float64)
     b = np.ascontiguousarray(np.abs(beta), np.float64)

@@ -23,6 +23,7 @@
 #ifndef ESC_H
 #define ESC_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -108,14 +109,25 @@ int esc_ctx_info(const esc_ctx *ctx, int64_t *dim, int32_t *elem_bytes, int32_t 
  * rows staged in LDS, walking windows of 2^W rows (one XCD per window) in P
  * planned passes (P = 1 when W = N), plus passes for terms no window held;
  * P = 5: windows chosen first to hold every term in as few passes as the
- * width W allows; P = 6: the same windows run in reverse order.  d_out is
- * overwritten. */
+ * width W allows; P = 6: the same windows run in reverse order; P = 7 / 8:
+ * the plans of 5 / 6 run by kernels generated for the lattice at run time
+ * (hiprtc) in which each lane holds 2^k rows over k tile sites, with
+ * 1000000 * k added to the configuration.  d_out is overwritten. */
 int esc_autotune(esc_ctx *ctx, const void *d_in, void *d_out, int reps, int *tile_bits, double *ms, void *stream);
 /* Select the H.psi kernel explicitly (a configuration as reported by
  * esc_autotune or listed by esc_kernel_configs). */
 int esc_set_tile_bits(esc_ctx *ctx, int tile_bits);
 /* List the kernel configurations of a context (*n = count; up to cap written). */
 int esc_kernel_configs(const esc_ctx *ctx, int32_t *configs, int32_t cap, int32_t *n);
+/* Generated-kernel configurations (P = 7 / 8 above) compile at run time on
+ * first use.  This builds one into the code-object cache without a device
+ * (ESC_JIT_CACHE, else jit_cache/ beside libesc.so), so that later contexts
+ * of the same lattice and term structure load it instead of compiling; the
+ * couplings are run-time arguments and do not enter the cache key.  source
+ * (may be NULL) receives the generated HIP source, truncated to cap - 1
+ * characters; *len its full length.  No reference counterpart. */
+int esc_kernel_precompile(const esc_model *model, int precision, int config, char *source, size_t cap,
+                          size_t *len);
 
 /* ---------------------------------------------------------------- kernels -- */
 /* d_out = H d_in.  Replaces Hamiltonian::MatrixVectorMult
