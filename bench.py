@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-cpu-coo", action="store_true",
                     help="skip the reference-algorithm (COO lists) CPU H.psi, which holds 29 GB of host memory")
     ap.add_argument("--no-spectrum", action="store_true", help="skip the JeJe spectrum timing")
@@ -86,19 +86,22 @@ def time_hpsi(eng, x, y, steps, warmup, copy_calib=False):
     return ms, wall
 
 
-# HBM bytes per H.psi (all passes) measured with rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE on this workload, FETCH scaled by 2.0 and WRITE by 1.0 (the
-# factors that make a plain one-vector read / write read its own size); the
-# summary with the commands is profiles/hpsi_n24_hbm_traffic.md
-# the configurations the autotuner picks for this workload, profiled under profiles/
-# (c128: the library default and the same windows in reverse order, which
-# trade places from machine to machine; the faster on the machine is timed)
-PROFILED_CONFIGS = {"c128": [541218, 631118], "c64": [531218]}
+# The H.psi kernel configurations of this workload: the library defaults at 24
+# sites (generated sub-cube kernels: complex128 2^3 rows per lane, 8-wave
+# workgroups on 2^12-row tiles; complex64 2^4 rows per lane on 2^13-row tiles;
+# cover-first plan over 2^18-row windows, two passes).
+PROFILED_CONFIGS = {"c128": [3731218], "c64": [4731318]}
 
+# HBM bytes per H.psi (both passes) from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE on `python bench.py --steps 20 --warmup 2 --no-cpu --no-spectrum
+# --no-n32 --no-batch`, FETCH scaled by 2.0 and WRITE by 1.0 (the factors that
+# make a plain one-vector read / write report its own size), from the per-pass
+# rows of the named profile summary.
 MEASURED_TRAFFIC = {
-    ("c128", 541218): 1845611520,   # 638890 KiB fetched x 2.0 + 524575 KiB written
-    ("c128", 631118): 1980452864,   # 704819 KiB fetched x 2.0 + 524398 KiB written
-    ("c64", 531218): 743146496,     # 231755 KiB fetched x 2.0 + 262219 KiB written
+    # (171444 + 404368) KiB fetched x 2.0 + (262566 + 262145) KiB written
+    ("c128", 3731218): (1716567040, "profiles/hpsi_cube_n24_hbm_traffic.md: esc_cube c128 (config 3731218)"),
+    # (91981 + 131281) KiB fetched x 2.0 + (131073 + 131072) KiB written
+    ("c64", 4731318): (725677056, "profiles/hpsi_cube_n24_hbm_traffic.md: esc_cube c64 (config 4731318)"),
 }
 
 
@@ -114,8 +117,32 @@ def time_lanczos_step(eng, phi, steps):
     return (time.perf_counter() - t0) * 1e3 / steps
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads():
+    """OpenMP threads of the host legs: OMP_NUM_THREADS, else the cores this
+    process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit():
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
 def cpu_baseline(reps):
-    """Oracle gather H.psi (FP64, OpenMP over the host cores) at the same N=24."""
+    """Oracle gather H.psi (FP64, OpenMP over the host cores) at the same N=24:
+    one warm-up, then the median of `reps` timed applications."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     m = O.Model(LX, LY, K=(K, K, K), J=J, G=G, B=O.field111(H111))
@@ -128,8 +155,10 @@ def cpu_baseline(reps):
         O.hpsi_gather(m, v)
         t.append(time.perf_counter() - t0)
     med = float(np.median(t))
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": 1.0 / med, "unit": "H.psi/s", "cores": cores, "kind": "port",
+    cores = host_threads()
+    return {"value": 1.0 / med, "unit": "H.psi/s", "cores": cores, "kind": "port", "cpu_model": cpu_model(),
+            "nproc": os.cpu_count(), "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "times_s": [round(x, 4) for x in t],
             "sample": f"{reps} FP64 gather H.psi at N=24 after 1 warm-up (median {med:.3f} s each), "
                       "oracle/esc_oracle.c orc_hpsi_gather, OpenMP"}
 
@@ -144,7 +173,7 @@ def cpu_coo_baseline(reps):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     m = O.Model(LX, LY, K=(K, K, K), B=O.field111(H111))
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cores = host_threads()
     t0 = time.perf_counter()
     h = O.CooHamiltonian(m)
     build = time.perf_counter() - t0
@@ -161,7 +190,7 @@ def cpu_coo_baseline(reps):
     fx, fy, _ = m.families(m.K)
     entries = (len(fx) + len(fy) + 2 * m.N) * m.dim
     return {"value": 1.0 / med, "unit": "H.psi/s", "cores": cores, "kind": "port",
-            "list_build_s": build, "list_entries": entries, "list_bytes": 24 * entries,
+            "times_s": [round(x, 4) for x in t], "list_build_s": build, "list_entries": entries, "list_bytes": 24 * entries,
             "sample": f"{reps} FP64 COO H.psi at N={m.N} (K=-1, h[111]=0.2) after 1 warm-up "
                       f"(median {med:.3f} s each; list build {build:.1f} s, not included), "
                       "oracle/esc_oracle.c orc_coo_build + orc_coo_apply, OpenMP"}
@@ -169,6 +198,9 @@ def cpu_coo_baseline(reps):
 
 def main():
     args = parse()
+    # host legs: threads bound close to their cores (read by OpenMP when the
+    # oracle library loads)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding "
               "of the 2^N vector is a design note, not a measured path", file=sys.stderr)
@@ -194,6 +226,7 @@ def main():
             eng.set_tile_bits(config)
         else:
             # outside the timed region: the faster of the profiled configurations
+            # (the first call of a generated configuration compiles or loads it)
             for cfg in PROFILED_CONFIGS[prec]:
                 eng.set_tile_bits(cfg)
                 probe[cfg] = time_hpsi(eng, x, y, 30, 3)[0]
@@ -201,11 +234,16 @@ def main():
             eng.set_tile_bits(config)
         ms, wall = time_hpsi(eng, x, y, args.steps, args.warmup, args.copy_calib and prec == "c128")
         bytes_alg = 2 * eng.dim * eng.elem_bytes
-        kname = f"k_hpsi_tile<{'float' if prec == 'c64' else 'double'}, TB, WPB>" if config else \
-            f"k_hpsi<{'float' if prec == 'c64' else 'double'}>"
+        if config // 100000 % 10 in (7, 8):
+            kname = "esc_cube_p0 + esc_cube_p1 (generated sub-cube passes)"
+        elif config:
+            kname = f"k_hpsi_tile<{'float' if prec == 'c64' else 'double'}, TB, WPB>"
+        else:
+            kname = f"k_hpsi<{'float' if prec == 'c64' else 'double'}>"
+        traffic, tsrc = MEASURED_TRAFFIC.get((prec, config), (None, None))
         res[prec] = {"ms": ms, "wall": wall, "bytes": bytes_alg, "gbs": bytes_alg / (ms * 1e-3) / 1e9,
                      "hpsi_per_s": 1e3 / ms, "config": config, "kernel": kname,
-                     "traffic": MEASURED_TRAFFIC.get((prec, config)),
+                     "traffic": traffic, "traffic_source": tsrc,
                      "probe_ms": {str(k): v for k, v in probe.items()}}
         res[prec]["step_ms"] = time_lanczos_step(eng, x, 200)
         del x, y
@@ -233,12 +271,16 @@ def main():
                    "kernel_config": r["config"], "kernel_config_probe_ms": r["probe_ms"]},
         "roofline": {"bound": "hbm", "achieved": r["gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": r["gbs"] / PEAK_HBM_GBS, "traffic": r["traffic"],
+                     "traffic_source": r["traffic_source"],
                      "kernel": r["kernel"], "bytes_per_launch": r["bytes"],
-                     "launch": "one H.psi = the passes of its kernel configuration, timed together"},
+                     "launch": "one H.psi = the passes of its kernel configuration, timed together with HIP "
+                               "events on the launch stream"},
         "c64": {"hpsi_per_s": res["c64"]["hpsi_per_s"], "ms_per_hpsi": res["c64"]["ms"],
                 "achieved_GBs": res["c64"]["gbs"], "frac": res["c64"]["gbs"] / PEAK_HBM_GBS,
                 "bytes_per_launch": res["c64"]["bytes"], "kernel": res["c64"]["kernel"],
-                "kernel_config": res["c64"]["config"], "traffic": res["c64"]["traffic"]},
+                "kernel_config": res["c64"]["config"], "traffic": res["c64"]["traffic"],
+                "traffic_source": res["c64"]["traffic_source"],
+                "speedup_vs_c128": r["ms"] / res["c64"]["ms"]},
     }
     # one Lanczos step priced at the 3 dim s bytes of a fully fused step (read v
     # and v_prev, write w)
@@ -265,7 +307,7 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_reps)
         if not args.no_cpu_coo:
-            out["cpu_baseline"]["ref_algorithm"] = cpu_coo_baseline(2)
+            out["cpu_baseline"]["ref_algorithm"] = cpu_coo_baseline(args.cpu_reps)
         if e2e_inputs is not None:
             out["cpu_baseline"]["jeje_end_to_end"] = cpu_jeje_e2e(e2e_inputs)
     else:
@@ -548,10 +590,23 @@ def _l1(a, b):
     return float(np.abs(a[:, 1:] - b[:, 1:]).sum() / np.abs(b[:, 1:]).sum())
 
 
+def _weights(tab):
+    """Integrated spectral weight of every JeJe column over the omega grid
+    (trapezoid rule): xx, yy, xy, yx, tt."""
+    return np.trapezoid(tab[:, 1:], tab[:, 0], axis=0)
+
+
 def spectrum_summary(spec):
     c128, c64 = spec["c128"], spec["c64"]
     e128, e64 = c128["E0"], c64["E0"]
-    # total spectral weight of a channel = |J_a gs|^2 (the zeroth moment)
+    # the complex64 pipeline end to end (its own ground state and its own
+    # 200-step fractions) against the complex128 one: integrated weight per
+    # column, relative to the largest column weight
+    w128, w64 = _weights(c128["table"]), _weights(c64["table"])
+    wscale = float(np.abs(w128).max())
+    werr = [float(abs(a - b) / wscale) for a, b in zip(w64, w128)]
+    w200 = float(np.abs(_weights(c64["table_200"]) - _weights(c128["table_200"])).max() / wscale)
+    # zeroth moment of a diagonal channel from the common state: |J_a gs|^2
     wt = max(abs(c64["cf"][op][2] - c128["cf"][op][2]) / c128["cf"][op][2] for op in ("Jex", "Jey"))
     lead = 0.0
     for op in ("Jex", "Jey"):
@@ -562,18 +617,25 @@ def spectrum_summary(spec):
     return {"jeje_seconds_c128": c128["seconds"], "jeje_seconds_c64": c64["seconds"],
             "E0_c128": e128, "E0_c64": e64, "E0_rel_err_c64": abs(e64 - e128) / abs(e128),
             "gs_residual_c128": c128["residual"], "gs_residual_c64": c64["residual"],
-            "spectral_weight_rel_err_c64": wt,
+            "spectral_weight_rel_err_c64": max(werr),
+            "spectral_weight_rel_err_c64_by_column": dict(zip(("xx", "yy", "xy", "yx", "tt"), werr)),
+            "spectral_weight_rel_err_c64_common_gs_200_steps": w200,
+            "zeroth_moment_rel_err_c64_common_gs": wt,
             "leading_alpha_beta_rel_err_c64": float(lead),
             "spectrum_l1_rel_err_c64_400_steps": _l1(c64["table_400"], c128["table_400"]),
             "spectrum_l1_rel_err_c64_200_steps": _l1(c64["table_200"], c128["table_200"]),
             "spectrum_l1_c128_200_vs_400_steps": _l1(c128["table_200"], c128["table_400"]),
             "spectrum_l1_rel_err_c64_own_gs": _l1(c64["table"], c128["table"]),
             "workload": "timed: GS 200 Lanczos steps + 1 restart (FP64 scalars) + JeJe xx, yy, xy, yx, tt by "
-                        "continued fractions, 200 steps each, eta=0.05, omega=linspace(0,6,2000).  Errors of "
-                        "complex64 against complex128 from the complex128 ground state: weight = |J_a gs|^2, "
-                        "leading alpha/beta = first 5 of the Jex and Jey recursions (relative to beta), L1 of the "
-                        "JeJe table at 400 steps (converged at eta=0.05) and at 200 steps (not converged: see the "
-                        "complex128 200-vs-400 figure); own_gs compares the two timed pipelines end to end"}
+                        "continued fractions, 200 steps each, eta=0.05, omega=linspace(0,6,2000).  "
+                        "spectral_weight_rel_err_c64: integral over the omega grid of each column of the "
+                        "complex64 pipeline's own table against the complex128 pipeline's (each from its own "
+                        "ground state, 200 steps), relative to the largest column weight; E0_rel_err_c64 from "
+                        "the same two runs.  From the complex128 ground state (converted): zeroth moment "
+                        "|J_a gs|^2, leading alpha/beta (first 5 of the Jex and Jey recursions, relative to "
+                        "beta), L1 of the table at 400 steps (converged at eta=0.05) and at 200 steps (not "
+                        "converged: see the complex128 200-vs-400 figure); own_gs = L1 of the two timed "
+                        "pipelines' tables"}
 
 
 if __name__ == "__main__":

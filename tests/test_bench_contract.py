@@ -32,18 +32,21 @@ def test_documented_flags_parse():
 
 
 def test_profiled_configurations_and_traffic_match_the_profiles():
+    """The configurations bench.py times are the ones the committed kernel
+    trace and traffic summaries cover, and roofline.traffic is (FETCH KiB x 2.0
+    + WRITE KiB) x 1024 of the named summary row, both passes."""
     b = load_bench()
     cfgs = b.PROFILED_CONFIGS
     assert set(b.MEASURED_TRAFFIC) == {(p, c) for p in cfgs for c in cfgs[p]}
-    trace = open(os.path.join(ROOT, "profiles", "hpsi_n24_kernel_trace.md")).read()
-    traffic = open(os.path.join(ROOT, "profiles", "hpsi_n24_hbm_traffic.md")).read()
-    for c128 in cfgs["c128"]:
-        assert f"{c128},{cfgs['c64'][0]}" in trace and f"{c128},{cfgs['c64'][0]}" in traffic
-    # per-H.psi totals: FETCH KiB x 2.0 + WRITE KiB, from the per-pass table
-    lines = traffic.splitlines()
-    for (prec, cfg), total in b.MEASURED_TRAFFIC.items():
-        i = next(n for n, ln in enumerate(lines) if ln.startswith("| k_hpsi_tile<") and f"(config {cfg})" in ln)
-        assert ("<double" in lines[i]) == (prec == "c128")
+    trace = open(os.path.join(ROOT, "profiles", "hpsi_cube_n24_kernel_trace.md")).read()
+    for prec in cfgs:
+        for c in cfgs[prec]:
+            assert f"| {prec} | {c} |" in trace
+    for (prec, cfg), (total, source) in b.MEASURED_TRAFFIC.items():
+        path, row = source.split(": ", 1)
+        lines = open(os.path.join(ROOT, path)).read().splitlines()
+        i = next(n for n, ln in enumerate(lines) if ln.startswith(f"| {row} |"))
+        assert prec in row and str(cfg) in row
         cells = [[c.strip() for c in ln.strip().strip("|").split("|")] for ln in lines[i:i + 2]]
         assert [c[1] for c in cells] == ["1", "2"]
         fetch = sum(int(c[2]) for c in cells)
