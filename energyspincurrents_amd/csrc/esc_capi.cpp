@@ -1955,6 +1955,7 @@ int esc_ctx_create(const esc_model *model, int precision, esc_ctx **out) {
         auto c = std::make_unique<esc_ctx>();
         ctx_init_host(c.get(), model, precision);
         hip_check(hipGetDevice(&c->device), "hipGetDevice");
+        hip_check(esc::scratch_init(), "no-allocation workspace address");
         if (hipMalloc(&c->d_htab, sizeof(esc::HTables)) != hipSuccess) throw EscError(ESC_ERR_ALLOC, "table allocation failed");
         hip_check(hipMemcpy(c->d_htab, &c->htab, sizeof(esc::HTables), hipMemcpyHostToDevice), "hipMemcpy");
         c->tile_bits = default_tile_bits(c.get());
@@ -2073,6 +2074,16 @@ int esc_dot(esc_ctx *ctx, const void *d_x, const void *d_y, double *out, void *s
         check_ptr(d_y, "d_y");
         check_ptr(out, "out");
         device_dot(ctx, d_x, d_y, out, (hipStream_t)stream);
+    });
+}
+
+int esc_dot_noalloc(esc_ctx *ctx, const void *d_x, const void *d_y, double *d_out, void *stream) {
+    return guarded([&] {
+        check_ctx(ctx);
+        check_ptr(d_x, "d_x");
+        check_ptr(d_y, "d_y");
+        check_ptr(d_out, "d_out");
+        hip_check(esc::launch_dot_noalloc(ctx->c64(), d_x, d_y, ctx->dim, d_out, (hipStream_t)stream), "dot");
     });
 }
 

@@ -189,6 +189,13 @@ hipError_t launch_rec_small(bool c64, const HTables *t, void *b0, void *b1, void
 // y += (coef[2k] + i coef[2k+1]) rec[0] x, k = *step
 hipError_t launch_axpy_rec_step(bool c64, const double *coef, const uint32_t *step, const double *rec, const void *x,
                                 void *y, uint64_t dim, hipStream_t s);
+// The no-allocation dot: the same kernels as launch_dot with the partial sums
+// in the library's fixed per-GPU array; result is caller device memory.
+// scratch_init resolves that array's address on the current device (once per
+// device; also done on the first launch_dot_noalloc there).
+constexpr size_t kScratchBytes = 2 * kRedBlocks * sizeof(double);
+hipError_t scratch_init();
+hipError_t launch_dot_noalloc(bool c64, const void *x, const void *y, uint64_t dim, double *result, hipStream_t s);
 // result[0] = |x|^2
 hipError_t launch_norm2(bool c64, const void *x, uint64_t dim, double *partials, double *result, hipStream_t s);
 // coef[m] = <V_m | w> for m < k (V_m = base + m * dim), FP64; result has 2k doubles

@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <mutex>
 
 namespace esc {
 namespace {
@@ -1218,6 +1219,55 @@ hipError_t launch_dot(bool c64, const void *x, const void *y, uint64_t dim, doub
                            (CV2p<double>)y, dim, partials);
     hipLaunchKernelGGL(k_reduce_final, dim3(2), dim3(kThreads), 0, s, partials, kRedBlocks, 2, result);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Reduction workspace that needs no allocation: one fixed device array
+// (kRedBlocks value pairs, 16 KiB), held by each GPU the library runs on until
+// the process ends.  Host code never passes the array's name to a kernel: its
+// device address is resolved once per device (scratch_init, with that device
+// current) and kept in g_scratch_ptr.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[2 * kRedBlocks];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxScratchDevices] = {};
+}  // namespace
+
+hipError_t scratch_init() {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return hipSuccess;
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = p;
+    return hipSuccess;
+}
+
+hipError_t launch_dot_noalloc(bool c64, const void *x, const void *y, uint64_t dim, double *result, hipStream_t s) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxScratchDevices) return hipErrorInvalidDevice;
+    void *p;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        p = g_scratch_ptr[dev];
+    }
+    if (!p) {
+        if ((e = scratch_init()) != hipSuccess) return e;
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        p = g_scratch_ptr[dev];
+    }
+    // every partial pair is written by k_dot_partial before k_reduce_final
+    // reads it, so nothing needs clearing between calls
+    return launch_dot(c64, x, y, dim, static_cast<double *>(p), result, s);
 }
 
 hipError_t launch_norm2(bool c64, const void *x, uint64_t dim, double *partials, double *result, hipStream_t s) {

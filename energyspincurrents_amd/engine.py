@@ -261,6 +261,17 @@ class Engine:
         check(lib().esc_dot(self._h, self._v(x, "x"), self._v(y, "y"), _np_ptr(r), _stream(stream)))
         return complex(r[0], r[1])
 
+    def dot_noalloc(self, x, y, out, stream=None):
+        """out[0], out[1] = Re, Im <x|y> in device memory (float64 tensor of 2),
+        computed without any allocation (esc_dot_noalloc)."""
+        import torch
+        if not (isinstance(out, torch.Tensor) and out.dtype == torch.float64 and out.numel() >= 2
+                and out.device.type == "cuda" and out.is_contiguous()):
+            raise ValueError("out: contiguous float64 device tensor of at least 2 elements")
+        check(lib().esc_dot_noalloc(self._h, self._v(x, "x"), self._v(y, "y"), ctypes.c_void_p(out.data_ptr()),
+                                    _stream(stream)))
+        return out
+
     def axpby(self, a: complex, x, b: complex, y, stream=None):
         av = np.array([a.real, a.imag]) if isinstance(a, complex) else np.array([float(a), 0.0])
         bv = np.array([b.real, b.imag]) if isinstance(b, complex) else np.array([float(b), 0.0])

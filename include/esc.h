@@ -148,6 +148,15 @@ int esc_start_vector(esc_ctx *ctx, uint64_t seed, int kind, void *d_out, void *s
  * deterministic.  Replaces Eigen's VectorXcd::dot as used at
  * src/DynLanczosSolver.h:76,95 and src/dynamics.h:250-262,370-381. */
 int esc_dot(esc_ctx *ctx, const void *d_x, const void *d_y, double *out, void *stream);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  d_out[0..1] = <d_x|d_y> written by the kernels into
+ * caller device memory (no host synchronisation, no allocation, the same bits
+ * as esc_dot).  The partial sums live in one fixed 16 KiB device array of the
+ * library per GPU (2 x 1024 doubles; the limit is that fixed size, which any
+ * vector length fits), shared by every call on that GPU: calls that overlap
+ * on one GPU race. */
+int esc_dot_noalloc(esc_ctx *ctx, const void *d_x, const void *d_y, double *d_out, void *stream);
 /* d_y = a d_x + b d_y, complex a = (a[0], a[1]), b likewise. */
 int esc_axpby(esc_ctx *ctx, const double *a, const void *d_x, const double *b, void *d_y, void *stream);
 /* Copy host complex<double> data into a context-precision device vector and back. */
