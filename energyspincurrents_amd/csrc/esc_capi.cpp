@@ -790,7 +790,8 @@ std::vector<esc::GTables> build_passes(const esc_ctx *c, int TB, int w, int P, i
 // several bond families (the chain's XYZ bonds) or a Kitaev-z diagonal whose
 // partial sums are not exact (the closed form would not equal the reference
 // order).
-bool cube_spec(const esc_ctx *c, int cfg, esc::CubeModel &m, std::vector<esc::CubePass> &passes, std::string *why) {
+bool cube_spec(const esc_ctx *c, int cfg, esc::CubeModel &m, std::vector<esc::CubePass> &passes, std::string *why,
+               bool do_plan = true) {
     const esc::HTables &h = c->htab;
     const int N = c->n_sites;
     const std::vector<HostTerm> terms = host_terms(c);
@@ -845,18 +846,14 @@ bool cube_spec(const esc_ctx *c, int cfg, esc::CubeModel &m, std::vector<esc::Cu
     }
     m.reg_sites = cfg_regs(cfg);
     m.waves = WPB;
-    passes.clear();
-    const std::vector<Plan> &pl = plan;
-    for (size_t i = 0; i < pl.size(); ++i) {
-        esc::CubePass cp;
-        window_order(N, pl[i], cp.wb, cp.cb);
-        cp.tile_bits = TB;
-        for (size_t t : pl[i].near) cp.terms.push_back(ct[t]);
-        for (size_t t : pl[i].far) cp.terms.push_back(ct[t]);
-        cp.first = i == 0;
-        cp.last = i + 1 == pl.size();
-        cp.diag = cp.first;
-        passes.push_back(std::move(cp));
+    if (!do_plan) return true;
+    // the generic kernel's cover-first tiles seed the sub-cube planner, with
+    // at most w - TB far sites per pass
+    std::vector<uint32_t> tiles;
+    for (const Plan &pl : plan) tiles.push_back(pl.T);
+    if (!esc::cube_plan(m, ct, TB, tiles, std::max(0, std::min(w, N) - TB), passes)) {
+        if (why) *why = "no sub-cube plan places every term";
+        return false;
     }
     return true;
 }
@@ -1069,7 +1066,7 @@ bool tile_config_fits(const esc_ctx *c, int config) {
         std::vector<esc::CubePass> passes;
         bool ok = false;
         try {
-            ok = cube_spec(c, config, m, passes, nullptr);
+            ok = cube_spec(c, config, m, passes, nullptr, false);
         } catch (const EscError &) {
             ok = false;
         }
@@ -1150,12 +1147,13 @@ std::vector<int> tile_candidates(const esc_ctx *c) {
 // with 2^12-row tiles (16 waves for c128, which moves the least HBM traffic of
 // the equally fast shapes at N = 24, 8 waves for c64).  A shape the context
 // cannot run falls back to a single pass over 2^11- or 2^10-row tiles.
-// From 19 sites the generated sub-cube kernels come first (N = 24: 0.36 ms
-// against 0.52 ms for the best generic shape in complex128, 0.21 against
-// 0.32 ms in complex64): complex128 2^3 rows per lane in 8-wave workgroups
-// on 2^12-row tiles, complex64 2^4 rows per lane on 2^13-row tiles; a model
-// they do not cover, or a missing run-time compiler, falls through to the
-// generic shapes.
+// From 19 sites the generated sub-cube kernels come first (N = 24: 0.35 ms
+// against 0.52 ms for the best generic shape in complex128, 0.20 against
+// 0.32 ms in complex64; N = 20: 25.6 against 40.1 us and 19.1 against
+// 28.8 us): complex128 2^3 rows per lane in 8-wave workgroups on 2^12-row
+// tiles; complex64 2^4 rows per lane, 4 waves on 2^12-row tiles up to 22
+// sites and 8 waves on 2^13-row tiles beyond.  A model they do not cover, or
+// a missing run-time compiler, falls through to the generic shapes.
 int default_tile_bits(esc_ctx *c) {
     const int N = c->n_sites;
     const bool c64 = c->c64();
@@ -1163,7 +1161,8 @@ int default_tile_bits(esc_ctx *c) {
     std::vector<int> pref;
     if (N >= 19) {
         const int w = N <= 22 ? 17 : 18;
-        pref.push_back(c64 ? make_config(kCubeFirst, 8, 13, w, 4) : make_config(kCubeFirst, 8, 12, w, 3));
+        pref.push_back(c64 ? (N <= 22 ? make_config(kCubeFirst, 4, 12, w, 4) : make_config(kCubeFirst, 8, 13, w, 4))
+                           : make_config(kCubeFirst, 8, 12, w, 3));
     }
     if (N <= 18) pref.push_back(make_config(1, 8, 10, N));
     else if (N <= 22) pref.push_back(c64 ? make_config(kCoverLast, 16, 12, 17) : make_config(kCoverFirst, 16, 11, 17));

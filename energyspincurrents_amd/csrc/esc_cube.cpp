@@ -127,6 +127,99 @@ void choose_registers(Layout &L, const std::vector<CubeTerm> &terms) {
 }
 
 // ---------------------------------------------------------------------------
+// Pass planning
+// ---------------------------------------------------------------------------
+// Cost of one pass: every distinct partner sub-cube is one load per row, a
+// far one (from L2, through the texture path) weighted three times a near
+// one (from LDS); the register sites are chosen for the pass's terms.
+long pass_cost(int N, uint32_t T, uint32_t F, int TB, int k, const std::vector<const CubeTerm *> &ts) {
+    Layout L;
+    L.N = N;
+    L.TB = TB;
+    L.k = k;
+    L.wb.clear();
+    for (int s = 0; s < 6; ++s) L.wb.push_back(s);
+    for (int s = 6; s < N; ++s)
+        if ((T >> s) & 1u) L.wb.push_back(s);
+    for (int s = 0; s < N; ++s)
+        if ((F >> s) & 1u) L.wb.push_back(s);
+    L.w = (int)L.wb.size();
+    std::vector<CubeTerm> terms;
+    for (const CubeTerm *t : ts) terms.push_back(*t);
+    choose_registers(L, terms);
+    std::vector<Outer> parts;
+    long cost = 0;
+    for (const CubeTerm &ct : terms) {
+        Outer o;
+        uint32_t rho;
+        split_mask(L, ct.mask, o, rho);
+        if (std::find(parts.begin(), parts.end(), o) == parts.end()) {
+            parts.push_back(o);
+            cost += o.own() ? 0 : (o.far ? 3 : 1);
+        }
+    }
+    return cost;
+}
+
+struct PlanEval {
+    long cost = -1;                       // -1: some term has no pass
+    std::vector<std::vector<int>> assign;  // term indices per pass
+    std::vector<uint32_t> far;             // far sites per pass
+};
+
+// place every term given the tiles: near where possible (the pass whose
+// current terms already touch the most of its sites), else far in the pass
+// that needs the fewest new far sites (at most max_far per pass); then the
+// summed pass costs
+PlanEval evaluate(int N, int TB, int k, int max_far, const std::vector<uint32_t> &T,
+                  const std::vector<CubeTerm> &terms) {
+    PlanEval ev;
+    const size_t P = T.size();
+    ev.assign.assign(P, {});
+    ev.far.assign(P, 0u);
+    std::vector<uint32_t> touched(P, 0u);
+    std::vector<int> farq;
+    for (size_t t = 0; t < terms.size(); ++t) {
+        int best = -1, bk = -1;
+        for (size_t p = 0; p < P; ++p) {
+            if (terms[t].mask & ~T[p]) continue;
+            const int key = __builtin_popcount(terms[t].mask & touched[p]);
+            if (key > bk) { bk = key; best = (int)p; }
+        }
+        if (best < 0) { farq.push_back((int)t); continue; }
+        ev.assign[best].push_back((int)t);
+        touched[best] |= terms[t].mask;
+    }
+    // far terms: repeatedly place the (term, pass) pair that adds the fewest
+    // far sites to its pass (ties: the pass with fewer terms)
+    while (!farq.empty()) {
+        int bt = -1, bp = -1, bk = 1 << 30;
+        for (size_t q = 0; q < farq.size(); ++q) {
+            const int t = farq[q];
+            for (size_t p = 0; p < P; ++p) {
+                const uint32_t nf = ev.far[p] | (terms[t].mask & ~T[p]);
+                if (__builtin_popcount(nf) > max_far) continue;
+                const int key = __builtin_popcount(nf & ~ev.far[p]) * 1024 + (int)ev.assign[p].size();
+                if (key < bk) { bk = key; bt = (int)q; bp = (int)p; }
+            }
+        }
+        if (bt < 0) return ev;
+        const int t = farq[bt];
+        ev.far[bp] |= terms[t].mask & ~T[bp];
+        ev.assign[bp].push_back(t);
+        farq.erase(farq.begin() + bt);
+    }
+    long cost = 0;
+    for (size_t p = 0; p < P; ++p) {
+        std::vector<const CubeTerm *> ts;
+        for (int t : ev.assign[p]) ts.push_back(&terms[t]);
+        cost += pass_cost(N, T[p], ev.far[p], TB, k, ts);
+    }
+    ev.cost = cost;
+    return ev;
+}
+
+// ---------------------------------------------------------------------------
 // Boolean expressions of row bits
 // ---------------------------------------------------------------------------
 struct BE {
@@ -421,6 +514,72 @@ void emit_pass(std::ostringstream &o, const CubeModel &m, const CubePass &p, int
 }
 
 }  // namespace
+
+bool cube_plan(const CubeModel &m, const std::vector<CubeTerm> &terms, int tile_bits, std::vector<uint32_t> tiles,
+               int max_far, std::vector<CubePass> &passes) {
+    const int N = m.n_sites, TB = tile_bits, k = m.reg_sites;
+    for (uint32_t &t : tiles) t |= 63u;
+    // plans are cached per lattice, term set, shape and seed tiles
+    std::string key = std::to_string(N) + ":" + std::to_string(TB) + ":" + std::to_string(k) + ":" +
+                      std::to_string(max_far);
+    for (uint32_t t : tiles) key += ":" + std::to_string(t);
+    for (const CubeTerm &t : terms)
+        key += "," + std::to_string(t.mask) + "/" + std::to_string(t.kind) + "/" + std::to_string(t.f) + "/" +
+               std::to_string(t.xp) + "/" + std::to_string(t.yp);
+    static std::mutex mu;
+    static std::map<std::string, std::vector<CubePass>> cache;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) { passes = it->second; return !passes.empty(); }
+    }
+    PlanEval cur = evaluate(N, TB, k, max_far, tiles, terms);
+    if (cur.cost < 0) return false;
+    // hill climbing: swap one tile site of one pass for a site outside it
+    for (bool gain = true; gain;) {
+        gain = false;
+        for (size_t p = 0; p < tiles.size(); ++p) {
+            for (int a = 6; a < N; ++a) {
+                if (!((tiles[p] >> a) & 1u)) continue;
+                for (int b = 6; b < N; ++b) {
+                    if ((tiles[p] >> b) & 1u) continue;
+                    std::vector<uint32_t> t2(tiles);
+                    t2[p] = (t2[p] & ~(1u << a)) | (1u << b);
+                    const PlanEval ev = evaluate(N, TB, k, max_far, t2, terms);
+                    if (ev.cost >= 0 && ev.cost < cur.cost) {
+                        tiles = t2;
+                        cur = ev;
+                        gain = true;
+                        break;
+                    }
+                }
+            }
+        }
+    }
+    passes.clear();
+    for (size_t p = 0; p < tiles.size(); ++p) {
+        if (cur.assign[p].empty()) continue;
+        CubePass cp;
+        for (int s = 0; s < 6; ++s) cp.wb.push_back(s);
+        for (int s = 6; s < N; ++s)
+            if ((tiles[p] >> s) & 1u) cp.wb.push_back(s);
+        for (int s = 0; s < N; ++s)
+            if ((cur.far[p] >> s) & 1u) cp.wb.push_back(s);
+        for (int s = 0; s < N; ++s)
+            if (!((tiles[p] | cur.far[p]) >> s & 1u)) cp.cb.push_back(s);
+        cp.tile_bits = TB;
+        for (int t : cur.assign[p]) cp.terms.push_back(terms[t]);
+        passes.push_back(std::move(cp));
+    }
+    for (size_t i = 0; i < passes.size(); ++i) {
+        passes[i].first = i == 0;
+        passes[i].last = i + 1 == passes.size();
+        passes[i].diag = passes[i].first;
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    cache[key] = passes;
+    return !passes.empty();
+}
 
 std::string cube_source(const CubeModel &m, const std::vector<CubePass> &passes) {
     std::ostringstream o;

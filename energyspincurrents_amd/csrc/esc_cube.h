@@ -68,6 +68,15 @@ enum CubeScalar {
 
 struct CubeProgram;
 
+// Passes of the generated kernels: starting from one tile per pass (sites,
+// the lane sites 0..5 implied), local search over the tile sites and the
+// placement of every term (near in a pass whose tile holds its mask, else
+// far in a pass with at most max_far sites outside its tile) minimises the
+// partner sub-cubes loaded per row, far ones weighted above near ones.  Each
+// pass's window is its tile plus the far sites its terms use.  False if some
+// term fits no pass.
+bool cube_plan(const CubeModel &m, const std::vector<CubeTerm> &terms, int tile_bits, std::vector<uint32_t> tiles,
+               int max_far, std::vector<CubePass> &passes);
 // The generated source of all passes (for tests and diagnostics).
 std::string cube_source(const CubeModel &m, const std::vector<CubePass> &passes);
 // Compile (or fetch from the process and disk caches) and load on the current

@@ -460,12 +460,16 @@ def precompile_kernel(model: SpinModel, precision="c128", config=None):
     code-object cache without a device (esc_kernel_precompile) and return its
     HIP source.  config None: the configuration a context of this size uses
     by default (complex128: 2^3 rows per lane, 8 waves, 2^12-row tiles;
-    complex64: 2^4 rows per lane, 2^13-row tiles; windows of 2^17 rows up to
-    22 sites, 2^18 beyond)."""
+    complex64: 2^4 rows per lane, 4 waves on 2^12-row tiles up to 22 sites,
+    8 waves on 2^13-row tiles beyond; at most 17 window sites up to 22 sites,
+    18 beyond)."""
     N = model.lattice.N
     if config is None:
         w = N if N <= 18 else (17 if N <= 22 else 18)
-        config = 4731300 + w if precision == "c64" else 3731200 + w
+        if precision == "c64":
+            config = (4721200 if N <= 22 else 4731300) + w
+        else:
+            config = 3731200 + w
     st = model.struct()
     n = ctypes.c_size_t()
     check(lib().esc_kernel_precompile(ctypes.byref(st), ESC_C64 if precision == "c64" else ESC_C128, int(config),
