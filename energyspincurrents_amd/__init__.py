@@ -6,7 +6,7 @@ include/esc.h); this package is the Python binding of that ABI.
 from ._capi import EscError, lib  # noqa: F401
 from .grad import SpectrumLoss, spectrum_loss  # noqa: F401
 from .engine import (Batch, Chain, Engine, Honeycomb, LanczosResult, SpinModel, cf_spectrum, field111,  # noqa: F401
-                     lehmann_spectrum, load_params, precompile_kernel, reference_omega, tridiag_eig)
+                     kernel_plan, lehmann_spectrum, load_params, precompile_kernel, reference_omega, tridiag_eig)
 
 __all__ = ["SpectrumLoss", "spectrum_loss", "Batch", "Chain", "Engine", "Honeycomb", "SpinModel", "LanczosResult", "EscError", "lib", "cf_spectrum",
-           "lehmann_spectrum", "precompile_kernel", "tridiag_eig", "load_params", "field111", "reference_omega"]
+           "kernel_plan", "lehmann_spectrum", "precompile_kernel", "tridiag_eig", "load_params", "field111", "reference_omega"]

@@ -29,7 +29,7 @@ EXPORTS = [
     "esc_autotune", "esc_set_tile_bits", "esc_kernel_configs", "esc_ground_state",
     "esc_batch_create", "esc_batch_destroy", "esc_batch_ground_state", "esc_batch_state", "esc_batch_dynamics",
     "esc_batch_autotune", "esc_spectrum_grad", "esc_two_point", "esc_chain", "esc_chain_connectors",
-    "esc_rixs", "esc_szsz", "esc_exact_diag", "esc_kernel_precompile", "esc_dot_noalloc",
+    "esc_rixs", "esc_szsz", "esc_exact_diag", "esc_kernel_precompile", "esc_dot_noalloc", "esc_kernel_plan",
 ]
 
 
@@ -106,6 +106,7 @@ def lib():
         "esc_lanczos": ([P, P, I, D, I, P, P, P, P, P, P, P], I),
         "esc_exact_diag": ([P, P, P, P], I),
         "esc_dot_noalloc": ([P, P, P, P, P], I),
+        "esc_kernel_plan": ([P, I, I, P, I, P, P, P, I, P], I),
         "esc_kernel_precompile": ([P, I, I, P, ctypes.c_size_t, P], I),
         "esc_ritz_overlaps": ([P, P, I, P, P], I),
         "esc_ritz_vector": ([P, I, P, P], I),

@@ -1985,6 +1985,75 @@ int esc_kernel_precompile(const esc_model *model, int precision, int config, cha
     });
 }
 
+int esc_kernel_plan(const esc_model *model, int precision, int config, int64_t *passes, int cap_passes,
+                    int *n_passes, int32_t *term_pass, uint32_t *term_mask, int cap_terms, int *n_terms) {
+    return guarded([&] {
+        check_ptr(n_passes, "n_passes");
+        check_ptr(n_terms, "n_terms");
+        esc_ctx c;
+        ctx_init_host(&c, model, precision);
+        if (config <= 0) throw EscError(ESC_ERR_ARG, "configuration 0 (one row per lane) has no passes");
+        const std::vector<HostTerm> terms = host_terms(&c);
+        struct Row { uint32_t W, T; int nnear, nfar; size_t lds; bool diag; };
+        std::vector<Row> rows;
+        std::vector<int> owner(terms.size(), -1);
+        auto own = [&](uint32_t mask, int p) {
+            for (size_t t = 0; t < terms.size(); ++t)
+                if (terms[t].mask == mask) {
+                    if (owner[t] >= 0) throw EscError(ESC_ERR_NUMERIC, "a term sits in two passes");
+                    owner[t] = p;
+                    return;
+                }
+            throw EscError(ESC_ERR_NUMERIC, "a pass applies a term the model does not have");
+        };
+        if (cfg_cube(config)) {
+            esc::CubeModel m;
+            std::vector<esc::CubePass> cp;
+            std::string why;
+            if (!tile_config_fits(&c, config) || !cube_spec(&c, config, m, cp, &why))
+                throw EscError(ESC_ERR_UNSUPPORTED, "configuration does not apply to this model" +
+                                                        (why.empty() ? std::string() : ": " + why));
+            for (size_t p = 0; p < cp.size(); ++p) {
+                Row r{0u, 0u, 0, 0, ((size_t)1 << cp[p].tile_bits) * c.esz, cp[p].diag};
+                for (size_t i = 0; i < cp[p].wb.size(); ++i) {
+                    r.W |= 1u << cp[p].wb[i];
+                    if ((int)i < cp[p].tile_bits) r.T |= 1u << cp[p].wb[i];
+                }
+                for (const esc::CubeTerm &t : cp[p].terms) {
+                    ((t.mask & ~r.T) ? r.nfar : r.nnear)++;
+                    own(t.mask, (int)p);
+                }
+                rows.push_back(r);
+            }
+        } else {
+            const int TB = cfg_tile_bits(config), w = cfg_window(config), P = cfg_passes(config);
+            if (!tile_config_fits(&c, config)) throw EscError(ESC_ERR_UNSUPPORTED, "configuration does not fit");
+            const std::vector<Plan> plan = plan_windows(&c, terms, TB, w, P);
+            const std::vector<esc::GTables> g = build_passes(&c, TB, w, P, cfg_waves(config));
+            if (g.size() != plan.size()) throw EscError(ESC_ERR_NUMERIC, "plan and tables disagree");
+            for (size_t p = 0; p < plan.size(); ++p) {
+                Row r{plan[p].W, plan[p].T, (int)plan[p].near.size(), (int)plan[p].far.size(),
+                      esc::tile_lds_bytes(c.c64(), TB, (int)g[p].cls_begin[4], (int)g[p].cls_begin[esc::kNumClasses]),
+                      g[p].first_pass != 0};
+                for (size_t t : plan[p].near) own(terms[t].mask, (int)p);
+                for (size_t t : plan[p].far) own(terms[t].mask, (int)p);
+                rows.push_back(r);
+            }
+        }
+        *n_passes = (int)rows.size();
+        *n_terms = (int)terms.size();
+        for (int p = 0; p < (int)rows.size() && passes && p < cap_passes; ++p) {
+            int64_t *o = passes + 6 * (size_t)p;
+            o[0] = rows[p].W; o[1] = rows[p].T; o[2] = rows[p].nnear; o[3] = rows[p].nfar;
+            o[4] = (int64_t)rows[p].lds; o[5] = rows[p].diag ? 1 : 0;
+        }
+        for (int t = 0; t < (int)terms.size() && t < cap_terms; ++t) {
+            if (term_pass) term_pass[t] = owner[t];
+            if (term_mask) term_mask[t] = terms[t].mask;
+        }
+    });
+}
+
 int esc_ctx_destroy(esc_ctx *ctx) {
     return guarded([&] { delete ctx; });
 }

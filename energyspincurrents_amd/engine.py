@@ -480,6 +480,29 @@ def precompile_kernel(model: SpinModel, precision="c128", config=None):
     return buf.value.decode()
 
 
+def kernel_plan(model: SpinModel, precision="c128", config=None):
+    """The pass plan of an H.psi kernel configuration without a device
+    (esc_kernel_plan): a list of per-pass dicts (window, tile, near, far,
+    lds_bytes, diag) and, per off-diagonal term, (mask, pass)."""
+    if config is None:
+        N = model.lattice.N
+        w = N if N <= 18 else (17 if N <= 22 else 18)
+        config = ((4721200 if N <= 22 else 4731300) + w) if precision == "c64" else 3731200 + w
+    st = model.struct()
+    prec = ESC_C64 if precision == "c64" else ESC_C128
+    npass, nterm = ctypes.c_int(), ctypes.c_int()
+    check(lib().esc_kernel_plan(ctypes.byref(st), prec, int(config), None, 0, ctypes.byref(npass), None, None, 0,
+                                ctypes.byref(nterm)))
+    rows = np.zeros(6 * max(npass.value, 1), np.int64)
+    tp = np.zeros(max(nterm.value, 1), np.int32)
+    tm = np.zeros(max(nterm.value, 1), np.uint32)
+    check(lib().esc_kernel_plan(ctypes.byref(st), prec, int(config), _np_ptr(rows), npass.value,
+                                ctypes.byref(npass), _np_ptr(tp), _np_ptr(tm), nterm.value, ctypes.byref(nterm)))
+    keys = ("window", "tile", "near", "far", "lds_bytes", "diag")
+    passes = [dict(zip(keys, (int(x) for x in rows[6 * p:6 * p + 6]))) for p in range(npass.value)]
+    return passes, [(int(tm[t]), int(tp[t])) for t in range(nterm.value)]
+
+
 def tridiag_eig(alpha, beta, vectors=True):
     a = np.ascontiguousarray(np.real(alpha), np.float64)
     b = np.ascontiguousarray(np.abs(beta), np.float64)

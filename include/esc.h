@@ -128,6 +128,16 @@ int esc_kernel_configs(const esc_ctx *ctx, int32_t *configs, int32_t cap, int32_
  * characters; *len its full length.  No reference counterpart. */
 int esc_kernel_precompile(const esc_model *model, int precision, int config, char *source, size_t cap,
                           size_t *len);
+/* The pass plan of an H.psi kernel configuration (> 0) for a model, without
+ * a device (no reference counterpart; for tests and diagnostics).  Pass p <
+ * *n_passes: passes[6p + 0] = window sites (bit mask), + 1 = tile sites,
+ * + 2 = near terms (mask inside the tile), + 3 = far terms, + 4 = LDS bytes
+ * of one workgroup's tile, + 5 = 1 if the pass adds the diagonal.  Term t <
+ * *n_terms (the off-diagonal terms: bonds in family order, then flips):
+ * term_pass[t] = its pass (-1: none), term_mask[t] = its flip mask.  Arrays
+ * may be NULL; at most cap_passes / cap_terms entries are written. */
+int esc_kernel_plan(const esc_model *model, int precision, int config, int64_t *passes, int cap_passes,
+                    int *n_passes, int32_t *term_pass, uint32_t *term_mask, int cap_terms, int *n_terms);
 
 /* ---------------------------------------------------------------- kernels -- */
 /* d_out = H d_in.  Replaces Hamiltonian::MatrixVectorMult
