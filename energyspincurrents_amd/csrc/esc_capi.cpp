@@ -3,9 +3,7 @@
 // Host orchestration only: device tables, workspaces, the Lanczos loop and the
 // FP64 small solves.  All vector work runs in the HIP kernels of
 // esc_kernels.hip; nothing here falls back to a CPU path.
-#include "esc.h"
-#include "esc_cube.h"
-#include "esc_internal.h"
+#include "esc_ctx.h"
 
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
@@ -29,45 +27,18 @@
 #include <thread>
 #include <vector>
 
-namespace {
+using namespace esc_impl;
 
+namespace esc_impl {
 thread_local std::string g_err;
-
-}  // namespace
+}  // namespace esc_impl
 
 namespace esc {
-void set_error(const char *msg) { g_err = msg; }
-void clear_error() { g_err.clear(); }
+void set_error(const char *msg) { esc_impl::g_err = msg; }
+void clear_error() { esc_impl::g_err.clear(); }
 }  // namespace esc
 
-namespace {
-
-struct EscError : std::runtime_error {
-    int code;
-    EscError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
-};
-
-inline void hip_check(hipError_t e, const char *what) {
-    if (e != hipSuccess) throw EscError(ESC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
-}
-
-template <typename F>
-int guarded(F &&f) {
-    try {
-        f();
-        g_err.clear();
-        return ESC_OK;
-    } catch (const EscError &e) {
-        g_err = e.what();
-        return e.code;
-    } catch (const std::bad_alloc &) {
-        g_err = "host allocation failed";
-        return ESC_ERR_ALLOC;
-    } catch (const std::exception &e) {
-        g_err = e.what();
-        return ESC_ERR_NUMERIC;
-    }
-}
+namespace esc_impl {
 
 // ------------------------------------------------------------------------
 // FP64 symmetric tridiagonal eigensolver: implicit QL with Wilkinson-type
@@ -184,8 +155,9 @@ double tridiag_lowest(int n, const double *a, const double *b) {
     return 0.5 * (lo + hi);
 }
 
-}  // namespace
+}  // namespace esc_impl
 
+namespace esc_impl {
 // One line on stderr per finished stage of a long call (a continued fraction,
 // a ground-state cycle) when the environment sets ESC_VERBOSE to anything but
 // "0"; silent otherwise.
@@ -205,105 +177,10 @@ void progress(const char *fmt, ...) {
     std::fprintf(stderr, "esc: [%.1f ms] %s\n", ms, buf);
     std::fflush(stderr);
 }
+}  // namespace esc_impl
 
-// ------------------------------------------------------------------------
-// Context
-// ------------------------------------------------------------------------
-// dense diagonalisation (Solver=ED) up to 2^13 states: 1 GiB of matrix
-constexpr int kMaxEdSites = 13;
 
-struct esc_ctx {
-    int n_sites = 0;
-    int precision = ESC_C128;
-    uint64_t dim = 0;
-    size_t esz = 16;
-    int device = 0;
-    esc_model model{};
-    std::vector<int32_t> bonds[3];
-    std::vector<int32_t> neigh;
-    esc::HTables htab{};
-    esc::HTables *d_htab = nullptr;
-    std::map<int64_t, esc::CurrentTable *> d_cur;   // cached per (op, site)
-    double *d_partials = nullptr;                   // reduction workspace
-    size_t partials_cap = 0;
-    double *d_result = nullptr;                     // small device results
-    size_t result_cap = 0;
-    void *d_vecs = nullptr;                         // Lanczos basis / work vectors
-    size_t vecs_cap = 0;                            // in vectors
-    int basis_n = 0;                                // vectors valid in the kept basis
-    std::vector<double> Z;                          // last tridiagonal eigenvectors
-    std::vector<double> evals;                      // last Ritz values (ascending)
-    int z_n = 0;
-    bool basis_kept = false;
-    void *d_aux = nullptr;                          // dynamics work vectors
-    size_t aux_cap = 0;
-    esc::CurrentTable *d_combo = nullptr;           // combined-operator table
-    struct PassDev {
-        esc::GTables *d;
-        int n_uniform, n_terms;
-    };
-    std::map<int, std::vector<PassDev>> d_pass;     // windowed tiled kernel per configuration
-    mutable std::map<int, bool> tile_fits;           // configuration fits the LDS
-    int tile_bits = 0;                              // kernel configuration: 0 = one row per lane,
-                                                    // else 100 * tile bits + window bits
-    double *d_cube_cf = nullptr;                    // coupling scalars of the generated kernels
-    std::map<int, std::shared_ptr<esc::CubeProgram>> cube;   // generated kernels per configuration
-    mutable std::map<int, std::string> cube_fail;   // configurations whose generation failed, and why
-
-    bool c64() const { return precision == ESC_C64; }
-
-    ~esc_ctx() {
-        if (d_htab) (void)hipFree(d_htab);
-        for (auto &kv : d_cur) (void)hipFree(kv.second);
-        if (d_partials) (void)hipFree(d_partials);
-        if (d_result) (void)hipFree(d_result);
-        if (d_vecs) (void)hipFree(d_vecs);
-        if (d_aux) (void)hipFree(d_aux);
-        if (d_combo) (void)hipFree(d_combo);
-        if (d_cube_cf) (void)hipFree(d_cube_cf);
-        for (auto &kv : d_pass)
-            for (auto &p : kv.second) (void)hipFree(p.d);
-    }
-    void ensure_aux(size_t nvec) {
-        if (nvec <= aux_cap) return;
-        if (d_aux) hip_check(hipFree(d_aux), "hipFree");
-        d_aux = nullptr;
-        aux_cap = 0;
-        if (hipMalloc(&d_aux, nvec * dim * esz) != hipSuccess)
-            throw EscError(ESC_ERR_ALLOC, "cannot allocate " + std::to_string(nvec) + " work vectors on the device");
-        aux_cap = nvec;
-    }
-    void *aux(size_t m) { return (char *)d_aux + m * dim * esz; }
-
-    void ensure_partials(size_t n) {
-        if (n <= partials_cap) return;
-        if (d_partials) hip_check(hipFree(d_partials), "hipFree");
-        d_partials = nullptr;
-        if (hipMalloc(&d_partials, n * sizeof(double)) != hipSuccess) throw EscError(ESC_ERR_ALLOC, "workspace allocation failed");
-        partials_cap = n;
-    }
-    void ensure_result(size_t n) {
-        if (n <= result_cap) return;
-        if (d_result) hip_check(hipFree(d_result), "hipFree");
-        d_result = nullptr;
-        if (hipMalloc(&d_result, n * sizeof(double)) != hipSuccess) throw EscError(ESC_ERR_ALLOC, "workspace allocation failed");
-        result_cap = n;
-    }
-    void ensure_vecs(size_t nvec) {
-        if (nvec <= vecs_cap) return;
-        if (d_vecs) hip_check(hipFree(d_vecs), "hipFree");
-        d_vecs = nullptr;
-        vecs_cap = 0;
-        basis_kept = false;
-        if (hipMalloc(&d_vecs, nvec * dim * esz) != hipSuccess)
-            throw EscError(ESC_ERR_ALLOC, "cannot allocate " + std::to_string(nvec) + " vectors of " +
-                                              std::to_string(dim * esz) + " bytes on the device");
-        vecs_cap = nvec;
-    }
-    void *vec(size_t m) { return (char *)d_vecs + m * dim * esz; }
-};
-
-namespace {
+namespace esc_impl {
 
 void build_tables(esc_ctx *c) {
     const esc_model &m = c->model;
@@ -1184,7 +1061,7 @@ int default_tile_bits(esc_ctx *c) {
     return 0;
 }
 
-void launch_hpsi_with(esc_ctx *c, int config, const void *in, void *out, hipStream_t s, double *dot_part = nullptr) {
+void launch_hpsi_with(esc_ctx *c, int config, const void *in, void *out, hipStream_t s, double *dot_part) {
     if (config > 0 && cfg_cube(config)) {
         esc::CubeProgram &p = cube_for(c, config);
         hip_check(esc::cube_launch(p, c->d_cube_cf, in, out, dot_part, c->dim, s), "generated hpsi launch");
@@ -1496,7 +1373,6 @@ int plain_recursion(esc_ctx *c, const void *start, int steps, double eps, const 
 }
 
 // ---- spectra from continued fractions (shared by one context and batches) ----
-using cplx = std::complex<double>;
 
 // column pairs of the output tables (src/dynamics.h:318-328,427-435); index
 // nc is the total channel
@@ -1509,10 +1385,6 @@ std::vector<std::pair<int, int>> spectrum_pairs(bool js, int nc) {
 // One continued-fraction run: phi = (sum_a coef_a J_a)|gs>.  kind 0: a
 // diagonal channel (a), 1: the total channel, 2 / 3: the polarisation runs
 // J_a + J_b and J_a + i J_b of a cross channel (a, b).
-struct CfRun {
-    std::vector<cplx> coef;
-    int kind, a, b;
-};
 
 std::vector<CfRun> cf_runs(bool js, int nc) {
     std::vector<CfRun> r;
@@ -1909,14 +1781,14 @@ void rixs_spectra(esc_ctx *g, esc_ctx *q, int method, int site, const void *d_gs
     }
 }
 
-}  // namespace
+}  // namespace esc_impl
 
 extern "C" {
 
-const char *esc_last_error(void) { return g_err.c_str(); }
+const char *esc_last_error(void) { return esc_impl::g_err.c_str(); }
 int esc_version(void) { return 100; }
 
-namespace {
+namespace esc_impl {
 // validated copy of the model and its host tables; no device work
 void ctx_init_host(esc_ctx *c, const esc_model *model, int precision) {
         check_ptr(model, "model");
@@ -1945,7 +1817,7 @@ void ctx_init_host(esc_ctx *c, const esc_model *model, int precision) {
         c->model.neigh = nullptr;
         build_tables(c);
 }
-}  // namespace
+}  // namespace esc_impl
 
 int esc_ctx_create(const esc_model *model, int precision, esc_ctx **out) {
     return guarded([&] {
@@ -2841,785 +2713,6 @@ int esc_lehmann_spectrum(int n, const double *evals, const double *Oa, const dou
             out[2 * q] = acc.real();
             out[2 * q + 1] = acc.imag();
         }
-    });
-}
-
-}  // extern "C"
-
-// ------------------------------------------------------------------------
-// Batches of independent systems (different sizes and couplings) solved in
-// lockstep: every Lanczos step of all instances is one captured HIP graph
-// (the instances' H.psi launches plus four batched kernels for the scalars),
-// and the scalars never leave the device until the run ends.
-// ------------------------------------------------------------------------
-struct esc_batch {
-    int n = 0;
-    bool c64 = false;
-    size_t esz = 16;
-    std::vector<esc_ctx *> ctx;
-    std::vector<void *> mem;              // per instance: slots 0..2, start, gs, phi
-    std::vector<double *> dotbuf;         // per instance: tiled H.psi dot partials, or null
-    std::vector<esc::BatchLane> lanes[3]; // per rotation phase
-    esc::BatchLane *d_lanes[3] = {nullptr, nullptr, nullptr};
-    uint32_t total_blocks = 0;
-    double *d_partials = nullptr, *d_alpha = nullptr, *d_beta = nullptr, *d_coef = nullptr;
-    uint32_t *d_step = nullptr;
-    int cap = 0;                          // step capacity of alpha / beta / coef (graphs bake it)
-    hipStream_t s = nullptr;
-    hipGraph_t graph[2][3] = {};
-    hipGraphExec_t exec[2][3] = {};
-    std::vector<double> e0;
-    bool have_gs = false;
-
-    void *slot(int i, int q) const { return (char *)mem[i] + (size_t)q * ctx[i]->dim * esz; }
-    void drop_graphs() {
-        for (int a = 0; a < 2; ++a)
-            for (int p = 0; p < 3; ++p) {
-                if (exec[a][p]) (void)hipGraphExecDestroy(exec[a][p]);
-                if (graph[a][p]) (void)hipGraphDestroy(graph[a][p]);
-                exec[a][p] = nullptr;
-                graph[a][p] = nullptr;
-            }
-    }
-    ~esc_batch() {
-        if (s) (void)hipStreamSynchronize(s);
-        drop_graphs();
-        for (void *m : mem) if (m) (void)hipFree(m);
-        for (double *d : dotbuf) if (d) (void)hipFree(d);
-        for (auto *d : d_lanes) if (d) (void)hipFree(d);
-        for (void *p : {(void *)d_partials, (void *)d_alpha, (void *)d_beta, (void *)d_coef, (void *)d_step})
-            if (p) (void)hipFree(p);
-        for (esc_ctx *c : ctx) delete c;
-        if (s) (void)hipStreamDestroy(s);
-    }
-};
-
-namespace {
-
-void batch_check(const esc_batch *b) {
-    if (!b) throw EscError(ESC_ERR_ARG, "null batch");
-}
-
-void batch_capacity(esc_batch *b, int steps) {
-    if (steps <= b->cap) return;
-    b->drop_graphs();
-    for (double **p : {&b->d_alpha, &b->d_beta, &b->d_coef}) {
-        if (*p) hip_check(hipFree(*p), "hipFree");
-        *p = nullptr;
-    }
-    const size_t n = (size_t)b->n;
-    if (hipMalloc(&b->d_alpha, n * steps * 2 * sizeof(double)) != hipSuccess ||
-        hipMalloc(&b->d_beta, n * (steps + 1) * sizeof(double)) != hipSuccess ||
-        hipMalloc(&b->d_coef, n * steps * sizeof(double)) != hipSuccess)
-        throw EscError(ESC_ERR_ALLOC, "batch scalar allocation failed");
-    b->cap = steps;
-}
-
-// Instances with a tiled H.psi get their dot <v|w> from the kernel's per-tile
-// partials (one buffer each, sized for the current kernel choice), so the
-// batched dot pass skips them.  Called when the kernel choices are set
-// (creation, autotuning), with no graph holding the old lanes.
-void batch_setup_dots(esc_batch *b) {
-    b->dotbuf.resize(b->n, nullptr);
-    for (int i = 0; i < b->n; ++i) {
-        esc_ctx *c = b->ctx[i];
-        if (b->dotbuf[i]) hip_check(hipFree(b->dotbuf[i]), "hipFree");
-        b->dotbuf[i] = nullptr;
-        uint32_t ndot = 0;
-        if (c->tile_bits > 0) {
-            ndot = (uint32_t)(c->dim >> cfg_tile_bits(c->tile_bits));
-            if (hipMalloc(&b->dotbuf[i], 2 * (size_t)ndot * sizeof(double)) != hipSuccess)
-                throw EscError(ESC_ERR_ALLOC, "batch dot buffer allocation failed");
-        }
-        for (int p = 0; p < 3; ++p) {
-            b->lanes[p][i].dotp = b->dotbuf[i];
-            b->lanes[p][i].ndot = ndot;
-        }
-    }
-    for (int p = 0; p < 3; ++p)
-        hip_check(hipMemcpy(b->d_lanes[p], b->lanes[p].data(), sizeof(esc::BatchLane) * b->n, hipMemcpyHostToDevice),
-                  "hipMemcpy");
-}
-
-// the launches of step k (phase k % 3) for all instances
-void batch_step_launches(esc_batch *b, int phase, bool accum) {
-    const int lda = b->cap, ldb = b->cap + 1, ldc = b->cap;
-    if (accum)
-        hip_check(esc::launch_batch_step(b->c64, esc::kBatchAccum, b->d_lanes[phase], b->n, b->total_blocks,
-                                         b->d_partials, b->d_alpha, lda, b->d_beta, ldb, b->d_coef, ldc, b->d_step,
-                                         b->s), "batch accumulate");
-    for (int i = 0; i < b->n; ++i) {
-        const esc::BatchLane &l = b->lanes[phase][i];
-        if (l.ndot > 0) launch_hpsi_with(b->ctx[i], b->ctx[i]->tile_bits, l.v, l.w, b->s, b->dotbuf[i]);
-        else run_hpsi(b->ctx[i], l.v, l.w, b->s);
-    }
-    for (int piece : {esc::kBatchDot, esc::kBatchUpdate})
-        hip_check(esc::launch_batch_step(b->c64, piece, b->d_lanes[phase], b->n, b->total_blocks, b->d_partials,
-                                         b->d_alpha, lda, b->d_beta, ldb, b->d_coef, ldc, b->d_step, b->s),
-                  "batch step");
-}
-
-// `steps` lockstep Lanczos steps of every instance from the unit vectors in
-// slot 1 (slot 0 is cleared); with accum, acc += coef[i][k] v_k into gs.
-// Returns alpha (real parts, n x steps) and beta (n x (steps + 1)).
-void batch_run(esc_batch *b, int steps, bool use_graph, const std::vector<double> *coef, std::vector<double> &alpha,
-               std::vector<double> &beta) {
-    batch_capacity(b, steps);
-    const bool accum = coef != nullptr;
-    for (int i = 0; i < b->n; ++i) {
-        hip_check(hipMemsetAsync(b->slot(i, 0), 0, b->ctx[i]->dim * b->esz, b->s), "clear");
-        if (accum) hip_check(hipMemsetAsync(b->slot(i, 4), 0, b->ctx[i]->dim * b->esz, b->s), "clear");
-    }
-    hip_check(hipMemsetAsync(b->d_beta, 0, (size_t)b->n * (b->cap + 1) * sizeof(double), b->s), "clear");
-    hip_check(hipMemsetAsync(b->d_step, 0, sizeof(uint32_t), b->s), "clear");
-    if (accum) {
-        std::vector<double> c((size_t)b->n * b->cap, 0.0);
-        for (int i = 0; i < b->n; ++i)
-            for (int k = 0; k < steps; ++k) c[(size_t)i * b->cap + k] = (*coef)[(size_t)i * steps + k];
-        hip_check(hipMemcpyAsync(b->d_coef, c.data(), c.size() * sizeof(double), hipMemcpyHostToDevice, b->s), "coef");
-    }
-    for (int k = 0; k < steps; ++k) {
-        const int p = k % 3;
-        if (!use_graph) {
-            batch_step_launches(b, p, accum);
-            continue;
-        }
-        if (!b->exec[accum][p]) {
-            hip_check(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal), "begin capture");
-            try {
-                batch_step_launches(b, p, accum);
-            } catch (...) {
-                hipGraph_t g = nullptr;
-                (void)hipStreamEndCapture(b->s, &g);
-                if (g) (void)hipGraphDestroy(g);
-                throw;
-            }
-            hip_check(hipStreamEndCapture(b->s, &b->graph[accum][p]), "end capture");
-            hip_check(hipGraphInstantiate(&b->exec[accum][p], b->graph[accum][p], nullptr, nullptr, 0), "instantiate");
-        }
-        hip_check(hipGraphLaunch(b->exec[accum][p], b->s), "graph launch");
-    }
-    std::vector<double> a2((size_t)b->n * b->cap * 2), bt((size_t)b->n * (b->cap + 1));
-    hip_check(hipMemcpyAsync(a2.data(), b->d_alpha, a2.size() * sizeof(double), hipMemcpyDeviceToHost, b->s), "alpha");
-    hip_check(hipMemcpyAsync(bt.data(), b->d_beta, bt.size() * sizeof(double), hipMemcpyDeviceToHost, b->s), "beta");
-    hip_check(hipStreamSynchronize(b->s), "sync");
-    alpha.assign((size_t)b->n * steps, 0.0);
-    beta.assign((size_t)b->n * (steps + 1), 0.0);
-    for (int i = 0; i < b->n; ++i) {
-        for (int k = 0; k < steps; ++k) alpha[(size_t)i * steps + k] = a2[2 * ((size_t)i * b->cap + k)];
-        for (int k = 0; k <= steps; ++k) beta[(size_t)i * (steps + 1) + k] = bt[(size_t)i * (b->cap + 1) + k];
-    }
-}
-
-// length of instance i's recursion: up to and including the first step whose
-// beta_{k+1} came out zero (exhausted Krylov space)
-int batch_length(const std::vector<double> &beta, int i, int steps) {
-    for (int k = 0; k < steps; ++k)
-        if (beta[(size_t)i * (steps + 1) + k + 1] == 0.0) return k + 1;
-    return steps;
-}
-
-// copy x / |x| of instance i into its slot 1 (norm2 returned)
-double batch_load_unit(esc_batch *b, int i, const void *x) {
-    esc_ctx *c = b->ctx[i];
-    const double n2 = device_norm2(c, x, b->s);
-    void *dst = b->slot(i, 1);
-    hip_check(hipMemcpyAsync(dst, x, c->dim * b->esz, hipMemcpyDeviceToDevice, b->s), "copy");
-    if (n2 > 0.0 && std::isfinite(n2)) hip_check(esc::launch_scale(b->c64, dst, 1.0 / std::sqrt(n2), c->dim, b->s), "scale");
-    return n2;
-}
-
-}  // namespace
-
-extern "C" {
-
-int esc_batch_create(int n, const esc_model *models, int precision, esc_batch **out) {
-    return guarded([&] {
-        check_ptr(models, "models");
-        check_ptr(out, "out");
-        *out = nullptr;
-        if (n < 1 || n > 1024) throw EscError(ESC_ERR_ARG, "n must be in [1, 1024]");
-        auto b = std::make_unique<esc_batch>();
-        b->n = n;
-        b->c64 = precision == ESC_C64;
-        b->esz = b->c64 ? 8 : 16;
-        hip_check(hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking), "hipStreamCreate");
-        for (int i = 0; i < n; ++i) {
-            esc_ctx *c = nullptr;
-            const int rc = esc_ctx_create(&models[i], precision, &c);
-            if (rc != ESC_OK) throw EscError(rc, "instance " + std::to_string(i) + ": " + esc_last_error());
-            b->ctx.push_back(c);
-            void *m = nullptr;
-            if (hipMalloc(&m, 6 * c->dim * b->esz) != hipSuccess)
-                throw EscError(ESC_ERR_ALLOC, "instance " + std::to_string(i) + ": cannot allocate its 6 vectors");
-            b->mem.push_back(m);
-        }
-        uint32_t blk = 0;
-        for (int p = 0; p < 3; ++p) b->lanes[p].resize(n);
-        for (int i = 0; i < n; ++i) {
-            const uint64_t dim = b->ctx[i]->dim;
-            uint64_t nb = (dim + 16 * (uint64_t)256 - 1) / (16 * (uint64_t)256);
-            nb = std::max<uint64_t>(1, std::min<uint64_t>(nb, 1024));
-            for (int p = 0; p < 3; ++p) {
-                esc::BatchLane &l = b->lanes[p][i];
-                l.vp = b->slot(i, p);
-                l.v = b->slot(i, (p + 1) % 3);
-                l.w = b->slot(i, (p + 2) % 3);
-                l.acc = b->slot(i, 4);
-                l.dim = dim;
-                l.blk0 = blk;
-                l.nblk = (uint32_t)nb;
-                l.dotp = nullptr;
-                l.ndot = 0;
-            }
-            blk += (uint32_t)nb;
-        }
-        b->total_blocks = blk;
-        for (int p = 0; p < 3; ++p) {
-            if (hipMalloc(&b->d_lanes[p], sizeof(esc::BatchLane) * n) != hipSuccess)
-                throw EscError(ESC_ERR_ALLOC, "batch table allocation failed");
-            hip_check(hipMemcpy(b->d_lanes[p], b->lanes[p].data(), sizeof(esc::BatchLane) * n, hipMemcpyHostToDevice),
-                      "hipMemcpy");
-        }
-        if (hipMalloc(&b->d_partials, (size_t)blk * 2 * sizeof(double)) != hipSuccess ||
-            hipMalloc(&b->d_step, sizeof(uint32_t)) != hipSuccess)
-            throw EscError(ESC_ERR_ALLOC, "batch workspace allocation failed");
-        batch_setup_dots(b.get());
-        b->e0.assign(n, 0.0);
-        *out = b.release();
-    });
-}
-
-int esc_batch_destroy(esc_batch *b) {
-    return guarded([&] { delete b; });
-}
-
-int esc_batch_ground_state(esc_batch *b, uint64_t seed, int steps, int use_graph, double *e0, int *iters,
-                           double *alpha, double *beta) {
-    return guarded([&] {
-        batch_check(b);
-        if (steps < 2 || steps > 4096) throw EscError(ESC_ERR_ARG, "steps must be in [2, 4096]");
-        b->have_gs = false;
-        // start vectors (Gaussian, seed + i), kept in slot 3 for the replay;
-        // one eager H.psi per instance builds its kernel tables before capture
-        for (int i = 0; i < b->n; ++i) {
-            esc_ctx *c = b->ctx[i];
-            hip_check(esc::launch_gaussian(b->c64, seed + (uint64_t)i, b->slot(i, 3), c->dim, b->s), "gaussian");
-            batch_load_unit(b, i, b->slot(i, 3));
-            hip_check(hipMemcpyAsync(b->slot(i, 3), b->slot(i, 1), c->dim * b->esz, hipMemcpyDeviceToDevice, b->s), "copy");
-            run_hpsi(c, b->slot(i, 1), b->slot(i, 2), b->s);
-        }
-        std::vector<double> al, bt;
-        batch_run(b, steps, use_graph != 0, nullptr, al, bt);
-        std::vector<double> coef((size_t)b->n * steps, 0.0);
-        for (int i = 0; i < b->n; ++i) {
-            const int n = batch_length(bt, i, steps);
-            std::vector<double> ev(n), Z((size_t)n * n);
-            if (tridiag_ql(n, al.data() + (size_t)i * steps, bt.data() + (size_t)i * (steps + 1), ev.data(), Z.data()) != 0)
-                throw EscError(ESC_ERR_NUMERIC, "tridiagonal QL did not converge");
-            b->e0[i] = ev[0];
-            for (int k = 0; k < n; ++k) coef[(size_t)i * steps + k] = Z[k];
-            if (e0) e0[i] = ev[0];
-            if (iters) iters[i] = n;
-        }
-        if (alpha) std::memcpy(alpha, al.data(), al.size() * sizeof(double));
-        if (beta) std::memcpy(beta, bt.data(), bt.size() * sizeof(double));
-        // replay from the same starts, forming |gs> = sum_k Z(k,0) v_k in slot 4
-        for (int i = 0; i < b->n; ++i)
-            hip_check(hipMemcpyAsync(b->slot(i, 1), b->slot(i, 3), b->ctx[i]->dim * b->esz, hipMemcpyDeviceToDevice, b->s),
-                      "copy");
-        std::vector<double> al2, bt2;
-        batch_run(b, steps, use_graph != 0, &coef, al2, bt2);
-        for (int i = 0; i < b->n; ++i) {
-            const double n2 = device_norm2(b->ctx[i], b->slot(i, 4), b->s);
-            hip_check(esc::launch_scale(b->c64, b->slot(i, 4), 1.0 / std::sqrt(n2), b->ctx[i]->dim, b->s), "scale");
-        }
-        hip_check(hipStreamSynchronize(b->s), "sync");
-        b->have_gs = true;
-    });
-}
-
-int esc_batch_autotune(esc_batch *b, int reps, int *configs) {
-    return guarded([&] {
-        batch_check(b);
-        b->drop_graphs();   // the captured steps bake each instance's kernel choice
-        b->have_gs = false;
-        for (int i = 0; i < b->n; ++i) {
-            esc_ctx *c = b->ctx[i];
-            hip_check(esc::launch_gaussian(b->c64, 0x5eedULL + (uint64_t)i, b->slot(i, 1), c->dim, b->s), "gaussian");
-            int cfg = 0;
-            double ms = 0.0;
-            const int rc = esc_autotune(c, b->slot(i, 1), b->slot(i, 2), reps, &cfg, &ms, b->s);
-            if (rc != ESC_OK) throw EscError(rc, "instance " + std::to_string(i) + ": " + esc_last_error());
-            if (configs) configs[i] = cfg;
-        }
-        hip_check(hipStreamSynchronize(b->s), "sync");
-        batch_setup_dots(b);
-    });
-}
-
-int esc_batch_state(esc_batch *b, int i, void *d_out, void *stream) {
-    return guarded([&] {
-        batch_check(b);
-        check_ptr(d_out, "d_out");
-        if (i < 0 || i >= b->n) throw EscError(ESC_ERR_ARG, "instance out of range");
-        if (!b->have_gs) throw EscError(ESC_ERR_ARG, "no ground states yet (esc_batch_ground_state)");
-        hip_check(hipMemcpyAsync(d_out, b->slot(i, 4), b->ctx[i]->dim * b->esz, hipMemcpyDeviceToDevice,
-                                 (hipStream_t)stream), "copy");
-    });
-}
-
-int esc_batch_dynamics(esc_batch *b, int kind, int cf_steps, double eta, const double *omega, int nw, int use_graph,
-                       double *tables, double *expect) {
-    return guarded([&] {
-        batch_check(b);
-        check_ptr(omega, "omega");
-        check_ptr(tables, "tables");
-        if (!b->have_gs) throw EscError(ESC_ERR_ARG, "no ground states yet (esc_batch_ground_state)");
-        if (kind != ESC_DYN_JSJS && kind != ESC_DYN_JEJE) throw EscError(ESC_ERR_ARG, "unknown dynamics kind");
-        if (cf_steps < 2 || cf_steps > 4096) throw EscError(ESC_ERR_ARG, "cf_steps must be in [2, 4096]");
-        if (nw < 1) throw EscError(ESC_ERR_ARG, "nw must be >= 1");
-        const bool js = kind == ESC_DYN_JSJS;
-        const std::vector<int> ops = js ? std::vector<int>{ESC_OP_JSX, ESC_OP_JSY, ESC_OP_JSZ}
-                                        : std::vector<int>{ESC_OP_JEX, ESC_OP_JEY};
-        const int nc = (int)ops.size(), ncols = js ? 14 : 5;
-        const int n = b->n;
-        std::vector<std::vector<esc::CurrentTable>> tabs(n, std::vector<esc::CurrentTable>(nc));
-        for (int i = 0; i < n; ++i) {
-            for (int a = 0; a < nc; ++a) host_current_table(b->ctx[i], ops[a], ESC_SITE_TOTAL, &tabs[i][a]);
-            if (!b->ctx[i]->d_combo && hipMalloc(&b->ctx[i]->d_combo, sizeof(esc::CurrentTable)) != hipSuccess)
-                throw EscError(ESC_ERR_ALLOC, "table allocation failed");
-        }
-        // phi_i = (sum_a c_a J_a)|gs_i> into slot 5
-        auto apply = [&](int i, const std::vector<cplx> &coef) {
-            esc_ctx *c = b->ctx[i];
-            esc::CurrentTable t;
-            std::memset(&t, 0, sizeof(t));
-            for (int a = 0; a < nc; ++a)
-                if (coef[a] != cplx(0.0, 0.0)) append_scaled(t, tabs[i][a], coef[a].real(), coef[a].imag());
-            hip_check(hipMemcpyAsync(c->d_combo, &t, sizeof(t), hipMemcpyHostToDevice, b->s), "table");
-            hip_check(b->c64 ? esc::launch_current_c64(c->d_combo, b->slot(i, 4), b->slot(i, 5), c->dim, b->s)
-                             : esc::launch_current_c128(c->d_combo, b->slot(i, 4), b->slot(i, 5), c->dim, b->s),
-                      "current launch");
-            hip_check(hipStreamSynchronize(b->s), "sync");   // the host table is reused
-        };
-        std::vector<std::vector<cplx>> ex(n, std::vector<cplx>(nc + 1, cplx(0.0, 0.0)));
-        for (int i = 0; i < n; ++i)
-            for (int a = 0; a < nc; ++a) {
-                std::vector<cplx> c(nc, cplx(0.0, 0.0));
-                c[a] = 1.0;
-                apply(i, c);
-                double d[2];
-                device_dot(b->ctx[i], b->slot(i, 4), b->slot(i, 5), d, b->s);
-                ex[i][a] = cplx(d[0], d[1]);
-                ex[i][nc] += ex[i][a];
-            }
-        const std::vector<CfRun> runs = cf_runs(js, nc);
-        std::vector<std::vector<std::vector<cplx>>> f(n);
-        for (const CfRun &r : runs) {
-            std::vector<double> n2(n);
-            for (int i = 0; i < n; ++i) {
-                apply(i, r.coef);
-                n2[i] = batch_load_unit(b, i, b->slot(i, 5));
-            }
-            std::vector<double> al, bt;
-            batch_run(b, cf_steps, use_graph != 0, nullptr, al, bt);
-            for (int i = 0; i < n; ++i) {
-                const int it = n2[i] > 0.0 ? batch_length(bt, i, cf_steps) : 0;
-                f[i].push_back(cf_eval(it, al.data() + (size_t)i * cf_steps, bt.data() + (size_t)i * (cf_steps + 1),
-                                       n2[i], b->e0[i], eta, omega, nw));
-            }
-        }
-        for (int i = 0; i < n; ++i) {
-            auto F = cf_assemble(js, nc, runs, f[i], nw);
-            write_spectrum_table(js, nc, ex[i], F, omega, nw, eta, tables + (size_t)i * nw * (1 + ncols));
-            if (expect)
-                for (int a = 0; a <= nc; ++a) {
-                    expect[2 * ((size_t)i * 4 + a)] = ex[i][a].real();
-                    expect[2 * ((size_t)i * 4 + a) + 1] = ex[i][a].imag();
-                }
-        }
-    });
-}
-
-}  // extern "C"
-
-// ------------------------------------------------------------------------
-// Gradients of a spectrum loss with respect to (K, J, Gamma, h) by a
-// hand-written adjoint of the ground-state problem, the Lanczos recursion
-// from phi = O|gs> and the continued fraction, with the recursion vectors
-// checkpointed every C steps and recomputed segment by segment.
-// ------------------------------------------------------------------------
-namespace {
-
-struct DevVecs {
-    std::vector<void *> p;
-    size_t bytes = 0;
-    explicit DevVecs(size_t b) : bytes(b) {}
-    void *get() {
-        void *x = nullptr;
-        if (hipMalloc(&x, bytes) != hipSuccess) throw EscError(ESC_ERR_ALLOC, "gradient work vector allocation failed");
-        p.push_back(x);
-        return x;
-    }
-    ~DevVecs() {
-        for (void *x : p) (void)hipFree(x);
-    }
-};
-
-struct CtxPtr {
-    esc_ctx *c = nullptr;
-    ~CtxPtr() { delete c; }
-};
-
-void make_ctx(const esc_model &m, CtxPtr &out) {
-    const int rc = esc_ctx_create(&m, ESC_C128, &out.c);
-    if (rc != ESC_OK) throw EscError(rc, esc_last_error());
-}
-
-// Loss L = sum_q (Im G(w_q) - target_q)^2 of the continued fraction
-// G(w) = n2 g_0, g_k = 1 / (z - a_k - b_{k+1}^2 g_{k+1}), z = w + e0 - i eta,
-// and its derivatives with respect to a_k, b_k, n2 and e0 (holomorphic
-// chain rule per frequency).
-double cf_loss_adjoint(int M, const double *a, const double *b, double n2, double e0, double eta, const double *omega,
-                       int nw, const double *target, double *spectrum, std::vector<double> &abar,
-                       std::vector<double> &bbar, double &n2bar, double &e0bar) {
-    abar.assign(M, 0.0);
-    bbar.assign(M + 1, 0.0);
-    n2bar = 0.0;
-    e0bar = 0.0;
-    double L = 0.0;
-    std::vector<cplx> g(M);
-    for (int q = 0; q < nw; ++q) {
-        const cplx z(omega[q] + e0, -eta);
-        for (int k = M - 1; k >= 0; --k) {
-            const cplx tail = (k + 1 < M) ? b[k + 1] * b[k + 1] * g[k + 1] : cplx(0.0, 0.0);
-            g[k] = 1.0 / (z - a[k] - tail);
-        }
-        const cplx G = n2 * g[0];
-        const double S = G.imag();
-        if (spectrum) spectrum[q] = S;
-        const double r = S - target[q];
-        L += r * r;
-        const double w = 2.0 * r;
-        cplx c = n2, dz(0.0, 0.0);
-        for (int k = 0; k < M; ++k) {
-            const cplx gk2 = g[k] * g[k];
-            abar[k] += w * (c * gk2).imag();
-            dz -= c * gk2;
-            if (k + 1 < M) {
-                bbar[k + 1] += w * (c * 2.0 * b[k + 1] * g[k + 1] * gk2).imag();
-                c = c * b[k + 1] * b[k + 1] * gk2;
-            }
-        }
-        n2bar += w * g[0].imag();
-        e0bar += w * dz.imag();
-    }
-    return L;
-}
-
-}  // namespace
-
-extern "C" {
-
-int esc_spectrum_grad(const esc_model *model, int op, int site, int gs_steps, int cf_steps, int checkpoint,
-                      double eta, const double *omega, int nw, const double *target, double *loss, double *spectrum,
-                      double *grad, double *e0_out, double *de0, void *stream) {
-    return guarded([&] {
-        check_ptr(model, "model");
-        check_ptr(omega, "omega");
-        check_ptr(target, "target");
-        check_ptr(grad, "grad");
-        if (nw < 1) throw EscError(ESC_ERR_ARG, "nw must be >= 1");
-        if (gs_steps < 2 || gs_steps > 4096 || cf_steps < 2 || cf_steps > 4096)
-            throw EscError(ESC_ERR_ARG, "steps must be in [2, 4096]");
-        if (checkpoint < 0) throw EscError(ESC_ERR_ARG, "checkpoint interval must be >= 0");
-        if (!(eta > 0.0)) throw EscError(ESC_ERR_ARG, "eta must be > 0");
-        // operator O and the order of its couplings in K (currents: Js ~ K, Je ~ K^2)
-        int kdeg = 0;
-        if (op == ESC_OP_JSX || op == ESC_OP_JSY || op == ESC_OP_JSZ) kdeg = 1;
-        else if (op == ESC_OP_JEX || op == ESC_OP_JEY) kdeg = 2;
-        else if (op < ESC_OP_SX || op > ESC_OP_SZ) throw EscError(ESC_ERR_ARG, "unknown operator");
-        if (kdeg == 0 && (site < 0 || site >= model->n_sites)) throw EscError(ESC_ERR_ARG, "site out of range");
-        const int osite = kdeg ? ESC_SITE_TOTAL : site;
-        hipStream_t s = (hipStream_t)stream;
-
-        // H and its parts dH/dK (along (1,1,1) in Kx,Ky,Kz), dH/dJ, dH/dGamma, dH/dh ([111] field)
-        CtxPtr H, P[4], Oplus, Ominus;
-        make_ctx(*model, H);
-        for (int p = 0; p < 4; ++p) {
-            esc_model m = *model;
-            m.K[0] = m.K[1] = m.K[2] = (p == 0) ? 1.0 : 0.0;
-            m.J = (p == 1) ? 1.0 : 0.0;
-            m.Gamma = (p == 2) ? 1.0 : 0.0;
-            const double hb = (p == 3) ? 1.0 / std::sqrt(3.0) : 0.0;
-            m.B[0] = m.B[1] = m.B[2] = hb;
-            make_ctx(m, P[p]);
-        }
-        if (kdeg) {   // dO/dK = (O(K + 1) - O(K - 1)) / 2, exact for couplings of degree <= 2
-            esc_model mp = *model, mm = *model;
-            for (int f = 0; f < 3; ++f) { mp.K[f] += 1.0; mm.K[f] -= 1.0; }
-            make_ctx(mp, Oplus);
-            make_ctx(mm, Ominus);
-        }
-        esc_ctx *c = H.c;
-        const uint64_t dim = c->dim;
-        DevVecs V(dim * c->esz);
-        auto apply = [&](esc_ctx *cx, const void *x, void *y) {
-            esc::CurrentTable *t = current_table(cx, op, osite);
-            hip_check(esc::launch_current_c128(t, x, y, dim, s), "operator");
-        };
-        auto axpby = [&](cplx a, const void *x, cplx b, void *y) {
-            hip_check(esc::launch_axpby(false, a.real(), a.imag(), x, b.real(), b.imag(), y, dim, s), "axpby");
-        };
-        auto dot = [&](const void *x, const void *y) {
-            double d[2];
-            device_dot(c, x, y, d, s);
-            return cplx(d[0], d[1]);
-        };
-
-        // ground state (FP64, replayed recursion with one restart) and E0
-        void *v0 = V.get(), *gs = V.get();
-        hip_check(esc::launch_gaussian(false, 0x9e3779b97f4a7c15ULL, v0, dim, s), "gaussian");
-        double e0 = 0.0, resid = 0.0;
-        int it = 0;
-        if (esc_ground_state(c, v0, gs_steps, 1e-14, 1, &e0, &resid, &it, gs, stream) != ESC_OK)
-            throw EscError(ESC_ERR_NUMERIC, std::string("ground state: ") + esc_last_error());
-        progress("gradient: ground state, %d H.psi", it);
-        void *t0 = V.get(), *t1 = V.get();
-        void *cg_r = V.get(), *cg_p = V.get(), *cg_ap = V.get();
-        // x = (H - e)^+ b on the complement of gs (b orthogonal to gs), by
-        // conjugate gradients; returns the relative residual reached
-        auto cg = [&](double e, const void *b, void *x, double tol) {
-            hip_check(hipMemsetAsync(x, 0, dim * c->esz, s), "clear");
-            hip_check(hipMemcpyAsync(cg_r, b, dim * c->esz, hipMemcpyDeviceToDevice, s), "copy");
-            hip_check(hipMemcpyAsync(cg_p, b, dim * c->esz, hipMemcpyDeviceToDevice, s), "copy");
-            double rr = device_norm2(c, cg_r, s);
-            const double bn = std::sqrt(rr);
-            if (!(bn > 0.0)) return 0.0;
-            int iter = 0;
-            for (; iter < 5000 && std::sqrt(rr) > tol * bn; ++iter) {
-                run_hpsi(c, cg_p, cg_ap, s);
-                axpby(-e, cg_p, 1.0, cg_ap);
-                axpby(-dot(gs, cg_ap), gs, 1.0, cg_ap);
-                const double pAp = dot(cg_p, cg_ap).real();
-                if (!(pAp > 0.0)) break;   // no positive curvature left: the solve is as good as it gets
-                const double a = rr / pAp;
-                axpby(a, cg_p, 1.0, x);
-                axpby(-a, cg_ap, 1.0, cg_r);
-                const double rr2 = device_norm2(c, cg_r, s);
-                axpby(1.0, cg_r, rr2 / rr, cg_p);
-                rr = rr2;
-            }
-            progress("conjugate gradients: %d iterations, relative residual %.2e", iter, std::sqrt(rr) / bn);
-            return std::sqrt(rr) / bn;
-        };
-        // residual correction of the ground state (Newton steps for the
-        // eigenvector): r = (H - e0) gs, gs -= (H - e0)^+ r, renormalise
-        for (int pass = 0; pass < 4; ++pass) {
-            run_hpsi(c, gs, t0, s);
-            e0 = dot(gs, t0).real();
-            axpby(-e0, gs, 1.0, t0);
-            resid = std::sqrt(device_norm2(c, t0, s));
-            progress("gradient: Newton pass %d, E0 %.15f, residual %.3e", pass, e0, resid);
-            if (resid < 1e-13 * std::max(1.0, std::fabs(e0))) break;
-            cg(e0, t0, t1, 1e-12);
-            axpby(-1.0, t1, 1.0, gs);
-            hip_check(esc::launch_scale(false, gs, 1.0 / std::sqrt(device_norm2(c, gs, s)), dim, s), "scale");
-        }
-        run_hpsi(c, gs, t0, s);
-        e0 = dot(gs, t0).real();
-        if (e0_out) *e0_out = e0;
-        // Hellmann-Feynman: dE0/dtheta_p = <gs|H_p|gs>
-        double hf[4];
-        for (int p = 0; p < 4; ++p) {
-            run_hpsi(P[p].c, gs, t0, s);
-            hf[p] = dot(gs, t0).real();
-            if (de0) de0[p] = hf[p];
-        }
-        // phi = O|gs>, v_0 = phi / |phi|
-        void *phi = V.get();
-        apply(c, gs, phi);
-        const double n2 = device_norm2(c, phi, s);
-        if (!(n2 > 0.0)) throw EscError(ESC_ERR_NUMERIC, "O|gs> vanishes");
-        const double nphi = std::sqrt(n2);
-
-        // forward recursion.  checkpoint > 0: three rotating vectors, the pair
-        // (v_{k-1}, v_k) saved every C steps and segments recomputed in the
-        // reverse sweep.  checkpoint == 0: every vector kept (in the context's
-        // Lanczos storage) and each new one reorthogonalised against all
-        // before it, so the floating-point recursion stays the exact-arithmetic
-        // one the adjoint differentiates (reorthogonalisation changes nothing
-        // in exact arithmetic, so the adjoint formulas are the same)
-        const bool full = checkpoint == 0;
-        const int C = full ? cf_steps : checkpoint;
-        if (full) {
-            c->ensure_vecs((size_t)cf_steps + 1);
-            c->basis_kept = false;
-        }
-        c->ensure_partials(2 * esc::kRedBlocks);
-        c->ensure_result(8);
-        auto step = [&](const void *vp, const void *v, void *w, double bk, double *ak, double *bk1) {
-            run_hpsi(c, v, w, s);
-            double a[2];
-            hip_check(esc::launch_dot(false, v, w, dim, c->d_partials, c->d_result, s), "dot");
-            sync_get(s, a, c->d_result, 2 * sizeof(double));
-            hip_check(esc::launch_lanczos_update(false, w, v, vp, c->d_result, bk, dim, c->d_partials, c->d_result + 2, s),
-                      "update");
-            double b2;
-            sync_get(s, &b2, c->d_result + 2, sizeof(double));
-            *ak = a[0];
-            *bk1 = std::sqrt(b2);
-            if (!std::isfinite(*ak) || !std::isfinite(*bk1)) throw EscError(ESC_ERR_NUMERIC, "non-finite recursion");
-            if (*bk1 > 1e-13 * std::max(1.0, std::fabs(a[0])))
-                hip_check(esc::launch_scale(false, w, 1.0 / *bk1, dim, s), "scale");
-        };
-        std::vector<double> al, be(1, 0.0);
-        std::vector<void *> ckp;        // ckp[2j] = v_{jC-1} (or zero), ckp[2j+1] = v_{jC}
-        void *zero = V.get();
-        hip_check(hipMemsetAsync(zero, 0, dim * c->esz, s), "clear");
-        void *rot[3] = {full ? zero : V.get(), full ? c->vec(0) : V.get(), full ? nullptr : V.get()};
-        hip_check(hipMemcpyAsync(rot[1], phi, dim * c->esz, hipMemcpyDeviceToDevice, s), "copy");
-        hip_check(esc::launch_scale(false, rot[1], 1.0 / nphi, dim, s), "scale");
-        if (!full) hip_check(hipMemsetAsync(rot[0], 0, dim * c->esz, s), "clear");
-        int M = 0;
-        for (int k = 0; k < cf_steps; ++k) {
-            void *vp = full ? (k > 0 ? c->vec((size_t)k - 1) : zero) : rot[k % 3];
-            void *v = full ? c->vec((size_t)k) : rot[(k + 1) % 3];
-            void *w = full ? c->vec((size_t)k + 1) : rot[(k + 2) % 3];
-            if (!full && k % C == 0) {
-                void *a = V.get(), *b = V.get();
-                hip_check(hipMemcpyAsync(a, vp, dim * c->esz, hipMemcpyDeviceToDevice, s), "checkpoint");
-                hip_check(hipMemcpyAsync(b, v, dim * c->esz, hipMemcpyDeviceToDevice, s), "checkpoint");
-                ckp.push_back(a);
-                ckp.push_back(b);
-            }
-            double ak, bk1;
-            step(k > 0 ? vp : zero, v, w, be[k], &ak, &bk1);
-            if (full && k > 0 && bk1 > 1e-13 * std::max(1.0, std::fabs(ak))) {
-                cgs_pass(c, k + 1, w, s);
-                cgs_pass(c, k + 1, w, s);
-                hip_check(esc::launch_norm2(false, w, dim, c->d_partials, c->d_result + 4, s), "norm2");
-                hip_check(esc::launch_normalize(false, w, c->d_result + 4, dim, s), "normalize");
-            }
-            al.push_back(ak);
-            be.push_back(bk1);
-            M = k + 1;
-            if (!(bk1 > 1e-13 * std::max(1.0, std::fabs(ak)))) break;   // Krylov space exhausted
-        }
-        progress("gradient: forward recursion, %d steps", M);
-        // loss and its adjoints with respect to alpha, beta, n2, e0
-        std::vector<double> abar, bbar;
-        double n2bar = 0.0, e0bar = 0.0;
-        const double L = cf_loss_adjoint(M, al.data(), be.data(), n2, e0, eta, omega, nw, target, spectrum, abar, bbar,
-                                         n2bar, e0bar);
-        if (loss) *loss = L;
-        double g[4] = {0.0, 0.0, 0.0, 0.0};
-        // the sweep's Re<u_bar|H_p v_k> go to device slots without a host
-        // round trip and are added to g afterwards, in the sweep's order
-        DevVecs GD(sizeof(double) * 8 * (size_t)std::max(M, 1));
-        double *gdot = static_cast<double *>(GD.get());
-        size_t ngdot = 0;
-        c->ensure_partials(2 * esc::kRedBlocks);
-
-        // reverse sweep over the recursion, segment by segment
-        std::vector<void *> seg(C + 2);
-        if (!full)
-            for (auto &x : seg) x = V.get();
-        void *vbar_next = V.get(), *vbar_k = V.get(), *vbar_prev = V.get(), *wbar = V.get(), *ubar = V.get();
-        hip_check(hipMemsetAsync(vbar_next, 0, dim * c->esz, s), "clear");
-        hip_check(hipMemsetAsync(vbar_k, 0, dim * c->esz, s), "clear");
-        const int nseg = (M + C - 1) / C;
-        for (int j = nseg - 1; j >= 0; --j) {
-            const int k0 = j * C, k1 = std::min(M, k0 + C);
-            // seg[0] = v_{k0-1}, seg[1 + i] = v_{k0 + i}, i = 0..k1-k0
-            if (full) {   // all vectors are resident
-                seg[0] = zero;
-                for (int i = 0; i <= k1 - k0; ++i) seg[1 + i] = c->vec((size_t)(k0 + i));
-            } else {
-                hip_check(hipMemcpyAsync(seg[0], ckp[2 * j], dim * c->esz, hipMemcpyDeviceToDevice, s), "restore");
-                hip_check(hipMemcpyAsync(seg[1], ckp[2 * j + 1], dim * c->esz, hipMemcpyDeviceToDevice, s), "restore");
-                for (int k = k0; k < k1; ++k) {
-                    double ak, bk1;
-                    step(k > 0 ? seg[k - k0] : zero, seg[k - k0 + 1], seg[k - k0 + 2], be[k], &ak, &bk1);
-                    if (ak != al[k] || bk1 != be[k + 1])
-                        throw EscError(ESC_ERR_NUMERIC, "recomputed recursion differs");
-                }
-            }
-            for (int k = k1 - 1; k >= k0; --k) {
-                const void *vkm1 = k > 0 ? seg[k - k0] : zero, *vk = seg[k - k0 + 1], *vk1 = seg[k - k0 + 2];
-                // w_bar = v_bar_{k+1} / b + (b_bar_{k+1} - Re<v_{k+1}|v_bar_{k+1}>/b) v_{k+1}
-                hip_check(hipMemsetAsync(wbar, 0, dim * c->esz, s), "clear");
-                const double b = be[k + 1];
-                if (k + 1 < M && b > 0.0) {
-                    const double pr = dot(vk1, vbar_next).real();
-                    axpby(1.0 / b, vbar_next, 0.0, wbar);
-                    axpby(bbar[k + 1] - pr / b, vk1, 1.0, wbar);
-                }
-                const double atot = abar[k] - dot(wbar, vk).real();
-                // u_bar = w_bar + a_tot v_k
-                hip_check(hipMemcpyAsync(ubar, wbar, dim * c->esz, hipMemcpyDeviceToDevice, s), "copy");
-                axpby(atot, vk, 1.0, ubar);
-                // v_bar_k += -a_k w_bar + a_tot H v_k + H u_bar
-                run_hpsi(c, vk, t0, s);
-                axpby(-al[k], wbar, 1.0, vbar_k);
-                axpby(atot, t0, 1.0, vbar_k);
-                run_hpsi(c, ubar, t1, s);
-                axpby(1.0, t1, 1.0, vbar_k);
-                // beta_k and v_{k-1}
-                hip_check(hipMemsetAsync(vbar_prev, 0, dim * c->esz, s), "clear");
-                if (k > 0) {
-                    bbar[k] += -dot(wbar, vkm1).real();
-                    axpby(-be[k], wbar, 0.0, vbar_prev);
-                }
-                // theta_bar_p += Re<u_bar|H_p v_k>
-                for (int p = 0; p < 4; ++p) {
-                    run_hpsi(P[p].c, vk, t0, s);
-                    hip_check(esc::launch_dot(false, ubar, t0, dim, c->d_partials, gdot + 2 * ngdot++, s), "dot");
-                }
-                std::swap(vbar_next, vbar_k);   // v_bar_k is complete
-                std::swap(vbar_k, vbar_prev);
-            }
-        }
-        {
-            std::vector<double> h(2 * ngdot);
-            if (ngdot) sync_get(s, h.data(), gdot, h.size() * sizeof(double));
-            for (size_t q = 0; q < ngdot; ++q) g[q % 4] += h[2 * q];
-        }
-        progress("gradient: reverse sweep");
-        // vbar_next holds v_bar_0: phi_bar = (v_bar_0 - v_0 Re<v_0|v_bar_0>) / |phi| + 2 n2_bar phi
-        void *v0u = seg[1];   // the last segment restored was the first: v_0
-        void *phibar = V.get();
-        const double pr0 = dot(v0u, vbar_next).real();
-        hip_check(hipMemsetAsync(phibar, 0, dim * c->esz, s), "clear");
-        axpby(1.0 / nphi, vbar_next, 0.0, phibar);
-        axpby(-pr0 / nphi, v0u, 1.0, phibar);
-        axpby(2.0 * n2bar, phi, 1.0, phibar);
-        // O's own dependence on K
-        if (kdeg) {
-            apply(Oplus.c, gs, t0);
-            apply(Ominus.c, gs, t1);
-            axpby(-1.0, t1, 1.0, t0);
-            g[0] += 0.5 * dot(phibar, t0).real();
-        }
-        // E0 through z: e0_bar dE0/dtheta_p (Hellmann-Feynman)
-        for (int p = 0; p < 4; ++p) g[p] += e0bar * hf[p];
-        // the ground state: gs_bar = O phi_bar; mu = (H - E0)^+ P gs_bar by CG on
-        // the complement of gs; theta_bar_p -= Re<mu|H_p gs>
-        void *gsbar = V.get();
-        apply(c, phibar, gsbar);
-        const cplx pg = dot(gs, gsbar);
-        axpby(-pg, gs, 1.0, gsbar);
-        void *mu = V.get();
-        if (cg(e0, gsbar, mu, 1e-11) > 1e-8)
-            throw EscError(ESC_ERR_NUMERIC, "CG with H - E0 did not converge (ground state not separated by a gap?)");
-        for (int p = 0; p < 4; ++p) {
-            run_hpsi(P[p].c, gs, t0, s);
-            g[p] -= dot(mu, t0).real();
-        }
-        hip_check(hipStreamSynchronize(s), "sync");
-        progress("gradient: done");
-        for (int p = 0; p < 4; ++p) grad[p] = g[p];
     });
 }
 

@@ -1,0 +1,198 @@
+// esc_ctx.h -- the device context behind the C ABI (esc_ctx) and the host
+// helpers its source files share: esc_capi.cpp (contexts, kernels, Lanczos,
+// spectra, small solves), esc_batch.cpp (lockstep batches) and esc_grad.cpp
+// (spectrum gradients).  Internal to the library.
+#pragma once
+#include "esc.h"
+#include "esc_cube.h"
+#include "esc_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace esc_impl {
+
+using cplx = std::complex<double>;
+
+// last error text of the calling thread (esc_last_error)
+extern thread_local std::string g_err;
+
+struct EscError : std::runtime_error {
+    int code;
+    EscError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+inline void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw EscError(ESC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename F>
+int guarded(F &&f) {
+    try {
+        f();
+        g_err.clear();
+        return ESC_OK;
+    } catch (const EscError &e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_err = "host allocation failed";
+        return ESC_ERR_ALLOC;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return ESC_ERR_NUMERIC;
+    }
+}
+
+// one stderr line per finished stage when ESC_VERBOSE is set
+void progress(const char *fmt, ...);
+
+// dense diagonalisation (Solver=ED) up to 2^13 states: 1 GiB of matrix
+constexpr int kMaxEdSites = 13;
+
+}  // namespace esc_impl
+
+// ------------------------------------------------------------------------
+// Context
+// ------------------------------------------------------------------------
+struct esc_ctx {
+    int n_sites = 0;
+    int precision = ESC_C128;
+    uint64_t dim = 0;
+    size_t esz = 16;
+    int device = 0;
+    esc_model model{};
+    std::vector<int32_t> bonds[3];
+    std::vector<int32_t> neigh;
+    esc::HTables htab{};
+    esc::HTables *d_htab = nullptr;
+    std::map<int64_t, esc::CurrentTable *> d_cur;   // cached per (op, site)
+    double *d_partials = nullptr;                   // reduction workspace
+    size_t partials_cap = 0;
+    double *d_result = nullptr;                     // small device results
+    size_t result_cap = 0;
+    void *d_vecs = nullptr;                         // Lanczos basis / work vectors
+    size_t vecs_cap = 0;                            // in vectors
+    int basis_n = 0;                                // vectors valid in the kept basis
+    std::vector<double> Z;                          // last tridiagonal eigenvectors
+    std::vector<double> evals;                      // last Ritz values (ascending)
+    int z_n = 0;
+    bool basis_kept = false;
+    void *d_aux = nullptr;                          // dynamics work vectors
+    size_t aux_cap = 0;
+    esc::CurrentTable *d_combo = nullptr;           // combined-operator table
+    struct PassDev {
+        esc::GTables *d;
+        int n_uniform, n_terms;
+    };
+    std::map<int, std::vector<PassDev>> d_pass;     // windowed tiled kernel per configuration
+    mutable std::map<int, bool> tile_fits;           // configuration fits the LDS
+    int tile_bits = 0;                              // kernel configuration: 0 = one row per lane,
+                                                    // else 100 * tile bits + window bits
+    double *d_cube_cf = nullptr;                    // coupling scalars of the generated kernels
+    std::map<int, std::shared_ptr<esc::CubeProgram>> cube;   // generated kernels per configuration
+    mutable std::map<int, std::string> cube_fail;   // configurations whose generation failed, and why
+
+    bool c64() const { return precision == ESC_C64; }
+
+    ~esc_ctx() {
+        if (d_htab) (void)hipFree(d_htab);
+        for (auto &kv : d_cur) (void)hipFree(kv.second);
+        if (d_partials) (void)hipFree(d_partials);
+        if (d_result) (void)hipFree(d_result);
+        if (d_vecs) (void)hipFree(d_vecs);
+        if (d_aux) (void)hipFree(d_aux);
+        if (d_combo) (void)hipFree(d_combo);
+        if (d_cube_cf) (void)hipFree(d_cube_cf);
+        for (auto &kv : d_pass)
+            for (auto &p : kv.second) (void)hipFree(p.d);
+    }
+    void ensure_aux(size_t nvec) {
+        if (nvec <= aux_cap) return;
+        if (d_aux) esc_impl::hip_check(hipFree(d_aux), "hipFree");
+        d_aux = nullptr;
+        aux_cap = 0;
+        if (hipMalloc(&d_aux, nvec * dim * esz) != hipSuccess)
+            throw esc_impl::EscError(ESC_ERR_ALLOC, "cannot allocate " + std::to_string(nvec) + " work vectors on the device");
+        aux_cap = nvec;
+    }
+    void *aux(size_t m) { return (char *)d_aux + m * dim * esz; }
+
+    void ensure_partials(size_t n) {
+        if (n <= partials_cap) return;
+        if (d_partials) esc_impl::hip_check(hipFree(d_partials), "hipFree");
+        d_partials = nullptr;
+        if (hipMalloc(&d_partials, n * sizeof(double)) != hipSuccess) throw esc_impl::EscError(ESC_ERR_ALLOC, "workspace allocation failed");
+        partials_cap = n;
+    }
+    void ensure_result(size_t n) {
+        if (n <= result_cap) return;
+        if (d_result) esc_impl::hip_check(hipFree(d_result), "hipFree");
+        d_result = nullptr;
+        if (hipMalloc(&d_result, n * sizeof(double)) != hipSuccess) throw esc_impl::EscError(ESC_ERR_ALLOC, "workspace allocation failed");
+        result_cap = n;
+    }
+    void ensure_vecs(size_t nvec) {
+        if (nvec <= vecs_cap) return;
+        if (d_vecs) esc_impl::hip_check(hipFree(d_vecs), "hipFree");
+        d_vecs = nullptr;
+        vecs_cap = 0;
+        basis_kept = false;
+        if (hipMalloc(&d_vecs, nvec * dim * esz) != hipSuccess)
+            throw esc_impl::EscError(ESC_ERR_ALLOC, "cannot allocate " + std::to_string(nvec) + " vectors of " +
+                                              std::to_string(dim * esz) + " bytes on the device");
+        vecs_cap = nvec;
+    }
+    void *vec(size_t m) { return (char *)d_vecs + m * dim * esz; }
+};
+
+namespace esc_impl {
+
+// argument and workspace helpers
+void check_ctx(const esc_ctx *c);
+void check_ptr(const void *p, const char *name);
+void sync_get(hipStream_t s, void *dst, const void *src, size_t n);
+void device_dot(esc_ctx *c, const void *x, const void *y, double out[2], hipStream_t s);
+double device_norm2(esc_ctx *c, const void *x, hipStream_t s);
+
+// H.psi with a given kernel configuration / the context's choice
+int cfg_tile_bits(int config);
+void launch_hpsi_with(esc_ctx *c, int config, const void *in, void *out, hipStream_t s, double *dot_part = nullptr);
+void run_hpsi(esc_ctx *c, const void *in, void *out, hipStream_t s);
+
+// one classical Gram-Schmidt pass of w against the first k kept vectors
+void cgs_pass(esc_ctx *c, int k, void *w, hipStream_t s);
+
+// operator tables (Pauli strings)
+void host_current_table(const esc_ctx *c, int op, int site, esc::CurrentTable *t);
+esc::CurrentTable *current_table(esc_ctx *c, int op, int site);
+void append_scaled(esc::CurrentTable &dst, const esc::CurrentTable &src, double cr, double ci);
+
+// FP64 small solves
+int tridiag_ql(int n, const double *a, const double *b, double *evals, double *Z);
+double tridiag_lowest(int n, const double *a, const double *b);
+
+// spectra from continued fractions: one run per channel / polarisation pair
+struct CfRun {
+    std::vector<cplx> coef;
+    int kind, a, b;
+};
+std::vector<std::pair<int, int>> spectrum_pairs(bool js, int nc);
+std::vector<CfRun> cf_runs(bool js, int nc);
+std::vector<cplx> cf_eval(int it, const double *al, const double *be, double n2, double e0, double eta,
+                          const double *omega, int nw);
+std::map<std::pair<int, int>, std::vector<cplx>> cf_assemble(bool js, int nc, const std::vector<CfRun> &runs,
+                                                             const std::vector<std::vector<cplx>> &f, int nw);
+void write_spectrum_table(bool js, int nc, const std::vector<cplx> &ex,
+                          std::map<std::pair<int, int>, std::vector<cplx>> &F, const double *omega, int nw,
+                          double eta, double *table);
+
+}  // namespace esc_impl

@@ -1,0 +1,402 @@
+// esc_grad.cpp -- gradients of a continued-fraction spectrum loss with
+// respect to (K, J, Gamma, h) by a hand-written adjoint of the Lanczos
+// recursion (esc_spectrum_grad).
+#include "esc_ctx.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <random>
+#include <set>
+#include <string>
+#include <vector>
+
+using namespace esc_impl;
+
+// ------------------------------------------------------------------------
+// Gradients of a spectrum loss with respect to (K, J, Gamma, h) by a
+// hand-written adjoint of the ground-state problem, the Lanczos recursion
+// from phi = O|gs> and the continued fraction, with the recursion vectors
+// checkpointed every C steps and recomputed segment by segment.
+// ------------------------------------------------------------------------
+namespace {
+
+struct DevVecs {
+    std::vector<void *> p;
+    size_t bytes = 0;
+    explicit DevVecs(size_t b) : bytes(b) {}
+    void *get() {
+        void *x = nullptr;
+        if (hipMalloc(&x, bytes) != hipSuccess) throw EscError(ESC_ERR_ALLOC, "gradient work vector allocation failed");
+        p.push_back(x);
+        return x;
+    }
+    ~DevVecs() {
+        for (void *x : p) (void)hipFree(x);
+    }
+};
+
+struct CtxPtr {
+    esc_ctx *c = nullptr;
+    ~CtxPtr() { delete c; }
+};
+
+void make_ctx(const esc_model &m, CtxPtr &out) {
+    const int rc = esc_ctx_create(&m, ESC_C128, &out.c);
+    if (rc != ESC_OK) throw EscError(rc, esc_last_error());
+}
+
+// Loss L = sum_q (Im G(w_q) - target_q)^2 of the continued fraction
+// G(w) = n2 g_0, g_k = 1 / (z - a_k - b_{k+1}^2 g_{k+1}), z = w + e0 - i eta,
+// and its derivatives with respect to a_k, b_k, n2 and e0 (holomorphic
+// chain rule per frequency).
+double cf_loss_adjoint(int M, const double *a, const double *b, double n2, double e0, double eta, const double *omega,
+                       int nw, const double *target, double *spectrum, std::vector<double> &abar,
+                       std::vector<double> &bbar, double &n2bar, double &e0bar) {
+    abar.assign(M, 0.0);
+    bbar.assign(M + 1, 0.0);
+    n2bar = 0.0;
+    e0bar = 0.0;
+    double L = 0.0;
+    std::vector<cplx> g(M);
+    for (int q = 0; q < nw; ++q) {
+        const cplx z(omega[q] + e0, -eta);
+        for (int k = M - 1; k >= 0; --k) {
+            const cplx tail = (k + 1 < M) ? b[k + 1] * b[k + 1] * g[k + 1] : cplx(0.0, 0.0);
+            g[k] = 1.0 / (z - a[k] - tail);
+        }
+        const cplx G = n2 * g[0];
+        const double S = G.imag();
+        if (spectrum) spectrum[q] = S;
+        const double r = S - target[q];
+        L += r * r;
+        const double w = 2.0 * r;
+        cplx c = n2, dz(0.0, 0.0);
+        for (int k = 0; k < M; ++k) {
+            const cplx gk2 = g[k] * g[k];
+            abar[k] += w * (c * gk2).imag();
+            dz -= c * gk2;
+            if (k + 1 < M) {
+                bbar[k + 1] += w * (c * 2.0 * b[k + 1] * g[k + 1] * gk2).imag();
+                c = c * b[k + 1] * b[k + 1] * gk2;
+            }
+        }
+        n2bar += w * g[0].imag();
+        e0bar += w * dz.imag();
+    }
+    return L;
+}
+
+}  // namespace
+
+extern "C" {
+
+int esc_spectrum_grad(const esc_model *model, int op, int site, int gs_steps, int cf_steps, int checkpoint,
+                      double eta, const double *omega, int nw, const double *target, double *loss, double *spectrum,
+                      double *grad, double *e0_out, double *de0, void *stream) {
+    return guarded([&] {
+        check_ptr(model, "model");
+        check_ptr(omega, "omega");
+        check_ptr(target, "target");
+        check_ptr(grad, "grad");
+        if (nw < 1) throw EscError(ESC_ERR_ARG, "nw must be >= 1");
+        if (gs_steps < 2 || gs_steps > 4096 || cf_steps < 2 || cf_steps > 4096)
+            throw EscError(ESC_ERR_ARG, "steps must be in [2, 4096]");
+        if (checkpoint < 0) throw EscError(ESC_ERR_ARG, "checkpoint interval must be >= 0");
+        if (!(eta > 0.0)) throw EscError(ESC_ERR_ARG, "eta must be > 0");
+        // operator O and the order of its couplings in K (currents: Js ~ K, Je ~ K^2)
+        int kdeg = 0;
+        if (op == ESC_OP_JSX || op == ESC_OP_JSY || op == ESC_OP_JSZ) kdeg = 1;
+        else if (op == ESC_OP_JEX || op == ESC_OP_JEY) kdeg = 2;
+        else if (op < ESC_OP_SX || op > ESC_OP_SZ) throw EscError(ESC_ERR_ARG, "unknown operator");
+        if (kdeg == 0 && (site < 0 || site >= model->n_sites)) throw EscError(ESC_ERR_ARG, "site out of range");
+        const int osite = kdeg ? ESC_SITE_TOTAL : site;
+        hipStream_t s = (hipStream_t)stream;
+
+        // H and its parts dH/dK (along (1,1,1) in Kx,Ky,Kz), dH/dJ, dH/dGamma, dH/dh ([111] field)
+        CtxPtr H, P[4], Oplus, Ominus;
+        make_ctx(*model, H);
+        for (int p = 0; p < 4; ++p) {
+            esc_model m = *model;
+            m.K[0] = m.K[1] = m.K[2] = (p == 0) ? 1.0 : 0.0;
+            m.J = (p == 1) ? 1.0 : 0.0;
+            m.Gamma = (p == 2) ? 1.0 : 0.0;
+            const double hb = (p == 3) ? 1.0 / std::sqrt(3.0) : 0.0;
+            m.B[0] = m.B[1] = m.B[2] = hb;
+            make_ctx(m, P[p]);
+        }
+        if (kdeg) {   // dO/dK = (O(K + 1) - O(K - 1)) / 2, exact for couplings of degree <= 2
+            esc_model mp = *model, mm = *model;
+            for (int f = 0; f < 3; ++f) { mp.K[f] += 1.0; mm.K[f] -= 1.0; }
+            make_ctx(mp, Oplus);
+            make_ctx(mm, Ominus);
+        }
+        esc_ctx *c = H.c;
+        const uint64_t dim = c->dim;
+        DevVecs V(dim * c->esz);
+        auto apply = [&](esc_ctx *cx, const void *x, void *y) {
+            esc::CurrentTable *t = current_table(cx, op, osite);
+            hip_check(esc::launch_current_c128(t, x, y, dim, s), "operator");
+        };
+        auto axpby = [&](cplx a, const void *x, cplx b, void *y) {
+            hip_check(esc::launch_axpby(false, a.real(), a.imag(), x, b.real(), b.imag(), y, dim, s), "axpby");
+        };
+        auto dot = [&](const void *x, const void *y) {
+            double d[2];
+            device_dot(c, x, y, d, s);
+            return cplx(d[0], d[1]);
+        };
+
+        // ground state (FP64, replayed recursion with one restart) and E0
+        void *v0 = V.get(), *gs = V.get();
+        hip_check(esc::launch_gaussian(false, 0x9e3779b97f4a7c15ULL, v0, dim, s), "gaussian");
+        double e0 = 0.0, resid = 0.0;
+        int it = 0;
+        if (esc_ground_state(c, v0, gs_steps, 1e-14, 1, &e0, &resid, &it, gs, stream) != ESC_OK)
+            throw EscError(ESC_ERR_NUMERIC, std::string("ground state: ") + esc_last_error());
+        progress("gradient: ground state, %d H.psi", it);
+        void *t0 = V.get(), *t1 = V.get();
+        void *cg_r = V.get(), *cg_p = V.get(), *cg_ap = V.get();
+        // x = (H - e)^+ b on the complement of gs (b orthogonal to gs), by
+        // conjugate gradients; returns the relative residual reached
+        auto cg = [&](double e, const void *b, void *x, double tol) {
+            hip_check(hipMemsetAsync(x, 0, dim * c->esz, s), "clear");
+            hip_check(hipMemcpyAsync(cg_r, b, dim * c->esz, hipMemcpyDeviceToDevice, s), "copy");
+            hip_check(hipMemcpyAsync(cg_p, b, dim * c->esz, hipMemcpyDeviceToDevice, s), "copy");
+            double rr = device_norm2(c, cg_r, s);
+            const double bn = std::sqrt(rr);
+            if (!(bn > 0.0)) return 0.0;
+            int iter = 0;
+            for (; iter < 5000 && std::sqrt(rr) > tol * bn; ++iter) {
+                run_hpsi(c, cg_p, cg_ap, s);
+                axpby(-e, cg_p, 1.0, cg_ap);
+                axpby(-dot(gs, cg_ap), gs, 1.0, cg_ap);
+                const double pAp = dot(cg_p, cg_ap).real();
+                if (!(pAp > 0.0)) break;   // no positive curvature left: the solve is as good as it gets
+                const double a = rr / pAp;
+                axpby(a, cg_p, 1.0, x);
+                axpby(-a, cg_ap, 1.0, cg_r);
+                const double rr2 = device_norm2(c, cg_r, s);
+                axpby(1.0, cg_r, rr2 / rr, cg_p);
+                rr = rr2;
+            }
+            progress("conjugate gradients: %d iterations, relative residual %.2e", iter, std::sqrt(rr) / bn);
+            return std::sqrt(rr) / bn;
+        };
+        // residual correction of the ground state (Newton steps for the
+        // eigenvector): r = (H - e0) gs, gs -= (H - e0)^+ r, renormalise
+        for (int pass = 0; pass < 4; ++pass) {
+            run_hpsi(c, gs, t0, s);
+            e0 = dot(gs, t0).real();
+            axpby(-e0, gs, 1.0, t0);
+            resid = std::sqrt(device_norm2(c, t0, s));
+            progress("gradient: Newton pass %d, E0 %.15f, residual %.3e", pass, e0, resid);
+            if (resid < 1e-13 * std::max(1.0, std::fabs(e0))) break;
+            cg(e0, t0, t1, 1e-12);
+            axpby(-1.0, t1, 1.0, gs);
+            hip_check(esc::launch_scale(false, gs, 1.0 / std::sqrt(device_norm2(c, gs, s)), dim, s), "scale");
+        }
+        run_hpsi(c, gs, t0, s);
+        e0 = dot(gs, t0).real();
+        if (e0_out) *e0_out = e0;
+        // Hellmann-Feynman: dE0/dtheta_p = <gs|H_p|gs>
+        double hf[4];
+        for (int p = 0; p < 4; ++p) {
+            run_hpsi(P[p].c, gs, t0, s);
+            hf[p] = dot(gs, t0).real();
+            if (de0) de0[p] = hf[p];
+        }
+        // phi = O|gs>, v_0 = phi / |phi|
+        void *phi = V.get();
+        apply(c, gs, phi);
+        const double n2 = device_norm2(c, phi, s);
+        if (!(n2 > 0.0)) throw EscError(ESC_ERR_NUMERIC, "O|gs> vanishes");
+        const double nphi = std::sqrt(n2);
+
+        // forward recursion.  checkpoint > 0: three rotating vectors, the pair
+        // (v_{k-1}, v_k) saved every C steps and segments recomputed in the
+        // reverse sweep.  checkpoint == 0: every vector kept (in the context's
+        // Lanczos storage) and each new one reorthogonalised against all
+        // before it, so the floating-point recursion stays the exact-arithmetic
+        // one the adjoint differentiates (reorthogonalisation changes nothing
+        // in exact arithmetic, so the adjoint formulas are the same)
+        const bool full = checkpoint == 0;
+        const int C = full ? cf_steps : checkpoint;
+        if (full) {
+            c->ensure_vecs((size_t)cf_steps + 1);
+            c->basis_kept = false;
+        }
+        c->ensure_partials(2 * esc::kRedBlocks);
+        c->ensure_result(8);
+        auto step = [&](const void *vp, const void *v, void *w, double bk, double *ak, double *bk1) {
+            run_hpsi(c, v, w, s);
+            double a[2];
+            hip_check(esc::launch_dot(false, v, w, dim, c->d_partials, c->d_result, s), "dot");
+            sync_get(s, a, c->d_result, 2 * sizeof(double));
+            hip_check(esc::launch_lanczos_update(false, w, v, vp, c->d_result, bk, dim, c->d_partials, c->d_result + 2, s),
+                      "update");
+            double b2;
+            sync_get(s, &b2, c->d_result + 2, sizeof(double));
+            *ak = a[0];
+            *bk1 = std::sqrt(b2);
+            if (!std::isfinite(*ak) || !std::isfinite(*bk1)) throw EscError(ESC_ERR_NUMERIC, "non-finite recursion");
+            if (*bk1 > 1e-13 * std::max(1.0, std::fabs(a[0])))
+                hip_check(esc::launch_scale(false, w, 1.0 / *bk1, dim, s), "scale");
+        };
+        std::vector<double> al, be(1, 0.0);
+        std::vector<void *> ckp;        // ckp[2j] = v_{jC-1} (or zero), ckp[2j+1] = v_{jC}
+        void *zero = V.get();
+        hip_check(hipMemsetAsync(zero, 0, dim * c->esz, s), "clear");
+        void *rot[3] = {full ? zero : V.get(), full ? c->vec(0) : V.get(), full ? nullptr : V.get()};
+        hip_check(hipMemcpyAsync(rot[1], phi, dim * c->esz, hipMemcpyDeviceToDevice, s), "copy");
+        hip_check(esc::launch_scale(false, rot[1], 1.0 / nphi, dim, s), "scale");
+        if (!full) hip_check(hipMemsetAsync(rot[0], 0, dim * c->esz, s), "clear");
+        int M = 0;
+        for (int k = 0; k < cf_steps; ++k) {
+            void *vp = full ? (k > 0 ? c->vec((size_t)k - 1) : zero) : rot[k % 3];
+            void *v = full ? c->vec((size_t)k) : rot[(k + 1) % 3];
+            void *w = full ? c->vec((size_t)k + 1) : rot[(k + 2) % 3];
+            if (!full && k % C == 0) {
+                void *a = V.get(), *b = V.get();
+                hip_check(hipMemcpyAsync(a, vp, dim * c->esz, hipMemcpyDeviceToDevice, s), "checkpoint");
+                hip_check(hipMemcpyAsync(b, v, dim * c->esz, hipMemcpyDeviceToDevice, s), "checkpoint");
+                ckp.push_back(a);
+                ckp.push_back(b);
+            }
+            double ak, bk1;
+            step(k > 0 ? vp : zero, v, w, be[k], &ak, &bk1);
+            if (full && k > 0 && bk1 > 1e-13 * std::max(1.0, std::fabs(ak))) {
+                cgs_pass(c, k + 1, w, s);
+                cgs_pass(c, k + 1, w, s);
+                hip_check(esc::launch_norm2(false, w, dim, c->d_partials, c->d_result + 4, s), "norm2");
+                hip_check(esc::launch_normalize(false, w, c->d_result + 4, dim, s), "normalize");
+            }
+            al.push_back(ak);
+            be.push_back(bk1);
+            M = k + 1;
+            if (!(bk1 > 1e-13 * std::max(1.0, std::fabs(ak)))) break;   // Krylov space exhausted
+        }
+        progress("gradient: forward recursion, %d steps", M);
+        // loss and its adjoints with respect to alpha, beta, n2, e0
+        std::vector<double> abar, bbar;
+        double n2bar = 0.0, e0bar = 0.0;
+        const double L = cf_loss_adjoint(M, al.data(), be.data(), n2, e0, eta, omega, nw, target, spectrum, abar, bbar,
+                                         n2bar, e0bar);
+        if (loss) *loss = L;
+        double g[4] = {0.0, 0.0, 0.0, 0.0};
+        // the sweep's Re<u_bar|H_p v_k> go to device slots without a host
+        // round trip and are added to g afterwards, in the sweep's order
+        DevVecs GD(sizeof(double) * 8 * (size_t)std::max(M, 1));
+        double *gdot = static_cast<double *>(GD.get());
+        size_t ngdot = 0;
+        c->ensure_partials(2 * esc::kRedBlocks);
+
+        // reverse sweep over the recursion, segment by segment
+        std::vector<void *> seg(C + 2);
+        if (!full)
+            for (auto &x : seg) x = V.get();
+        void *vbar_next = V.get(), *vbar_k = V.get(), *vbar_prev = V.get(), *wbar = V.get(), *ubar = V.get();
+        hip_check(hipMemsetAsync(vbar_next, 0, dim * c->esz, s), "clear");
+        hip_check(hipMemsetAsync(vbar_k, 0, dim * c->esz, s), "clear");
+        const int nseg = (M + C - 1) / C;
+        for (int j = nseg - 1; j >= 0; --j) {
+            const int k0 = j * C, k1 = std::min(M, k0 + C);
+            // seg[0] = v_{k0-1}, seg[1 + i] = v_{k0 + i}, i = 0..k1-k0
+            if (full) {   // all vectors are resident
+                seg[0] = zero;
+                for (int i = 0; i <= k1 - k0; ++i) seg[1 + i] = c->vec((size_t)(k0 + i));
+            } else {
+                hip_check(hipMemcpyAsync(seg[0], ckp[2 * j], dim * c->esz, hipMemcpyDeviceToDevice, s), "restore");
+                hip_check(hipMemcpyAsync(seg[1], ckp[2 * j + 1], dim * c->esz, hipMemcpyDeviceToDevice, s), "restore");
+                for (int k = k0; k < k1; ++k) {
+                    double ak, bk1;
+                    step(k > 0 ? seg[k - k0] : zero, seg[k - k0 + 1], seg[k - k0 + 2], be[k], &ak, &bk1);
+                    if (ak != al[k] || bk1 != be[k + 1])
+                        throw EscError(ESC_ERR_NUMERIC, "recomputed recursion differs");
+                }
+            }
+            for (int k = k1 - 1; k >= k0; --k) {
+                const void *vkm1 = k > 0 ? seg[k - k0] : zero, *vk = seg[k - k0 + 1], *vk1 = seg[k - k0 + 2];
+                // w_bar = v_bar_{k+1} / b + (b_bar_{k+1} - Re<v_{k+1}|v_bar_{k+1}>/b) v_{k+1}
+                hip_check(hipMemsetAsync(wbar, 0, dim * c->esz, s), "clear");
+                const double b = be[k + 1];
+                if (k + 1 < M && b > 0.0) {
+                    const double pr = dot(vk1, vbar_next).real();
+                    axpby(1.0 / b, vbar_next, 0.0, wbar);
+                    axpby(bbar[k + 1] - pr / b, vk1, 1.0, wbar);
+                }
+                const double atot = abar[k] - dot(wbar, vk).real();
+                // u_bar = w_bar + a_tot v_k
+                hip_check(hipMemcpyAsync(ubar, wbar, dim * c->esz, hipMemcpyDeviceToDevice, s), "copy");
+                axpby(atot, vk, 1.0, ubar);
+                // v_bar_k += -a_k w_bar + a_tot H v_k + H u_bar
+                run_hpsi(c, vk, t0, s);
+                axpby(-al[k], wbar, 1.0, vbar_k);
+                axpby(atot, t0, 1.0, vbar_k);
+                run_hpsi(c, ubar, t1, s);
+                axpby(1.0, t1, 1.0, vbar_k);
+                // beta_k and v_{k-1}
+                hip_check(hipMemsetAsync(vbar_prev, 0, dim * c->esz, s), "clear");
+                if (k > 0) {
+                    bbar[k] += -dot(wbar, vkm1).real();
+                    axpby(-be[k], wbar, 0.0, vbar_prev);
+                }
+                // theta_bar_p += Re<u_bar|H_p v_k>
+                for (int p = 0; p < 4; ++p) {
+                    run_hpsi(P[p].c, vk, t0, s);
+                    hip_check(esc::launch_dot(false, ubar, t0, dim, c->d_partials, gdot + 2 * ngdot++, s), "dot");
+                }
+                std::swap(vbar_next, vbar_k);   // v_bar_k is complete
+                std::swap(vbar_k, vbar_prev);
+            }
+        }
+        {
+            std::vector<double> h(2 * ngdot);
+            if (ngdot) sync_get(s, h.data(), gdot, h.size() * sizeof(double));
+            for (size_t q = 0; q < ngdot; ++q) g[q % 4] += h[2 * q];
+        }
+        progress("gradient: reverse sweep");
+        // vbar_next holds v_bar_0: phi_bar = (v_bar_0 - v_0 Re<v_0|v_bar_0>) / |phi| + 2 n2_bar phi
+        void *v0u = seg[1];   // the last segment restored was the first: v_0
+        void *phibar = V.get();
+        const double pr0 = dot(v0u, vbar_next).real();
+        hip_check(hipMemsetAsync(phibar, 0, dim * c->esz, s), "clear");
+        axpby(1.0 / nphi, vbar_next, 0.0, phibar);
+        axpby(-pr0 / nphi, v0u, 1.0, phibar);
+        axpby(2.0 * n2bar, phi, 1.0, phibar);
+        // O's own dependence on K
+        if (kdeg) {
+            apply(Oplus.c, gs, t0);
+            apply(Ominus.c, gs, t1);
+            axpby(-1.0, t1, 1.0, t0);
+            g[0] += 0.5 * dot(phibar, t0).real();
+        }
+        // E0 through z: e0_bar dE0/dtheta_p (Hellmann-Feynman)
+        for (int p = 0; p < 4; ++p) g[p] += e0bar * hf[p];
+        // the ground state: gs_bar = O phi_bar; mu = (H - E0)^+ P gs_bar by CG on
+        // the complement of gs; theta_bar_p -= Re<mu|H_p gs>
+        void *gsbar = V.get();
+        apply(c, phibar, gsbar);
+        const cplx pg = dot(gs, gsbar);
+        axpby(-pg, gs, 1.0, gsbar);
+        void *mu = V.get();
+        if (cg(e0, gsbar, mu, 1e-11) > 1e-8)
+            throw EscError(ESC_ERR_NUMERIC, "CG with H - E0 did not converge (ground state not separated by a gap?)");
+        for (int p = 0; p < 4; ++p) {
+            run_hpsi(P[p].c, gs, t0, s);
+            g[p] -= dot(mu, t0).real();
+        }
+        hip_check(hipStreamSynchronize(s), "sync");
+        progress("gradient: done");
+        for (int p = 0; p < 4; ++p) grad[p] = g[p];
+    });
+}
+
+}  // extern "C"
+
