@@ -10,6 +10,7 @@ Usage: python -m energyspincurrents_amd.build  [--no-cli] [--no-oracle]
 """
 from __future__ import annotations
 
+import math
 import os
 import subprocess
 import sys
@@ -75,8 +76,12 @@ def warm_jit_cache():
     sys.path.insert(0, ROOT)
     import energyspincurrents_amd as esc
     kjgh = dict(K=(-1.0, -1.0, -1.0), J=0.1, Gamma=0.2, B=esc.field111(0.2))
+    # the batched scan's term structure: K = cos(phi), J = sin(phi), field
+    # along [111] (no Gamma; the diagonal summed bond by bond)
+    scan = dict(K=(math.cos(1.0),) * 3, J=math.sin(1.0), B=esc.field111(0.3))
     jobs = [((3, 8), kjgh, "c128"), ((3, 8), kjgh, "c64"), ((2, 10), kjgh, "c128"), ((2, 10), kjgh, "c64"),
-            ((4, 8), dict(K=(-1.0, -1.0, -1.0), B=esc.field111(0.3)), "c64")]
+            ((4, 8), dict(K=(-1.0, -1.0, -1.0), B=esc.field111(0.3)), "c64"), ((2, 10), scan, "c64"),
+            ((3, 8), scan, "c64")]
     for (lx, ly), kw, prec in jobs:
         esc.precompile_kernel(esc.SpinModel(esc.Honeycomb(lx, ly), **kw), prec)
         print(f"+ generated H.psi kernels: {lx * ly} sites, {prec}", flush=True)

@@ -1,6 +1,7 @@
 // esc_batch.cpp -- many independent systems solved in lockstep (esc_batch_*):
 // one Lanczos step of all instances per replayed HIP graph.
 #include "esc_ctx.h"
+#include "esc_tridiag.h"
 
 #include <hip/hip_runtime.h>
 
@@ -42,6 +43,24 @@ struct esc_batch {
     hipGraphExec_t exec[2][3] = {};
     std::vector<double> e0;
     bool have_gs = false;
+    // device workspaces of the solves, grown on demand: the tridiagonal
+    // solves, energies, lengths, kept alpha / beta rows, norms, start-vector
+    // norms, expectation values, fractions, omegas and operator tables
+    struct DevBuf {
+        void *p = nullptr;
+        size_t bytes = 0;
+        void *get(size_t n) {
+            if (n > bytes) {
+                if (p) hip_check(hipFree(p), "hipFree");
+                p = nullptr;
+                bytes = 0;
+                if (hipMalloc(&p, n) != hipSuccess) throw EscError(ESC_ERR_ALLOC, "batch workspace allocation failed");
+                bytes = n;
+            }
+            return p;
+        }
+    };
+    DevBuf work, e0_dev, iters_dev, keep_alpha, keep_beta, norms, n2_dev, ex_dev, f_dev, omega_dev, tabs_dev;
 
     void *slot(int i, int q) const { return (char *)mem[i] + (size_t)q * ctx[i]->dim * esz; }
     void drop_graphs() {
@@ -61,6 +80,9 @@ struct esc_batch {
         for (auto *d : d_lanes) if (d) (void)hipFree(d);
         for (void *p : {(void *)d_partials, (void *)d_alpha, (void *)d_beta, (void *)d_coef, (void *)d_step})
             if (p) (void)hipFree(p);
+        for (DevBuf *d : {&work, &e0_dev, &iters_dev, &keep_alpha, &keep_beta, &norms, &n2_dev, &ex_dev, &f_dev,
+                          &omega_dev, &tabs_dev})
+            if (d->p) (void)hipFree(d->p);
         for (esc_ctx *c : ctx) delete c;
         if (s) (void)hipStreamDestroy(s);
     }
@@ -132,24 +154,17 @@ void batch_step_launches(esc_batch *b, int phase, bool accum) {
 }
 
 // `steps` lockstep Lanczos steps of every instance from the unit vectors in
-// slot 1 (slot 0 is cleared); with accum, acc += coef[i][k] v_k into gs.
-// Returns alpha (real parts, n x steps) and beta (n x (steps + 1)).
-void batch_run(esc_batch *b, int steps, bool use_graph, const std::vector<double> *coef, std::vector<double> &alpha,
-               std::vector<double> &beta) {
+// slot 1 (slot 0 is cleared); with accum, acc += coef[i][k] v_k into gs, the
+// coefficients read from d_coef (row i at i * cap).  alpha and beta stay in
+// d_alpha / d_beta: nothing waits for the device.
+void batch_run(esc_batch *b, int steps, bool use_graph, bool accum) {
     batch_capacity(b, steps);
-    const bool accum = coef != nullptr;
     for (int i = 0; i < b->n; ++i) {
         hip_check(hipMemsetAsync(b->slot(i, 0), 0, b->ctx[i]->dim * b->esz, b->s), "clear");
         if (accum) hip_check(hipMemsetAsync(b->slot(i, 4), 0, b->ctx[i]->dim * b->esz, b->s), "clear");
     }
     hip_check(hipMemsetAsync(b->d_beta, 0, (size_t)b->n * (b->cap + 1) * sizeof(double), b->s), "clear");
     hip_check(hipMemsetAsync(b->d_step, 0, sizeof(uint32_t), b->s), "clear");
-    if (accum) {
-        std::vector<double> c((size_t)b->n * b->cap, 0.0);
-        for (int i = 0; i < b->n; ++i)
-            for (int k = 0; k < steps; ++k) c[(size_t)i * b->cap + k] = (*coef)[(size_t)i * steps + k];
-        hip_check(hipMemcpyAsync(b->d_coef, c.data(), c.size() * sizeof(double), hipMemcpyHostToDevice, b->s), "coef");
-    }
     for (int k = 0; k < steps; ++k) {
         const int p = k % 3;
         if (!use_graph) {
@@ -171,9 +186,15 @@ void batch_run(esc_batch *b, int steps, bool use_graph, const std::vector<double
         }
         hip_check(hipGraphLaunch(b->exec[accum][p], b->s), "graph launch");
     }
+}
+
+// host copies of alpha (real parts, n x steps) and beta (n x (steps + 1))
+// from device rows of leading dimensions cap and cap + 1 (waits for the device)
+void batch_read(esc_batch *b, const double *d_alpha, const double *d_beta, int steps, std::vector<double> &alpha,
+                std::vector<double> &beta) {
     std::vector<double> a2((size_t)b->n * b->cap * 2), bt((size_t)b->n * (b->cap + 1));
-    hip_check(hipMemcpyAsync(a2.data(), b->d_alpha, a2.size() * sizeof(double), hipMemcpyDeviceToHost, b->s), "alpha");
-    hip_check(hipMemcpyAsync(bt.data(), b->d_beta, bt.size() * sizeof(double), hipMemcpyDeviceToHost, b->s), "beta");
+    hip_check(hipMemcpyAsync(a2.data(), d_alpha, a2.size() * sizeof(double), hipMemcpyDeviceToHost, b->s), "alpha");
+    hip_check(hipMemcpyAsync(bt.data(), d_beta, bt.size() * sizeof(double), hipMemcpyDeviceToHost, b->s), "beta");
     hip_check(hipStreamSynchronize(b->s), "sync");
     alpha.assign((size_t)b->n * steps, 0.0);
     beta.assign((size_t)b->n * (steps + 1), 0.0);
@@ -183,22 +204,15 @@ void batch_run(esc_batch *b, int steps, bool use_graph, const std::vector<double
     }
 }
 
-// length of instance i's recursion: up to and including the first step whose
-// beta_{k+1} came out zero (exhausted Krylov space)
-int batch_length(const std::vector<double> &beta, int i, int steps) {
-    for (int k = 0; k < steps; ++k)
-        if (beta[(size_t)i * (steps + 1) + k + 1] == 0.0) return k + 1;
-    return steps;
-}
-
-// copy x / |x| of instance i into its slot 1 (norm2 returned)
-double batch_load_unit(esc_batch *b, int i, const void *x) {
+// x / |x| of instance i into its slot 1, |x|^2 into the device double n2
+// (x is copied unscaled when the norm is zero or not finite)
+void batch_load_unit(esc_batch *b, int i, const void *x, double *n2) {
     esc_ctx *c = b->ctx[i];
-    const double n2 = device_norm2(c, x, b->s);
+    c->ensure_partials(2 * esc::kRedBlocks);
+    hip_check(esc::launch_norm2(b->c64, x, c->dim, c->d_partials, n2, b->s), "norm2");
     void *dst = b->slot(i, 1);
     hip_check(hipMemcpyAsync(dst, x, c->dim * b->esz, hipMemcpyDeviceToDevice, b->s), "copy");
-    if (n2 > 0.0 && std::isfinite(n2)) hip_check(esc::launch_scale(b->c64, dst, 1.0 / std::sqrt(n2), c->dim, b->s), "scale");
-    return n2;
+    hip_check(esc::launch_normalize(b->c64, dst, n2, c->dim, b->s), "normalize");
 }
 
 }  // namespace
@@ -272,41 +286,57 @@ int esc_batch_ground_state(esc_batch *b, uint64_t seed, int steps, int use_graph
         batch_check(b);
         if (steps < 2 || steps > 4096) throw EscError(ESC_ERR_ARG, "steps must be in [2, 4096]");
         b->have_gs = false;
+        const int n = b->n;
+        batch_capacity(b, steps);
+        double *norms = (double *)b->norms.get(2 * (size_t)n * sizeof(double));
         // start vectors (Gaussian, seed + i), kept in slot 3 for the replay;
         // one eager H.psi per instance builds its kernel tables before capture
-        for (int i = 0; i < b->n; ++i) {
+        for (int i = 0; i < n; ++i) {
             esc_ctx *c = b->ctx[i];
             hip_check(esc::launch_gaussian(b->c64, seed + (uint64_t)i, b->slot(i, 3), c->dim, b->s), "gaussian");
-            batch_load_unit(b, i, b->slot(i, 3));
+            batch_load_unit(b, i, b->slot(i, 3), norms + i);
             hip_check(hipMemcpyAsync(b->slot(i, 3), b->slot(i, 1), c->dim * b->esz, hipMemcpyDeviceToDevice, b->s), "copy");
             run_hpsi(c, b->slot(i, 1), b->slot(i, 2), b->s);
         }
-        std::vector<double> al, bt;
-        batch_run(b, steps, use_graph != 0, nullptr, al, bt);
-        std::vector<double> coef((size_t)b->n * steps, 0.0);
-        for (int i = 0; i < b->n; ++i) {
-            const int n = batch_length(bt, i, steps);
-            std::vector<double> ev(n), Z((size_t)n * n);
-            if (tridiag_ql(n, al.data() + (size_t)i * steps, bt.data() + (size_t)i * (steps + 1), ev.data(), Z.data()) != 0)
-                throw EscError(ESC_ERR_NUMERIC, "tridiagonal QL did not converge");
-            b->e0[i] = ev[0];
-            for (int k = 0; k < n; ++k) coef[(size_t)i * steps + k] = Z[k];
-            if (e0) e0[i] = ev[0];
-            if (iters) iters[i] = n;
-        }
-        if (alpha) std::memcpy(alpha, al.data(), al.size() * sizeof(double));
-        if (beta) std::memcpy(beta, bt.data(), bt.size() * sizeof(double));
+        batch_run(b, steps, use_graph != 0, false);
+        // the recursion's rows are kept for the caller; the lowest eigenpair
+        // of each T (Sturm bisection, inverse iteration) is solved on the
+        // device straight into the replay coefficients
+        const size_t na = (size_t)n * b->cap * 2 * sizeof(double), nb = (size_t)n * (b->cap + 1) * sizeof(double);
+        double *ka = (double *)b->keep_alpha.get(na), *kb = (double *)b->keep_beta.get(nb);
+        hip_check(hipMemcpyAsync(ka, b->d_alpha, na, hipMemcpyDeviceToDevice, b->s), "copy");
+        hip_check(hipMemcpyAsync(kb, b->d_beta, nb, hipMemcpyDeviceToDevice, b->s), "copy");
+        double *d_e0 = (double *)b->e0_dev.get((size_t)n * sizeof(double));
+        int *d_it = (int *)b->iters_dev.get((size_t)n * sizeof(int));
+        double *work = (double *)b->work.get((size_t)n * 6 * steps * sizeof(double));
+        hip_check(esc::launch_batch_tridiag(n, b->d_alpha, b->cap, b->d_beta, b->cap + 1, steps, work, b->d_coef, b->cap,
+                                            d_e0, d_it, b->s),
+                  "batch tridiagonal");
         // replay from the same starts, forming |gs> = sum_k Z(k,0) v_k in slot 4
-        for (int i = 0; i < b->n; ++i)
+        for (int i = 0; i < n; ++i)
             hip_check(hipMemcpyAsync(b->slot(i, 1), b->slot(i, 3), b->ctx[i]->dim * b->esz, hipMemcpyDeviceToDevice, b->s),
                       "copy");
-        std::vector<double> al2, bt2;
-        batch_run(b, steps, use_graph != 0, &coef, al2, bt2);
-        for (int i = 0; i < b->n; ++i) {
-            const double n2 = device_norm2(b->ctx[i], b->slot(i, 4), b->s);
-            hip_check(esc::launch_scale(b->c64, b->slot(i, 4), 1.0 / std::sqrt(n2), b->ctx[i]->dim, b->s), "scale");
+        batch_run(b, steps, use_graph != 0, true);
+        for (int i = 0; i < n; ++i) {
+            esc_ctx *c = b->ctx[i];
+            c->ensure_partials(2 * esc::kRedBlocks);
+            hip_check(esc::launch_norm2(b->c64, b->slot(i, 4), c->dim, c->d_partials, norms + n + i, b->s), "norm2");
+            hip_check(esc::launch_normalize(b->c64, b->slot(i, 4), norms + n + i, c->dim, b->s), "normalize");
         }
-        hip_check(hipStreamSynchronize(b->s), "sync");
+        // the one wait of the call
+        std::vector<int> it(n);
+        b->e0.assign(n, 0.0);
+        hip_check(hipMemcpyAsync(b->e0.data(), d_e0, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, b->s), "e0");
+        hip_check(hipMemcpyAsync(it.data(), d_it, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, b->s), "iters");
+        std::vector<double> al, bt;
+        batch_read(b, ka, kb, steps, al, bt);
+        for (int i = 0; i < n; ++i)
+            if (!std::isfinite(b->e0[i]))
+                throw EscError(ESC_ERR_NUMERIC, "instance " + std::to_string(i) + ": non-finite tridiagonal matrix");
+        if (e0) std::memcpy(e0, b->e0.data(), (size_t)n * sizeof(double));
+        if (iters) std::memcpy(iters, it.data(), (size_t)n * sizeof(int));
+        if (alpha) std::memcpy(alpha, al.data(), al.size() * sizeof(double));
+        if (beta) std::memcpy(beta, bt.data(), bt.size() * sizeof(double));
         b->have_gs = true;
     });
 }
@@ -356,59 +386,78 @@ int esc_batch_dynamics(esc_batch *b, int kind, int cf_steps, double eta, const d
                                         : std::vector<int>{ESC_OP_JEX, ESC_OP_JEY};
         const int nc = (int)ops.size(), ncols = js ? 14 : 5;
         const int n = b->n;
-        std::vector<std::vector<esc::CurrentTable>> tabs(n, std::vector<esc::CurrentTable>(nc));
-        for (int i = 0; i < n; ++i) {
-            for (int a = 0; a < nc; ++a) host_current_table(b->ctx[i], ops[a], ESC_SITE_TOTAL, &tabs[i][a]);
-            if (!b->ctx[i]->d_combo && hipMalloc(&b->ctx[i]->d_combo, sizeof(esc::CurrentTable)) != hipSuccess)
-                throw EscError(ESC_ERR_ALLOC, "table allocation failed");
-        }
-        // phi_i = (sum_a c_a J_a)|gs_i> into slot 5
-        auto apply = [&](int i, const std::vector<cplx> &coef) {
-            esc_ctx *c = b->ctx[i];
-            esc::CurrentTable t;
-            std::memset(&t, 0, sizeof(t));
-            for (int a = 0; a < nc; ++a)
-                if (coef[a] != cplx(0.0, 0.0)) append_scaled(t, tabs[i][a], coef[a].real(), coef[a].imag());
-            hip_check(hipMemcpyAsync(c->d_combo, &t, sizeof(t), hipMemcpyHostToDevice, b->s), "table");
-            hip_check(b->c64 ? esc::launch_current_c64(c->d_combo, b->slot(i, 4), b->slot(i, 5), c->dim, b->s)
-                             : esc::launch_current_c128(c->d_combo, b->slot(i, 4), b->slot(i, 5), c->dim, b->s),
-                      "current launch");
-            hip_check(hipStreamSynchronize(b->s), "sync");   // the host table is reused
-        };
-        std::vector<std::vector<cplx>> ex(n, std::vector<cplx>(nc + 1, cplx(0.0, 0.0)));
-        for (int i = 0; i < n; ++i)
-            for (int a = 0; a < nc; ++a) {
-                std::vector<cplx> c(nc, cplx(0.0, 0.0));
-                c[a] = 1.0;
-                apply(i, c);
-                double d[2];
-                device_dot(b->ctx[i], b->slot(i, 4), b->slot(i, 5), d, b->s);
-                ex[i][a] = cplx(d[0], d[1]);
-                ex[i][nc] += ex[i][a];
-            }
         const std::vector<CfRun> runs = cf_runs(js, nc);
-        std::vector<std::vector<std::vector<cplx>>> f(n);
-        for (const CfRun &r : runs) {
-            std::vector<double> n2(n);
-            for (int i = 0; i < n; ++i) {
-                apply(i, r.coef);
-                n2[i] = batch_load_unit(b, i, b->slot(i, 5));
-            }
-            std::vector<double> al, bt;
-            batch_run(b, cf_steps, use_graph != 0, nullptr, al, bt);
-            for (int i = 0; i < n; ++i) {
-                const int it = n2[i] > 0.0 ? batch_length(bt, i, cf_steps) : 0;
-                f[i].push_back(cf_eval(it, al.data() + (size_t)i * cf_steps, bt.data() + (size_t)i * (cf_steps + 1),
-                                       n2[i], b->e0[i], eta, omega, nw));
+        const int R = (int)runs.size();
+        // every run's operator table for every instance, uploaded once
+        std::vector<esc::CurrentTable> tabs((size_t)n * R);
+        for (int i = 0; i < n; ++i) {
+            std::vector<esc::CurrentTable> comp(nc);
+            for (int a = 0; a < nc; ++a) host_current_table(b->ctx[i], ops[a], ESC_SITE_TOTAL, &comp[a]);
+            for (int r = 0; r < R; ++r) {
+                esc::CurrentTable &t = tabs[(size_t)i * R + r];
+                std::memset(&t, 0, sizeof(t));
+                for (int a = 0; a < nc; ++a) {
+                    const cplx c = runs[r].coef[a];
+                    if (c != cplx(0.0, 0.0)) append_scaled(t, comp[a], c.real(), c.imag());
+                }
             }
         }
+        auto *d_tabs = (esc::CurrentTable *)b->tabs_dev.get(tabs.size() * sizeof(esc::CurrentTable));
+        hip_check(hipMemcpyAsync(d_tabs, tabs.data(), tabs.size() * sizeof(esc::CurrentTable), hipMemcpyHostToDevice,
+                                 b->s), "tables");
+        auto *d_omega = (double *)b->omega_dev.get((size_t)nw * sizeof(double));
+        hip_check(hipMemcpyAsync(d_omega, omega, (size_t)nw * sizeof(double), hipMemcpyHostToDevice, b->s), "omega");
+        auto *d_e0 = (double *)b->e0_dev.get((size_t)n * sizeof(double));
+        hip_check(hipMemcpyAsync(d_e0, b->e0.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, b->s), "e0");
+        auto *d_n2 = (double *)b->n2_dev.get((size_t)R * n * sizeof(double));
+        auto *d_ex = (double *)b->ex_dev.get((size_t)n * nc * 2 * sizeof(double));
+        void *d_f = b->f_dev.get((size_t)R * n * nw * 2 * sizeof(double));
+        for (int r = 0; r < R; ++r) {
+            // phi_i = (sum_a c_a J_a)|gs_i> into slot 5; the diagonal runs also
+            // give <gs_i|J_a|gs_i>
+            for (int i = 0; i < n; ++i) {
+                esc_ctx *c = b->ctx[i];
+                const esc::CurrentTable *t = d_tabs + (size_t)i * R + r;
+                hip_check(b->c64 ? esc::launch_current_c64(t, b->slot(i, 4), b->slot(i, 5), c->dim, b->s)
+                                 : esc::launch_current_c128(t, b->slot(i, 4), b->slot(i, 5), c->dim, b->s),
+                          "current launch");
+                if (runs[r].kind == 0) {
+                    c->ensure_partials(2 * esc::kRedBlocks);
+                    hip_check(esc::launch_dot(b->c64, b->slot(i, 4), b->slot(i, 5), c->dim, c->d_partials,
+                                              d_ex + 2 * ((size_t)i * nc + runs[r].a), b->s),
+                              "dot");
+                }
+                batch_load_unit(b, i, b->slot(i, 5), d_n2 + (size_t)r * n + i);
+            }
+            batch_run(b, cf_steps, use_graph != 0, false);
+            hip_check(esc::launch_batch_cf(n, b->d_alpha, b->cap, b->d_beta, b->cap + 1, cf_steps,
+                                           d_n2 + (size_t)r * n, d_e0, eta, d_omega, nw,
+                                           (char *)d_f + (size_t)r * n * nw * 2 * sizeof(double), b->s),
+                      "batch continued fractions");
+        }
+        // the one wait of the call: fractions and expectation values
+        std::vector<cplx> fh((size_t)R * n * nw);
+        std::vector<double> exh((size_t)n * nc * 2);
+        hip_check(hipMemcpyAsync(fh.data(), d_f, fh.size() * sizeof(cplx), hipMemcpyDeviceToHost, b->s), "fractions");
+        hip_check(hipMemcpyAsync(exh.data(), d_ex, exh.size() * sizeof(double), hipMemcpyDeviceToHost, b->s), "ex");
+        hip_check(hipStreamSynchronize(b->s), "sync");
         for (int i = 0; i < n; ++i) {
-            auto F = cf_assemble(js, nc, runs, f[i], nw);
-            write_spectrum_table(js, nc, ex[i], F, omega, nw, eta, tables + (size_t)i * nw * (1 + ncols));
+            std::vector<cplx> ex(nc + 1, cplx(0.0, 0.0));
+            for (int a = 0; a < nc; ++a) {
+                ex[a] = cplx(exh[2 * ((size_t)i * nc + a)], exh[2 * ((size_t)i * nc + a) + 1]);
+                ex[nc] += ex[a];
+            }
+            std::vector<std::vector<cplx>> f(R);
+            for (int r = 0; r < R; ++r) {
+                const cplx *src = fh.data() + ((size_t)r * n + i) * nw;
+                f[r].assign(src, src + nw);
+            }
+            auto F = cf_assemble(js, nc, runs, f, nw);
+            write_spectrum_table(js, nc, ex, F, omega, nw, eta, tables + (size_t)i * nw * (1 + ncols));
             if (expect)
                 for (int a = 0; a <= nc; ++a) {
-                    expect[2 * ((size_t)i * 4 + a)] = ex[i][a].real();
-                    expect[2 * ((size_t)i * 4 + a) + 1] = ex[i][a].imag();
+                    expect[2 * ((size_t)i * 4 + a)] = ex[a].real();
+                    expect[2 * ((size_t)i * 4 + a) + 1] = ex[a].imag();
                 }
         }
     });

@@ -4,6 +4,7 @@
 // FP64 small solves.  All vector work runs in the HIP kernels of
 // esc_kernels.hip; nothing here falls back to a CPU path.
 #include "esc_ctx.h"
+#include "esc_tridiag.h"
 
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
@@ -111,48 +112,14 @@ int tridiag_ql(int n, const double *a, const double *b, double *evals, double *Z
     return 0;
 }
 
-// Lowest eigenvalue only (the per-iteration convergence test), by bisection
-// on the Sturm count: the number of eigenvalues below x is the number of
-// negative pivots q_i = (a_i - x) - b_i^2 / q_{i-1} of T - x I.  O(n) per
-// halving instead of the O(n^2) of a QL solve; backward stable like QL, the
-// result is within a few ulp of |lambda| (or 2^-128 of the Gershgorin width).
+// Lowest eigenvalue only (the per-iteration convergence test): Sturm-count
+// bisection (esc::td_lowest), O(n) per halving instead of the O(n^2) of a QL
+// solve; the batched solves run the same routine on the device.
 double tridiag_lowest(int n, const double *a, const double *b) {
-    if (n == 1) return a[0];
-    double lo = std::numeric_limits<double>::infinity(), hi = -lo, bmax = 0.0;
-    for (int i = 0; i < n; ++i) {
-        const double r = (i > 0 ? std::fabs(b[i]) : 0.0) + (i + 1 < n ? std::fabs(b[i + 1]) : 0.0);
-        lo = std::min(lo, a[i] - r);
-        hi = std::max(hi, a[i] + r);
-        if (i > 0) bmax = std::max(bmax, b[i] * b[i]);
-    }
-    if (!std::isfinite(lo) || !std::isfinite(hi)) throw EscError(ESC_ERR_NUMERIC, "non-finite tridiagonal matrix");
-    // pivots below pivmin in magnitude are replaced by -pivmin
-    const double pivmin = std::numeric_limits<double>::min() * std::max(1.0, bmax);
-    auto below = [&](double x) {   // eigenvalues < x
-        int c = 0;
-        double q = a[0] - x;
-        for (int i = 0;;) {
-            if (std::fabs(q) < pivmin) q = -pivmin;
-            if (q < 0.0) ++c;
-            if (++i == n) break;
-            q = (a[i] - x) - (b[i] * b[i]) / q;
-        }
-        return c;
-    };
-    // eigenvalues lie in [lo, hi]: widen by a margin so below(lo) == 0
-    const double margin = 2.0 * std::numeric_limits<double>::epsilon() * std::max(std::fabs(lo), std::fabs(hi)) +
-                          2.0 * pivmin;
-    lo -= margin;
-    hi += margin;
-    for (int it = 0; it < 256; ++it) {
-        const double mid = 0.5 * (lo + hi);
-        if (!(mid > lo && mid < hi)) break;   // interval at machine resolution
-        if (hi - lo <= 4.0 * std::numeric_limits<double>::epsilon() * std::max(std::fabs(lo), std::fabs(hi)))
-            break;
-        if (below(mid) >= 1) hi = mid;
-        else lo = mid;
-    }
-    return 0.5 * (lo + hi);
+    bool ok = false;
+    const double e = esc::td_lowest(n, a, b, &ok);
+    if (!ok) throw EscError(ESC_ERR_NUMERIC, "non-finite tridiagonal matrix");
+    return e;
 }
 
 }  // namespace esc_impl
@@ -701,11 +668,18 @@ bool cube_spec(const esc_ctx *c, int cfg, esc::CubeModel &m, std::vector<esc::Cu
     const int TB = cfg_tile_bits(cfg), w = cfg_window(cfg), P = cfg_passes(cfg) - 2, WPB = cfg_waves(cfg);
     const std::vector<Plan> plan = plan_windows(c, terms, TB, w, P);
     const std::vector<esc::GTables> g = build_passes(c, TB, w, P, 4);
-    if (!g[0].diag_closed) {
-        if (why) *why = "the Kitaev-z diagonal needs the bond-by-bond sum";
-        return false;
-    }
     std::memset(&m, 0, sizeof(m));
+    if (!g[0].diag_closed) {
+        if (h.n_zdiag > 64 || h.n_jdiag > 96) {
+            if (why) *why = "too many diagonal bonds for the bond-by-bond sum";
+            return false;
+        }
+        m.seq_diag = true;
+        m.n_zpair = h.n_zdiag;
+        m.n_jpair = h.n_jdiag;
+        for (uint32_t q = 0; q < h.n_zdiag; ++q) m.zpair[q] = h.zdiag[q];
+        for (uint32_t q = 0; q < h.n_jdiag; ++q) m.jpair[q] = h.jdiag[q];
+    }
     m.n_sites = N;
     m.c64 = c->c64();
     for (int f = 0; f < 3; ++f) m.bond_cplx[f] = h.q[f] != 0.0;
@@ -932,6 +906,28 @@ esc::CubeProgram &cube_for(esc_ctx *c, int config) {
     }
     c->cube[config] = prog;
     return *prog;
+}
+
+// Compile the generated programs of several configurations into the process
+// and disk caches on up to 16 host threads, so that a tuner loading them one
+// after another does not wait for one compilation after another.  Failures
+// are left for cube_for to report.
+void cube_compile_parallel(const esc_ctx *c, const std::vector<int> &configs) {
+    std::vector<int> todo;
+    for (int cfg : configs)
+        if (cfg > 0 && cfg_cube(cfg) && !c->cube.count(cfg) && !c->cube_fail.count(cfg)) todo.push_back(cfg);
+    if (todo.size() < 2) return;
+    host_parallel_for((int)todo.size(), 1, [&](int b, int e) {
+        for (int q = b; q < e; ++q) {
+            try {
+                esc::CubeModel m;
+                std::vector<esc::CubePass> passes;
+                std::string why;
+                if (cube_spec(c, todo[q], m, passes, &why)) (void)esc::cube_precompile(m, passes, &why);
+            } catch (...) {
+            }
+        }
+    });
 }
 
 // whether every pass of a configuration fits the LDS of one CU (cached)
@@ -1408,56 +1404,17 @@ std::vector<CfRun> cf_runs(bool js, int nc) {
     return r;
 }
 
-// n2 <phi|(z - H)^-1|phi>/n2 from the recursion coefficients, z = w + e0 - i eta
-// fn(begin, end) over contiguous chunks of [0, n) on up to 16 host threads
-// (the caller takes the first chunk), at least `grain` items per chunk
-template <typename F>
-void host_parallel_for(int n, int grain, const F &fn) {
-    const unsigned hw = std::thread::hardware_concurrency();
-    int nt = (int)std::min(hw ? hw : 1u, 16u);
-    nt = std::max(1, std::min(nt, n / std::max(1, grain)));
-    if (nt <= 1) {
-        fn(0, n);
-        return;
-    }
-    const int chunk = (n + nt - 1) / nt;
-    std::vector<std::thread> th;
-    try {
-        for (int t = 1; t < nt; ++t) {
-            const int b = t * chunk, e = std::min(n, b + chunk);
-            if (b < e) th.emplace_back([&fn, b, e] { fn(b, e); });
-        }
-    } catch (...) {
-        for (auto &x : th) x.join();
-        throw;
-    }
-    fn(0, std::min(n, chunk));
-    for (auto &x : th) x.join();
-}
-
 // norm2 <phi|(z - H)^-1|phi>, z = omega + e0 - i eta, from the recursion
-// coefficients, on every omega.  Each level is 1 / (x + i y) with y <= -eta
-// (the tail's imaginary part is never negative), so the plain formula
-// (x - i y) / (x^2 + y^2) stays in range; the omegas run on the host threads.
+// coefficients, on every omega (esc::td_cf, the routine the batched solves
+// run on the device); the omegas run on the host threads.
 std::vector<cplx> cf_eval(int it, const double *al, const double *be, double n2, double e0, double eta,
                           const double *omega, int nw) {
     std::vector<cplx> f(nw);
     host_parallel_for(nw, 128, [&](int q0, int q1) {
         for (int q = q0; q < q1; ++q) {
-            const double zr = omega[q] + e0, zi = -eta;
-            double gr = 0.0, gi = 0.0;
-            for (int k = it - 1; k >= 0; --k) {
-                double xr = zr - al[k], xi = zi;
-                if (k + 1 < it) {
-                    const double b2 = be[k + 1] * be[k + 1];
-                    xr -= b2 * gr;
-                    xi -= b2 * gi;
-                }
-                const double d = xr * xr + xi * xi;
-                gr = xr / d;
-                gi = -xi / d;
-            }
-            f[q] = cplx(n2 * gr, n2 * gi);
+            double fr, fi;
+            esc::td_cf(it, al, 1, be, n2, omega[q] + e0, -eta, &fr, &fi);
+            f[q] = cplx(fr, fi);
         }
     });
     return f;
@@ -2610,6 +2567,7 @@ int esc_autotune(esc_ctx *ctx, const void *d_in, void *d_out, int reps, int *til
         hipStream_t s = (hipStream_t)stream;
         std::vector<int> cand = tile_candidates(ctx);
         cand.push_back(0);   // one-row-per-lane kernel
+        cube_compile_parallel(ctx, cand);
         hipEvent_t e0, e1;
         hip_check(hipEventCreate(&e0), "hipEventCreate");
         hip_check(hipEventCreate(&e1), "hipEventCreate");
@@ -2665,6 +2623,86 @@ int esc_tridiag_eig(int n, const double *a, const double *b, double *evals, doub
         check_ptr(b, "b");
         check_ptr(evals, "evals");
         if (tridiag_ql(n, a, b, evals, Z) != 0) throw EscError(ESC_ERR_NUMERIC, "tridiagonal QL did not converge");
+    });
+}
+
+int esc_tridiag_lowest(int n, const double *a, const double *b, double *e0, double *z, int on_device) {
+    return guarded([&] {
+        if (n < 1 || n > 4096) throw EscError(ESC_ERR_ARG, "n must be in [1, 4096]");
+        check_ptr(a, "a");
+        check_ptr(b, "b");
+        check_ptr(e0, "e0");
+        // b[0] is not part of T; a zero b[k] (k >= 1) would end the recursion
+        // there, so the routines see the first n rows only when none is zero
+        for (int k = 1; k < n; ++k)
+            if (b[k] == 0.0) throw EscError(ESC_ERR_ARG, "b[k] == 0 for some 1 <= k < n: split the matrix first");
+        if (!on_device) {
+            bool ok = false;
+            *e0 = esc::td_lowest(n, a, b, &ok);
+            if (!ok) throw EscError(ESC_ERR_NUMERIC, "non-finite tridiagonal matrix");
+            if (z) {
+                std::vector<double> work(5 * (size_t)n);
+                esc::td_vector(n, a, b, *e0, work.data(), z);
+            }
+            return;
+        }
+        // one instance of the batched kernel: alpha as (re, im) pairs, beta
+        // with beta[n] = 0 so that the recursion's length is n
+        std::vector<double> a2(2 * (size_t)n, 0.0), bt((size_t)n + 1, 0.0);
+        for (int k = 0; k < n; ++k) a2[2 * (size_t)k] = a[k];
+        for (int k = 1; k < n; ++k) bt[k] = b[k];
+        bt[0] = 1.0;
+        double *d = nullptr;
+        const size_t nd = 2 * (size_t)n + ((size_t)n + 1) + 6 * (size_t)n + (size_t)n + 1;
+        if (hipMalloc(&d, nd * sizeof(double) + sizeof(int)) != hipSuccess) throw EscError(ESC_ERR_ALLOC, "allocation failed");
+        struct Free { double *p; ~Free() { (void)hipFree(p); } } fr{d};
+        double *da = d, *db = da + 2 * n, *dw = db + n + 1, *dz = dw + 6 * (size_t)n, *de = dz + n;
+        int *di = (int *)(de + 1);
+        hip_check(hipMemcpy(da, a2.data(), a2.size() * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+        hip_check(hipMemcpy(db, bt.data(), bt.size() * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+        hip_check(esc::launch_batch_tridiag(1, da, n, db, n + 1, n, dw, dz, n, de, di, nullptr), "tridiagonal kernel");
+        hip_check(hipMemcpy(e0, de, sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
+        if (!std::isfinite(*e0)) throw EscError(ESC_ERR_NUMERIC, "non-finite tridiagonal matrix");
+        if (z) hip_check(hipMemcpy(z, dz, (size_t)n * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
+    });
+}
+
+int esc_cf_eval(int n, const double *a, const double *b, double norm2, double e0, double eta, const double *omega,
+                int nw, double *out, int on_device) {
+    return guarded([&] {
+        if (n < 0 || n > 4096) throw EscError(ESC_ERR_ARG, "n must be in [0, 4096]");
+        if (nw < 1) throw EscError(ESC_ERR_ARG, "nw must be >= 1");
+        check_ptr(a, "a");
+        check_ptr(b, "b");
+        check_ptr(omega, "omega");
+        check_ptr(out, "out");
+        if (!on_device) {
+            const std::vector<cplx> f = cf_eval(norm2 > 0.0 ? n : 0, a, b, norm2, e0, eta, omega, nw);
+            std::memcpy(out, f.data(), (size_t)nw * sizeof(cplx));
+            return;
+        }
+        // one instance of the batched kernel; its length is n through a zero
+        // beta[n] and nonzero stand-ins for zeros inside the first n
+        const int st = std::max(n, 1);
+        std::vector<double> a2(2 * (size_t)st, 0.0), bt((size_t)st + 1, 0.0);
+        for (int k = 0; k < n; ++k) a2[2 * (size_t)k] = a[k];
+        for (int k = 1; k < n; ++k) bt[k] = b[k];
+        for (int k = 1; k < n; ++k)
+            if (bt[k] == 0.0) throw EscError(ESC_ERR_ARG, "b[k] == 0 for some 1 <= k < n: the fraction ends there");
+        bt[0] = 1.0;
+        const double n2 = n > 0 ? norm2 : 0.0;
+        double *d = nullptr;
+        const size_t nd = 2 * (size_t)st + ((size_t)st + 1) + 2 + (size_t)nw + 2 * (size_t)nw;
+        if (hipMalloc(&d, nd * sizeof(double)) != hipSuccess) throw EscError(ESC_ERR_ALLOC, "allocation failed");
+        struct Free { double *p; ~Free() { (void)hipFree(p); } } fr{d};
+        double *da = d, *db = da + 2 * st, *dn = db + st + 1, *de = dn + 1, *dw = de + 1, *df = dw + nw;
+        hip_check(hipMemcpy(da, a2.data(), a2.size() * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+        hip_check(hipMemcpy(db, bt.data(), bt.size() * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+        hip_check(hipMemcpy(dn, &n2, sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+        hip_check(hipMemcpy(de, &e0, sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+        hip_check(hipMemcpy(dw, omega, (size_t)nw * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+        hip_check(esc::launch_batch_cf(1, da, st, db, st + 1, st, dn, de, eta, dw, nw, df, nullptr), "fraction kernel");
+        hip_check(hipMemcpy(out, df, 2 * (size_t)nw * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
     });
 }
 

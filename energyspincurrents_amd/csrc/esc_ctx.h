@@ -9,12 +9,14 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <complex>
 #include <cstdint>
 #include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -177,6 +179,32 @@ esc::CurrentTable *current_table(esc_ctx *c, int op, int site);
 void append_scaled(esc::CurrentTable &dst, const esc::CurrentTable &src, double cr, double ci);
 
 // FP64 small solves
+// fn(begin, end) over contiguous chunks of [0, n) on up to 16 host threads
+// (the caller takes the first chunk), at least `grain` items per chunk
+template <typename F>
+void host_parallel_for(int n, int grain, const F &fn) {
+    const unsigned hw = std::thread::hardware_concurrency();
+    int nt = (int)std::min(hw ? hw : 1u, 16u);
+    nt = std::max(1, std::min(nt, n / std::max(1, grain)));
+    if (nt <= 1) {
+        fn(0, n);
+        return;
+    }
+    const int chunk = (n + nt - 1) / nt;
+    std::vector<std::thread> th;
+    try {
+        for (int t = 1; t < nt; ++t) {
+            const int b = t * chunk, e = std::min(n, b + chunk);
+            if (b < e) th.emplace_back([&fn, b, e] { fn(b, e); });
+        }
+    } catch (...) {
+        for (auto &x : th) x.join();
+        throw;
+    }
+    fn(0, std::min(n, chunk));
+    for (auto &x : th) x.join();
+}
+
 int tridiag_ql(int n, const double *a, const double *b, double *evals, double *Z);
 double tridiag_lowest(int n, const double *a, const double *b);
 

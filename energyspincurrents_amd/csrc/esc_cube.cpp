@@ -588,10 +588,24 @@ std::string cube_source(const CubeModel &m, const std::vector<CubePass> &passes)
     o << "typedef " << (m.c64 ? "float" : "double") << " R;\n";
     o << "typedef " << (m.c64 ? "float2" : "double2") << " V2;\n";
     o << "typedef R NV2 __attribute__((ext_vector_type(2)));\n";
-    // diagonal (closed form; used only when every partial sum of the Kitaev-z
-    // part is exact, so it equals the reference's bond-by-bond sum)
+    // diagonal: the closed form where every partial sum of the Kitaev-z part is
+    // exact (it then equals the reference's bond-by-bond sum), else that sum
+    // itself, z bonds then Heisenberg bonds in the reference's order
     o << "__device__ __forceinline__ double esc_diag(unsigned r, double kz4, double j4, double bz) {\n";
     o << "#pragma clang fp contract(off)\n";
+    if (m.seq_diag) {
+        o << "  double d = 0.0;\n";
+        for (uint32_t q = 0; q < m.n_zpair; ++q)
+            o << "  d += (((r >> " << (m.zpair[q] & 0xffu) << ") ^ (r >> " << ((m.zpair[q] >> 8) & 0xffu)
+              << ")) & 1u) ? -kz4 : kz4;\n";
+        for (uint32_t q = 0; q < m.n_jpair; ++q)
+            o << "  d += (((r >> " << (m.jpair[q] & 0xffu) << ") ^ (r >> " << ((m.jpair[q] >> 8) & 0xffu)
+              << ")) & 1u) ? -j4 : j4;\n";
+        o << "  d = d + bz * (0.5 * (double)(" << m.n_sites << " - 2 * (int)__popc(r)));\n";
+        o << "  (void)j4;\n  return d;\n}\n\n";
+        for (size_t i = 0; i < passes.size(); ++i) emit_pass(o, m, passes[i], (int)i);
+        return o.str();
+    }
     o << "  unsigned cz = 0;\n";
     for (uint32_t q = 0; q < m.n_zgrp; ++q)
         o << "  cz += __popc((r ^ (r >> " << m.zgrp_d[q] << ")) & " << hex32(m.zgrp_m[q]) << ");\n";

@@ -51,6 +51,11 @@ struct CubeModel {
     // diagonal in closed form: z and Heisenberg bonds counted by distance groups
     uint32_t nz, nj, n_zgrp, n_jgrp;
     uint32_t zgrp_d[32], zgrp_m[32], jgrp_d[32], jgrp_m[32];
+    // or bond by bond in the reference's order (when the closed form's
+    // partial sums would round differently): pairs i | j << 8
+    bool seq_diag;
+    uint32_t n_zpair, n_jpair;
+    uint32_t zpair[64], jpair[96];
     int reg_sites;          // k: rows per lane = 2^k
     int waves;              // waves per workgroup
 };

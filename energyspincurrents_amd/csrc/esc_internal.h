@@ -158,7 +158,8 @@ hipError_t launch_axpby(bool c64, double ar, double ai, const void *x, double br
 // w = w - alpha v - beta u ; result[0] = |w|^2     (alpha complex, beta real)
 hipError_t launch_lanczos_update(bool c64, void *w, const void *v, const void *u, const double *d_alpha,
                                  double beta, uint64_t dim, double *partials, double *result, hipStream_t s);
-// x *= 1 / sqrt(result[0]) with result read on the device (no host round trip)
+// x *= 1 / sqrt(result[0]) with result read on the device (no host round trip);
+// x is left as it is when result[0] is not positive and finite
 hipError_t launch_normalize(bool c64, void *x, const double *d_norm2, uint64_t dim, hipStream_t s);
 // x *= scale (real)
 hipError_t launch_scale(bool c64, void *x, double scale, uint64_t dim, hipStream_t s);
@@ -227,6 +228,19 @@ constexpr int kBatchAccum = 2;    // acc += coef[i][k] v_k
 hipError_t launch_batch_step(bool c64, int piece, const BatchLane *d_lanes, int n, uint32_t total_blocks,
                              double *partials, double *alpha, int lda, double *beta, int ldb, const double *coef,
                              int ldc, uint32_t *step, hipStream_t s);
+// One thread per instance i < n: the length of its recorded recursion, the
+// lowest eigenvalue of its tridiagonal matrix (alpha[2 (i lda + k)] real
+// parts, beta[i ldb + k]) into e0[i], its length into iters[i], and the unit
+// eigenvector into coef[i ldc + k] (zero beyond the length).  work: 6 steps
+// doubles per instance.
+hipError_t launch_batch_tridiag(int n, const double *alpha, int lda, const double *beta, int ldb, int steps,
+                                double *work, double *coef, int ldc, double *e0, int *iters, hipStream_t s);
+// The continued fraction of every instance i < n on every omega q < nw, as
+// complex doubles f[i nw + q], from the same alpha / beta rows, the norms
+// n2[i] of the recursions' start vectors and the energies e0[i].
+hipError_t launch_batch_cf(int n, const double *alpha, int lda, const double *beta, int ldb, int steps,
+                           const double *n2, const double *e0, double eta, const double *omega, int nw, void *f,
+                           hipStream_t s);
 // convert FP64 complex device buffer to c64 (or copy for c128)
 hipError_t launch_convert_from_c128(bool c64, const void *src, void *dst, uint64_t dim, hipStream_t s);
 hipError_t launch_convert_to_c128(bool c64, const void *src, void *dst, uint64_t dim, hipStream_t s);

@@ -5,6 +5,7 @@
 // the basis state r (bit k = site k, bit 0 = spin up).  Every output row is
 // owned by exactly one lane, so no atomics are used anywhere.
 #include "esc_internal.h"
+#include "esc_tridiag.h"
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -802,7 +803,9 @@ __global__ __launch_bounds__(kSmallThreads) void k_rec_small(const HTables *__re
 template <typename R>
 __global__ __launch_bounds__(kThreads) void k_normalize(typename Vec2<R>::T *__restrict__ x,
                                                         const double *__restrict__ d_norm2, uint64_t dim) {
-    const double s = 1.0 / sqrt(d_norm2[0]);
+    const double n2 = d_norm2[0];
+    if (!(n2 > 0.0) || !(n2 < INFINITY)) return;   // a zero or non-finite norm leaves x as it is
+    const double s = 1.0 / sqrt(n2);
     const uint64_t stride = (uint64_t)gridDim.x * kThreads;
     for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < dim; i += stride) {
         typename Vec2<R>::T o = x[i];
@@ -1476,6 +1479,66 @@ hipError_t launch_batch_step(bool c64, int piece, const BatchLane *d_lanes, int 
     default:
         return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+// ---- the tridiagonal matrices and continued fractions of a batch ----
+// One thread per instance: the recursion's length, the lowest eigenvalue of
+// its T and the eigenvector (the replay coefficients, zero beyond the length).
+__global__ __launch_bounds__(64) void k_batch_tridiag(int n, const double *__restrict__ alpha, int lda,
+                                                      const double *__restrict__ beta, int ldb, int steps,
+                                                      double *__restrict__ work, double *__restrict__ coef, int ldc,
+                                                      double *__restrict__ e0, int *__restrict__ iters) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double *bt = beta + (size_t)i * ldb;
+    const int len = td_length(bt, steps);
+    double *a = work + (size_t)i * 6 * steps;
+    for (int k = 0; k < len; ++k) a[k] = alpha[2 * ((size_t)i * lda + k)];
+    bool ok = false;
+    const double lam = td_lowest(len, a, bt, &ok);
+    double *x = coef + (size_t)i * ldc;
+    if (ok) td_vector(len, a, bt, lam, a + steps, x);
+    else
+        for (int k = 0; k < len; ++k) x[k] = 0.0;
+    for (int k = len; k < steps; ++k) x[k] = 0.0;
+    e0[i] = ok ? lam : NAN;
+    iters[i] = len;
+}
+
+// Continued fractions of every instance on every omega: grid (omega blocks,
+// instances); f[i * nw + q] = n2_i <phi_i|(z - H_i)^-1|phi_i> / n2_i at
+// z = omega_q + e0_i - i eta (no recursion when n2_i is not positive).
+__global__ __launch_bounds__(256) void k_batch_cf(const double *__restrict__ alpha, int lda,
+                                                  const double *__restrict__ beta, int ldb, int steps,
+                                                  const double *__restrict__ n2, const double *__restrict__ e0,
+                                                  double eta, const double *__restrict__ omega, int nw,
+                                                  double2 *__restrict__ f) {
+    const int i = blockIdx.y;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nw) return;
+    const double *bt = beta + (size_t)i * ldb;
+    const double nn = n2[i];
+    const int it = nn > 0.0 ? td_length(bt, steps) : 0;
+    double fr, fi;
+    td_cf(it, alpha + 2 * (size_t)i * lda, 2, bt, nn, omega[q] + e0[i], -eta, &fr, &fi);
+    f[(size_t)i * nw + q] = make_double2(fr, fi);
+}
+
+hipError_t launch_batch_tridiag(int n, const double *alpha, int lda, const double *beta, int ldb, int steps,
+                                double *work, double *coef, int ldc, double *e0, int *iters, hipStream_t s) {
+    if (n < 1 || steps < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_batch_tridiag, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, n, alpha, lda, beta, ldb,
+                       steps, work, coef, ldc, e0, iters);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_cf(int n, const double *alpha, int lda, const double *beta, int ldb, int steps,
+                           const double *n2, const double *e0, double eta, const double *omega, int nw, void *f,
+                           hipStream_t s) {
+    if (n < 1 || nw < 1 || n > 65535) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_batch_cf, dim3((unsigned)((nw + 255) / 256), (unsigned)n), dim3(256), 0, s, alpha, lda, beta,
+                       ldb, steps, n2, e0, eta, omega, nw, (double2 *)f);
     return hipGetLastError();
 }
 

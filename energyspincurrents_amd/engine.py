@@ -513,6 +513,35 @@ def tridiag_eig(alpha, beta, vectors=True):
     return ev, (Z.reshape(n, n).T.copy() if vectors else None)
 
 
+def tridiag_lowest(alpha, beta, device=False):
+    """Lowest eigenpair (e0, z) of the tridiagonal matrix of a recursion
+    (esc_tridiag_lowest): alpha[0..n-1], beta[1..n-1] (beta[0] ignored); on
+    the host, or with device=True by the batched solves' kernel."""
+    a = np.ascontiguousarray(np.real(alpha), np.float64)
+    b = np.ascontiguousarray(np.abs(beta), np.float64)
+    n = len(a)
+    if len(b) < n:
+        raise ValueError("beta needs n entries (beta[0] ignored)")
+    e0 = ctypes.c_double()
+    z = np.zeros(n)
+    check(lib().esc_tridiag_lowest(n, _np_ptr(a), _np_ptr(b), ctypes.byref(e0), _np_ptr(z), int(bool(device))))
+    return e0.value, z
+
+
+def cf_eval(alpha, beta, norm2, e0, eta, omega, device=False):
+    """The raw continued fraction of a recursion on every omega (esc_cf_eval),
+    on the host threads or, with device=True, by the batched spectra's kernel."""
+    a = np.ascontiguousarray(np.real(alpha), np.float64)
+    b = np.ascontiguousarray(beta, np.float64)
+    if len(b) < len(a):
+        raise ValueError("beta needs n entries (beta[0] ignored)")
+    om = np.ascontiguousarray(omega, np.float64)
+    out = np.zeros(2 * len(om))
+    check(lib().esc_cf_eval(len(a), _np_ptr(a), _np_ptr(b), float(norm2), float(e0), float(eta), _np_ptr(om),
+                            len(om), _np_ptr(out), int(bool(device))))
+    return out[0::2] + 1j * out[1::2]
+
+
 def cf_spectrum(alpha, beta, norm2, e0, eta, omega, elastic=None):
     a = np.ascontiguousarray(alpha, np.float64)
     b = np.ascontiguousarray(beta, np.float64)

@@ -341,6 +341,25 @@ int esc_params_load(const char *path, esc_params *out);
  * (may be NULL for eigenvalues only). */
 int esc_tridiag_eig(int n, const double *a, const double *b, double *evals, double *Z);
 
+/* Lowest eigenpair of T (a, b as above, every b[k] != 0 for 1 <= k < n):
+ * Sturm-count bisection for the eigenvalue, inverse iteration with a
+ * partially pivoted LU of T - e0 I for the unit eigenvector z (may be NULL;
+ * first nonzero component positive; inside a cluster of equal eigenvalues,
+ * one unit vector of the cluster).  on_device = 0 runs the routine on the
+ * host, 1 the kernel the batched ground states use on the current device;
+ * both compute the same bits.  The ground-state half of TriDiag
+ * (src/DynLanczosSolver.h:371-414): E0 and the coefficients of |gs>. */
+int esc_tridiag_lowest(int n, const double *a, const double *b, double *e0, double *z, int on_device);
+
+/* The raw continued fraction n2 <phi|(z - H)^-1|phi> / n2 of a recursion of
+ * n levels (a[0..n-1], b[1..n-1], every b[k] != 0), z = w + e0 - i eta, on
+ * nw frequencies; out interleaved complex, zero unless norm2 > 0.  on_device = 0: host threads;
+ * 1: the kernel of the batched spectra on the current device; both compute
+ * the same bits.  The fraction DynJsJs / DynJeJe's CF mode sums
+ * (src/dynamics.h:270-336,389-455). */
+int esc_cf_eval(int n, const double *a, const double *b, double norm2, double e0, double eta,
+                const double *omega, int nw, double *out, int on_device);
+
 /* Continued fraction: G(w) = norm2 / (z - a0 - b1^2/(z - a1 - ...)),
  * z = w + e0 - i eta, for nw frequencies.  Optional elastic subtraction of
  * |<J>|^2/(w - i eta) when elastic != NULL (elastic[0] + i elastic[1] =

@@ -88,3 +88,40 @@ def test_batch_c64_energies_match_c128():
     assert len(cfg) == len(ms)
     e64, _, _, _ = b64.ground_state(seed=9, steps=200)
     assert np.all(np.abs(e64 - e128) < 1e-6 * np.abs(e128))
+
+
+def test_device_tridiagonal_and_fraction_routines_equal_the_host():
+    """The kernels the batch runs for its ground states (lowest eigenpair of
+    each T) and spectra (continued fractions) compute the host routines'
+    bits: random matrices and the tridiagonal matrix of a plain 8-site run
+    that repeats E0."""
+    rng = np.random.default_rng(33)
+    m = O.Model(2, 4, K=(-1.0, -1.0, -1.0), B=O.field111(0.3))
+    al, be, n2 = O.lanczos_plain(lambda x: O.hpsi_gather(m, x), rng.standard_normal(m.dim) + 0j, 300)
+    cases = [(al, be[:len(al)], n2)]
+    for n in (1, 2, 7, 200):
+        cases.append((rng.standard_normal(n), np.r_[0.0, 0.1 + np.abs(rng.standard_normal(n - 1))], 1.3))
+    om = np.linspace(-2.0, 8.0, 1000)
+    for a, b, nn in cases:
+        print(f"[device tridiagonal] n={len(a)}", flush=True)
+        e_h, z_h = esc.tridiag_lowest(a, b)
+        e_d, z_d = esc.tridiag_lowest(a, b, device=True)
+        assert e_h == e_d and np.array_equal(z_h, z_d)
+        f_h = esc.cf_eval(a, b, nn, -e_h, 0.05, om)
+        f_d = esc.cf_eval(a, b, nn, -e_h, 0.05, om, device=True)
+        assert np.array_equal(f_h, f_d)
+
+
+def test_batch_ground_state_energies_are_the_lowest_eigenvalues_of_the_returned_recursions():
+    """E0 and the replay coefficients come from the device routine on the
+    device's own alpha / beta rows: equal, bit for bit, to the host routine
+    on the rows the call returns, and within rounding of LAPACK."""
+    ms = MIXED[:4]
+    b = esc.Batch([spin_model(m) for m in ms], "c128")
+    e0, it, al, be = b.ground_state(seed=5, steps=120)
+    for i in range(len(ms)):
+        n = int(it[i])
+        e_h, _ = esc.tridiag_lowest(al[i, :n], be[i, :n])
+        assert e0[i] == e_h, (i, e0[i], e_h)
+        T = np.diag(al[i, :n]) + np.diag(be[i, 1:n], 1) + np.diag(be[i, 1:n], -1)
+        assert abs(e0[i] - np.linalg.eigvalsh(T)[0]) < 1e-12 * max(1.0, abs(e0[i]))

@@ -5,8 +5,9 @@ Checks: every default configuration compiles for the benchmark lattices in
 both precisions; a generated pass applies each term exactly once, and the
 passes together apply every bond and flip of the model once (counted from
 the source's own term comments); models the generated kernels do not cover
-(the chain's XYZ pairs, a non-exact Kitaev-z diagonal) are refused with
-ESC_ERR_UNSUPPORTED; no packed-FP32 instruction appears in the code object.
+(the chain's XYZ pairs) are refused with ESC_ERR_UNSUPPORTED; a diagonal whose
+closed form would round differently is summed bond by bond; no packed-FP32
+instruction appears in the code object.
 """
 import os
 import re
@@ -63,9 +64,19 @@ def test_unsupported_models_are_refused():
     with pytest.raises(esc.EscError) as e:
         esc.precompile_kernel(esc.SpinModel(esc.Chain(20), K=(1.0, 0.5, 0.25)), "c128")
     assert e.value.code == -6
-    with pytest.raises(esc.EscError) as e:       # Kz = 0.3: partial sums of +-Kz/4 are not exact
-        esc.precompile_kernel(model(2, 10, K=(-1.0, 0.7, 0.3)), "c128")
-    assert e.value.code == -6
+
+
+def test_inexact_closed_form_diagonal_is_summed_bond_by_bond():
+    """Kz = 0.3: the partial sums of +-Kz/4 are not exact, so the generated
+    diagonal adds the z bonds, then the Heisenberg bonds, one by one in the
+    reference's order (10 + 30 at 20 sites); Kz = -1 keeps the closed form."""
+    src = esc.precompile_kernel(model(2, 10, K=(-1.0, 0.7, 0.3), J=0.1), "c128")
+    body = src[src.index("esc_diag("):src.index("}", src.index("esc_diag("))]
+    assert body.count("? -kz4 : kz4") == 10 and body.count("? -j4 : j4") == 30
+    assert "__popc((r ^" not in body
+    src = esc.precompile_kernel(model(2, 10, K=(-1.0, -1.0, -1.0), J=0.1), "c128")
+    body = src[src.index("esc_diag("):src.index("}", src.index("esc_diag("))]
+    assert "? -kz4 : kz4" not in body and "__popc((r ^" in body
 
 
 def test_code_object_has_no_packed_fp32(jit_cache):
