@@ -30,7 +30,7 @@ EXPORTS = [
     "esc_batch_create", "esc_batch_destroy", "esc_batch_ground_state", "esc_batch_state", "esc_batch_dynamics",
     "esc_batch_autotune", "esc_spectrum_grad", "esc_two_point", "esc_chain", "esc_chain_connectors",
     "esc_rixs", "esc_szsz", "esc_exact_diag", "esc_kernel_precompile", "esc_dot_noalloc", "esc_kernel_plan",
-    "esc_tridiag_lowest", "esc_cf_eval",
+    "esc_tridiag_lowest", "esc_cf_eval", "esc_table_grad",
 ]
 
 
@@ -119,6 +119,7 @@ def lib():
         "esc_batch_state": ([P, I, P, P], I),
         "esc_batch_autotune": ([P, I, P], I),
         "esc_spectrum_grad": ([P, I, I, I, I, I, D, P, I, P, P, P, P, P, P, P], I),
+        "esc_table_grad": ([P, I, I, I, I, D, P, I, P, P, P, P, P, P, P], I),
         "esc_two_point": ([P, I, I, P, P, P], I),
         "esc_chain": ([I, I, P], I),
         "esc_rixs": ([P, P, I, I, P, D, I, D, D, P, I, P, P], I),

@@ -4,6 +4,8 @@ SpectrumLoss evaluates L(theta) = sum_q (S(w_q) - target_q)^2 with
 S(w) = Im <phi|(w + E0 - i eta - H(theta))^-1|phi>, phi = O|gs(theta)>, and
 its gradient through the library's adjoint (esc_spectrum_grad).
 spectrum_loss(theta, target, spec) wraps it as a torch.autograd.Function.
+TableLoss does the same for a loss over a whole JsJs or JeJe table
+(esc_table_grad).
 """
 from __future__ import annotations
 
@@ -11,7 +13,7 @@ import ctypes
 
 import numpy as np
 
-from ._capi import OPS, check, lib
+from ._capi import DYN_KINDS, OPS, check, lib
 from .engine import Honeycomb, SpinModel, _np_ptr, field111
 
 
@@ -52,6 +54,37 @@ class SpectrumLoss:
                                       _np_ptr(target), ctypes.byref(loss), _np_ptr(spec), _np_ptr(grad),
                                       ctypes.byref(e0), _np_ptr(de0), None))
         return {"loss": loss.value, "grad": grad, "spectrum": spec, "e0": e0.value, "de0": de0}
+
+
+class TableLoss(SpectrumLoss):
+    """A loss over the whole JsJs or JeJe table (esc_table_grad):
+    L = sum over frequencies and columns of (T - target)^2, T the table the
+    CF-mode spectra write (JsJs: 14 columns, JeJe: 5, without the omega
+    column); target has shape (nw, ncols)."""
+
+    def __init__(self, lattice: Honeycomb, omega, eta=0.1, kind="JeJe", gs_steps=300, cf_steps=120, checkpoint=10):
+        super().__init__(lattice, omega, eta=eta, gs_steps=gs_steps, cf_steps=cf_steps, checkpoint=checkpoint)
+        if kind not in ("JsJs", "JeJe"):
+            raise ValueError("kind must be JsJs or JeJe")
+        self.kind = kind
+        self.ncols = 14 if kind == "JsJs" else 5
+
+    def evaluate(self, theta, target):
+        """Returns dict(loss, grad[4], table (nw, 1 + ncols), e0, de0[4])."""
+        target = np.ascontiguousarray(target, np.float64)
+        if target.shape != (len(self.omega), self.ncols):
+            raise ValueError(f"target must have shape ({len(self.omega)}, {self.ncols})")
+        m = self.model(theta)   # holds the arrays the struct points into
+        st = m.struct()
+        loss, e0 = ctypes.c_double(), ctypes.c_double()
+        table = np.zeros((len(self.omega), 1 + self.ncols))
+        grad = np.zeros(4)
+        de0 = np.zeros(4)
+        check(lib().esc_table_grad(ctypes.byref(st), DYN_KINDS[self.kind], self.gs_steps, self.cf_steps,
+                                   self.checkpoint, self.eta, _np_ptr(self.omega), len(self.omega), _np_ptr(target),
+                                   ctypes.byref(loss), _np_ptr(table), _np_ptr(grad), ctypes.byref(e0), _np_ptr(de0),
+                                   None))
+        return {"loss": loss.value, "grad": grad, "table": table, "e0": e0.value, "de0": de0}
 
 
 def _function():

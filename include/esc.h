@@ -433,6 +433,22 @@ int esc_spectrum_grad(const esc_model *model, int op, int site, int gs_steps, in
                       double eta, const double *omega, int nw, const double *target, double *loss,
                       double *spectrum, double *grad, double *e0, double *de0, void *stream);
 
+/* The same gradient for a loss over a whole JsJs or JeJe table (kind
+ * ESC_DYN_JSJS / ESC_DYN_JEJE): L = sum_q sum_c (T_c(w_q) - target[q ncols
+ * + c])^2 over the table's columns c (JsJs: 14, the real and imaginary parts
+ * of xx, yy, zz, xy, xz, yz, tt; JeJe: 5, Im of xx, yy, xy, yx, tt), T as
+ * esc_dynamics writes it in CF mode: the continued fractions of O_r|gs> for
+ * the runs O_r = J_a, J_a + J_b, J_a + i J_b, sum_a J_a, the cross channels by
+ * polarisation, and the elastic terms conj(<J_a>) <J_b> / (w - i eta)
+ * subtracted (src/dynamics.h:222-460).  Every run contributes through its
+ * own recursion adjoint; the ground state's adjoint (one CG solve) collects
+ * O_r^dagger phi_bar_r of all runs and the expectation values' 2 J_a|gs>
+ * terms.  table: nw x (1 + ncols), first column omega (may be NULL).  Other
+ * arguments and outputs as esc_spectrum_grad.  No reference counterpart. */
+int esc_table_grad(const esc_model *model, int kind, int gs_steps, int cf_steps, int checkpoint, double eta,
+                   const double *omega, int nw, const double *target, double *loss, double *table, double *grad,
+                   double *e0, double *de0, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,10 +7,9 @@ same gradient bit for bit, and the torch.autograd wrapper.
 The loss is the output of a truncated Lanczos recursion; its late
 coefficients amplify rounding (the recursion loses orthogonality), so the
 loss computed in floating point carries noise that finite differences
-divide by the step.  Where the recursion is short enough that the loss is
-smooth at 1e-12 (N=8 with 20 steps, N=20 with 40), or reorthogonalised
-(checkpoint = 0), the bar is 1e-6 / 1e-5 relative plus ten times the finite
-differences' own spread between the two steps."""
+divide by the step.  Bars: 1e-6 relative at N=8 (20 steps), 1e-5 at N=20
+with 40 steps, 5e-4 at N=20 with 120 steps.  The table losses (every column
+of the JsJs / JeJe tables) are checked the same way at N=8."""
 import numpy as np
 import pytest
 
@@ -86,17 +85,15 @@ def test_torch_autograd_function():
     assert np.allclose(tgt.grad.numpy(), -2.0 * (r["spectrum"] - target), rtol=1e-12, atol=1e-14)
 
 
-@pytest.mark.parametrize("cf_steps,checkpoint,bar", [(40, 10, 1e-5), (120, 0, 1e-2), (120, 10, 2e-2)])
+@pytest.mark.parametrize("cf_steps,checkpoint,bar", [(40, 10, 1e-5), (120, 0, 5e-4), (120, 10, 5e-4)])
 def test_n20_study_gradient(cf_steps, checkpoint, bar):
     """The 20-site inverse problem: K = -1 + d, J, Gamma, h random, 1000
     frequencies, eta = 0.1; finite differences of the loss in FP64 with steps
-    1e-4 and 5e-5.  Bar on |grad - fd| / |fd| over the four components: 1e-5
-    for 40 steps (measured 7e-7).  For the 120-step study the reverse sweep
-    through the long recursion sums terms that grow with the step index and
-    cancel, and loses digits (measured 6e-3 with reorthogonalised vectors,
-    1e-2 with checkpoints and lost orthogonality; a numpy restatement of the
-    same adjoint shows the same growth with the recursion length), so the
-    bars are 1e-2 and 2e-2."""
+    1e-4 and 5e-5, Richardson-combined.  Bar on |grad - fd| / |fd| over the
+    four components: 1e-5 for 40 steps (measured 5.8e-7); 5e-4 for the
+    120-step study, reorthogonalised or checkpointed (measured 5.3e-5 and
+    2.7e-5; the finite differences' own spread between the two steps is
+    1e-5 to 2e-5 of the gradient there)."""
     rng = np.random.default_rng(20)
     theta = np.array([-1.0 + rng.uniform(0, 0.2), rng.uniform(0, 0.2), rng.uniform(0, 0.2), rng.uniform(0, 0.4)])
     w = np.linspace(0.0, 6.0, 1000)
@@ -113,3 +110,47 @@ def test_n20_study_gradient(cf_steps, checkpoint, bar):
     err = np.linalg.norm(r["grad"] - d) / np.linalg.norm(d)
     print(f"  relative error {err:.3g}", flush=True)
     assert err < bar
+
+
+@pytest.mark.parametrize("kind", ["JeJe", "JsJs"])
+def test_table_loss_gradient_matches_finite_differences(kind):
+    """Loss over the whole table (esc_table_grad): every column, the cross
+    channels by polarisation and the elastic terms, N=8, 20 steps."""
+    w = np.linspace(0.0, 6.0, 150)
+    spec = esc.TableLoss(esc.Honeycomb(2, 4), w, eta=0.1, kind=kind, gs_steps=300, cf_steps=20, checkpoint=10)
+    target = np.random.default_rng(2).uniform(0.0, 0.5, (len(w), spec.ncols))
+    print(f"[table grad N=8] {kind}", flush=True)
+    r = spec.evaluate(THETA, target)
+    assert r["table"].shape == (len(w), 1 + spec.ncols) and np.array_equal(r["table"][:, 0], w)
+    assert abs(r["loss"] - ((r["table"][:, 1:] - target) ** 2).sum()) < 1e-12 * r["loss"]
+    for p in range(4):
+        d = fd(spec, target, THETA, p, 1e-4)
+        print(f"  theta[{p}] adjoint {r['grad'][p]:.12g} fd {d:.12g}", flush=True)
+        assert abs(r["grad"][p] - d) < 1e-6 * max(1.0, abs(d)), f"theta[{p}]: adjoint {r['grad'][p]} fd {d}"
+
+
+def test_table_loss_columns_match_the_spectra_entry_point():
+    """The table the gradient call evaluates is the CF-mode JeJe table of
+    esc_dynamics from the same ground state: same E0, columns within 1e-8."""
+    w = np.linspace(0.0, 6.0, 150)
+    spec = esc.TableLoss(esc.Honeycomb(2, 4), w, eta=0.1, kind="JeJe", gs_steps=300, cf_steps=20, checkpoint=10)
+    r = spec.evaluate(THETA, np.zeros((len(w), 5)))
+    eng = esc.Engine(spec.model(THETA), "c128")
+    gs, e0, _, _ = eng.ground_state(eng.start_vector(3, "Gaussian"), steps=300, restarts=1)
+    assert abs(e0 - r["e0"]) < 1e-10 * abs(e0)
+    tab, _ = eng.dynamics("JeJe", "CF", gs, w, 0.1, e0=r["e0"], cf_steps=20)
+    assert np.abs(tab - r["table"]).max() < 1e-8 * np.abs(tab[:, 1:]).max()
+
+
+def test_hellmann_feynman_at_20_sites():
+    """dE0/dtheta = <gs|dH/dtheta|gs> against finite differences of E0 on the
+    20-site cluster of the inverse-problem study."""
+    rng = np.random.default_rng(20)
+    theta = np.array([-1.0 + rng.uniform(0, 0.2), rng.uniform(0, 0.2), rng.uniform(0, 0.2), rng.uniform(0, 0.4)])
+    w = np.linspace(0.0, 6.0, 50)
+    spec = esc.SpectrumLoss(esc.Honeycomb(2, 10), w, eta=0.1, op="Jex", gs_steps=300, cf_steps=4, checkpoint=2)
+    r = spec.evaluate(theta, np.zeros_like(w))
+    for p in range(4):
+        d = fd(spec, np.zeros_like(w), theta, p, 1e-4, key="e0")
+        print(f"[HF N=20] theta[{p}] {r['de0'][p]:.12g} fd {d:.12g}", flush=True)
+        assert abs(r["de0"][p] - d) < 1e-7 * max(1.0, abs(d))
