@@ -127,13 +127,11 @@ double tridiag_lowest(int n, const double *a, const double *b) {
 namespace esc_impl {
 // One line on stderr per finished stage of a long call (a continued fraction,
 // a ground-state cycle) when the environment sets ESC_VERBOSE to anything but
-// "0"; silent otherwise.
+// "0" (read at every call, so a caller may switch it on and off); silent
+// otherwise.
 void progress(const char *fmt, ...) {
-    static const bool on = [] {
-        const char *v = std::getenv("ESC_VERBOSE");
-        return v != nullptr && v[0] != '\0' && std::strcmp(v, "0") != 0;
-    }();
-    if (!on) return;
+    const char *v = std::getenv("ESC_VERBOSE");
+    if (v == nullptr || v[0] == '\0' || std::strcmp(v, "0") == 0) return;
     static const auto t0 = std::chrono::steady_clock::now();
     char buf[256];
     va_list ap;
@@ -498,12 +496,17 @@ void window_order(int N, const Plan &pl, std::vector<int> &wb, std::vector<int> 
         if (!((pl.W >> b) & 1u)) cb.push_back(b);
 }
 
-std::vector<esc::GTables> build_passes(const esc_ctx *c, int TB, int w, int P, int WPB) {
+std::vector<esc::GTables> build_passes(const esc_ctx *c, int TB, int w, int P, int WPB,
+                                       const std::vector<HostTerm> *custom = nullptr) {
     const int N = c->n_sites;
     if (w > N) w = N;
     if (TB - 6 > 7 || WPB < 1 || ((1 << (TB - 6)) % WPB) != 0) throw EscError(ESC_ERR_UNSUPPORTED, "tile/wave shape");
-    const esc::HTables &h = c->htab;
-    const std::vector<HostTerm> terms = host_terms(c);
+    esc::HTables h = c->htab;
+    if (custom) {   // an operator without a diagonal: every diagonal coupling zero
+        h.kz4 = h.j4 = h.bz = 0.0;
+        h.n_zdiag = h.n_jdiag = 0;
+    }
+    const std::vector<HostTerm> terms = custom ? *custom : host_terms(c);
     const std::vector<Plan> plan = plan_windows(c, terms, TB, w, P);
 
     std::vector<esc::GTables> passes;
@@ -629,11 +632,11 @@ std::vector<esc::GTables> build_passes(const esc_ctx *c, int TB, int w, int P, i
 }
 
 // The generated sub-cube kernels (esc_cube.cpp) for configuration cfg: the
-// cover-first plan of the generic tiled kernel, its terms in bond / flip form.
-// Returns false (with the reason) for models they do not cover: a pair in
-// several bond families (the chain's XYZ bonds) or a Kitaev-z diagonal whose
-// partial sums are not exact (the closed form would not equal the reference
-// order).
+// cover-first plan of the generic tiled kernel seeds the sub-cube planner, its
+// terms in bond / flip form.  Returns false (with the reason) for models they
+// do not cover: a pair in several bond families (the chain's XYZ bonds).  A
+// Kitaev-z diagonal whose closed form would round differently is summed bond
+// by bond in the reference's order.
 bool cube_spec(const esc_ctx *c, int cfg, esc::CubeModel &m, std::vector<esc::CubePass> &passes, std::string *why,
                bool do_plan = true) {
     const esc::HTables &h = c->htab;
@@ -834,6 +837,196 @@ esc::CurrentTable *current_table(esc_ctx *c, int op, int site) {
     hip_check(hipMemcpy(d, t.get(), sizeof(esc::CurrentTable), hipMemcpyHostToDevice), "hipMemcpy");
     c->d_cur[key] = d;
     return d;
+}
+
+// One Pauli string of a current table as a gather term of the tiled kernel:
+// y[r] += w phase(r) x[r ^ m], m the sites of its Sx / Sy factors, phase(r)
+// the product of the factors applied first to last as k_current applies
+// them, tabulated over the bits of r on the sites of its Sy / Sz factors (at
+// most three).  The entries are exact (w times +-2^-n or +-i 2^-n).  False
+// for strings with no flip or more than three phase sites.
+bool string_gather_term(const esc::OpFactor *f, int n, double wr, double wi, HostTerm &t) {
+    uint32_t m = 0;
+    int sel[3], ns = 0;
+    for (int q = 0; q < n; ++q) {
+        if (f[q].op != 2) m ^= 1u << f[q].site;
+        if (f[q].op != 0) {
+            bool have = false;
+            for (int u = 0; u < ns; ++u) have = have || sel[u] == f[q].site;
+            if (!have) {
+                if (ns == 3) return false;
+                sel[ns++] = f[q].site;
+            }
+        }
+    }
+    if (m == 0) return false;
+    if (ns == 0) sel[ns++] = __builtin_ctz(m);
+    while (ns < 3) { sel[ns] = sel[0]; ++ns; }
+    t.mask = m;
+    t.b0 = sel[0]; t.b1 = sel[1]; t.b2 = sel[2];
+    t.cplx = false;
+    for (int e = 0; e < 8; ++e) {
+        uint32_t r = 0;
+        for (int q = 2; q >= 0; --q) r = (r & ~(1u << sel[q])) | ((uint32_t)((e >> q) & 1) << sel[q]);
+        uint32_t k = r ^ m;
+        double cr = 1.0, ci = 0.0;
+        for (int q = 0; q < n; ++q) {
+            const uint32_t site = (uint32_t)f[q].site, bit = (k >> site) & 1u;
+            if (f[q].op == 0) {
+                cr *= 0.5; ci *= 0.5; k ^= 1u << site;
+            } else if (f[q].op == 1) {
+                const double h = bit ? -0.5 : 0.5;
+                const double nr = -ci * h, ni = cr * h;
+                cr = nr; ci = ni; k ^= 1u << site;
+            } else {
+                const double h = bit ? -0.5 : 0.5;
+                cr *= h; ci *= h;
+            }
+        }
+        t.coef[e][0] = wr * cr - wi * ci;
+        t.coef[e][1] = wr * ci + wi * cr;
+        if (t.coef[e][1] != 0.0) t.cplx = true;
+    }
+    return true;
+}
+
+// the value of a gather term's coefficient for row r
+const double *term_value(const HostTerm &t, uint32_t r) {
+    const int e = (int)((r >> t.b0) & 1u) | (int)(((r >> t.b1) & 1u) << 1) | (int)(((r >> t.b2) & 1u) << 2);
+    return t.coef[e];
+}
+
+// All strings of a current table as gather terms, terms with the same mask
+// merged while their phase sites number at most three.  False when some
+// string does not fit (then k_current applies the table).
+bool current_gather_terms(const esc::CurrentTable &ct, std::vector<HostTerm> &out) {
+    out.clear();
+    for (int p = 0; p < ct.n_pairs; ++p) {
+        const esc::CurrentPair &cp = ct.pair[p];
+        for (int side = 0; side < 2; ++side) {
+            const int n = side ? cp.nb : cp.na;
+            if (n == 0) continue;
+            HostTerm t;
+            const double wr = side ? -cp.wb_re : cp.wa_re, wi = side ? -cp.wb_im : cp.wa_im;
+            if (!string_gather_term(side ? cp.fb : cp.fa, n, wr, wi, t)) return false;
+            bool merged = false;
+            for (HostTerm &o : out) {
+                if (o.mask != t.mask) continue;
+                int u[3], nu = 0;
+                for (int site : {o.b0, o.b1, o.b2, t.b0, t.b1, t.b2}) {
+                    bool have = false;
+                    for (int v = 0; v < nu; ++v) have = have || u[v] == site;
+                    if (!have) {
+                        if (nu == 3) { nu = 4; break; }
+                        u[nu++] = site;
+                    }
+                }
+                if (nu > 3) continue;
+                while (nu < 3) { u[nu] = u[0]; ++nu; }
+                HostTerm mt = o;
+                mt.b0 = u[0]; mt.b1 = u[1]; mt.b2 = u[2];
+                mt.cplx = false;
+                for (int e = 0; e < 8; ++e) {
+                    uint32_t r = 0;
+                    for (int q = 2; q >= 0; --q) r = (r & ~(1u << u[q])) | ((uint32_t)((e >> q) & 1) << u[q]);
+                    const double *a = term_value(o, r), *b = term_value(t, r);
+                    mt.coef[e][0] = a[0] + b[0];
+                    mt.coef[e][1] = a[1] + b[1];
+                    if (mt.coef[e][1] != 0.0) mt.cplx = true;
+                }
+                o = mt;
+                merged = true;
+                break;
+            }
+            if (!merged) out.push_back(t);
+        }
+    }
+    return !out.empty();
+}
+
+uint64_t table_hash(const esc::CurrentTable &t) {
+    uint64_t h = 1469598103934665603ull;
+    const unsigned char *p = reinterpret_cast<const unsigned char *>(&t.pair[0]);
+    const size_t n = (size_t)std::max(0, t.n_pairs) * sizeof(esc::CurrentPair);
+    for (size_t i = 0; i < n; ++i) { h ^= p[i]; h *= 1099511628211ull; }
+    h ^= (uint64_t)(uint32_t)t.n_pairs;
+    h *= 1099511628211ull;
+    return h;
+}
+
+void apply_current_table(esc_ctx *c, const esc::CurrentTable &ht, const esc::CurrentTable *dt, const void *x, void *y,
+                         hipStream_t s) {
+    if (x == y) throw EscError(ESC_ERR_ARG, "operator input and output must differ");
+    const int N = c->n_sites;
+    const esc_ctx::CurPass *cp = nullptr;
+    if (N >= 17) {
+        const uint64_t key = table_hash(ht);
+        auto it = c->d_curpass.find(key);
+        if (it == c->d_curpass.end()) {
+            const auto t0 = std::chrono::steady_clock::now();
+            esc_ctx::CurPass e;
+            std::vector<HostTerm> terms;
+            if (current_gather_terms(ht, terms)) {
+                // the generic kernel's shapes for this size and precision
+                const bool c64 = c->c64();
+                std::vector<int> pref;
+                if (N <= 18) pref.push_back(make_config(1, 8, 10, N));
+                else if (N <= 22) pref.push_back(c64 ? make_config(kCoverLast, 16, 12, 17) : make_config(kCoverFirst, 16, 11, 17));
+                else pref.push_back(c64 ? make_config(kCoverFirst, 8, 12, 18) : make_config(kCoverFirst, 16, 12, 18));
+                pref.push_back(make_config(1, 8, 11, N));
+                pref.push_back(make_config(1, 4, 10, N));
+                for (int cfg : pref) {
+                    const int tb = cfg_tile_bits(cfg), waves = cfg_waves(cfg);
+                    if (N < tb || !esc::tile_shape_supported(c64, tb, waves)) continue;
+                    std::vector<esc::GTables> host;
+                    try {
+                        host = build_passes(c, tb, cfg_window(cfg), cfg_passes(cfg), waves, &terms);
+                    } catch (const EscError &) {
+                        continue;
+                    }
+                    bool fits = true;
+                    for (const auto &g : host)
+                        fits = fits && esc::tile_lds_bytes(c64, tb, (int)g.cls_begin[4],
+                                                           (int)g.cls_begin[esc::kNumClasses]) <= esc::kMaxTileLds;
+                    if (!fits) continue;
+                    for (const auto &g : host) {
+                        esc::GTables *d = nullptr;
+                        if (hipMalloc(&d, sizeof(esc::GTables)) != hipSuccess) {
+                            for (auto &q : e.pass) (void)hipFree(q.d);
+                            throw EscError(ESC_ERR_ALLOC, "table allocation failed");
+                        }
+                        e.pass.push_back({d, (int)g.cls_begin[4], (int)g.cls_begin[esc::kNumClasses]});
+                        hip_check(hipMemcpy(d, &g, sizeof(esc::GTables), hipMemcpyHostToDevice), "hipMemcpy");
+                    }
+                    e.config = cfg;
+                    break;
+                }
+            }
+            progress("operator table of %d pairs: %d gather terms, %s (%.1f ms)", ht.n_pairs, (int)terms.size(),
+                     e.config ? "tiled kernel" : "k_current",
+                     std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+            it = c->d_curpass.emplace(key, std::move(e)).first;
+        }
+        if (it->second.config) cp = &it->second;
+    }
+    if (cp) {
+        const int tb = cfg_tile_bits(cp->config), waves = cfg_waves(cp->config);
+        for (const auto &p : cp->pass)
+            hip_check(esc::launch_hpsi_tile(c->c64(), p.d, c->d_htab, tb, waves, p.n_uniform, p.n_terms, x, y, nullptr,
+                                            c->dim, s),
+                      "tiled operator launch");
+        return;
+    }
+    const esc::CurrentTable *d = dt;
+    if (!d) {
+        if (!c->d_combo && hipMalloc(&c->d_combo, sizeof(esc::CurrentTable)) != hipSuccess)
+            throw EscError(ESC_ERR_ALLOC, "table allocation failed");
+        hip_check(hipMemcpyAsync(c->d_combo, &ht, sizeof(esc::CurrentTable), hipMemcpyHostToDevice, s), "table");
+        hip_check(hipStreamSynchronize(s), "sync");   // the host table may go out of scope
+        d = c->d_combo;
+    }
+    hip_check(c->c64() ? esc::launch_current_c64(d, x, y, c->dim, s) : esc::launch_current_c128(d, x, y, c->dim, s),
+              "current launch");
 }
 
 void check_ctx(const esc_ctx *c) {
@@ -1914,9 +2107,9 @@ int esc_apply_op(esc_ctx *ctx, int op, int site, const void *d_in, void *d_out, 
         if (d_in == d_out) throw EscError(ESC_ERR_ARG, "d_in and d_out must not overlap");
         esc::CurrentTable *t = current_table(ctx, op, site);
         hipStream_t s = (hipStream_t)stream;
-        hip_check(ctx->c64() ? esc::launch_current_c64(t, d_in, d_out, ctx->dim, s)
-                             : esc::launch_current_c128(t, d_in, d_out, ctx->dim, s),
-                  "current launch");
+        auto ht = std::make_unique<esc::CurrentTable>();
+        host_current_table(ctx, op, site, ht.get());
+        apply_current_table(ctx, *ht, t, d_in, d_out, s);
     });
 }
 
@@ -2408,9 +2601,7 @@ int esc_dynamics(esc_ctx *ctx, int kind, int method, const void *d_gs, double e0
             for (int a = 0; a < nc; ++a)
                 if (coef[a] != cplx(0.0, 0.0)) append_scaled(t, tabs[a], coef[a].real(), coef[a].imag());
             hip_check(hipMemcpy(ctx->d_combo, &t, sizeof(t), hipMemcpyHostToDevice), "hipMemcpy");
-            hip_check(ctx->c64() ? esc::launch_current_c64(ctx->d_combo, d_gs, ctx->aux(0), ctx->dim, s)
-                                 : esc::launch_current_c128(ctx->d_combo, d_gs, ctx->aux(0), ctx->dim, s),
-                      "current launch");
+            apply_current_table(ctx, t, ctx->d_combo, d_gs, ctx->aux(0), s);
         };
         auto unit = [&](int a) {
             std::vector<cplx> c(nc, cplx(0.0, 0.0));

@@ -96,6 +96,13 @@ struct esc_ctx {
         int n_uniform, n_terms;
     };
     std::map<int, std::vector<PassDev>> d_pass;     // windowed tiled kernel per configuration
+    // operator tables applied by the windowed tiled kernel, per table content
+    // (hash of the host table): configuration and passes, or no entry = k_current
+    struct CurPass {
+        int config = 0;
+        std::vector<PassDev> pass;
+    };
+    std::map<uint64_t, CurPass> d_curpass;
     mutable std::map<int, bool> tile_fits;           // configuration fits the LDS
     int tile_bits = 0;                              // kernel configuration: 0 = one row per lane,
                                                     // else 100 * tile bits + window bits
@@ -116,6 +123,8 @@ struct esc_ctx {
         if (d_cube_cf) (void)hipFree(d_cube_cf);
         for (auto &kv : d_pass)
             for (auto &p : kv.second) (void)hipFree(p.d);
+        for (auto &kv : d_curpass)
+            for (auto &p : kv.second.pass) (void)hipFree(p.d);
     }
     void ensure_aux(size_t nvec) {
         if (nvec <= aux_cap) return;
@@ -176,6 +185,11 @@ void cgs_pass(esc_ctx *c, int k, void *w, hipStream_t s);
 // operator tables (Pauli strings)
 void host_current_table(const esc_ctx *c, int op, int site, esc::CurrentTable *t);
 esc::CurrentTable *current_table(esc_ctx *c, int op, int site);
+// y = O x for the operator of host table ht (device copy dt, may be null):
+// from 17 sites the windowed tiled kernel with the table's Pauli strings as
+// gather terms, else (or when a string does not fit that form) k_current
+void apply_current_table(esc_ctx *c, const esc::CurrentTable &ht, const esc::CurrentTable *dt, const void *x, void *y,
+                         hipStream_t s);
 void append_scaled(esc::CurrentTable &dst, const esc::CurrentTable &src, double cr, double ci);
 
 // FP64 small solves

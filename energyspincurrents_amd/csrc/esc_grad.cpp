@@ -169,10 +169,8 @@ void grad_core(const esc_model &model, const std::vector<int> &base_ops, int osi
     DevVecs TB(host.size() * sizeof(esc::CurrentTable));
     auto *d_tab = static_cast<esc::CurrentTable *>(TB.get());
     hip_check(hipMemcpy(d_tab, host.data(), host.size() * sizeof(esc::CurrentTable), hipMemcpyHostToDevice), "tables");
-    auto tab = [&](int x, int i) { return d_tab + (size_t)x * ntab + i; };
-    auto apply = [&](const esc::CurrentTable *t, const void *x, void *y) {
-        hip_check(esc::launch_current_c128(t, x, y, dim, s), "operator");
-    };
+    auto tab = [&](int x, int i) { return (size_t)x * ntab + i; };
+    auto apply = [&](size_t t, const void *x, void *y) { apply_current_table(c, host[t], d_tab + t, x, y, s); };
     auto axpby = [&](cplx a, const void *x, cplx b, void *y) {
         hip_check(esc::launch_axpby(false, a.real(), a.imag(), x, b.real(), b.imag(), y, dim, s), "axpby");
     };
