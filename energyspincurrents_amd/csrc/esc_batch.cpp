@@ -417,10 +417,8 @@ int esc_batch_dynamics(esc_batch *b, int kind, int cf_steps, double eta, const d
             // give <gs_i|J_a|gs_i>
             for (int i = 0; i < n; ++i) {
                 esc_ctx *c = b->ctx[i];
-                const esc::CurrentTable *t = d_tabs + (size_t)i * R + r;
-                hip_check(b->c64 ? esc::launch_current_c64(t, b->slot(i, 4), b->slot(i, 5), c->dim, b->s)
-                                 : esc::launch_current_c128(t, b->slot(i, 4), b->slot(i, 5), c->dim, b->s),
-                          "current launch");
+                apply_current_table(c, tabs[(size_t)i * R + r], d_tabs + (size_t)i * R + r, b->slot(i, 4), b->slot(i, 5),
+                                    b->s);
                 if (runs[r].kind == 0) {
                     c->ensure_partials(2 * esc::kRedBlocks);
                     hip_check(esc::launch_dot(b->c64, b->slot(i, 4), b->slot(i, 5), c->dim, c->d_partials,
