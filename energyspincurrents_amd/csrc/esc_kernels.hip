@@ -1029,21 +1029,40 @@ __global__ __launch_bounds__(kThreads) void k_b_update(const BatchLane *__restri
     const double c = batch_scale(brow, k), cp = k > 0 ? batch_scale(brow, k - 1) : 0.0;
     const double ar = alpha[2 * ((size_t)inst * lda + k)] * c, ai = alpha[2 * ((size_t)inst * lda + k) + 1] * c;
     const double b = brow[k] * cp;
-    const V2 *v = (const V2 *)l.v, *u = (const V2 *)l.vp;
-    V2 *w = (V2 *)l.w;
     double acc[1] = {0.0};
     const uint64_t stride = (uint64_t)l.nblk * kThreads;
-    for (uint64_t i = (uint64_t)(blockIdx.x - l.blk0) * kThreads + threadIdx.x; i < l.dim; i += stride) {
-        const double vr = v[i].x, vi = v[i].y;
-        double wr = c * (double)w[i].x - (ar * vr - ai * vi) - b * (double)u[i].x;
-        double wi = c * (double)w[i].y - (ar * vi + ai * vr) - b * (double)u[i].y;
-        V2 o;
-        o.x = (R)wr;
-        o.y = (R)wi;
-        w[i] = o;
-        wr = (double)o.x;
-        wi = (double)o.y;
-        acc[0] += wr * wr + wi * wi;
+    const uint64_t first = (uint64_t)(blockIdx.x - l.blk0) * kThreads + threadIdx.x;
+    if constexpr (sizeof(R) == 4) {
+        // complex64: two elements per lane (16-byte accesses); dim is a power
+        // of two >= 2
+        const float4 *v = (const float4 *)l.v, *u = (const float4 *)l.vp;
+        float4 *w = (float4 *)l.w;
+        for (uint64_t i = first; i < l.dim / 2; i += stride) {
+            const float4 x = v[i], y = w[i], z = u[i];
+            const double r0 = c * (double)y.x - (ar * (double)x.x - ai * (double)x.y) - b * (double)z.x;
+            const double i0 = c * (double)y.y - (ar * (double)x.y + ai * (double)x.x) - b * (double)z.y;
+            const double r1 = c * (double)y.z - (ar * (double)x.z - ai * (double)x.w) - b * (double)z.z;
+            const double i1 = c * (double)y.w - (ar * (double)x.w + ai * (double)x.z) - b * (double)z.w;
+            const float4 o = make_float4((float)r0, (float)i0, (float)r1, (float)i1);
+            w[i] = o;
+            acc[0] += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
+            acc[0] += (double)o.z * (double)o.z + (double)o.w * (double)o.w;
+        }
+    } else {
+        const V2 *v = (const V2 *)l.v, *u = (const V2 *)l.vp;
+        V2 *w = (V2 *)l.w;
+        for (uint64_t i = first; i < l.dim; i += stride) {
+            const double vr = v[i].x, vi = v[i].y;
+            double wr = c * (double)w[i].x - (ar * vr - ai * vi) - b * (double)u[i].x;
+            double wi = c * (double)w[i].y - (ar * vi + ai * vr) - b * (double)u[i].y;
+            V2 o;
+            o.x = (R)wr;
+            o.y = (R)wi;
+            w[i] = o;
+            wr = (double)o.x;
+            wi = (double)o.y;
+            acc[0] += wr * wr + wi * wi;
+        }
     }
     block_reduce_store<1>(acc, partials + blockIdx.x);
 }
