@@ -226,10 +226,13 @@ def main():
             eng.set_tile_bits(config)
         else:
             # outside the timed region: the faster of the profiled configurations
-            # (the first call of a generated configuration compiles or loads it)
+            # (the first call of a generated configuration compiles or loads it).
+            # 300 applications each (about 0.1 s): the GPU's clocks settle over
+            # the first tens of milliseconds of load, and the first 50 H.psi run
+            # about 10 % slower than the next 50
             for cfg in PROFILED_CONFIGS[prec]:
                 eng.set_tile_bits(cfg)
-                probe[cfg] = time_hpsi(eng, x, y, 30, 3)[0]
+                probe[cfg] = time_hpsi(eng, x, y, 300, 3)[0]
             config = min(probe, key=probe.get)
             eng.set_tile_bits(config)
         ms, wall = time_hpsi(eng, x, y, args.steps, args.warmup, args.copy_calib and prec == "c128")

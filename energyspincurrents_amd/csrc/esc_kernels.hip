@@ -591,20 +591,46 @@ __device__ __forceinline__ void lanczos_update_body(typename Vec2<R>::T *__restr
                                                     double bi, double e, uint64_t dim, double *__restrict__ partials) {
     double acc[1] = {0.0};
     const uint64_t stride = (uint64_t)gridDim.x * kThreads;
-    for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < dim; i += stride) {
-        const double vr = v[i].x, vi = v[i].y;
-        double wr = a * (double)w[i].x + (br * vr - bi * vi);
-        double wi = a * (double)w[i].y + (br * vi + bi * vr);
-        if (u != nullptr) {
-            wr += e * (double)u[i].x;
-            wi += e * (double)u[i].y;
+    const uint64_t first = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    if constexpr (sizeof(R) == 4) {
+        // complex64: two elements per lane (16-byte accesses); dim is a power
+        // of two >= 2
+        const float4 *v4 = (const float4 *)v, *u4 = (const float4 *)u;
+        float4 *w4 = (float4 *)w;
+        for (uint64_t i = first; i < dim / 2; i += stride) {
+            const float4 x = v4[i], y = w4[i];
+            double r0 = a * (double)y.x + (br * (double)x.x - bi * (double)x.y);
+            double i0 = a * (double)y.y + (br * (double)x.y + bi * (double)x.x);
+            double r1 = a * (double)y.z + (br * (double)x.z - bi * (double)x.w);
+            double i1 = a * (double)y.w + (br * (double)x.w + bi * (double)x.z);
+            if (u != nullptr) {
+                const float4 z = u4[i];
+                r0 += e * (double)z.x;
+                i0 += e * (double)z.y;
+                r1 += e * (double)z.z;
+                i1 += e * (double)z.w;
+            }
+            const float4 o = make_float4((float)r0, (float)i0, (float)r1, (float)i1);
+            w4[i] = o;
+            acc[0] += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
+            acc[0] += (double)o.z * (double)o.z + (double)o.w * (double)o.w;
         }
-        typename Vec2<R>::T o;
-        o.x = (R)wr;
-        o.y = (R)wi;
-        w[i] = o;
-        const double sr = o.x, si = o.y;
-        acc[0] += sr * sr + si * si;
+    } else {
+        for (uint64_t i = first; i < dim; i += stride) {
+            const double vr = v[i].x, vi = v[i].y;
+            double wr = a * (double)w[i].x + (br * vr - bi * vi);
+            double wi = a * (double)w[i].y + (br * vi + bi * vr);
+            if (u != nullptr) {
+                wr += e * (double)u[i].x;
+                wi += e * (double)u[i].y;
+            }
+            typename Vec2<R>::T o;
+            o.x = (R)wr;
+            o.y = (R)wi;
+            w[i] = o;
+            const double sr = o.x, si = o.y;
+            acc[0] += sr * sr + si * si;
+        }
     }
     block_reduce_store<1>(acc, partials + blockIdx.x);
 }
