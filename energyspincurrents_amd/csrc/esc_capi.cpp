@@ -1494,11 +1494,14 @@ void cf_run(esc_ctx *ctx, const void *d_phi, int steps, void *work[3], double *a
 // Lanczos without reorthogonalisation from the unit vector `start`, in the
 // context's vectors 0..2, with the exit rule of esc_lanczos (|dE0| < eps or
 // `steps` iterations; also when the Krylov space is exhausted).  Returns the
-// number n of alphas.  With coef != nullptr the same recursion is replayed
-// (deterministic kernels: the same vectors bit for bit) for exactly n_fixed
-// vectors, accumulating acc += coef[k] v_k (complex coefficients).
+// number n of alphas.  With resid_floor > 0 it also stops once the lowest
+// Ritz pair's residual bound beta_n |z_{n-1}| falls to resid_floor |T|.  With
+// coef != nullptr the same recursion is replayed (deterministic kernels: the
+// same vectors bit for bit) for exactly n_fixed vectors, accumulating
+// acc += coef[k] v_k (complex coefficients).
 int plain_recursion(esc_ctx *c, const void *start, int steps, double eps, const std::complex<double> *coef,
-                    int n_fixed, void *acc, std::vector<double> &ar, std::vector<double> &bt, hipStream_t s) {
+                    int n_fixed, void *acc, std::vector<double> &ar, std::vector<double> &bt, hipStream_t s,
+                    double resid_floor = 0.0) {
     const uint64_t dim = c->dim;
     c->ensure_partials(2 * esc::kRedBlocks);
     const int cap = coef ? n_fixed : steps;
@@ -1535,7 +1538,8 @@ int plain_recursion(esc_ctx *c, const void *start, int steps, double eps, const 
     }
     RecRunner run(c, s, c->vec(0), c->vec(1), c->vec(2), (uint32_t)cap);
     run.start(1.0);
-    double E0 = 1.0;
+    double E0 = 1.0, tnorm = 0.0;
+    std::vector<double> z, work;
     int checked = 0;
     for (;;) {
         const int k = run.advance(steps) - 1;
@@ -1554,6 +1558,19 @@ int plain_recursion(esc_ctx *c, const void *start, int steps, double eps, const 
                 const double D0 = tridiag_lowest(n, ar.data(), bt.data());
                 if (std::fabs(E0 - D0) < eps) done = true;
                 E0 = D0;
+            }
+            // residual floor: the lowest Ritz pair's residual |H y - theta y| =
+            // beta_n |z_{n-1}| has reached what the vector storage can hold
+            tnorm = std::max(tnorm, std::fabs(a0) + std::fabs(bt[n - 1]) + std::fabs(b));
+            if (!done && resid_floor > 0.0 && n >= 2) {
+                bool ok = false;
+                const double th = esc::td_lowest(n, ar.data(), bt.data(), &ok);
+                if (ok) {
+                    z.resize(n);
+                    work.resize(5 * (size_t)n);
+                    esc::td_vector(n, ar.data(), bt.data(), th, work.data(), z.data());
+                    if (b * std::fabs(z[n - 1]) <= resid_floor * tnorm) done = true;
+                }
             }
             if (done) return n;
         }
@@ -2428,7 +2445,11 @@ int esc_ground_state(esc_ctx *ctx, const void *d_v0, int steps, double eps, int 
         long total = 0;
         for (int cyc = 0; cyc <= restarts; ++cyc) {
             std::vector<double> ar, bt;
-            const int n = plain_recursion(ctx, start, steps, eps, nullptr, 0, nullptr, ar, bt, s);
+            // restarts stop once the Ritz residual reaches the storage's
+            // precision (a few units of 2^-24 or 2^-53 of |T|): the vector
+            // cannot hold a better state
+            const double floor = cyc > 0 ? 4.0 * (c64 ? 5.96e-8 : 1.11e-16) : 0.0;
+            const int n = plain_recursion(ctx, start, steps, eps, nullptr, 0, nullptr, ar, bt, s, floor);
             total += n;
             progress("ground state cycle %d: recursion of %d steps", cyc, n);
             std::vector<double> ev(n), Z((size_t)n * n);
@@ -2437,7 +2458,7 @@ int esc_ground_state(esc_ctx *ctx, const void *d_v0, int steps, double eps, int 
             std::vector<std::complex<double>> coef(Z.begin(), Z.begin() + n);   // column 0: the lowest Ritz pair
             hip_check(hipMemsetAsync(d_gs, 0, dim * ctx->esz, s), "clear");
             std::vector<double> ar2, bt2;
-            plain_recursion(ctx, start, steps, eps, coef.data(), n, d_gs, ar2, bt2, s);
+            plain_recursion(ctx, start, steps, eps, coef.data(), n, d_gs, ar2, bt2, s, 0.0);
             total += n - 1;
             progress("ground state cycle %d: replayed", cyc);
             const double ng = device_norm2(ctx, d_gs, s);

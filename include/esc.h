@@ -205,7 +205,9 @@ int esc_exact_diag(esc_ctx *ctx, double *evals, void *d_gs, void *stream);
  * are deterministic, so the replay produces the same vectors -- accumulating
  * |gs> = sum_m Z(m,0) v_m; later cycles restart from the previous |gs>
  * (`restarts` extra cycles; each restart removes the drift that rounding in
- * complex64 storage leaves in the Ritz vector).  e0 = <gs|H|gs>, resid =
+ * complex64 storage leaves in the Ritz vector, and stops early once the
+ * lowest Ritz pair's residual bound reaches four units of the storage
+ * precision times |T|).  e0 = <gs|H|gs>, resid =
  * |H gs - e0 gs| (FP64 reductions), iters = H.psi applications in all.
  * Replaces the ground-state use of Lanczos_Nirav("GS") where the basis
  * cannot be stored (src/DynLanczosSolver.h:38-239). */
