@@ -98,10 +98,10 @@ PROFILED_CONFIGS = {"c128": [3731218], "c64": [4731318]}
 # make a plain one-vector read / write report its own size), from the per-pass
 # rows of the named profile summary.
 MEASURED_TRAFFIC = {
-    # (171444 + 404368) KiB fetched x 2.0 + (262566 + 262145) KiB written
-    ("c128", 3731218): (1716567040, "profiles/hpsi_cube_n24_hbm_traffic.md: esc_cube c128 (config 3731218)"),
-    # (91981 + 131281) KiB fetched x 2.0 + (131073 + 131072) KiB written
-    ("c64", 4731318): (725677056, "profiles/hpsi_cube_n24_hbm_traffic.md: esc_cube c64 (config 4731318)"),
+    # (212053 + 366101) KiB fetched x 2.0 + (262264 + 262144) KiB written
+    ("c128", 3731218): (1721053184, "profiles/hpsi_cube_n24_hbm_traffic.md: esc_cube c128 (config 3731218)"),
+    # (93554 + 134664) KiB fetched x 2.0 + (131077 + 131076) KiB written
+    ("c64", 4731318): (735835136, "profiles/hpsi_cube_n24_hbm_traffic.md: esc_cube c64 (config 4731318)"),
 }
 
 
