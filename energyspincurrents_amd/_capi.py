@@ -30,7 +30,7 @@ EXPORTS = [
     "esc_batch_create", "esc_batch_destroy", "esc_batch_ground_state", "esc_batch_state", "esc_batch_dynamics",
     "esc_batch_autotune", "esc_spectrum_grad", "esc_two_point", "esc_chain", "esc_chain_connectors",
     "esc_rixs", "esc_szsz", "esc_exact_diag", "esc_kernel_precompile", "esc_dot_noalloc", "esc_kernel_plan",
-    "esc_tridiag_lowest", "esc_cf_eval", "esc_table_grad",
+    "esc_tridiag_lowest", "esc_cf_eval", "esc_table_grad", "esc_tridiag_eigenpairs",
 ]
 
 
@@ -133,6 +133,7 @@ def lib():
         "esc_tridiag_eig": ([I, P, P, P, P], I),
         "esc_cf_spectrum": ([I, P, P, D, D, D, P, I, P, P], I),
         "esc_tridiag_lowest": ([I, P, P, P, P, I], I),
+        "esc_tridiag_eigenpairs": ([I, P, P, P, P, I], I),
         "esc_cf_eval": ([I, P, P, D, D, D, P, I, P, I], I),
         "esc_lehmann_spectrum": ([I, P, P, P, P, D, P, I, P], I),
         "esc_autotune": ([P, P, P, I, P, P, P], I),

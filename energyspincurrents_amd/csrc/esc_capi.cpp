@@ -112,6 +112,42 @@ int tridiag_ql(int n, const double *a, const double *b, double *evals, double *Z
     return 0;
 }
 
+// All eigenpairs of T (esc_tridiag.h: Sturm bisection, inverse iteration
+// orthogonalised within clusters of close eigenvalues), on the device (one
+// workgroup) or the host; both compute the same bits.  Z column-major,
+// column j = eigenvector j.
+void tridiag_eigenpairs(int n, const double *a, const double *b, double *ev, double *Z, bool on_device,
+                        hipStream_t s) {
+    if (n < 1 || n > esc::kMaxTridiag) throw EscError(ESC_ERR_ARG, "tridiagonal size must be in [1, 4096]");
+    if (!on_device) {
+        double lo, hi, piv;
+        if (!esc::td_range(n, a, b, &lo, &hi, &piv)) throw EscError(ESC_ERR_NUMERIC, "non-finite tridiagonal matrix");
+        for (int k = 0; k < n; ++k) ev[k] = n == 1 ? a[0] : esc::td_kth(n, a, b, k, lo, hi, piv);
+        const double tn = esc::td_eig_norm(n, a, b);
+        std::vector<double> work(5 * (size_t)n);
+        for (int j = 0; j < n; ++j)
+            if (esc::td_eig_cluster_start(ev, j, tn)) esc::td_eig_vectors(n, a, b, ev, j, tn, work.data(), Z);
+        return;
+    }
+    const size_t nd = 2 * (size_t)n + (size_t)n + (size_t)n * n + (size_t)esc::kTridiagThreads * 5 * n + 1;
+    double *d = nullptr;
+    if (hipMalloc(&d, nd * sizeof(double)) != hipSuccess) throw EscError(ESC_ERR_ALLOC, "tridiagonal workspace");
+    struct Free { double *p; ~Free() { (void)hipFree(p); } } fr{d};
+    double *da = d, *db = da + n, *dev = db + n, *dz = dev + n, *dw = dz + (size_t)n * n;
+    int *dst = reinterpret_cast<int *>(dw + (size_t)esc::kTridiagThreads * 5 * n);
+    std::vector<double> bb(b, b + n);
+    bb[0] = 0.0;
+    hip_check(hipMemcpyAsync(da, a, (size_t)n * sizeof(double), hipMemcpyHostToDevice, s), "hipMemcpy");
+    hip_check(hipMemcpyAsync(db, bb.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, s), "hipMemcpy");
+    hip_check(esc::launch_tridiag_eig(n, da, db, dev, dz, dw, dst, s), "tridiagonal kernel");
+    int st = 0;
+    hip_check(hipMemcpyAsync(&st, dst, sizeof(int), hipMemcpyDeviceToHost, s), "hipMemcpy");
+    hip_check(hipMemcpyAsync(ev, dev, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s), "hipMemcpy");
+    hip_check(hipMemcpyAsync(Z, dz, (size_t)n * n * sizeof(double), hipMemcpyDeviceToHost, s), "hipMemcpy");
+    hip_check(hipStreamSynchronize(s), "sync");
+    if (st) throw EscError(ESC_ERR_NUMERIC, "non-finite tridiagonal matrix");
+}
+
 // Lowest eigenvalue only (the per-iteration convergence test): Sturm-count
 // bisection (esc::td_lowest), O(n) per halving instead of the O(n^2) of a QL
 // solve; the batched solves run the same routine on the device.
@@ -1892,8 +1928,7 @@ void rixs_spectra(esc_ctx *g, esc_ctx *q, int method, int site, const void *d_gs
         std::vector<double> ar, bt;
         const int n = plain_recursion(q, start, cf_steps, 0.0, nullptr, 0, nullptr, ar, bt, s);
         std::vector<double> ev(n), Z((size_t)n * n);
-        if (tridiag_ql(n, ar.data(), bt.data(), ev.data(), Z.data()) != 0)
-            throw EscError(ESC_ERR_NUMERIC, "tridiagonal QL did not converge");
+        tridiag_eigenpairs(n, ar.data(), bt.data(), ev.data(), Z.data(), true, s);
         std::vector<cplx> y(n, cplx(0.0, 0.0));
         for (int j = 0; j < n; ++j) {
             const cplx f = std::sqrt(n2) * Z[(size_t)j * n] * inv_complex(0.0 - ev[j], gamma);
@@ -2328,8 +2363,7 @@ int esc_lanczos(esc_ctx *ctx, const void *d_v0, int steps, double eps, int mode,
         for (int k = 0; k <= n; ++k) beta[k] = bt[k];
         ctx->Z.assign((size_t)n * n, 0.0);
         std::vector<double> ev(n);
-        if (tridiag_ql(n, ar.data(), bt.data(), ev.data(), ctx->Z.data()) != 0)
-            throw EscError(ESC_ERR_NUMERIC, "tridiagonal QL did not converge");
+        tridiag_eigenpairs(n, ar.data(), bt.data(), ev.data(), ctx->Z.data(), true, s);
         ctx->z_n = n;
         ctx->evals = ev;
         if (evals) std::memcpy(evals, ev.data(), sizeof(double) * n);
@@ -2452,10 +2486,14 @@ int esc_ground_state(esc_ctx *ctx, const void *d_v0, int steps, double eps, int 
             const int n = plain_recursion(ctx, start, steps, eps, nullptr, 0, nullptr, ar, bt, s, floor);
             total += n;
             progress("ground state cycle %d: recursion of %d steps", cyc, n);
-            std::vector<double> ev(n), Z((size_t)n * n);
-            if (tridiag_ql(n, ar.data(), bt.data(), ev.data(), Z.data()) != 0)
-                throw EscError(ESC_ERR_NUMERIC, "tridiagonal QL did not converge");
-            std::vector<std::complex<double>> coef(Z.begin(), Z.begin() + n);   // column 0: the lowest Ritz pair
+            // the lowest Ritz pair (the routine the batched ground states run on
+            // the device): its eigenvector gives the replay coefficients
+            bool ok = false;
+            const double lam = esc::td_lowest(n, ar.data(), bt.data(), &ok);
+            if (!ok) throw EscError(ESC_ERR_NUMERIC, "non-finite tridiagonal matrix");
+            std::vector<double> z(n), work(5 * (size_t)n);
+            esc::td_vector(n, ar.data(), bt.data(), lam, work.data(), z.data());
+            std::vector<std::complex<double>> coef(z.begin(), z.end());
             hip_check(hipMemsetAsync(d_gs, 0, dim * ctx->esz, s), "clear");
             std::vector<double> ar2, bt2;
             plain_recursion(ctx, start, steps, eps, coef.data(), n, d_gs, ar2, bt2, s, 0.0);
@@ -2835,6 +2873,17 @@ int esc_tridiag_eig(int n, const double *a, const double *b, double *evals, doub
         check_ptr(b, "b");
         check_ptr(evals, "evals");
         if (tridiag_ql(n, a, b, evals, Z) != 0) throw EscError(ESC_ERR_NUMERIC, "tridiagonal QL did not converge");
+    });
+}
+
+int esc_tridiag_eigenpairs(int n, const double *a, const double *b, double *evals, double *Z, int on_device) {
+    return guarded([&] {
+        check_ptr(a, "a");
+        check_ptr(b, "b");
+        check_ptr(evals, "evals");
+        check_ptr(Z, "Z");
+        if (n < 1 || n > esc::kMaxTridiag) throw EscError(ESC_ERR_ARG, "n must be in [1, 4096]");
+        tridiag_eigenpairs(n, a, b, evals, Z, on_device != 0, nullptr);
     });
 }
 

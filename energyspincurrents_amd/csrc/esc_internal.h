@@ -241,6 +241,15 @@ hipError_t launch_batch_tridiag(int n, const double *alpha, int lda, const doubl
 hipError_t launch_batch_cf(int n, const double *alpha, int lda, const double *beta, int ldb, int steps,
                            const double *n2, const double *e0, double eta, const double *omega, int nw, void *f,
                            hipStream_t s);
+// All eigenpairs of the tridiagonal matrix a[0..n-1], b[1..n-1] (device
+// arrays): ascending eigenvalues ev[n], column j of Z (n x n, column-major)
+// the unit eigenvector j; work holds kTridiagThreads * 5 * n doubles;
+// status[0] = 1 for a non-finite matrix.  Sturm bisection and inverse
+// iteration, orthogonalised within clusters of close eigenvalues.
+constexpr int kMaxTridiag = 4096;
+constexpr int kTridiagThreads = 256;
+hipError_t launch_tridiag_eig(int n, const double *a, const double *b, double *ev, double *Z, double *work,
+                              int *status, hipStream_t s);
 // convert FP64 complex device buffer to c64 (or copy for c128)
 hipError_t launch_convert_from_c128(bool c64, const void *src, void *dst, uint64_t dim, hipStream_t s);
 hipError_t launch_convert_to_c128(bool c64, const void *src, void *dst, uint64_t dim, hipStream_t s);

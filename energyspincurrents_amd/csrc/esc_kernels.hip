@@ -1587,4 +1587,36 @@ hipError_t launch_batch_cf(int n, const double *alpha, int lda, const double *be
     return hipGetLastError();
 }
 
+// ---- all eigenpairs of one tridiagonal matrix (n <= kMaxTridiag) ----
+// One workgroup: every thread bisects for its eigenvalues, then takes the
+// clusters that start at its indices and computes their vectors in order.
+__global__ __launch_bounds__(256) void k_tridiag_eig(int n, const double *__restrict__ a,
+                                                     const double *__restrict__ b, double *__restrict__ ev,
+                                                     double *__restrict__ Z, double *__restrict__ work,
+                                                     int *__restrict__ status) {
+    __shared__ double s_ev[kMaxTridiag];
+    __shared__ double s_lo, s_hi, s_piv, s_tn;
+    __shared__ int s_ok;
+    if (threadIdx.x == 0) {
+        s_ok = td_range(n, a, b, &s_lo, &s_hi, &s_piv) ? 1 : 0;
+        s_tn = td_eig_norm(n, a, b);
+        status[0] = s_ok ? 0 : 1;
+    }
+    __syncthreads();
+    if (!s_ok) return;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) s_ev[k] = n == 1 ? a[0] : td_kth(n, a, b, k, s_lo, s_hi, s_piv);
+    __syncthreads();
+    for (int k = threadIdx.x; k < n; k += blockDim.x) ev[k] = s_ev[k];
+    for (int j = threadIdx.x; j < n; j += blockDim.x)
+        if (td_eig_cluster_start(s_ev, j, s_tn))
+            td_eig_vectors(n, a, b, s_ev, j, s_tn, work + (size_t)threadIdx.x * 5 * n, Z);
+}
+
+hipError_t launch_tridiag_eig(int n, const double *a, const double *b, double *ev, double *Z, double *work,
+                              int *status, hipStream_t s) {
+    if (n < 1 || n > kMaxTridiag) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_tridiag_eig, dim3(1), dim3(kTridiagThreads), 0, s, n, a, b, ev, Z, work, status);
+    return hipGetLastError();
+}
+
 }  // namespace esc

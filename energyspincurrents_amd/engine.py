@@ -528,6 +528,21 @@ def tridiag_lowest(alpha, beta, device=False):
     return e0.value, z
 
 
+def tridiag_eigenpairs(alpha, beta, device=False):
+    """Every eigenpair (evals, Z) of the tridiagonal matrix of a recursion
+    (esc_tridiag_eigenpairs): bisection and inverse iteration, on the host or
+    with device=True on the current device; Z[:, j] is eigenvector j."""
+    a = np.ascontiguousarray(np.real(alpha), np.float64)
+    b = np.ascontiguousarray(np.abs(beta), np.float64)
+    n = len(a)
+    if len(b) < n:
+        raise ValueError("beta needs n entries (beta[0] ignored)")
+    ev = np.zeros(n)
+    Z = np.zeros(n * n)
+    check(lib().esc_tridiag_eigenpairs(n, _np_ptr(a), _np_ptr(b), _np_ptr(ev), _np_ptr(Z), int(bool(device))))
+    return ev, Z.reshape(n, n).T.copy()
+
+
 def cf_eval(alpha, beta, norm2, e0, eta, omega, device=False):
     """The raw continued fraction of a recursion on every omega (esc_cf_eval),
     on the host threads or, with device=True, by the batched spectra's kernel."""

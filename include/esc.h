@@ -343,6 +343,16 @@ int esc_params_load(const char *path, esc_params *out);
  * (may be NULL for eigenvalues only). */
 int esc_tridiag_eig(int n, const double *a, const double *b, double *evals, double *Z);
 
+/* Every eigenpair of T (a, b as above): Sturm-count bisection for the
+ * eigenvalues, inverse iteration for the vectors, each iterate
+ * orthogonalised against the earlier vectors of its cluster (consecutive
+ * eigenvalues within 1e-3 |T|_1, LAPACK dstein's criterion); ascending
+ * evals, Z column-major (column j = eigenvector j, first nonzero component
+ * positive).  on_device = 0: the host; 1: one workgroup on the current
+ * device -- the solver esc_lanczos and the RIXS fractions use -- with the
+ * same bits.  n <= 4096.  TriDiag (src/DynLanczosSolver.h:371-414). */
+int esc_tridiag_eigenpairs(int n, const double *a, const double *b, double *evals, double *Z, int on_device);
+
 /* Lowest eigenpair of T (a, b as above, every b[k] != 0 for 1 <= k < n):
  * Sturm-count bisection for the eigenvalue, inverse iteration with a
  * partially pivoted LU of T - e0 I for the unit eigenvector z (may be NULL;

@@ -125,3 +125,21 @@ def test_batch_ground_state_energies_are_the_lowest_eigenvalues_of_the_returned_
         assert e0[i] == e_h, (i, e0[i], e_h)
         T = np.diag(al[i, :n]) + np.diag(be[i, 1:n], 1) + np.diag(be[i, 1:n], -1)
         assert abs(e0[i] - np.linalg.eigvalsh(T)[0]) < 1e-12 * max(1.0, abs(e0[i]))
+
+
+def test_device_all_eigenpairs_equal_the_host():
+    """The tridiagonal eigensolver esc_lanczos and the RIXS fractions run on
+    the device (bisection + clustered inverse iteration) computes the host
+    routine's bits, on random matrices, Wilkinson's W21+ (close pairs) and a
+    plain recursion's matrix with repeated eigenvalues."""
+    rng = np.random.default_rng(41)
+    m = O.Model(2, 4, K=(-1.0, -1.0, -1.0), B=O.field111(0.3))
+    al, be, _ = O.lanczos_plain(lambda x: O.hpsi_gather(m, x), rng.standard_normal(m.dim) + 0j, 300)
+    cases = [(al, be[:len(al)]), (np.abs(10.0 - np.arange(21.0)), np.r_[0.0, np.ones(20)])]
+    for n in (1, 3, 64, 400):
+        cases.append((rng.standard_normal(n), np.r_[0.0, 0.05 + np.abs(rng.standard_normal(n - 1))]))
+    for a, b in cases:
+        print(f"[device eigenpairs] n={len(a)}", flush=True)
+        ev_h, Z_h = esc.tridiag_eigenpairs(a, b)
+        ev_d, Z_d = esc.tridiag_eigenpairs(a, b, device=True)
+        assert np.array_equal(ev_h, ev_d) and np.array_equal(Z_h, Z_d)

@@ -121,3 +121,43 @@ def test_continued_fraction_equals_dense_resolvent():
     got = esc.cf_eval(al, be[:len(al)], n2, 0.0, eta, om)
     ref = np.array([np.vdot(phi, np.linalg.solve((w - 1j * eta) * np.eye(60) - H, phi)) for w in om])
     assert np.abs(got - ref).max() < 1e-8 * np.abs(ref).max()
+
+
+def check_eigenpairs(a, b, ev, Z, tol_res=1e-13, tol_orth=1e-12):
+    T = tmat(a, b)
+    w = np.linalg.eigvalsh(T)
+    tn = np.abs(T).sum(axis=1).max()
+    assert np.abs(ev - w).max() <= 16 * np.finfo(float).eps * tn
+    assert np.linalg.norm(T @ Z - Z * ev, axis=0).max() <= tol_res * tn
+    assert np.abs(Z.T @ Z - np.eye(len(a))).max() <= tol_orth
+
+
+@pytest.mark.parametrize("n,scale", [(1, 1.0), (2, 1.0), (50, 1.0), (300, 1e-2)])
+def test_all_eigenpairs_match_lapack(n, scale):
+    rng = np.random.default_rng(100 + n)
+    a = scale * rng.standard_normal(n)
+    b = np.r_[0.0, scale * (0.05 + np.abs(rng.standard_normal(n - 1)))]
+    ev, Z = esc.tridiag_eigenpairs(a, b)
+    check_eigenpairs(a, b, ev, Z)
+    assert np.all(Z[np.argmax(Z != 0, axis=0), np.arange(n)] > 0)
+
+
+def test_all_eigenpairs_of_wilkinson_matrix_with_close_pairs():
+    """W21+: diagonal |10 - k|, off-diagonal 1; its largest eigenvalues come in
+    pairs that agree to 1e-14, so the clusters' vectors must be
+    orthogonalised against each other."""
+    a = np.abs(10.0 - np.arange(21.0))
+    b = np.r_[0.0, np.ones(20)]
+    ev, Z = esc.tridiag_eigenpairs(a, b)
+    assert ev[-1] - ev[-2] < 1e-12
+    check_eigenpairs(a, b, ev, Z)
+
+
+def test_all_eigenpairs_of_a_plain_recursion_with_repeated_eigenvalues():
+    m = O.Model(2, 4, K=(-1.0, -1.0, -1.0), B=O.field111(0.3))
+    H = O.hamiltonian_dense_kron(m)
+    rng = np.random.default_rng(7)
+    al, be, _ = O.lanczos_plain(lambda x: H @ x, rng.standard_normal(m.dim) + 0j, 300)
+    n = len(al)
+    ev, Z = esc.tridiag_eigenpairs(al, be[:n])
+    check_eigenpairs(al, be[:n], ev, Z)
