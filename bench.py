@@ -196,6 +196,15 @@ def cpu_coo_baseline(reps):
                       "oracle/esc_oracle.c orc_coo_build + orc_coo_apply, OpenMP"}
 
 
+_T0 = time.perf_counter()
+
+
+def progress(stage):
+    """One stderr line per finished stage (the JSON line stays the only
+    stdout output)."""
+    print(f"[bench] {stage} done at {time.perf_counter() - _T0:.1f} s", file=sys.stderr, flush=True)
+
+
 def main():
     args = parse()
     # host legs: threads bound close to their cores (read by OpenMP when the
@@ -248,10 +257,13 @@ def main():
                      "hpsi_per_s": 1e3 / ms, "config": config, "kernel": kname,
                      "traffic": traffic, "traffic_source": tsrc,
                      "probe_ms": {str(k): v for k, v in probe.items()}}
+        progress(f"{prec} H.psi")
         res[prec]["step_ms"] = time_lanczos_step(eng, x, 200)
+        progress(f"{prec} Lanczos step")
         del x, y
         if not args.no_spectrum:
             spec[prec] = spectrum_run(eng, None if prec == "c128" else spec["c128"]["gs"])
+            progress(f"{prec} spectrum")
         del eng
         torch.cuda.empty_cache()
 
@@ -300,8 +312,10 @@ def main():
         out["spectrum"] = spectrum_summary(spec)
     if not args.no_n32:
         out["n32"] = n32_run(esc)
+        progress("32-site H.psi")
     if not args.no_batch:
         out["batch"] = batch_run(esc)
+        progress("batched scan")
     if args.pipelines:
         out["pipelines"] = pipelines_run(esc)
     e2e_inputs = None
@@ -309,8 +323,10 @@ def main():
         out["jeje_end_to_end"], e2e_inputs = jeje_e2e_run(esc)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_reps)
+        progress("CPU gather baseline")
         if not args.no_cpu_coo:
             out["cpu_baseline"]["ref_algorithm"] = cpu_coo_baseline(args.cpu_reps)
+            progress("CPU reference-algorithm baseline")
         if e2e_inputs is not None:
             out["cpu_baseline"]["jeje_end_to_end"] = cpu_jeje_e2e(e2e_inputs)
     else:

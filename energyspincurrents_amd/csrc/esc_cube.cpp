@@ -27,7 +27,7 @@ namespace esc {
 namespace {
 
 constexpr uint32_t kCubeMaxGrid = 1u << 22;
-constexpr int kFarAhead = 2;   // far sub-cubes in flight ahead of their terms
+constexpr int kFarAhead = 1;   // far sub-cubes in flight ahead of their terms
 
 // ---------------------------------------------------------------------------
 // Layout of one pass
@@ -301,6 +301,72 @@ std::string scatter(const std::string &var, const std::vector<int> &bits, int fr
     return o.str();
 }
 
+// Diagonal of a lane's 2^k rows, added to the accumulators as d(r) psi(r)
+// (after the previous passes' output when the pass is not the first).  The
+// rows are R0 | X with X the slot's row bits, so for a bond-distance group
+// (r ^ (r >> d)) & m = A ^ C with A = (R0 ^ (R0 >> d)) & m formed once per
+// lane and C = (X ^ (X >> d)) & m a constant of the slot; groups whose masks
+// are disjoint share one word, and popc(r) = popc(R0) + popc(X).  The counts
+// are the closed form's integers, so d(r) is the same double.
+void emit_diag(std::ostringstream &o, const CubeModel &m, const Layout &L, int S, bool first) {
+    o << "    const unsigned R0 = base | grw | lane;\n";
+    auto acc = [&](int x, const std::string &d) {
+        const std::string X = std::to_string(x);
+        if (first)
+            o << "      ar[" << X << "] = " << d << " * p[" << X << "].x; ai[" << X << "] = " << d << " * p[" << X << "].y;\n";
+        else
+            o << "      ar[" << X << "] = fma(" << d << ", p[" << X << "].x, ar[" << X << "]); ai[" << X << "] = fma(" << d
+              << ", p[" << X << "].y, ai[" << X << "]);\n";
+    };
+    if (m.seq_diag) {
+        for (int x = 0; x < S; ++x) {
+            o << "    { const R d = (R)esc_diag(R0 | " << hex32(L.slot_row((uint32_t)x)) << ", cf[13], cf[14], cf[15]);\n";
+            acc(x, "d");
+            o << "    }\n";
+        }
+        return;
+    }
+    // pack groups with disjoint masks into words
+    struct Word { uint32_t mask = 0; std::vector<std::pair<uint32_t, uint32_t>> grp; };
+    auto pack = [](const uint32_t *gd, const uint32_t *gm, uint32_t ng) {
+        std::vector<Word> w;
+        for (uint32_t q = 0; q < ng; ++q) {
+            size_t i = 0;
+            while (i < w.size() && (w[i].mask & gm[q])) ++i;
+            if (i == w.size()) w.emplace_back();
+            w[i].mask |= gm[q];
+            w[i].grp.push_back({gd[q], gm[q]});
+        }
+        return w;
+    };
+    const std::vector<Word> zw = pack(m.zgrp_d, m.zgrp_m, m.n_zgrp);
+    const std::vector<Word> jw = m.nj ? pack(m.jgrp_d, m.jgrp_m, m.n_jgrp) : std::vector<Word>();
+    auto word_expr = [](const std::string &r, const Word &w) {
+        std::string e = "0u";
+        for (const auto &g : w.grp)
+            e += " | ((" + r + " ^ (" + r + " >> " + std::to_string(g.first) + ")) & " + hex32(g.second) + ")";
+        return e;
+    };
+    auto word_const = [](uint32_t X, const Word &w) {
+        uint32_t c = 0;
+        for (const auto &g : w.grp) c |= (X ^ (X >> g.first)) & g.second;
+        return c;
+    };
+    for (size_t i = 0; i < zw.size(); ++i) o << "    const unsigned zw" << i << " = " << word_expr("R0", zw[i]) << ";\n";
+    for (size_t i = 0; i < jw.size(); ++i) o << "    const unsigned jw" << i << " = " << word_expr("R0", jw[i]) << ";\n";
+    o << "    const unsigned pc0 = __popc(R0);\n";
+    for (int x = 0; x < S; ++x) {
+        const uint32_t X = L.slot_row((uint32_t)x);
+        std::string cz = "0u", cj = "0u";
+        for (size_t i = 0; i < zw.size(); ++i) cz += " + __popc(zw" + std::to_string(i) + " ^ " + hex32(word_const(X, zw[i])) + ")";
+        for (size_t i = 0; i < jw.size(); ++i) cj += " + __popc(jw" + std::to_string(i) + " ^ " + hex32(word_const(X, jw[i])) + ")";
+        o << "    { const R d = (R)esc_diag_c(" << cz << ", " << cj << ", pc0 + " << __builtin_popcount(X)
+          << "u, cf[13], cf[14], cf[15]);\n";
+        acc(x, "d");
+        o << "    }\n";
+    }
+}
+
 void emit_pass(std::ostringstream &o, const CubeModel &m, const CubePass &p, int index) {
     Layout L;
     L.N = m.n_sites;
@@ -394,23 +460,17 @@ void emit_pass(std::ostringstream &o, const CubeModel &m, const CubePass &p, int
     o << "    {\n      const unsigned va = lane | gix;\n";
     for (int x = 0; x < S; ++x) o << "      p[" << x << "] = tile[va + " << hex32(L.slot_off(x)) << "];\n";
     o << "    }\n";
-    // accumulator: the diagonal (first pass), else zero; a later pass reads
-    // the previous passes' output here and adds it after its own terms, so
-    // the loads are in flight during the term work
+    // accumulators: zero in the first pass; a later pass starts from the
+    // previous passes' output (streamed loads, in flight while the first
+    // terms run).  The pass that holds the diagonal adds d(r) psi(r) first.
     for (int x = 0; x < S; ++x) {
-        if (p.diag && p.first)
-            o << "    { const R d = (R)esc_diag(base | grw | lane | " << hex32(L.slot_row(x))
-              << ", cf[13], cf[14], cf[15]); ar[" << x << "] = d * p[" << x << "].x; ai[" << x << "] = d * p[" << x
-              << "].y; }\n";
-        else
+        if (p.first)
             o << "    ar[" << x << "] = 0; ai[" << x << "] = 0;\n";
+        else
+            o << "    { const NV2 v = __builtin_nontemporal_load(reinterpret_cast<const NV2 *>(out + (base | grw | "
+              << hex32(L.slot_row(x)) << ") + lane)); ar[" << x << "] = v.x; ai[" << x << "] = v.y; }\n";
     }
-    if (!p.first) {
-        o << "    NV2 prev[" << S << "];\n";
-        for (int x = 0; x < S; ++x)
-            o << "    prev[" << x << "] = __builtin_nontemporal_load(reinterpret_cast<const NV2 *>(out + (base | grw | "
-              << hex32(L.slot_row(x)) << ") + lane));\n";
-    }
+    if (p.diag) emit_diag(o, m, L, S, p.first);
     auto emit_terms = [&](const std::vector<std::pair<CubeTerm, uint32_t>> &ts, const std::string &src) {
         for (const auto &pr : ts) {
             const CubeTerm &t = pr.first;
@@ -485,9 +545,6 @@ void emit_pass(std::ostringstream &o, const CubeModel &m, const CubePass &p, int
         while (!pending.empty() && (done_far + 1) * nearg.size() <= (i + 1) * farg.size()) retire_far();
     }
     while (!pending.empty()) retire_far();
-    if (!p.first)
-        for (int x = 0; x < S; ++x)
-            o << "    ar[" << x << "] += prev[" << x << "].x; ai[" << x << "] += prev[" << x << "].y;\n";
     // store (streamed) and the tile's share of <in|out> from the staged rows
     for (int x = 0; x < S; ++x) {
         o << "    { NV2 v = {ar[" << x << "], ai[" << x << "]}; __builtin_nontemporal_store(v, reinterpret_cast<NV2 *>(out + "
@@ -574,7 +631,7 @@ bool cube_plan(const CubeModel &m, const std::vector<CubeTerm> &terms, int tile_
     for (size_t i = 0; i < passes.size(); ++i) {
         passes[i].first = i == 0;
         passes[i].last = i + 1 == passes.size();
-        passes[i].diag = passes[i].first;
+        passes[i].diag = passes[i].last;
     }
     std::lock_guard<std::mutex> lk(mu);
     cache[key] = passes;
@@ -591,9 +648,9 @@ std::string cube_source(const CubeModel &m, const std::vector<CubePass> &passes)
     // diagonal: the closed form where every partial sum of the Kitaev-z part is
     // exact (it then equals the reference's bond-by-bond sum), else that sum
     // itself, z bonds then Heisenberg bonds in the reference's order
-    o << "__device__ __forceinline__ double esc_diag(unsigned r, double kz4, double j4, double bz) {\n";
-    o << "#pragma clang fp contract(off)\n";
     if (m.seq_diag) {
+        o << "__device__ __forceinline__ double esc_diag(unsigned r, double kz4, double j4, double bz) {\n";
+        o << "#pragma clang fp contract(off)\n";
         o << "  double d = 0.0;\n";
         for (uint32_t q = 0; q < m.n_zpair; ++q)
             o << "  d += (((r >> " << (m.zpair[q] & 0xffu) << ") ^ (r >> " << ((m.zpair[q] >> 8) & 0xffu)
@@ -606,18 +663,15 @@ std::string cube_source(const CubeModel &m, const std::vector<CubePass> &passes)
         for (size_t i = 0; i < passes.size(); ++i) emit_pass(o, m, passes[i], (int)i);
         return o.str();
     }
-    o << "  unsigned cz = 0;\n";
-    for (uint32_t q = 0; q < m.n_zgrp; ++q)
-        o << "  cz += __popc((r ^ (r >> " << m.zgrp_d[q] << ")) & " << hex32(m.zgrp_m[q]) << ");\n";
+    // the closed form from the three counts: anti-aligned z bonds, anti-aligned
+    // Heisenberg bonds, down spins (emit_diag forms them per row)
+    o << "__device__ __forceinline__ double esc_diag_c(unsigned cz, unsigned cj, unsigned pc, double kz4, double j4, "
+         "double bz) {\n";
+    o << "#pragma clang fp contract(off)\n";
     o << "  double d = kz4 * (double)(" << (int)m.nz << " - 2 * (int)cz);\n";
-    if (m.nj) {
-        o << "  unsigned cj = 0;\n";
-        for (uint32_t q = 0; q < m.n_jgrp; ++q)
-            o << "  cj += __popc((r ^ (r >> " << m.jgrp_d[q] << ")) & " << hex32(m.jgrp_m[q]) << ");\n";
-        o << "  d = d + j4 * (double)(" << (int)m.nj << " - 2 * (int)cj);\n";
-    }
-    o << "  d = d + bz * (0.5 * (double)(" << m.n_sites << " - 2 * (int)__popc(r)));\n";
-    o << "  (void)j4;\n  return d;\n}\n\n";
+    if (m.nj) o << "  d = d + j4 * (double)(" << (int)m.nj << " - 2 * (int)cj);\n";
+    o << "  d = d + bz * (0.5 * (double)(" << m.n_sites << " - 2 * (int)pc));\n";
+    o << "  (void)cj; (void)j4;\n  return d;\n}\n\n";
     for (size_t i = 0; i < passes.size(); ++i) emit_pass(o, m, passes[i], (int)i);
     return o.str();
 }

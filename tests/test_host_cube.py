@@ -73,10 +73,50 @@ def test_inexact_closed_form_diagonal_is_summed_bond_by_bond():
     src = esc.precompile_kernel(model(2, 10, K=(-1.0, 0.7, 0.3), J=0.1), "c128")
     body = src[src.index("esc_diag("):src.index("}", src.index("esc_diag("))]
     assert body.count("? -kz4 : kz4") == 10 and body.count("? -j4 : j4") == 30
-    assert "__popc((r ^" not in body
+    assert "esc_diag_c" not in src
     src = esc.precompile_kernel(model(2, 10, K=(-1.0, -1.0, -1.0), J=0.1), "c128")
-    body = src[src.index("esc_diag("):src.index("}", src.index("esc_diag("))]
-    assert "? -kz4 : kz4" not in body and "__popc((r ^" in body
+    assert "? -kz4 : kz4" not in src and "esc_diag(" not in src
+    # closed form: per-lane words of (R0 ^ (R0 >> d)) & m, one popcount per
+    # word and row of the lane's sub-cube
+    assert "esc_diag_c(" in src and "const unsigned zw0 = 0u | ((R0 ^ (R0 >>" in src
+
+
+def test_closed_form_diagonal_counts_match_bond_by_bond():
+    """The generated closed form counts anti-aligned bonds per row from words
+    formed once per lane, (R0 ^ (R0 >> d)) & m, xor a constant of the slot;
+    evaluate those emitted expressions on the host for random rows and compare
+    with the bond-by-bond counts of the lattice."""
+    mdl = model(3, 8, K=(-1.0, -1.0, -1.0), J=0.1, B=esc.field111(0.2))
+    src = esc.precompile_kernel(mdl, "c128", 3731218)
+    last = src[src.rindex("// pass "):]
+    words = dict(re.findall(r"const unsigned ((?:zw|jw)\d+) = (0u(?: \| \(\(R0 \^ \(R0 >> \d+\)\) & 0x[0-9a-f]+u\))+);", last))
+    calls = re.findall(r"esc_diag_c\(([^,]+), ([^,]+), pc0 \+ (\d+)u,", last)
+    slot_rows = [int(x, 16) for x in re.findall(r"out \+ \(base \| grw \| 0x([0-9a-f]+)u\) \+ lane\)\); ar\[\d+\] = v\.x", last)]
+    assert len(calls) == len(slot_rows) == 8 and words
+    om = O.Model(3, 8, K=(-1.0, -1.0, -1.0), J=0.1, B=O.field111(0.2))
+    px, py, pz = om.bonds            # families x, y, z (src/Hamiltonian.h:230-328)
+    bonds_z = [tuple(b) for b in pz]
+    bonds_all = [tuple(b) for fam in (px, py, pz) for b in fam]
+    assert len(bonds_z) == 12 and len(bonds_all) == 36
+    rng = np.random.default_rng(5)
+    free = 0xffffff
+    for X in slot_rows:
+        free &= ~X
+    def ev(expr, R0):
+        return eval(expr.replace("u", "").replace("R0", str(R0)))
+    for _ in range(200):
+        R0 = int(rng.integers(0, 1 << 24)) & free
+        vals = {k: ev(e, R0) for k, e in words.items()}
+        for (cz_e, cj_e, pcd), X in zip(calls, slot_rows):
+            r = R0 | X
+            def cnt(e):
+                tot = 0
+                for w, c in re.findall(r"__popc\(((?:zw|jw)\d+) \^ 0x([0-9a-f]+)u\)", e):
+                    tot += bin(vals[w] ^ int(c, 16)).count("1")
+                return tot
+            assert cnt(cz_e) == sum(((r >> i) ^ (r >> j)) & 1 for i, j in bonds_z)
+            assert cnt(cj_e) == sum(((r >> i) ^ (r >> j)) & 1 for i, j in bonds_all)
+            assert bin(R0).count("1") + int(pcd) == bin(r).count("1")
 
 
 def test_code_object_has_no_packed_fp32(jit_cache):
