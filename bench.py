@@ -93,15 +93,15 @@ def time_hpsi(eng, x, y, steps, warmup, copy_calib=False):
 PROFILED_CONFIGS = {"c128": [3731218], "c64": [4731318]}
 
 # HBM bytes per H.psi (both passes) from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE on `python bench.py --steps 20 --warmup 2 --no-cpu --no-spectrum
+# WRITE_SIZE on `python bench.py --steps 3 --warmup 1 --no-cpu --no-spectrum
 # --no-n32 --no-batch`, FETCH scaled by 2.0 and WRITE by 1.0 (the factors that
 # make a plain one-vector read / write report its own size), from the per-pass
 # rows of the named profile summary.
 MEASURED_TRAFFIC = {
-    # (212053 + 366101) KiB fetched x 2.0 + (262264 + 262144) KiB written
-    ("c128", 3731218): (1721053184, "profiles/hpsi_cube_n24_hbm_traffic.md: esc_cube c128 (config 3731218)"),
-    # (93554 + 134664) KiB fetched x 2.0 + (131077 + 131076) KiB written
-    ("c64", 4731318): (735835136, "profiles/hpsi_cube_n24_hbm_traffic.md: esc_cube c64 (config 4731318)"),
+    # (206863 + 361841) KiB fetched x 2.0 + (262192 + 262149) KiB written
+    ("c128", 3731218): (1701630976, "profiles/hpsi_cube_n24_hbm_traffic.md: esc_cube c128 (config 3731218)"),
+    # (95073 + 139348) KiB fetched x 2.0 + (131102 + 131142) KiB written
+    ("c64", 4731318): (748632064, "profiles/hpsi_cube_n24_hbm_traffic.md: esc_cube c64 (config 4731318)"),
 }
 
 
