@@ -69,22 +69,63 @@ def build_oracle():
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
 
 
+def cube_candidates(n_sites):
+    """The generated-kernel configurations a context of n_sites may try (the
+    list of cube_candidates in esc_capi.cpp): 2^12- and 2^13-row tiles, 2^3
+    or 2^4 rows per lane, 4 or 8 waves, both window orders."""
+    w = n_sites if n_sites <= 18 else (17 if n_sites <= 22 else 18)
+    out = []
+    for tb in (12, 13):
+        if tb > w:
+            continue
+        for k in (3, 4):
+            for v in (2, 3):
+                for P in (7, 8):
+                    if w == n_sites and P == 8:
+                        continue
+                    out.append(1000000 * k + 100000 * P + 10000 * v + 100 * tb + w)
+    return out
+
+
 def warm_jit_cache():
     """Compile the generated H.psi kernels of the benchmark lattices into
     jit_cache/ beside libesc.so, so that contexts of these models load them
-    instead of compiling (the couplings are not part of the cache key)."""
+    instead of compiling (the couplings are not part of the cache key): the
+    default configurations of the single-system workloads, and every
+    candidate of the lattices that bench.py autotunes (the batched scan's
+    16-, 18-, 20- and 24-site clusters, the 32-site cluster)."""
     sys.path.insert(0, ROOT)
+    from concurrent.futures import ProcessPoolExecutor
     import energyspincurrents_amd as esc
     kjgh = dict(K=(-1.0, -1.0, -1.0), J=0.1, Gamma=0.2, B=esc.field111(0.2))
     # the batched scan's term structure: K = cos(phi), J = sin(phi), field
     # along [111] (no Gamma; the diagonal summed bond by bond)
     scan = dict(K=(math.cos(1.0),) * 3, J=math.sin(1.0), B=esc.field111(0.3))
-    jobs = [((3, 8), kjgh, "c128"), ((3, 8), kjgh, "c64"), ((2, 10), kjgh, "c128"), ((2, 10), kjgh, "c64"),
-            ((4, 8), dict(K=(-1.0, -1.0, -1.0), B=esc.field111(0.3)), "c64"), ((2, 10), scan, "c64"),
-            ((3, 8), scan, "c64")]
-    for (lx, ly), kw, prec in jobs:
-        esc.precompile_kernel(esc.SpinModel(esc.Honeycomb(lx, ly), **kw), prec)
-        print(f"+ generated H.psi kernels: {lx * ly} sites, {prec}", flush=True)
+    n32 = dict(K=(-1.0, -1.0, -1.0), B=esc.field111(0.3))
+    jobs = [((3, 8), kjgh, "c128", None), ((3, 8), kjgh, "c64", None), ((2, 10), kjgh, "c128", None),
+            ((2, 10), kjgh, "c64", None)]
+    for (lx, ly), kw in (((2, 8), scan), ((3, 6), scan), ((2, 10), scan), ((3, 8), scan), ((4, 8), n32)):
+        jobs += [((lx, ly), kw, "c64", cfg) for cfg in cube_candidates(lx * ly)]
+
+    # the run-time compiler serialises within a process: one process per job
+    with ProcessPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        done = list(ex.map(_precompile_job, jobs))
+    print(f"+ generated H.psi kernels: {sum(done)} configurations of the benchmark lattices", flush=True)
+
+
+def _precompile_job(job):
+    """One generated-kernel configuration into the disk cache; 1 if it applies
+    to the lattice, 0 if not (ESC_ERR_UNSUPPORTED)."""
+    sys.path.insert(0, ROOT)
+    import energyspincurrents_amd as esc
+    (lx, ly), kw, prec, cfg = job
+    try:
+        esc.precompile_kernel(esc.SpinModel(esc.Honeycomb(lx, ly), **kw), prec, cfg)
+    except esc.EscError as e:
+        if e.code != -6:   # -6: the configuration does not apply to this lattice
+            raise
+        return 0
+    return 1
 
 
 def build(cli=True, oracle=True):

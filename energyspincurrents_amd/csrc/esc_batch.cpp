@@ -61,6 +61,21 @@ struct esc_batch {
         }
     };
     DevBuf work, e0_dev, iters_dev, keep_alpha, keep_beta, norms, n2_dev, ex_dev, f_dev, omega_dev, tabs_dev;
+    // Instances that run the same generated program (same lattice, term
+    // structure and configuration) share one launch per pass: grid y = member,
+    // each with its own coupling scalars, vectors and dot partials.
+    struct Group {
+        std::vector<int> inst;
+        esc::CubeProgram *prog = nullptr;
+        uint64_t dim = 0;
+        DevBuf ptrs;   // per rotation phase: ins, outs, dots (3 x 3 x members pointers)
+        DevBuf cfs;    // members x kCubeScalars doubles
+        const void *const *ins(int phase) { return (const void *const *)ptrs.p + (size_t)phase * 3 * inst.size(); }
+        void *const *outs(int phase) { return (void *const *)ptrs.p + ((size_t)phase * 3 + 1) * inst.size(); }
+        double *const *dots(int phase) { return (double *const *)ptrs.p + ((size_t)phase * 3 + 2) * inst.size(); }
+    };
+    std::vector<std::unique_ptr<Group>> groups;
+    std::vector<int> group_of;            // instance -> group, or -1
 
     void *slot(int i, int q) const { return (char *)mem[i] + (size_t)q * ctx[i]->dim * esz; }
     void drop_graphs() {
@@ -83,6 +98,9 @@ struct esc_batch {
         for (DevBuf *d : {&work, &e0_dev, &iters_dev, &keep_alpha, &keep_beta, &norms, &n2_dev, &ex_dev, &f_dev,
                           &omega_dev, &tabs_dev})
             if (d->p) (void)hipFree(d->p);
+        for (auto &g : groups)
+            for (DevBuf *d : {&g->ptrs, &g->cfs})
+                if (d->p) (void)hipFree(d->p);
         for (esc_ctx *c : ctx) delete c;
         if (s) (void)hipStreamDestroy(s);
     }
@@ -109,6 +127,138 @@ void batch_capacity(esc_batch *b, int steps) {
     b->cap = steps;
 }
 
+// The generated program of instance i's kernel choice, or null.
+esc::CubeProgram *instance_program(esc_batch *b, int i) {
+    esc_ctx *c = b->ctx[i];
+    if (c->tile_bits <= 0 || !cfg_cube(c->tile_bits)) return nullptr;
+    return &cube_for(c, c->tile_bits);
+}
+
+// Groups of two or more instances whose kernel choices are the same
+// generated program (cube_program shares one program per source and device),
+// with their per-phase pointer tables and coupling scalars on the device.
+void batch_setup_groups(esc_batch *b) {
+    for (auto &g : b->groups)
+        for (esc_batch::DevBuf *d : {&g->ptrs, &g->cfs})
+            if (d->p) hip_check(hipFree(d->p), "hipFree");
+    b->groups.clear();
+    b->group_of.assign(b->n, -1);
+    std::map<std::pair<esc::CubeProgram *, uint64_t>, std::vector<int>> by;
+    for (int i = 0; i < b->n; ++i)
+        if (esc::CubeProgram *p = instance_program(b, i)) by[{p, b->ctx[i]->dim}].push_back(i);
+    for (auto &kv : by) {
+        if (kv.second.size() < 2) continue;
+        auto g = std::make_unique<esc_batch::Group>();
+        g->inst = kv.second;
+        g->prog = kv.first.first;
+        g->dim = kv.first.second;
+        const size_t m = g->inst.size();
+        std::vector<const void *> h(9 * m);
+        for (int p = 0; p < 3; ++p)
+            for (size_t q = 0; q < m; ++q) {
+                const int i = g->inst[q];
+                h[(size_t)p * 3 * m + q] = b->lanes[p][i].v;
+                h[((size_t)p * 3 + 1) * m + q] = b->lanes[p][i].w;
+                h[((size_t)p * 3 + 2) * m + q] = b->dotbuf[i];
+            }
+        hip_check(hipMemcpy(g->ptrs.get(h.size() * sizeof(void *)), h.data(), h.size() * sizeof(void *),
+                            hipMemcpyHostToDevice), "hipMemcpy");
+        double *cfs = (double *)g->cfs.get(m * esc::kCubeScalars * sizeof(double));
+        for (size_t q = 0; q < m; ++q)
+            hip_check(hipMemcpy(cfs + q * esc::kCubeScalars, b->ctx[g->inst[q]]->d_cube_cf,
+                                esc::kCubeScalars * sizeof(double), hipMemcpyDeviceToDevice), "hipMemcpy");
+        for (int i : g->inst) b->group_of[i] = (int)b->groups.size();
+        b->groups.push_back(std::move(g));
+    }
+}
+
+// Group tuning: instances whose generated programs coincide (same lattice and
+// term structure: the first candidate configuration gives one program) are
+// timed as one batched launch per candidate configuration, from their slot 1
+// into slot 2; the fastest configuration replaces the members' own choices
+// when it beats the sum of their best single-instance times.
+void batch_tune_groups(esc_batch *b, int reps, const std::vector<double> &solo) {
+    std::map<std::pair<esc::CubeProgram *, uint64_t>, std::vector<int>> cls;
+    for (int i = 0; i < b->n; ++i) {
+        const std::vector<int> cand = cube_candidates(b->ctx[i]);
+        if (cand.empty()) continue;
+        try {
+            cls[{&cube_for(b->ctx[i], cand[0]), b->ctx[i]->dim}].push_back(i);
+        } catch (const EscError &) {
+            continue;   // no generated kernels for this instance
+        }
+    }
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    esc_batch::DevBuf ptrs, cfs;
+    try {
+        for (auto &kv : cls) {
+            const std::vector<int> &mem = kv.second;
+            const size_t m = mem.size();
+            if (m < 2) continue;
+            double best_ms = 0.0;
+            for (int i : mem) best_ms += solo[i];
+            int best = 0;
+            std::vector<const void *> h(3 * m);
+            for (size_t q = 0; q < m; ++q) {
+                h[q] = b->slot(mem[q], 1);
+                h[m + q] = b->slot(mem[q], 2);
+                h[2 * m + q] = nullptr;
+            }
+            void **d = (void **)ptrs.get(h.size() * sizeof(void *));
+            hip_check(hipMemcpy(d, h.data(), h.size() * sizeof(void *), hipMemcpyHostToDevice), "hipMemcpy");
+            double *dcf = (double *)cfs.get(m * esc::kCubeScalars * sizeof(double));
+            for (int cfg : cube_candidates(b->ctx[mem[0]])) {
+                esc::CubeProgram *p = nullptr;
+                bool same = true;
+                try {
+                    for (int i : mem) {
+                        esc::CubeProgram *pi = &cube_for(b->ctx[i], cfg);
+                        if (!p) p = pi;
+                        same = same && pi == p;
+                    }
+                } catch (const EscError &) {
+                    same = false;
+                }
+                if (!same) continue;
+                for (size_t q = 0; q < m; ++q)
+                    hip_check(hipMemcpyAsync(dcf + q * esc::kCubeScalars, b->ctx[mem[q]]->d_cube_cf,
+                                             esc::kCubeScalars * sizeof(double), hipMemcpyDeviceToDevice, b->s),
+                              "copy");
+                auto launch = [&] {
+                    hip_check(esc::cube_launch_batched(*p, dcf, (const void *const *)d, (void *const *)d + m,
+                                                       (double *const *)d + 2 * m, b->ctx[mem[0]]->dim, (int)m, b->s),
+                              "generated hpsi launch (batched)");
+                };
+                launch();   // warm-up
+                hip_check(hipEventRecord(e0, b->s), "hipEventRecord");
+                for (int r = 0; r < reps; ++r) launch();
+                hip_check(hipEventRecord(e1, b->s), "hipEventRecord");
+                hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+                float t = 0.0f;
+                hip_check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
+                if ((double)t / reps < best_ms) {
+                    best_ms = (double)t / reps;
+                    best = cfg;
+                }
+            }
+            if (best)
+                for (int i : mem) b->ctx[i]->tile_bits = best;
+        }
+    } catch (...) {
+        for (esc_batch::DevBuf *x : {&ptrs, &cfs})
+            if (x->p) (void)hipFree(x->p);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    for (esc_batch::DevBuf *x : {&ptrs, &cfs})
+        if (x->p) (void)hipFree(x->p);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
+
 // Instances with a tiled H.psi get their dot <v|w> from the kernel's per-tile
 // partials (one buffer each, sized for the current kernel choice), so the
 // batched dot pass skips them.  Called when the kernel choices are set
@@ -133,6 +283,7 @@ void batch_setup_dots(esc_batch *b) {
     for (int p = 0; p < 3; ++p)
         hip_check(hipMemcpy(b->d_lanes[p], b->lanes[p].data(), sizeof(esc::BatchLane) * b->n, hipMemcpyHostToDevice),
                   "hipMemcpy");
+    batch_setup_groups(b);
 }
 
 // the launches of step k (phase k % 3) for all instances
@@ -142,7 +293,12 @@ void batch_step_launches(esc_batch *b, int phase, bool accum) {
         hip_check(esc::launch_batch_step(b->c64, esc::kBatchAccum, b->d_lanes[phase], b->n, b->total_blocks,
                                          b->d_partials, b->d_alpha, lda, b->d_beta, ldb, b->d_coef, ldc, b->d_step,
                                          b->s), "batch accumulate");
+    for (auto &g : b->groups)
+        hip_check(esc::cube_launch_batched(*g->prog, (const double *)g->cfs.p, g->ins(phase), g->outs(phase),
+                                           g->dots(phase), g->dim, (int)g->inst.size(), b->s),
+                  "generated hpsi launch (batched)");
     for (int i = 0; i < b->n; ++i) {
+        if (b->group_of[i] >= 0) continue;
         const esc::BatchLane &l = b->lanes[phase][i];
         if (l.ndot > 0) launch_hpsi_with(b->ctx[i], b->ctx[i]->tile_bits, l.v, l.w, b->s, b->dotbuf[i]);
         else run_hpsi(b->ctx[i], l.v, l.w, b->s);
@@ -346,6 +502,7 @@ int esc_batch_autotune(esc_batch *b, int reps, int *configs) {
         batch_check(b);
         b->drop_graphs();   // the captured steps bake each instance's kernel choice
         b->have_gs = false;
+        std::vector<double> solo(b->n, 0.0);
         for (int i = 0; i < b->n; ++i) {
             esc_ctx *c = b->ctx[i];
             hip_check(esc::launch_gaussian(b->c64, 0x5eedULL + (uint64_t)i, b->slot(i, 1), c->dim, b->s), "gaussian");
@@ -353,8 +510,11 @@ int esc_batch_autotune(esc_batch *b, int reps, int *configs) {
             double ms = 0.0;
             const int rc = esc_autotune(c, b->slot(i, 1), b->slot(i, 2), reps, &cfg, &ms, b->s);
             if (rc != ESC_OK) throw EscError(rc, "instance " + std::to_string(i) + ": " + esc_last_error());
-            if (configs) configs[i] = cfg;
+            solo[i] = ms;
         }
+        batch_tune_groups(b, reps, solo);
+        if (configs)
+            for (int i = 0; i < b->n; ++i) configs[i] = b->ctx[i]->tile_bits;
         hip_check(hipStreamSynchronize(b->s), "sync");
         batch_setup_dots(b);
     });

@@ -176,6 +176,11 @@ double device_norm2(esc_ctx *c, const void *x, hipStream_t s);
 
 // H.psi with a given kernel configuration / the context's choice
 int cfg_tile_bits(int config);
+bool cfg_cube(int config);
+// the generated program of a configuration (compiled or fetched from the
+// caches), its scalars uploaded to c->d_cube_cf
+esc::CubeProgram &cube_for(esc_ctx *c, int config);
+std::vector<int> cube_candidates(const esc_ctx *c);
 void launch_hpsi_with(esc_ctx *c, int config, const void *in, void *out, hipStream_t s, double *dot_part = nullptr);
 void run_hpsi(esc_ctx *c, const void *in, void *out, hipStream_t s);
 

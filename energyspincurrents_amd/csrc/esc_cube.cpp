@@ -413,14 +413,16 @@ void emit_pass(std::ostringstream &o, const CubeModel &m, const CubePass &p, int
                                      : " f" + std::to_string(pr.first.k));
         o << "\n";
     }
-    o << "extern \"C\" __global__ void __launch_bounds__(" << WPB * 64 << ") esc_cube_p" << index
+    // the pass body (b = tile block index); the kernels below call it for one
+    // system, or for system blockIdx.y of a batch of systems that share this
+    // lattice and term structure (own couplings, vectors and partials each)
+    o << "__device__ __forceinline__ void esc_cube_body" << index
       << "(const double *__restrict__ cf, const V2 *__restrict__ in, V2 *__restrict__ out, "
-         "double *__restrict__ dot_part, unsigned block_base) {\n";
+         "double *__restrict__ dot_part, const unsigned b) {\n";
     o << "  __shared__ V2 tile[" << (1 << TB) << "];\n";
     o << "  __shared__ double red[2][" << WPB << "];\n";
     o << "  const unsigned lane = threadIdx.x & 63u;\n";
     o << "  const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);\n";
-    o << "  const unsigned b = blockIdx.x + block_base;\n";
     if (xcdw)
         o << "  const unsigned seq = b / 8u, wi = (seq / " << bpw << "u) * 8u + b % 8u, tl = seq % " << bpw << "u;\n";
     else
@@ -568,6 +570,16 @@ void emit_pass(std::ostringstream &o, const CubeModel &m, const CubePass &p, int
         o << "      dot_part[2 * (size_t)b] = x; dot_part[2 * (size_t)b + 1] = y;\n    }\n  }\n";
     }
     o << "}\n\n";
+    o << "extern \"C\" __global__ void __launch_bounds__(" << WPB * 64 << ") esc_cube_p" << index
+      << "(const double *__restrict__ cf, const V2 *__restrict__ in, V2 *__restrict__ out, "
+         "double *__restrict__ dot_part, unsigned block_base) {\n";
+    o << "  esc_cube_body" << index << "(cf, in, out, dot_part, blockIdx.x + block_base);\n}\n\n";
+    o << "extern \"C\" __global__ void __launch_bounds__(" << WPB * 64 << ") esc_cube_b" << index
+      << "(const double *__restrict__ cfs, const V2 *const *__restrict__ ins, V2 *const *__restrict__ outs, "
+         "double *const *__restrict__ dots, unsigned block_base) {\n";
+    o << "  const unsigned inst = blockIdx.y;\n";
+    o << "  esc_cube_body" << index << "(cfs + " << (int)kCubeScalars
+      << "u * inst, ins[inst], outs[inst], dots[inst], blockIdx.x + block_base);\n}\n\n";
 }
 
 }  // namespace
@@ -681,7 +693,7 @@ std::string cube_source(const CubeModel &m, const std::vector<CubePass> &passes)
 // ---------------------------------------------------------------------------
 struct CubeProgram {
     hipModule_t module = nullptr;
-    std::vector<hipFunction_t> fn;
+    std::vector<hipFunction_t> fn, fnb;   // per pass: one system, a batch of systems
     int tile_bits = 0, threads = 0;
     ~CubeProgram() {
         if (module) (void)hipModuleUnload(module);
@@ -836,6 +848,12 @@ std::shared_ptr<CubeProgram> cube_program(const CubeModel &m, const std::vector<
             return nullptr;
         }
         prog->fn.push_back(f);
+        const std::string bname = "esc_cube_b" + std::to_string(i);
+        if (hipModuleGetFunction(&f, prog->module, bname.c_str()) != hipSuccess) {
+            if (err) *err = "generated kernel " + bname + " missing";
+            return nullptr;
+        }
+        prog->fnb.push_back(f);
     }
     prog->tile_bits = passes[0].tile_bits;
     prog->threads = m.waves * 64;
@@ -845,6 +863,28 @@ std::shared_ptr<CubeProgram> cube_program(const CubeModel &m, const std::vector<
 }
 
 int cube_tile_bits(const CubeProgram &p) { return p.tile_bits; }
+
+hipError_t cube_launch_batched(const CubeProgram &p, const double *d_cfs, const void *const *d_ins,
+                               void *const *d_outs, double *const *d_dots, uint64_t dim, int ninst, hipStream_t s) {
+    const uint64_t ntiles64 = dim >> p.tile_bits;
+    if (ntiles64 == 0 || ntiles64 > 0x7fffffffull || ninst < 1 || ninst > 65535) return hipErrorInvalidValue;
+    const uint32_t ntiles = (uint32_t)ntiles64;
+    for (size_t i = 0; i < p.fnb.size(); ++i) {
+        for (uint32_t base = 0; base < ntiles; base += kCubeMaxGrid) {
+            const uint32_t n = (ntiles - base) < kCubeMaxGrid ? (ntiles - base) : kCubeMaxGrid;
+            const double *a0 = d_cfs;
+            const void *const *a1 = d_ins;
+            void *const *a2 = d_outs;
+            double *const *a3 = d_dots;
+            uint32_t a4 = base;
+            void *args[] = {&a0, &a1, &a2, &a3, &a4};
+            const hipError_t e =
+                hipModuleLaunchKernel(p.fnb[i], n, (unsigned)ninst, 1, (unsigned)p.threads, 1, 1, 0, s, args, nullptr);
+            if (e != hipSuccess) return e;
+        }
+    }
+    return hipSuccess;
+}
 
 hipError_t cube_launch(const CubeProgram &p, const double *d_scalars, const void *in, void *out, double *dot_part,
                        uint64_t dim, hipStream_t s) {

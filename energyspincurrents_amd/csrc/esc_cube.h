@@ -96,5 +96,11 @@ bool cube_precompile(const CubeModel &m, const std::vector<CubePass> &passes, st
 hipError_t cube_launch(const CubeProgram &p, const double *d_scalars, const void *in, void *out, double *dot_part,
                        uint64_t dim, hipStream_t s);
 int cube_tile_bits(const CubeProgram &p);
+// All passes for ninst systems of one program at once (grid y = system):
+// d_cfs holds kCubeScalars coupling scalars per system, d_ins / d_outs /
+// d_dots one device pointer per system (d_dots as dot_part above, entries
+// may be null).
+hipError_t cube_launch_batched(const CubeProgram &p, const double *d_cfs, const void *const *d_ins,
+                               void *const *d_outs, double *const *d_dots, uint64_t dim, int ninst, hipStream_t s);
 
 }  // namespace esc

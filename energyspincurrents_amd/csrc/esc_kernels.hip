@@ -1064,12 +1064,23 @@ __global__ __launch_bounds__(kThreads) void k_b_alpha(const BatchLane *__restric
 }
 
 // w = c_k w - alpha_k c_k v - beta_k c_{k-1} vp (w holds H u_k); partial |w|^2
-template <typename R>
+// Streamed loads and stores (non-temporal hint) for the vector passes that
+// touch each element once.
+template <bool NT, typename T> __device__ __forceinline__ T ld_s(const T *p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <bool NT, typename T> __device__ __forceinline__ void st_s(T *p, T v) {
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+constexpr int kUnroll = 4;   // elements per lane in flight in the batched vector passes
+
+template <typename R, bool NT>
 __global__ __launch_bounds__(kThreads) void k_b_update(const BatchLane *__restrict__ L, int n,
                                                        const double *__restrict__ alpha, int lda,
                                                        const double *__restrict__ beta, int ldb,
                                                        const uint32_t *__restrict__ step, double *__restrict__ partials) {
-    using V2 = typename Vec2<R>::T;
     const int inst = batch_instance(L, n, blockIdx.x);
     const BatchLane &l = L[inst];
     const uint32_t k = *step;
@@ -1078,38 +1089,73 @@ __global__ __launch_bounds__(kThreads) void k_b_update(const BatchLane *__restri
     const double ar = alpha[2 * ((size_t)inst * lda + k)] * c, ai = alpha[2 * ((size_t)inst * lda + k) + 1] * c;
     const double b = brow[k] * cp;
     double acc[1] = {0.0};
-    const uint64_t stride = (uint64_t)l.nblk * kThreads;
-    const uint64_t first = (uint64_t)(blockIdx.x - l.blk0) * kThreads + threadIdx.x;
+    // kUnroll elements per lane per round, kThreads apart (coalesced)
+    const uint64_t span = (uint64_t)kThreads * kUnroll;
+    const uint64_t b0 = (uint64_t)(blockIdx.x - l.blk0) * span, bstride = (uint64_t)l.nblk * span;
     if constexpr (sizeof(R) == 4) {
-        // complex64: two elements per lane (16-byte accesses); dim is a power
+        // complex64: two elements per lane access (16 bytes); dim is a power
         // of two >= 2
-        const float4 *v = (const float4 *)l.v, *u = (const float4 *)l.vp;
-        float4 *w = (float4 *)l.w;
-        for (uint64_t i = first; i < l.dim / 2; i += stride) {
-            const float4 x = v[i], y = w[i], z = u[i];
-            const double r0 = c * (double)y.x - (ar * (double)x.x - ai * (double)x.y) - b * (double)z.x;
-            const double i0 = c * (double)y.y - (ar * (double)x.y + ai * (double)x.x) - b * (double)z.y;
-            const double r1 = c * (double)y.z - (ar * (double)x.z - ai * (double)x.w) - b * (double)z.z;
-            const double i1 = c * (double)y.w - (ar * (double)x.w + ai * (double)x.z) - b * (double)z.w;
-            const float4 o = make_float4((float)r0, (float)i0, (float)r1, (float)i1);
-            w[i] = o;
-            acc[0] += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
-            acc[0] += (double)o.z * (double)o.z + (double)o.w * (double)o.w;
+        typedef float F4 __attribute__((ext_vector_type(4)));
+        const F4 *v = (const F4 *)l.v, *u = (const F4 *)l.vp;
+        F4 *w = (F4 *)l.w;
+        const uint64_t n4 = l.dim / 2;
+        for (uint64_t base = b0; base < n4; base += bstride) {
+            F4 x[kUnroll], y[kUnroll], z[kUnroll];
+#pragma unroll
+            for (int q = 0; q < kUnroll; ++q) {
+                const uint64_t i = base + (uint64_t)q * kThreads + threadIdx.x;
+                if (i < n4) {
+                    x[q] = ld_s<NT>(v + i);
+                    y[q] = ld_s<NT>(w + i);
+                    z[q] = ld_s<NT>(u + i);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < kUnroll; ++q) {
+                const uint64_t i = base + (uint64_t)q * kThreads + threadIdx.x;
+                if (i >= n4) continue;
+                const double r0 = c * (double)y[q].x - (ar * (double)x[q].x - ai * (double)x[q].y) - b * (double)z[q].x;
+                const double i0 = c * (double)y[q].y - (ar * (double)x[q].y + ai * (double)x[q].x) - b * (double)z[q].y;
+                const double r1 = c * (double)y[q].z - (ar * (double)x[q].z - ai * (double)x[q].w) - b * (double)z[q].z;
+                const double i1 = c * (double)y[q].w - (ar * (double)x[q].w + ai * (double)x[q].z) - b * (double)z[q].w;
+                F4 o;
+                o.x = (float)r0;
+                o.y = (float)i0;
+                o.z = (float)r1;
+                o.w = (float)i1;
+                st_s<NT>(w + i, o);
+                acc[0] += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
+                acc[0] += (double)o.z * (double)o.z + (double)o.w * (double)o.w;
+            }
         }
     } else {
-        const V2 *v = (const V2 *)l.v, *u = (const V2 *)l.vp;
-        V2 *w = (V2 *)l.w;
-        for (uint64_t i = first; i < l.dim; i += stride) {
-            const double vr = v[i].x, vi = v[i].y;
-            double wr = c * (double)w[i].x - (ar * vr - ai * vi) - b * (double)u[i].x;
-            double wi = c * (double)w[i].y - (ar * vi + ai * vr) - b * (double)u[i].y;
-            V2 o;
-            o.x = (R)wr;
-            o.y = (R)wi;
-            w[i] = o;
-            wr = (double)o.x;
-            wi = (double)o.y;
-            acc[0] += wr * wr + wi * wi;
+        typedef double D2 __attribute__((ext_vector_type(2)));
+        const D2 *v = (const D2 *)l.v, *u = (const D2 *)l.vp;
+        D2 *w = (D2 *)l.w;
+        for (uint64_t base = b0; base < l.dim; base += bstride) {
+            D2 x[kUnroll], y[kUnroll], z[kUnroll];
+#pragma unroll
+            for (int q = 0; q < kUnroll; ++q) {
+                const uint64_t i = base + (uint64_t)q * kThreads + threadIdx.x;
+                if (i < l.dim) {
+                    x[q] = ld_s<NT>(v + i);
+                    y[q] = ld_s<NT>(w + i);
+                    z[q] = ld_s<NT>(u + i);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < kUnroll; ++q) {
+                const uint64_t i = base + (uint64_t)q * kThreads + threadIdx.x;
+                if (i >= l.dim) continue;
+                const double vr = x[q].x, vi = x[q].y;
+                const double wr = c * y[q].x - (ar * vr - ai * vi) - b * z[q].x;
+                const double wi = c * y[q].y - (ar * vi + ai * vr) - b * z[q].y;
+                D2 o;
+                o.x = wr;
+                o.y = wi;
+                st_s<NT>(w + i, o);
+                acc[0] += wr * wr + wi * wi;
+            }
         }
     }
     block_reduce_store<1>(acc, partials + blockIdx.x);
@@ -1138,23 +1184,38 @@ __global__ __launch_bounds__(kThreads) void k_b_beta(const BatchLane *__restrict
 
 // acc += coef[i][k] v_k = coef[i][k] c_k u_k  (replay of a recursion forming
 // a Ritz vector)
-template <typename R>
+template <typename R, bool NT>
 __global__ __launch_bounds__(kThreads) void k_b_accum(const BatchLane *__restrict__ L, int n,
                                                       const double *__restrict__ coef, int ldc,
                                                       const double *__restrict__ beta, int ldb,
                                                       const uint32_t *__restrict__ step) {
-    using V2 = typename Vec2<R>::T;
+    typedef R RV __attribute__((ext_vector_type(2)));
     const int inst = batch_instance(L, n, blockIdx.x);
     const BatchLane &l = L[inst];
     const double c = coef[(size_t)inst * ldc + *step] * batch_scale(beta + (size_t)inst * ldb, *step);
-    const V2 *v = (const V2 *)l.v;
-    V2 *a = (V2 *)l.acc;
-    const uint64_t stride = (uint64_t)l.nblk * kThreads;
-    for (uint64_t i = (uint64_t)(blockIdx.x - l.blk0) * kThreads + threadIdx.x; i < l.dim; i += stride) {
-        V2 o = a[i];
-        o.x = (R)((double)o.x + c * (double)v[i].x);
-        o.y = (R)((double)o.y + c * (double)v[i].y);
-        a[i] = o;
+    const RV *v = (const RV *)l.v;
+    RV *a = (RV *)l.acc;
+    const uint64_t span = (uint64_t)kThreads * kUnroll;
+    const uint64_t b0 = (uint64_t)(blockIdx.x - l.blk0) * span, bstride = (uint64_t)l.nblk * span;
+    for (uint64_t base = b0; base < l.dim; base += bstride) {
+        RV x[kUnroll], y[kUnroll];
+#pragma unroll
+        for (int q = 0; q < kUnroll; ++q) {
+            const uint64_t i = base + (uint64_t)q * kThreads + threadIdx.x;
+            if (i < l.dim) {
+                x[q] = ld_s<NT>(v + i);
+                y[q] = ld_s<NT>(a + i);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kUnroll; ++q) {
+            const uint64_t i = base + (uint64_t)q * kThreads + threadIdx.x;
+            if (i >= l.dim) continue;
+            RV o;
+            o.x = (R)((double)y[q].x + c * (double)x[q].x);
+            o.y = (R)((double)y[q].y + c * (double)x[q].y);
+            st_s<NT>(a + i, o);
+        }
     }
 }
 
@@ -1525,11 +1586,12 @@ hipError_t launch_batch_step(bool c64, int piece, const BatchLane *d_lanes, int 
                            (const double *)beta, ldb, (const uint32_t *)step);
         break;
     case kBatchUpdate:
+        // streamed accesses: each element is read and written once per step
         if (c64)
-            hipLaunchKernelGGL(k_b_update<float>, g, t, 0, s, d_lanes, n, (const double *)alpha, lda,
+            hipLaunchKernelGGL((k_b_update<float, true>), g, t, 0, s, d_lanes, n, (const double *)alpha, lda,
                                (const double *)beta, ldb, (const uint32_t *)step, partials);
         else
-            hipLaunchKernelGGL(k_b_update<double>, g, t, 0, s, d_lanes, n, (const double *)alpha, lda,
+            hipLaunchKernelGGL((k_b_update<double, true>), g, t, 0, s, d_lanes, n, (const double *)alpha, lda,
                                (const double *)beta, ldb, (const uint32_t *)step, partials);
         hipLaunchKernelGGL(k_b_beta, gi, t, 0, s, d_lanes, (const double *)partials, (const double *)alpha, lda, beta,
                            ldb, (const uint32_t *)step);
@@ -1537,11 +1599,11 @@ hipError_t launch_batch_step(bool c64, int piece, const BatchLane *d_lanes, int 
         break;
     case kBatchAccum:
         if (c64)
-            hipLaunchKernelGGL(k_b_accum<float>, g, t, 0, s, d_lanes, n, coef, ldc, (const double *)beta, ldb,
+            hipLaunchKernelGGL((k_b_accum<float, true>), g, t, 0, s, d_lanes, n, coef, ldc, (const double *)beta, ldb,
                                (const uint32_t *)step);
         else
-            hipLaunchKernelGGL(k_b_accum<double>, g, t, 0, s, d_lanes, n, coef, ldc, (const double *)beta, ldb,
-                               (const uint32_t *)step);
+            hipLaunchKernelGGL((k_b_accum<double, true>), g, t, 0, s, d_lanes, n, coef, ldc, (const double *)beta,
+                               ldb, (const uint32_t *)step);
         break;
     default:
         return hipErrorInvalidValue;

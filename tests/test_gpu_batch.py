@@ -143,3 +143,19 @@ def test_device_all_eigenpairs_equal_the_host():
         ev_h, Z_h = esc.tridiag_eigenpairs(a, b)
         ev_d, Z_d = esc.tridiag_eigenpairs(a, b, device=True)
         assert np.array_equal(ev_h, ev_d) and np.array_equal(Z_h, Z_d)
+
+
+def test_shared_program_instances_equal_single_instance_batches():
+    """Instances of one lattice and term structure (20 sites, K = cos(phi),
+    J = sin(phi), [111] field) run their generated H.psi passes as one launch
+    per pass (grid y = instance); an instance with Gamma has another program
+    and runs alone.  Each instance's recursion and energy equal, bit for bit,
+    those of a batch holding that instance only."""
+    ms = [O.Model(2, 10, **kjh(0.4, 0.2)), O.Model(2, 10, **kjh(2.2, 0.35)),
+          O.Model(2, 10, K=(-1.0, -1.0, -1.0), J=0.1, G=0.2, B=O.field111(0.2)), O.Model(2, 10, **kjh(4.1, 0.1))]
+    print("[batch shared programs] 4 x N=20 c128", flush=True)
+    e, it, al, be = esc.Batch([spin_model(m) for m in ms], "c128").ground_state(seed=5, steps=60)
+    for i, m in enumerate(ms):
+        e1, it1, al1, be1 = esc.Batch([spin_model(m)], "c128").ground_state(seed=5 + i, steps=60)
+        assert e1[0] == e[i] and it1[0] == it[i], f"instance {i}"
+        assert np.array_equal(al1[0], al[i]) and np.array_equal(be1[0], be[i]), f"instance {i}"
