@@ -88,9 +88,10 @@ def time_hpsi(eng, x, y, steps, warmup, copy_calib=False):
 
 # The H.psi kernel configurations of this workload: the library defaults at 24
 # sites (generated sub-cube kernels: complex128 2^3 rows per lane, 8-wave
-# workgroups on 2^12-row tiles; complex64 2^4 rows per lane on 2^13-row tiles;
-# cover-first plan over 2^18-row windows, two passes).
-PROFILED_CONFIGS = {"c128": [3731218], "c64": [4731318]}
+# workgroups on 2^12-row tiles; complex64 row pairs, 2^4 rows per lane on
+# 2^13-row tiles, windows in reverse order; cover-first plan over 2^18-row
+# windows, two passes).
+PROFILED_CONFIGS = {"c128": [3731218], "c64": [14831318]}
 
 # HBM bytes per H.psi (both passes) from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE on `python bench.py --steps 3 --warmup 1 --no-cpu --no-spectrum
@@ -98,10 +99,10 @@ PROFILED_CONFIGS = {"c128": [3731218], "c64": [4731318]}
 # make a plain one-vector read / write report its own size), from the per-pass
 # rows of the named profile summary.
 MEASURED_TRAFFIC = {
-    # (206863 + 361841) KiB fetched x 2.0 + (262192 + 262149) KiB written
-    ("c128", 3731218): (1701630976, "profiles/hpsi_cube_n24_hbm_traffic.md: esc_cube c128 (config 3731218)"),
-    # (95073 + 139348) KiB fetched x 2.0 + (131102 + 131142) KiB written
-    ("c64", 4731318): (748632064, "profiles/hpsi_cube_n24_hbm_traffic.md: esc_cube c64 (config 4731318)"),
+    # (206529 + 355911) KiB fetched x 2.0 + (262235 + 262154) KiB written
+    ("c128", 3731218): (1688851456, "profiles/hpsi_cube_n24_hbm_traffic.md: esc_cube c128 (config 3731218)"),
+    # (81305 + 134345) KiB fetched x 2.0 + (131153 + 131171) KiB written
+    ("c64", 14831318): (710270976, "profiles/hpsi_cube_n24_hbm_traffic.md: esc_cube c64 (config 14831318)"),
 }
 
 
@@ -246,7 +247,7 @@ def main():
             eng.set_tile_bits(config)
         ms, wall = time_hpsi(eng, x, y, args.steps, args.warmup, args.copy_calib and prec == "c128")
         bytes_alg = 2 * eng.dim * eng.elem_bytes
-        if config // 100000 % 10 in (7, 8):
+        if config // 100000 % 10 in (7, 8):   # generated kernels (1xxxxxxx: over row pairs)
             kname = "esc_cube_p0 + esc_cube_p1 (generated sub-cube passes)"
         elif config:
             kname = f"k_hpsi_tile<{'float' if prec == 'c64' else 'double'}, TB, WPB>"

@@ -367,15 +367,17 @@ constexpr int kCoverLast = 6;    // cover-first, windows in reverse order
 constexpr int kCubeFirst = 7;
 constexpr int kCubeLast = 8;
 int cfg_passes(int config) { return (config / 100000) % 10; }
-int cfg_regs(int config) { return config / 1000000; }
+int cfg_regs(int config) { return (config / 1000000) % 10; }
+// generated kernels over row pairs (complex64: 16-byte lane accesses)
+bool cfg_paired(int config) { return (config / 10000000) % 10 == 1; }
 bool cfg_cube(int config) { return cfg_passes(config) == kCubeFirst || cfg_passes(config) == kCubeLast; }
 int cfg_waves(int config) { return 1 << ((config / 10000) % 10); }
 int cfg_tile_bits(int config) { return (config / 100) % 100; }
 int cfg_window(int config) { return config % 100; }
-int make_config(int P, int waves, int tb, int w, int k = 0) {
+int make_config(int P, int waves, int tb, int w, int k = 0, bool paired = false) {
     int v = 0;
     while ((1 << v) < waves) ++v;
-    return 1000000 * k + 100000 * P + 10000 * v + 100 * tb + w;
+    return (paired ? 10000000 : 0) + 1000000 * k + 100000 * P + 10000 * v + 100 * tb + w;
 }
 
 // One planned pass: window sites W, tile sites T (lane sites 0..5 included)
@@ -735,6 +737,11 @@ bool cube_spec(const esc_ctx *c, int cfg, esc::CubeModel &m, std::vector<esc::Cu
         m.jgrp_m[q] = g[0].jgrp_m[q];
     }
     m.reg_sites = cfg_regs(cfg);
+    m.paired = cfg_paired(cfg);
+    if (m.paired && !m.c64) {
+        if (why) *why = "row pairs are a complex64 layout";
+        return false;
+    }
     m.waves = WPB;
     if (!do_plan) return true;
     // the generic kernel's cover-first tiles seed the sub-cube planner, with
@@ -1173,8 +1180,9 @@ bool tile_config_fits(const esc_ctx *c, int config) {
             ok = false;
         }
         const int TB = cfg_tile_bits(config), k = cfg_regs(config), waves = cfg_waves(config);
+        const int P = cfg_paired(config) ? 1 : 0;
         ok = ok && k >= 1 && TB - 6 - k >= 0 && ((1 << (TB - 6 - k)) % waves) == 0 &&
-             ((size_t)1 << TB) * c->esz <= 131072;
+             ((1 << (TB - 6 - P)) % waves) == 0 && ((size_t)1 << TB) * c->esz <= 131072;
         c->tile_fits[config] = ok;
         return ok;
     }
@@ -1192,9 +1200,10 @@ bool tile_config_fits(const esc_ctx *c, int config) {
     return ok;
 }
 
-// generated sub-cube configurations: 2^12-row tiles (2^13 for complex64),
-// 2^3 or 2^4 rows per lane, 4 or 8 waves, the cover-first windows of the
-// generic kernel in both orders.  ESC_NO_JIT=1 leaves them out.
+// generated sub-cube configurations: 2^12- and 2^13-row tiles, 2^3 or 2^4
+// rows per lane, 4 or 8 waves, the cover-first windows of the generic kernel
+// in both orders; complex64 with and without row pairs.  ESC_NO_JIT=1 leaves
+// them out.
 std::vector<int> cube_candidates(const esc_ctx *c) {
     std::vector<int> v;
     const char *nj = std::getenv("ESC_NO_JIT");
@@ -1206,11 +1215,12 @@ std::vector<int> cube_candidates(const esc_ctx *c) {
         if (tb > w) continue;
         for (int k : {3, 4})
             for (int waves : {4, 8})
-                for (int P : {kCubeFirst, kCubeLast}) {
-                    if (w == N && P == kCubeLast) continue;
-                    const int cfg = make_config(P, waves, tb, w, k);
-                    if (tile_config_fits(c, cfg)) v.push_back(cfg);
-                }
+                for (int P : {kCubeFirst, kCubeLast})
+                    for (bool paired : {false, true}) {
+                        if ((w == N && P == kCubeLast) || (paired && !c->c64())) continue;
+                        const int cfg = make_config(P, waves, tb, w, k, paired);
+                        if (tile_config_fits(c, cfg)) v.push_back(cfg);
+                    }
     }
     return v;
 }
@@ -1249,13 +1259,16 @@ std::vector<int> tile_candidates(const esc_ctx *c) {
 // with 2^12-row tiles (16 waves for c128, which moves the least HBM traffic of
 // the equally fast shapes at N = 24, 8 waves for c64).  A shape the context
 // cannot run falls back to a single pass over 2^11- or 2^10-row tiles.
-// From 19 sites the generated sub-cube kernels come first (N = 24: 0.35 ms
-// against 0.52 ms for the best generic shape in complex128, 0.20 against
-// 0.32 ms in complex64; N = 20: 25.6 against 40.1 us and 19.1 against
+// From 19 sites the generated sub-cube kernels come first (N = 24: 0.31 ms
+// against 0.52 ms for the best generic shape in complex128, 0.15 against
+// 0.32 ms in complex64; N = 20: 25.6 against 40.1 us and 14.3 against
 // 28.8 us): complex128 2^3 rows per lane in 8-wave workgroups on 2^12-row
-// tiles; complex64 2^4 rows per lane, 4 waves on 2^12-row tiles up to 22
-// sites and 8 waves on 2^13-row tiles beyond.  A model they do not cover, or
-// a missing run-time compiler, falls through to the generic shapes.
+// tiles; complex64 8 waves, over row pairs with 2^3 rows per lane on
+// 2^12-row tiles up to 22 sites and 2^4 rows per lane on 2^13-row tiles with
+// the windows in reverse order up to 26 sites, the same without row pairs
+// beyond (N = 32: 70 against 82 ms; all its plans take three passes).  A
+// model they do not cover, or a missing run-time compiler, falls through to
+// the generic shapes.
 int default_tile_bits(esc_ctx *c) {
     const int N = c->n_sites;
     const bool c64 = c->c64();
@@ -1263,8 +1276,10 @@ int default_tile_bits(esc_ctx *c) {
     std::vector<int> pref;
     if (N >= 19) {
         const int w = N <= 22 ? 17 : 18;
-        pref.push_back(c64 ? (N <= 22 ? make_config(kCubeFirst, 4, 12, w, 4) : make_config(kCubeFirst, 8, 13, w, 4))
-                           : make_config(kCubeFirst, 8, 12, w, 3));
+        if (c64 && N <= 22) pref.push_back(make_config(kCubeFirst, 8, 12, w, 3, true));
+        else if (c64 && N <= 26) pref.push_back(make_config(kCubeLast, 8, 13, w, 4, true));
+        else if (c64) pref.push_back(make_config(kCubeLast, 8, 13, w, 4));
+        else pref.push_back(make_config(kCubeFirst, 8, 12, w, 3));
     }
     if (N <= 18) pref.push_back(make_config(1, 8, 10, N));
     else if (N <= 22) pref.push_back(c64 ? make_config(kCoverLast, 16, 12, 17) : make_config(kCoverFirst, 16, 11, 17));
@@ -2824,7 +2839,12 @@ int esc_autotune(esc_ctx *ctx, const void *d_in, void *d_out, int reps, int *til
         int best = ctx->tile_bits;
         float best_ms = 1e30f;
         for (int tb : cand) {
-            launch_hpsi_with(ctx, tb, d_in, d_out, s);   // warm-up (also builds tables)
+            try {
+                launch_hpsi_with(ctx, tb, d_in, d_out, s);   // warm-up (also builds tables)
+            } catch (const EscError &e) {
+                if (e.code != ESC_ERR_UNSUPPORTED) throw;
+                continue;   // a generated configuration with no plan or program for this model
+            }
             hip_check(hipEventRecord(e0, s), "hipEventRecord");
             for (int r = 0; r < reps; ++r) launch_hpsi_with(ctx, tb, d_in, d_out, s);
             hip_check(hipEventRecord(e1, s), "hipEventRecord");

@@ -32,22 +32,24 @@ constexpr int kFarAhead = 1;   // far sub-cubes in flight ahead of their terms
 // ---------------------------------------------------------------------------
 // Layout of one pass
 // ---------------------------------------------------------------------------
-// Row bits: lane sites 0..5; tile-index bit 6 + q holds tile site wb[6 + q];
-// the register sites are k of those positions (slot bit b of a lane's
-// sub-cube = position regpos[b]), the others are group positions (bit q of
-// the group index = position grppos[q]); far sites wb[TB..w-1] and window-id
-// sites cb are fixed per workgroup.
+// Row bits: lane sites P..P+5 (P = 0, or 1 for row pairs); tile-index bit q
+// holds window site wb[q] (wb[q] = q for q < P + 6); the register sites are k
+// of the tile positions below TB (slot bit b of a lane's sub-cube = position
+// regpos[b]; with row pairs regpos[0] = 0, site 0, so a lane's slots 2y and
+// 2y + 1 are adjacent rows and move as one 16-byte access), the others are
+// group positions (bit q of the group index = position grppos[q]); far sites
+// wb[TB..w-1] and window-id sites cb are fixed per workgroup.
 enum Role { kLane, kReg, kGrp, kBase };
 
 struct Layout {
-    int N = 0, TB = 0, w = 0, k = 0;
+    int N = 0, TB = 0, w = 0, k = 0, P = 0;
     std::vector<int> wb, cb, regpos, grppos;
     int role[64];
     int idx[64];     // slot bit (kReg) or group bit (kGrp) of a site
 
     void finish() {
         for (int s = 0; s < 64; ++s) { role[s] = kBase; idx[s] = -1; }
-        for (int s = 0; s < 6; ++s) role[s] = kLane;
+        for (int s = P; s < P + 6; ++s) role[s] = kLane;
         for (int b = 0; b < (int)regpos.size(); ++b) { role[wb[regpos[b]]] = kReg; idx[wb[regpos[b]]] = b; }
         for (int q = 0; q < (int)grppos.size(); ++q) { role[wb[grppos[q]]] = kGrp; idx[wb[grppos[q]]] = q; }
     }
@@ -91,20 +93,27 @@ void split_mask(const Layout &L, uint32_t mask, Outer &o, uint32_t &rho) {
     }
 }
 
-// register sites of a pass: the k tile positions (outside the lane sites)
-// that leave the fewest distinct outer parts, then the most terms whose mask
-// lies inside them, then the lowest positions
+// register sites of a pass: the k tile positions (outside the lane sites;
+// with row pairs site 0 and k - 1 others) that leave the fewest distinct
+// outer parts, then the most terms whose mask lies inside them, then the
+// lowest positions
+void set_registers(Layout &L, uint32_t sel) {
+    const int npos = L.TB - 6 - L.P;
+    L.regpos.clear();
+    L.grppos.clear();
+    if (L.P) L.regpos.push_back(0);
+    for (int q = 0; q < npos; ++q) ((sel >> q) & 1u ? L.regpos : L.grppos).push_back(6 + L.P + q);
+    L.finish();
+}
+
 void choose_registers(Layout &L, const std::vector<CubeTerm> &terms) {
-    const int npos = L.TB - 6;
+    const int npos = L.TB - 6 - L.P;
     long best = -1;
     uint32_t best_sel = 0;
     for (uint32_t sel = 0; sel < (1u << npos); ++sel) {
-        if (__builtin_popcount(sel) != L.k) continue;
+        if (__builtin_popcount(sel) != L.k - L.P) continue;
         Layout t = L;
-        t.regpos.clear();
-        t.grppos.clear();
-        for (int q = 0; q < npos; ++q) ((sel >> q) & 1u ? t.regpos : t.grppos).push_back(6 + q);
-        t.finish();
+        set_registers(t, sel);
         std::vector<Outer> parts;
         long inside = 0, cost = 0;
         for (const CubeTerm &ct : terms) {
@@ -120,10 +129,7 @@ void choose_registers(Layout &L, const std::vector<CubeTerm> &terms) {
         const long key = (100000 - cost) * 1000 + inside;
         if (key > best) { best = key; best_sel = sel; }
     }
-    L.regpos.clear();
-    L.grppos.clear();
-    for (int q = 0; q < npos; ++q) ((best_sel >> q) & 1u ? L.regpos : L.grppos).push_back(6 + q);
-    L.finish();
+    set_registers(L, best_sel);
 }
 
 // ---------------------------------------------------------------------------
@@ -132,14 +138,15 @@ void choose_registers(Layout &L, const std::vector<CubeTerm> &terms) {
 // Cost of one pass: every distinct partner sub-cube is one load per row, a
 // far one (from L2, through the texture path) weighted three times a near
 // one (from LDS); the register sites are chosen for the pass's terms.
-long pass_cost(int N, uint32_t T, uint32_t F, int TB, int k, const std::vector<const CubeTerm *> &ts) {
+long pass_cost(int N, int P, uint32_t T, uint32_t F, int TB, int k, const std::vector<const CubeTerm *> &ts) {
     Layout L;
     L.N = N;
     L.TB = TB;
     L.k = k;
+    L.P = P;
     L.wb.clear();
-    for (int s = 0; s < 6; ++s) L.wb.push_back(s);
-    for (int s = 6; s < N; ++s)
+    for (int s = 0; s < 6 + P; ++s) L.wb.push_back(s);
+    for (int s = 6 + P; s < N; ++s)
         if ((T >> s) & 1u) L.wb.push_back(s);
     for (int s = 0; s < N; ++s)
         if ((F >> s) & 1u) L.wb.push_back(s);
@@ -171,7 +178,7 @@ struct PlanEval {
 // current terms already touch the most of its sites), else far in the pass
 // that needs the fewest new far sites (at most max_far per pass); then the
 // summed pass costs
-PlanEval evaluate(int N, int TB, int k, int max_far, const std::vector<uint32_t> &T,
+PlanEval evaluate(int N, int pair, int TB, int k, int max_far, const std::vector<uint32_t> &T,
                   const std::vector<CubeTerm> &terms) {
     PlanEval ev;
     const size_t P = T.size();
@@ -213,7 +220,7 @@ PlanEval evaluate(int N, int TB, int k, int max_far, const std::vector<uint32_t>
     for (size_t p = 0; p < P; ++p) {
         std::vector<const CubeTerm *> ts;
         for (int t : ev.assign[p]) ts.push_back(&terms[t]);
-        cost += pass_cost(N, T[p], ev.far[p], TB, k, ts);
+        cost += pass_cost(N, pair, T[p], ev.far[p], TB, k, ts);
     }
     ev.cost = cost;
     return ev;
@@ -309,7 +316,7 @@ std::string scatter(const std::string &var, const std::vector<int> &bits, int fr
 // are disjoint share one word, and popc(r) = popc(R0) + popc(X).  The counts
 // are the closed form's integers, so d(r) is the same double.
 void emit_diag(std::ostringstream &o, const CubeModel &m, const Layout &L, int S, bool first) {
-    o << "    const unsigned R0 = base | grw | lane;\n";
+    o << "    const unsigned R0 = base | grw | ln;\n";
     auto acc = [&](int x, const std::string &d) {
         const std::string X = std::to_string(x);
         if (first)
@@ -375,10 +382,11 @@ void emit_pass(std::ostringstream &o, const CubeModel &m, const CubePass &p, int
     L.k = m.reg_sites;
     L.wb = p.wb;
     L.cb = p.cb;
+    L.P = m.paired ? 1 : 0;
     choose_registers(L, p.terms);
-    const int TB = L.TB, k = L.k, S = 1 << k, WPB = m.waves;
+    const int TB = L.TB, k = L.k, S = 1 << k, WPB = m.waves, P = L.P;
     const int ngb = TB - 6 - k;                 // group index bits
-    const int ngrp_stage = 1 << (TB - 6);       // 64-row groups of the tile
+    const int ngrp_stage = 1 << (TB - 6 - P);   // groups of 64 lane accesses (2^P rows each) of the tile
     const int nsub = 1 << ngb;                  // sub-cubes per tile
     const int NG = nsub / WPB;                  // sub-cubes per wave
     const int nwin = 1 << (int)L.cb.size();
@@ -419,9 +427,12 @@ void emit_pass(std::ostringstream &o, const CubeModel &m, const CubePass &p, int
     o << "__device__ __forceinline__ void esc_cube_body" << index
       << "(const double *__restrict__ cf, const V2 *__restrict__ in, V2 *__restrict__ out, "
          "double *__restrict__ dot_part, const unsigned b) {\n";
-    o << "  __shared__ V2 tile[" << (1 << TB) << "];\n";
+    o << "  __shared__ __attribute__((aligned(16))) V2 tile[" << (1 << TB) << "];\n";
     o << "  __shared__ double red[2][" << WPB << "];\n";
     o << "  const unsigned lane = threadIdx.x & 63u;\n";
+    // lane bits in row (and tile-index) units: row pairs put site 0 below them
+    o << "  const unsigned ln = lane << " << P << "u;\n";
+    if (P) o << "  F4 *tile4 = reinterpret_cast<F4 *>(tile); (void)tile4;\n";
     o << "  const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);\n";
     if (xcdw)
         o << "  const unsigned seq = b / 8u, wi = (seq / " << bpw << "u) * 8u + b % 8u, tl = seq % " << bpw << "u;\n";
@@ -433,7 +444,11 @@ void emit_pass(std::ostringstream &o, const CubeModel &m, const CubePass &p, int
     o << "#pragma unroll\n";
     o << "  for (unsigned i = 0; i < " << ngrp_stage / WPB << "u; ++i) {\n";
     o << "    const unsigned g = wave + " << WPB << "u * i;\n";
-    o << "    tile[(g << 6) | lane] = in[base | (" << scatter("g", L.wb, 6, TB - 6) << ") | lane];\n";
+    if (P)
+        o << "    tile4[(g << 6) | lane] = *reinterpret_cast<const F4 *>(in + (base | ("
+          << scatter("g", L.wb, 6 + P, TB - 6 - P) << ") | ln));\n";
+    else
+        o << "    tile[(g << 6) | lane] = in[base | (" << scatter("g", L.wb, 6, TB - 6) << ") | lane];\n";
     o << "  }\n";
     o << "  __syncthreads();\n";
     o << "  const R A0 = (R)cf[0], B0 = (R)cf[1], Q0 = (R)cf[2], A1 = (R)cf[3], B1 = (R)cf[4], Q1 = (R)cf[5];\n";
@@ -441,7 +456,8 @@ void emit_pass(std::ostringstream &o, const CubeModel &m, const CubePass &p, int
          "Im = (R)cf[12];\n";
     o << "  (void)A0; (void)B0; (void)Q0; (void)A1; (void)B1; (void)Q1; (void)A2; (void)B2; (void)Q2;\n";
     o << "  (void)Rp; (void)Rm; (void)Ip; (void)Im;\n";
-    for (int s = 0; s < 6; ++s) o << "  const bool L" << s << " = (lane >> " << s << ") & 1u; (void)L" << s << ";\n";
+    for (int s = P; s < P + 6; ++s)
+        o << "  const bool L" << s << " = (lane >> " << s - P << ") & 1u; (void)L" << s << ";\n";
     for (int s = 0; s < L.N; ++s)
         if (L.role[s] == kBase)
             o << "  const bool U" << s << " = (base >> " << s << ") & 1u; (void)U" << s << ";\n";
@@ -457,20 +473,46 @@ void emit_pass(std::ostringstream &o, const CubeModel &m, const CubePass &p, int
         o << "    const unsigned grw = " << scatter("gi", gr, 0, ngb) << ";\n";
         for (int q = 0; q < ngb; ++q) o << "    const bool G" << q << " = (gi >> " << q << ") & 1u; (void)G" << q << ";\n";
     }
+    // a sub-cube from the tile (tile index va of slot 0) or from memory (row
+    // src + slot row + lo); with row pairs slots 2y, 2y + 1 are one 16-byte access
+    auto lds_cube = [&](const std::string &name, const std::string &va) {
+        for (int y = 0; y < S; y += 1 + P) {
+            if (P)
+                o << "      { const F4 t = tile4[(" << va << " + " << hex32(L.slot_off(y)) << ") >> 1]; " << name << "["
+                  << y << "] = make_float2(t.x, t.y); " << name << "[" << y + 1 << "] = make_float2(t.z, t.w); }\n";
+            else
+                o << "      " << name << "[" << y << "] = tile[" << va << " + " << hex32(L.slot_off(y)) << "];\n";
+        }
+    };
+    auto glb_cube = [&](const std::string &name, const std::string &src, const std::string &lo) {
+        for (int y = 0; y < S; y += 1 + P) {
+            if (P)
+                o << "      { const F4 t = *reinterpret_cast<const F4 *>(" << src << " + " << hex32(L.slot_row(y)) << " + "
+                  << lo << "); " << name << "[" << y << "] = make_float2(t.x, t.y); " << name << "[" << y + 1
+                  << "] = make_float2(t.z, t.w); }\n";
+            else
+                o << "      " << name << "[" << y << "] = (" << src << " + " << hex32(L.slot_row(y)) << ")[" << lo << "];\n";
+        }
+    };
     o << "    R ar[" << S << "], ai[" << S << "];\n";
     o << "    V2 p[" << S << "];\n";
-    o << "    {\n      const unsigned va = lane | gix;\n";
-    for (int x = 0; x < S; ++x) o << "      p[" << x << "] = tile[va + " << hex32(L.slot_off(x)) << "];\n";
+    o << "    {\n      const unsigned va = ln | gix;\n";
+    lds_cube("p", "va");
     o << "    }\n";
     // accumulators: zero in the first pass; a later pass starts from the
     // previous passes' output (streamed loads, in flight while the first
     // terms run).  The pass that holds the diagonal adds d(r) psi(r) first.
-    for (int x = 0; x < S; ++x) {
-        if (p.first)
-            o << "    ar[" << x << "] = 0; ai[" << x << "] = 0;\n";
-        else
+    for (int x = 0; x < S; x += 1 + P) {
+        if (p.first) {
+            for (int y = x; y <= x + P; ++y) o << "    ar[" << y << "] = 0; ai[" << y << "] = 0;\n";
+        } else if (P) {
+            o << "    { const F4 v = __builtin_nontemporal_load(reinterpret_cast<const F4 *>(out + (base | grw | "
+              << hex32(L.slot_row(x)) << " | ln))); ar[" << x << "] = v.x; ai[" << x << "] = v.y; ar[" << x + 1
+              << "] = v.z; ai[" << x + 1 << "] = v.w; }\n";
+        } else {
             o << "    { const NV2 v = __builtin_nontemporal_load(reinterpret_cast<const NV2 *>(out + (base | grw | "
               << hex32(L.slot_row(x)) << ") + lane)); ar[" << x << "] = v.x; ai[" << x << "] = v.y; }\n";
+        }
     }
     if (p.diag) emit_diag(o, m, L, S, p.first);
     auto emit_terms = [&](const std::vector<std::pair<CubeTerm, uint32_t>> &ts, const std::string &src) {
@@ -502,14 +544,13 @@ void emit_pass(std::ostringstream &o, const CubeModel &m, const CubePass &p, int
         for (int qq = 0; qq < ngb; ++qq)
             if ((ou.grp >> qq) & 1u) { gx |= 1u << L.grppos[qq]; grr |= 1u << L.wb[L.grppos[qq]]; }
         if (ou.far == 0) {
-            o << "    { const unsigned va = (lane ^ " << hex32(ou.lane) << ") | (gix ^ " << hex32(gx) << ");\n";
-            for (int y = 0; y < S; ++y) o << "      " << name << "[" << y << "] = tile[va + " << hex32(L.slot_off(y)) << "];\n";
+            o << "    { const unsigned va = (ln ^ " << hex32(ou.lane) << ") | (gix ^ " << hex32(gx) << ");\n";
+            lds_cube(name, "va");
             o << "    }\n";
         } else {
-            o << "    { const unsigned lo = lane ^ " << hex32(ou.lane) << ";\n";
+            o << "    { const unsigned lo = ln ^ " << hex32(ou.lane) << ";\n";
             o << "      const V2 *src = in + ((base ^ " << hex32(ou.far) << ") | (grw ^ " << hex32(grr) << "));\n";
-            for (int y = 0; y < S; ++y)
-                o << "      " << name << "[" << y << "] = (src + " << hex32(L.slot_row(y)) << ")[lo];\n";
+            glb_cube(name, "src", "lo");
             o << "    }\n";
         }
     };
@@ -548,15 +589,21 @@ void emit_pass(std::ostringstream &o, const CubeModel &m, const CubePass &p, int
     }
     while (!pending.empty()) retire_far();
     // store (streamed) and the tile's share of <in|out> from the staged rows
-    for (int x = 0; x < S; ++x) {
-        o << "    { NV2 v = {ar[" << x << "], ai[" << x << "]}; __builtin_nontemporal_store(v, reinterpret_cast<NV2 *>(out + "
-          << "(base | grw | " << hex32(L.slot_row(x)) << ") + lane)); }\n";
+    for (int x = 0; x < S; x += 1 + P) {
+        if (P)
+            o << "    { F4 v; v.x = ar[" << x << "]; v.y = ai[" << x << "]; v.z = ar[" << x + 1 << "]; v.w = ai[" << x + 1
+              << "]; __builtin_nontemporal_store(v, reinterpret_cast<F4 *>(out + (base | grw | " << hex32(L.slot_row(x))
+              << " | ln))); }\n";
+        else
+            o << "    { NV2 v = {ar[" << x << "], ai[" << x << "]}; __builtin_nontemporal_store(v, reinterpret_cast<NV2 *>(out + "
+              << "(base | grw | " << hex32(L.slot_row(x)) << ") + lane)); }\n";
     }
     if (p.last) {
-        o << "    if (dot_part) {\n      const unsigned va = lane | gix;\n";
+        o << "    if (dot_part) {\n      const unsigned va = ln | gix;\n      V2 q[" << S << "];\n";
+        lds_cube("q", "va");
         for (int x = 0; x < S; ++x)
-            o << "      { const V2 v = tile[va + " << hex32(L.slot_off(x)) << "]; const double xr = v.x, xi = v.y, yr = ar["
-              << x << "], yi = ai[" << x << "]; dr += xr * yr + xi * yi; di += xr * yi - xi * yr; }\n";
+            o << "      { const double xr = q[" << x << "].x, xi = q[" << x << "].y, yr = ar[" << x << "], yi = ai[" << x
+              << "]; dr += xr * yr + xi * yi; di += xr * yi - xi * yr; }\n";
         o << "    }\n";
     }
     o << "  }\n";
@@ -586,11 +633,18 @@ void emit_pass(std::ostringstream &o, const CubeModel &m, const CubePass &p, int
 
 bool cube_plan(const CubeModel &m, const std::vector<CubeTerm> &terms, int tile_bits, std::vector<uint32_t> tiles,
                int max_far, std::vector<CubePass> &passes) {
-    const int N = m.n_sites, TB = tile_bits, k = m.reg_sites;
-    for (uint32_t &t : tiles) t |= 63u;
+    const int N = m.n_sites, TB = tile_bits, k = m.reg_sites, P = m.paired ? 1 : 0;
+    // every tile holds the lane sites (and site 0 below them with row pairs);
+    // a seed tile that gains one drops its highest other sites
+    const uint32_t fixed = (1u << (6 + P)) - 1u;
+    if (N < 6 + P || TB < 6 + P) return false;
+    for (uint32_t &t : tiles) {
+        t |= fixed;
+        for (int s = N - 1; s >= 6 + P && __builtin_popcount(t) > TB; --s) t &= ~(1u << s);
+    }
     // plans are cached per lattice, term set, shape and seed tiles
     std::string key = std::to_string(N) + ":" + std::to_string(TB) + ":" + std::to_string(k) + ":" +
-                      std::to_string(max_far);
+                      std::to_string(max_far) + ":" + std::to_string(P);
     for (uint32_t t : tiles) key += ":" + std::to_string(t);
     for (const CubeTerm &t : terms)
         key += "," + std::to_string(t.mask) + "/" + std::to_string(t.kind) + "/" + std::to_string(t.f) + "/" +
@@ -602,19 +656,19 @@ bool cube_plan(const CubeModel &m, const std::vector<CubeTerm> &terms, int tile_
         auto it = cache.find(key);
         if (it != cache.end()) { passes = it->second; return !passes.empty(); }
     }
-    PlanEval cur = evaluate(N, TB, k, max_far, tiles, terms);
+    PlanEval cur = evaluate(N, P, TB, k, max_far, tiles, terms);
     if (cur.cost < 0) return false;
     // hill climbing: swap one tile site of one pass for a site outside it
     for (bool gain = true; gain;) {
         gain = false;
         for (size_t p = 0; p < tiles.size(); ++p) {
-            for (int a = 6; a < N; ++a) {
+            for (int a = 6 + P; a < N; ++a) {
                 if (!((tiles[p] >> a) & 1u)) continue;
-                for (int b = 6; b < N; ++b) {
+                for (int b = 6 + P; b < N; ++b) {
                     if ((tiles[p] >> b) & 1u) continue;
                     std::vector<uint32_t> t2(tiles);
                     t2[p] = (t2[p] & ~(1u << a)) | (1u << b);
-                    const PlanEval ev = evaluate(N, TB, k, max_far, t2, terms);
+                    const PlanEval ev = evaluate(N, P, TB, k, max_far, t2, terms);
                     if (ev.cost >= 0 && ev.cost < cur.cost) {
                         tiles = t2;
                         cur = ev;
@@ -629,8 +683,8 @@ bool cube_plan(const CubeModel &m, const std::vector<CubeTerm> &terms, int tile_
     for (size_t p = 0; p < tiles.size(); ++p) {
         if (cur.assign[p].empty()) continue;
         CubePass cp;
-        for (int s = 0; s < 6; ++s) cp.wb.push_back(s);
-        for (int s = 6; s < N; ++s)
+        for (int s = 0; s < 6 + P; ++s) cp.wb.push_back(s);
+        for (int s = 6 + P; s < N; ++s)
             if ((tiles[p] >> s) & 1u) cp.wb.push_back(s);
         for (int s = 0; s < N; ++s)
             if ((cur.far[p] >> s) & 1u) cp.wb.push_back(s);
@@ -657,6 +711,7 @@ std::string cube_source(const CubeModel &m, const std::vector<CubePass> &passes)
     o << "typedef " << (m.c64 ? "float" : "double") << " R;\n";
     o << "typedef " << (m.c64 ? "float2" : "double2") << " V2;\n";
     o << "typedef R NV2 __attribute__((ext_vector_type(2)));\n";
+    if (m.paired) o << "typedef float F4 __attribute__((ext_vector_type(4)));\n";
     // diagonal: the closed form where every partial sum of the Kitaev-z part is
     // exact (it then equals the reference's bond-by-bond sum), else that sum
     // itself, z bonds then Heisenberg bonds in the reference's order
@@ -797,9 +852,10 @@ bool code_for(const std::string &src, const std::string &key, std::vector<char> 
 bool check_model(const CubeModel &m, const std::vector<CubePass> &passes, std::string *err) {
     for (const CubePass &p : passes) {
         const size_t esz = m.c64 ? 8 : 16;
+        const int P = m.paired ? 1 : 0;
         const int ngb = p.tile_bits - 6 - m.reg_sites;
-        if (ngb < 0 || (1 << ngb) % m.waves != 0 || ((size_t)1 << p.tile_bits) * esz > 131072 ||
-            (1 << (p.tile_bits - 6)) % m.waves != 0 || (int)p.wb.size() < p.tile_bits) {
+        if (ngb < 0 || m.reg_sites < P || (1 << ngb) % m.waves != 0 || ((size_t)1 << p.tile_bits) * esz > 131072 ||
+            p.tile_bits < 6 + P || (1 << (p.tile_bits - 6 - P)) % m.waves != 0 || (int)p.wb.size() < p.tile_bits) {
             if (err) *err = "tile shape does not fit the sub-cube kernel";
             return false;
         }

@@ -58,6 +58,8 @@ struct CubeModel {
     uint32_t zpair[64], jpair[96];
     int reg_sites;          // k: rows per lane = 2^k
     int waves;              // waves per workgroup
+    bool paired;            // row pairs: site 0 is a register site under lanes on sites 1..6,
+                            // so the lane moves rows 2y, 2y + 1 as one 16-byte access (complex64)
 };
 
 // coupling scalars of the kernels (double, device memory), in this order

@@ -233,8 +233,10 @@ class Engine:
     def kernel_configs(self):
         """H.psi kernel configurations: 0 (one row per lane) or
         100000 * planned passes (5: cover-first planner, 6: its windows in
-        reverse order) + 10000 * log2(waves
-        per workgroup) + 100 * tile bits + window bits."""
+        reverse order; 7, 8: the generated sub-cube kernels over those plans)
+        + 10000 * log2(waves per workgroup) + 100 * tile bits + window bits,
+        plus, for the generated kernels, 1000000 * log2(rows per lane) and
+        10000000 when they run over row pairs (complex64, 16-byte accesses)."""
         n = ctypes.c_int32()
         check(lib().esc_kernel_configs(self._h, None, 0, ctypes.byref(n)))
         buf = (ctypes.c_int32 * max(n.value, 1))()
@@ -455,21 +457,28 @@ class Batch:
         return tables, (expect[:, 0::2] + 1j * expect[:, 1::2])[:, :nexp]
 
 
+def default_kernel_config(n_sites, precision="c128"):
+    """The generated-kernel configuration a context of n_sites sites takes by
+    default from 19 sites (default_tile_bits in esc_capi.cpp); also used for
+    smaller lattices, where the window holds every site."""
+    w = n_sites if n_sites <= 18 else (17 if n_sites <= 22 else 18)
+    if precision == "c64":   # 1xxxxxxx: over row pairs (16-byte accesses)
+        return (13731200 if n_sites <= 22 else (14831300 if n_sites <= 26 else 4831300)) + w
+    return 3731200 + w
+
+
 def precompile_kernel(model: SpinModel, precision="c128", config=None):
     """Build a generated H.psi kernel configuration for this lattice into the
     code-object cache without a device (esc_kernel_precompile) and return its
     HIP source.  config None: the configuration a context of this size uses
     by default (complex128: 2^3 rows per lane, 8 waves, 2^12-row tiles;
-    complex64: 2^4 rows per lane, 4 waves on 2^12-row tiles up to 22 sites,
-    8 waves on 2^13-row tiles beyond; at most 17 window sites up to 22 sites,
-    18 beyond)."""
+    complex64: 2^3 rows per lane on 2^12-row tiles over row pairs up to 22
+    sites, 2^4 rows per lane on 2^13-row tiles with the windows in reverse
+    order beyond (over row pairs up to 26 sites), 8 waves; at most 17 window
+    sites up to 22 sites, 18 beyond)."""
     N = model.lattice.N
     if config is None:
-        w = N if N <= 18 else (17 if N <= 22 else 18)
-        if precision == "c64":
-            config = (4721200 if N <= 22 else 4731300) + w
-        else:
-            config = 3731200 + w
+        config = default_kernel_config(N, precision)
     st = model.struct()
     n = ctypes.c_size_t()
     check(lib().esc_kernel_precompile(ctypes.byref(st), ESC_C64 if precision == "c64" else ESC_C128, int(config),
@@ -485,9 +494,7 @@ def kernel_plan(model: SpinModel, precision="c128", config=None):
     (esc_kernel_plan): a list of per-pass dicts (window, tile, near, far,
     lds_bytes, diag) and, per off-diagonal term, (mask, pass)."""
     if config is None:
-        N = model.lattice.N
-        w = N if N <= 18 else (17 if N <= 22 else 18)
-        config = ((4721200 if N <= 22 else 4731300) + w) if precision == "c64" else 3731200 + w
+        config = default_kernel_config(model.lattice.N, precision)
     st = model.struct()
     prec = ESC_C64 if precision == "c64" else ESC_C128
     npass, nterm = ctypes.c_int(), ctypes.c_int()
