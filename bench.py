@@ -88,10 +88,10 @@ def time_hpsi(eng, x, y, steps, warmup, copy_calib=False):
 
 # The H.psi kernel configurations of this workload: the library defaults at 24
 # sites (generated sub-cube kernels: complex128 2^3 rows per lane, 8-wave
-# workgroups on 2^12-row tiles; complex64 row pairs, 2^4 rows per lane on
-# 2^13-row tiles, windows in reverse order; cover-first plan over 2^18-row
-# windows, two passes).
-PROFILED_CONFIGS = {"c128": [3731218], "c64": [14831318]}
+# workgroups on 2^12-row tiles; complex64 row pairs, 2^4 rows per lane,
+# 4-wave workgroups on 2^12-row tiles, windows in reverse order; cover-first
+# plan over 2^18-row windows, two passes).
+PROFILED_CONFIGS = {"c128": [3731218], "c64": [14821218]}
 
 # HBM bytes per H.psi (both passes) from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE on `python bench.py --steps 3 --warmup 1 --no-cpu --no-spectrum
@@ -99,10 +99,10 @@ PROFILED_CONFIGS = {"c128": [3731218], "c64": [14831318]}
 # make a plain one-vector read / write report its own size), from the per-pass
 # rows of the named profile summary.
 MEASURED_TRAFFIC = {
-    # (206529 + 355911) KiB fetched x 2.0 + (262235 + 262154) KiB written
-    ("c128", 3731218): (1688851456, "profiles/hpsi_cube_n24_hbm_traffic.md: esc_cube c128 (config 3731218)"),
-    # (81305 + 134345) KiB fetched x 2.0 + (131153 + 131171) KiB written
-    ("c64", 14831318): (710270976, "profiles/hpsi_cube_n24_hbm_traffic.md: esc_cube c64 (config 14831318)"),
+    # (184446 + 385158) KiB fetched x 2.0 + (262166 + 262183) KiB written
+    ("c128", 3731218): (1703482368, "profiles/hpsi_cube_n24_hbm_traffic.md: esc_cube c128 (config 3731218)"),
+    # (88583 + 160886) KiB fetched x 2.0 + (131105 + 131079) KiB written
+    ("c64", 14821218): (779388928, "profiles/hpsi_cube_n24_hbm_traffic.md: esc_cube c64 (config 14821218)"),
 }
 
 

@@ -368,16 +368,18 @@ constexpr int kCubeFirst = 7;
 constexpr int kCubeLast = 8;
 int cfg_passes(int config) { return (config / 100000) % 10; }
 int cfg_regs(int config) { return (config / 1000000) % 10; }
-// generated kernels over row pairs (complex64: 16-byte lane accesses)
-bool cfg_paired(int config) { return (config / 10000000) % 10 == 1; }
+// the layout digit of the generated kernels (0..3): bit 0 = over row pairs
+// (complex64: 16-byte lane accesses), bit 1 = lanes on six contiguous sites
+bool cfg_paired(int config) { return ((config / 10000000) % 10) & 1; }
+bool cfg_contiguous(int config) { return ((config / 10000000) % 10) & 2; }
 bool cfg_cube(int config) { return cfg_passes(config) == kCubeFirst || cfg_passes(config) == kCubeLast; }
 int cfg_waves(int config) { return 1 << ((config / 10000) % 10); }
 int cfg_tile_bits(int config) { return (config / 100) % 100; }
 int cfg_window(int config) { return config % 100; }
-int make_config(int P, int waves, int tb, int w, int k = 0, bool paired = false) {
+int make_config(int P, int waves, int tb, int w, int k = 0, bool paired = false, bool contiguous = false) {
     int v = 0;
     while ((1 << v) < waves) ++v;
-    return (paired ? 10000000 : 0) + 1000000 * k + 100000 * P + 10000 * v + 100 * tb + w;
+    return 10000000 * ((paired ? 1 : 0) | (contiguous ? 2 : 0)) + 1000000 * k + 100000 * P + 10000 * v + 100 * tb + w;
 }
 
 // One planned pass: window sites W, tile sites T (lane sites 0..5 included)
@@ -738,6 +740,7 @@ bool cube_spec(const esc_ctx *c, int cfg, esc::CubeModel &m, std::vector<esc::Cu
     }
     m.reg_sites = cfg_regs(cfg);
     m.paired = cfg_paired(cfg);
+    m.contiguous = cfg_contiguous(cfg);
     if (m.paired && !m.c64) {
         if (why) *why = "row pairs are a complex64 layout";
         return false;
@@ -1202,8 +1205,9 @@ bool tile_config_fits(const esc_ctx *c, int config) {
 
 // generated sub-cube configurations: 2^12- and 2^13-row tiles, 2^3 or 2^4
 // rows per lane, 4 or 8 waves, the cover-first windows of the generic kernel
-// in both orders; complex64 with and without row pairs.  ESC_NO_JIT=1 leaves
-// them out.
+// in both orders; complex64 with and without row pairs; from 25 sites on
+// (vectors of 256 MiB and more) the contiguous lane layout besides the free
+// one.  ESC_NO_JIT=1 leaves them out.
 std::vector<int> cube_candidates(const esc_ctx *c) {
     std::vector<int> v;
     const char *nj = std::getenv("ESC_NO_JIT");
@@ -1216,11 +1220,13 @@ std::vector<int> cube_candidates(const esc_ctx *c) {
         for (int k : {3, 4})
             for (int waves : {4, 8})
                 for (int P : {kCubeFirst, kCubeLast})
-                    for (bool paired : {false, true}) {
-                        if ((w == N && P == kCubeLast) || (paired && !c->c64())) continue;
-                        const int cfg = make_config(P, waves, tb, w, k, paired);
-                        if (tile_config_fits(c, cfg)) v.push_back(cfg);
-                    }
+                    for (bool paired : {false, true})
+                        for (bool contiguous : {false, true}) {
+                            if ((w == N && P == kCubeLast) || (paired && !c->c64()) || (contiguous && N < 25))
+                                continue;
+                            const int cfg = make_config(P, waves, tb, w, k, paired, contiguous);
+                            if (tile_config_fits(c, cfg)) v.push_back(cfg);
+                        }
     }
     return v;
 }
@@ -1259,16 +1265,19 @@ std::vector<int> tile_candidates(const esc_ctx *c) {
 // with 2^12-row tiles (16 waves for c128, which moves the least HBM traffic of
 // the equally fast shapes at N = 24, 8 waves for c64).  A shape the context
 // cannot run falls back to a single pass over 2^11- or 2^10-row tiles.
-// From 19 sites the generated sub-cube kernels come first (N = 24: 0.31 ms
+// From 19 sites the generated sub-cube kernels come first (N = 24: 0.29 ms
 // against 0.52 ms for the best generic shape in complex128, 0.15 against
-// 0.32 ms in complex64; N = 20: 25.6 against 40.1 us and 14.3 against
+// 0.32 ms in complex64; N = 20: 18.8 against 40.1 us and 13.9 against
 // 28.8 us): complex128 2^3 rows per lane in 8-wave workgroups on 2^12-row
-// tiles; complex64 8 waves, over row pairs with 2^3 rows per lane on
-// 2^12-row tiles up to 22 sites and 2^4 rows per lane on 2^13-row tiles with
-// the windows in reverse order up to 26 sites, the same without row pairs
-// beyond (N = 32: 70 against 82 ms; all its plans take three passes).  A
-// model they do not cover, or a missing run-time compiler, falls through to
-// the generic shapes.
+// tiles (N = 24: 289 us with free lane sites, 307 us with contiguous ones);
+// complex64 over row pairs, with 2^3 rows per lane in 8-wave workgroups on
+// 2^12-row tiles up to 22 sites, 2^4 rows per lane in 4-wave workgroups on
+// 2^12-row tiles with the windows in reverse order up to 26 sites (N = 24:
+// 154 against 162 us for 8 waves on 2^13-row tiles), and 2^3 rows per lane
+// in 8-wave workgroups on 2^13-row tiles, windows in reverse order, beyond
+// (N = 32: 66.6 ms; 72 ms without row pairs with contiguous lanes, 84 ms
+// with 2^4 rows per lane).  A model they do not cover, or a missing run-time
+// compiler, falls through to the generic shapes.
 int default_tile_bits(esc_ctx *c) {
     const int N = c->n_sites;
     const bool c64 = c->c64();
@@ -1277,8 +1286,8 @@ int default_tile_bits(esc_ctx *c) {
     if (N >= 19) {
         const int w = N <= 22 ? 17 : 18;
         if (c64 && N <= 22) pref.push_back(make_config(kCubeFirst, 8, 12, w, 3, true));
-        else if (c64 && N <= 26) pref.push_back(make_config(kCubeLast, 8, 13, w, 4, true));
-        else if (c64) pref.push_back(make_config(kCubeLast, 8, 13, w, 4));
+        else if (c64 && N <= 26) pref.push_back(make_config(kCubeLast, 4, 12, w, 4, true));
+        else if (c64) pref.push_back(make_config(kCubeLast, 8, 13, w, 3, true));
         else pref.push_back(make_config(kCubeFirst, 8, 12, w, 3));
     }
     if (N <= 18) pref.push_back(make_config(1, 8, 10, N));
@@ -2878,8 +2887,11 @@ int esc_set_tile_bits(esc_ctx *ctx, int tile_bits) {
     return guarded([&] {
         check_ctx(ctx);
         if (tile_bits != 0) {
+            // any generated-kernel configuration that plans for this context,
+            // else one of the candidates
             const auto cand = tile_candidates(ctx);
-            if (std::find(cand.begin(), cand.end(), tile_bits) == cand.end())
+            const bool cube_ok = cfg_cube(tile_bits) && tile_bits < 40000000 && tile_config_fits(ctx, tile_bits);
+            if (!cube_ok && std::find(cand.begin(), cand.end(), tile_bits) == cand.end())
                 throw EscError(ESC_ERR_ARG, "tile size not available for this context");
         }
         ctx->tile_bits = tile_bits;

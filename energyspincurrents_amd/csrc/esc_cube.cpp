@@ -32,24 +32,29 @@ constexpr int kFarAhead = 1;   // far sub-cubes in flight ahead of their terms
 // ---------------------------------------------------------------------------
 // Layout of one pass
 // ---------------------------------------------------------------------------
-// Row bits: lane sites P..P+5 (P = 0, or 1 for row pairs); tile-index bit q
-// holds window site wb[q] (wb[q] = q for q < P + 6); the register sites are k
-// of the tile positions below TB (slot bit b of a lane's sub-cube = position
-// regpos[b]; with row pairs regpos[0] = 0, site 0, so a lane's slots 2y and
-// 2y + 1 are adjacent rows and move as one 16-byte access), the others are
-// group positions (bit q of the group index = position grppos[q]); far sites
-// wb[TB..w-1] and window-id sites cb are fixed per workgroup.
+// Tile-index bit q holds window site wb[q].  Positions P..P+5 (P = 0, or 1
+// for row pairs) are the lanes: lane bit i is site wb[P + i].  Lane bits
+// 0..run-1 are the sites P..P+run-1 above the pair bit, so 2^run lanes read
+// one run of rows: 64 bytes (run = 2 with 16-byte lane accesses, 3 with 8-byte
+// ones) in the free layout, whose other lane sites are any tile sites, or the
+// whole wave's 512 or 1024 bytes in the contiguous layout (run = 6).
+// The register sites are k of the tile positions (slot bit b of a lane's
+// sub-cube = position regpos[b]; with row pairs regpos[0] = 0, site 0, so a
+// lane's slots 2y and 2y + 1 are adjacent rows and move as one 16-byte
+// access), the others are group positions (bit q of the group index =
+// position grppos[q]); far sites wb[TB..w-1] and window-id sites cb are fixed
+// per workgroup.
 enum Role { kLane, kReg, kGrp, kBase };
 
 struct Layout {
-    int N = 0, TB = 0, w = 0, k = 0, P = 0;
+    int N = 0, TB = 0, w = 0, k = 0, P = 0, run = 6;
     std::vector<int> wb, cb, regpos, grppos;
     int role[64];
-    int idx[64];     // slot bit (kReg) or group bit (kGrp) of a site
+    int idx[64];     // lane bit (kLane), slot bit (kReg) or group bit (kGrp) of a site
 
     void finish() {
         for (int s = 0; s < 64; ++s) { role[s] = kBase; idx[s] = -1; }
-        for (int s = P; s < P + 6; ++s) role[s] = kLane;
+        for (int i = 0; i < 6; ++i) { role[wb[P + i]] = kLane; idx[wb[P + i]] = i; }
         for (int b = 0; b < (int)regpos.size(); ++b) { role[wb[regpos[b]]] = kReg; idx[wb[regpos[b]]] = b; }
         for (int q = 0; q < (int)grppos.size(); ++q) { role[wb[grppos[q]]] = kGrp; idx[wb[grppos[q]]] = q; }
     }
@@ -79,13 +84,25 @@ struct Outer {
     bool own() const { return lane == 0 && grp == 0 && far == 0; }
 };
 
+// lane sites of the run: all six in the contiguous layout, else a 64-byte run
+// (two sites of 16-byte lane accesses, three of 8-byte ones)
+int lane_run(const CubeModel &m) { return m.contiguous ? 6 : (m.c64 && !m.paired ? 3 : 2); }
+
+// row bits of the lane bits `lanes` of layout L
+uint32_t lane_rows(const Layout &L, uint32_t lanes) {
+    uint32_t r = 0;
+    for (int i = 0; i < 6; ++i)
+        if ((lanes >> i) & 1u) r |= 1u << L.wb[L.P + i];
+    return r;
+}
+
 void split_mask(const Layout &L, uint32_t mask, Outer &o, uint32_t &rho) {
     o = Outer();
     rho = 0;
     for (int s = 0; s < L.N; ++s) {
         if (!((mask >> s) & 1u)) continue;
         switch (L.role[s]) {
-        case kLane: o.lane |= 1u << s; break;
+        case kLane: o.lane |= 1u << L.idx[s]; break;
         case kReg: rho |= 1u << L.idx[s]; break;
         case kGrp: o.grp |= 1u << L.idx[s]; break;
         default: o.far |= 1u << s; break;
@@ -93,27 +110,56 @@ void split_mask(const Layout &L, uint32_t mask, Outer &o, uint32_t &rho) {
     }
 }
 
-// register sites of a pass: the k tile positions (outside the lane sites;
-// with row pairs site 0 and k - 1 others) that leave the fewest distinct
-// outer parts, then the most terms whose mask lies inside them, then the
-// lowest positions
-void set_registers(Layout &L, uint32_t sel) {
-    const int npos = L.TB - 6 - L.P;
+// Layout of a pass's tile sites (wb[0..TB-1], any order on entry): site 0
+// below the lanes with row pairs; lanes = sites P..P+run-1 (the run) and
+// 6 - run more; the register sites (site 0 and k - 1 others with row pairs);
+// the rest are group sites.  `sel` picks the registers among `cand` (the tile
+// sites that are neither site 0 with row pairs nor a run site); the lanes take
+// the four lowest of the others (a lane or a group bit costs the same LDS
+// load, so that choice does not change the plan's cost).
+void set_layout(Layout &L, const std::vector<int> &cand, uint32_t sel) {
+    std::vector<int> regs, rest;
+    for (size_t q = 0; q < cand.size(); ++q) ((sel >> q) & 1u ? regs : rest).push_back(cand[q]);
+    std::vector<int> tile;
+    if (L.P) tile.push_back(0);
+    for (int i = 0; i < L.run; ++i) tile.push_back(L.P + i);
+    for (int i = 0; i < 6 - L.run; ++i) tile.push_back(rest[i]);
+    std::vector<int> others(regs.begin(), regs.end());
+    others.insert(others.end(), rest.begin() + (6 - L.run), rest.end());
+    std::sort(others.begin(), others.end());
+    tile.insert(tile.end(), others.begin(), others.end());
+    for (int q = 0; q < L.TB; ++q) L.wb[q] = tile[q];
     L.regpos.clear();
     L.grppos.clear();
     if (L.P) L.regpos.push_back(0);
-    for (int q = 0; q < npos; ++q) ((sel >> q) & 1u ? L.regpos : L.grppos).push_back(6 + L.P + q);
+    for (int q = 6 + L.P; q < L.TB; ++q)
+        (std::find(regs.begin(), regs.end(), L.wb[q]) != regs.end() ? L.regpos : L.grppos).push_back(q);
     L.finish();
 }
 
+std::vector<int> layout_candidates(const Layout &L) {
+    std::vector<int> cand;
+    for (int q = 0; q < L.TB; ++q) {
+        const int s = L.wb[q];
+        if ((L.P && s == 0) || (s >= L.P && s < L.P + L.run)) continue;
+        cand.push_back(s);
+    }
+    std::sort(cand.begin(), cand.end());
+    return cand;
+}
+
+// The register choice that leaves the fewest distinct outer parts (far ones
+// weighted 3x), then the most terms whose mask lies inside the registers,
+// then the lowest sites.
 void choose_registers(Layout &L, const std::vector<CubeTerm> &terms) {
-    const int npos = L.TB - 6 - L.P;
+    const std::vector<int> cand = layout_candidates(L);
+    const int npos = (int)cand.size();
     long best = -1;
     uint32_t best_sel = 0;
     for (uint32_t sel = 0; sel < (1u << npos); ++sel) {
         if (__builtin_popcount(sel) != L.k - L.P) continue;
         Layout t = L;
-        set_registers(t, sel);
+        set_layout(t, cand, sel);
         std::vector<Outer> parts;
         long inside = 0, cost = 0;
         for (const CubeTerm &ct : terms) {
@@ -129,7 +175,7 @@ void choose_registers(Layout &L, const std::vector<CubeTerm> &terms) {
         const long key = (100000 - cost) * 1000 + inside;
         if (key > best) { best = key; best_sel = sel; }
     }
-    set_registers(L, best_sel);
+    set_layout(L, cand, best_sel);
 }
 
 // ---------------------------------------------------------------------------
@@ -138,15 +184,16 @@ void choose_registers(Layout &L, const std::vector<CubeTerm> &terms) {
 // Cost of one pass: every distinct partner sub-cube is one load per row, a
 // far one (from L2, through the texture path) weighted three times a near
 // one (from LDS); the register sites are chosen for the pass's terms.
-long pass_cost(int N, int P, uint32_t T, uint32_t F, int TB, int k, const std::vector<const CubeTerm *> &ts) {
+long pass_cost(int N, int P, int run, uint32_t T, uint32_t F, int TB, int k,
+               const std::vector<const CubeTerm *> &ts) {
     Layout L;
     L.N = N;
     L.TB = TB;
     L.k = k;
     L.P = P;
+    L.run = run;
     L.wb.clear();
-    for (int s = 0; s < 6 + P; ++s) L.wb.push_back(s);
-    for (int s = 6 + P; s < N; ++s)
+    for (int s = 0; s < N; ++s)
         if ((T >> s) & 1u) L.wb.push_back(s);
     for (int s = 0; s < N; ++s)
         if ((F >> s) & 1u) L.wb.push_back(s);
@@ -178,7 +225,7 @@ struct PlanEval {
 // current terms already touch the most of its sites), else far in the pass
 // that needs the fewest new far sites (at most max_far per pass); then the
 // summed pass costs
-PlanEval evaluate(int N, int pair, int TB, int k, int max_far, const std::vector<uint32_t> &T,
+PlanEval evaluate(int N, int pair, int run, int TB, int k, int max_far, const std::vector<uint32_t> &T,
                   const std::vector<CubeTerm> &terms) {
     PlanEval ev;
     const size_t P = T.size();
@@ -220,7 +267,7 @@ PlanEval evaluate(int N, int pair, int TB, int k, int max_far, const std::vector
     for (size_t p = 0; p < P; ++p) {
         std::vector<const CubeTerm *> ts;
         for (int t : ev.assign[p]) ts.push_back(&terms[t]);
-        cost += pass_cost(N, pair, T[p], ev.far[p], TB, k, ts);
+        cost += pass_cost(N, pair, run, T[p], ev.far[p], TB, k, ts);
     }
     ev.cost = cost;
     return ev;
@@ -316,7 +363,7 @@ std::string scatter(const std::string &var, const std::vector<int> &bits, int fr
 // are disjoint share one word, and popc(r) = popc(R0) + popc(X).  The counts
 // are the closed form's integers, so d(r) is the same double.
 void emit_diag(std::ostringstream &o, const CubeModel &m, const Layout &L, int S, bool first) {
-    o << "    const unsigned R0 = base | grw | ln;\n";
+    o << "    const unsigned R0 = base | grw | lrow;\n";
     auto acc = [&](int x, const std::string &d) {
         const std::string X = std::to_string(x);
         if (first)
@@ -383,6 +430,7 @@ void emit_pass(std::ostringstream &o, const CubeModel &m, const CubePass &p, int
     L.wb = p.wb;
     L.cb = p.cb;
     L.P = m.paired ? 1 : 0;
+    L.run = lane_run(m);
     choose_registers(L, p.terms);
     const int TB = L.TB, k = L.k, S = 1 << k, WPB = m.waves, P = L.P;
     const int ngb = TB - 6 - k;                 // group index bits
@@ -430,8 +478,9 @@ void emit_pass(std::ostringstream &o, const CubeModel &m, const CubePass &p, int
     o << "  __shared__ __attribute__((aligned(16))) V2 tile[" << (1 << TB) << "];\n";
     o << "  __shared__ double red[2][" << WPB << "];\n";
     o << "  const unsigned lane = threadIdx.x & 63u;\n";
-    // lane bits in row (and tile-index) units: row pairs put site 0 below them
+    // lane bits in tile-index units (positions P..P+5) and in row units
     o << "  const unsigned ln = lane << " << P << "u;\n";
+    o << "  const unsigned lrow = " << scatter("lane", L.wb, P, 6) << ";\n";
     if (P) o << "  F4 *tile4 = reinterpret_cast<F4 *>(tile); (void)tile4;\n";
     o << "  const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);\n";
     if (xcdw)
@@ -446,9 +495,9 @@ void emit_pass(std::ostringstream &o, const CubeModel &m, const CubePass &p, int
     o << "    const unsigned g = wave + " << WPB << "u * i;\n";
     if (P)
         o << "    tile4[(g << 6) | lane] = *reinterpret_cast<const F4 *>(in + (base | ("
-          << scatter("g", L.wb, 6 + P, TB - 6 - P) << ") | ln));\n";
+          << scatter("g", L.wb, 6 + P, TB - 6 - P) << ") | lrow));\n";
     else
-        o << "    tile[(g << 6) | lane] = in[base | (" << scatter("g", L.wb, 6, TB - 6) << ") | lane];\n";
+        o << "    tile[(g << 6) | lane] = in[base | (" << scatter("g", L.wb, 6, TB - 6) << ") | lrow];\n";
     o << "  }\n";
     o << "  __syncthreads();\n";
     o << "  const R A0 = (R)cf[0], B0 = (R)cf[1], Q0 = (R)cf[2], A1 = (R)cf[3], B1 = (R)cf[4], Q1 = (R)cf[5];\n";
@@ -456,8 +505,8 @@ void emit_pass(std::ostringstream &o, const CubeModel &m, const CubePass &p, int
          "Im = (R)cf[12];\n";
     o << "  (void)A0; (void)B0; (void)Q0; (void)A1; (void)B1; (void)Q1; (void)A2; (void)B2; (void)Q2;\n";
     o << "  (void)Rp; (void)Rm; (void)Ip; (void)Im;\n";
-    for (int s = P; s < P + 6; ++s)
-        o << "  const bool L" << s << " = (lane >> " << s - P << ") & 1u; (void)L" << s << ";\n";
+    for (int i = 0; i < 6; ++i)
+        o << "  const bool L" << L.wb[P + i] << " = (lane >> " << i << ") & 1u; (void)L" << L.wb[P + i] << ";\n";
     for (int s = 0; s < L.N; ++s)
         if (L.role[s] == kBase)
             o << "  const bool U" << s << " = (base >> " << s << ") & 1u; (void)U" << s << ";\n";
@@ -507,11 +556,11 @@ void emit_pass(std::ostringstream &o, const CubeModel &m, const CubePass &p, int
             for (int y = x; y <= x + P; ++y) o << "    ar[" << y << "] = 0; ai[" << y << "] = 0;\n";
         } else if (P) {
             o << "    { const F4 v = __builtin_nontemporal_load(reinterpret_cast<const F4 *>(out + (base | grw | "
-              << hex32(L.slot_row(x)) << " | ln))); ar[" << x << "] = v.x; ai[" << x << "] = v.y; ar[" << x + 1
+              << hex32(L.slot_row(x)) << " | lrow))); ar[" << x << "] = v.x; ai[" << x << "] = v.y; ar[" << x + 1
               << "] = v.z; ai[" << x + 1 << "] = v.w; }\n";
         } else {
             o << "    { const NV2 v = __builtin_nontemporal_load(reinterpret_cast<const NV2 *>(out + (base | grw | "
-              << hex32(L.slot_row(x)) << ") + lane)); ar[" << x << "] = v.x; ai[" << x << "] = v.y; }\n";
+              << hex32(L.slot_row(x)) << " | lrow))); ar[" << x << "] = v.x; ai[" << x << "] = v.y; }\n";
         }
     }
     if (p.diag) emit_diag(o, m, L, S, p.first);
@@ -544,11 +593,11 @@ void emit_pass(std::ostringstream &o, const CubeModel &m, const CubePass &p, int
         for (int qq = 0; qq < ngb; ++qq)
             if ((ou.grp >> qq) & 1u) { gx |= 1u << L.grppos[qq]; grr |= 1u << L.wb[L.grppos[qq]]; }
         if (ou.far == 0) {
-            o << "    { const unsigned va = (ln ^ " << hex32(ou.lane) << ") | (gix ^ " << hex32(gx) << ");\n";
+            o << "    { const unsigned va = (ln ^ " << hex32(ou.lane << P) << ") | (gix ^ " << hex32(gx) << ");\n";
             lds_cube(name, "va");
             o << "    }\n";
         } else {
-            o << "    { const unsigned lo = ln ^ " << hex32(ou.lane) << ";\n";
+            o << "    { const unsigned lo = lrow ^ " << hex32(lane_rows(L, ou.lane)) << ";\n";
             o << "      const V2 *src = in + ((base ^ " << hex32(ou.far) << ") | (grw ^ " << hex32(grr) << "));\n";
             glb_cube(name, "src", "lo");
             o << "    }\n";
@@ -593,10 +642,10 @@ void emit_pass(std::ostringstream &o, const CubeModel &m, const CubePass &p, int
         if (P)
             o << "    { F4 v; v.x = ar[" << x << "]; v.y = ai[" << x << "]; v.z = ar[" << x + 1 << "]; v.w = ai[" << x + 1
               << "]; __builtin_nontemporal_store(v, reinterpret_cast<F4 *>(out + (base | grw | " << hex32(L.slot_row(x))
-              << " | ln))); }\n";
+              << " | lrow))); }\n";
         else
             o << "    { NV2 v = {ar[" << x << "], ai[" << x << "]}; __builtin_nontemporal_store(v, reinterpret_cast<NV2 *>(out + "
-              << "(base | grw | " << hex32(L.slot_row(x)) << ") + lane)); }\n";
+              << "(base | grw | " << hex32(L.slot_row(x)) << " | lrow))); }\n";
     }
     if (p.last) {
         o << "    if (dot_part) {\n      const unsigned va = ln | gix;\n      V2 q[" << S << "];\n";
@@ -633,18 +682,21 @@ void emit_pass(std::ostringstream &o, const CubeModel &m, const CubePass &p, int
 
 bool cube_plan(const CubeModel &m, const std::vector<CubeTerm> &terms, int tile_bits, std::vector<uint32_t> tiles,
                int max_far, std::vector<CubePass> &passes) {
-    const int N = m.n_sites, TB = tile_bits, k = m.reg_sites, P = m.paired ? 1 : 0;
-    // every tile holds the lane sites (and site 0 below them with row pairs);
-    // a seed tile that gains one drops its highest other sites
-    const uint32_t fixed = (1u << (6 + P)) - 1u;
+    const int N = m.n_sites, TB = tile_bits, k = m.reg_sites, P = m.paired ? 1 : 0, run = lane_run(m);
+    // every tile holds the sites below and of the lanes' run; a 64-byte run
+    // also the next site, so its 128-byte line lies in the tile (sites 0..2
+    // with 16-byte rows, else 0..3); a seed tile that gains one of them drops
+    // its highest other sites
+    const uint32_t fixed = run == 6 ? (1u << (6 + P)) - 1u : (1u << (P + run + 1)) - 1u;
     if (N < 6 + P || TB < 6 + P) return false;
     for (uint32_t &t : tiles) {
         t |= fixed;
-        for (int s = N - 1; s >= 6 + P && __builtin_popcount(t) > TB; --s) t &= ~(1u << s);
+        for (int s = N - 1; s >= 0 && __builtin_popcount(t) > TB; --s)
+            if (!((fixed >> s) & 1u)) t &= ~(1u << s);
     }
     // plans are cached per lattice, term set, shape and seed tiles
     std::string key = std::to_string(N) + ":" + std::to_string(TB) + ":" + std::to_string(k) + ":" +
-                      std::to_string(max_far) + ":" + std::to_string(P);
+                      std::to_string(max_far) + ":" + std::to_string(P) + ":" + std::to_string(run);
     for (uint32_t t : tiles) key += ":" + std::to_string(t);
     for (const CubeTerm &t : terms)
         key += "," + std::to_string(t.mask) + "/" + std::to_string(t.kind) + "/" + std::to_string(t.f) + "/" +
@@ -656,19 +708,19 @@ bool cube_plan(const CubeModel &m, const std::vector<CubeTerm> &terms, int tile_
         auto it = cache.find(key);
         if (it != cache.end()) { passes = it->second; return !passes.empty(); }
     }
-    PlanEval cur = evaluate(N, P, TB, k, max_far, tiles, terms);
+    PlanEval cur = evaluate(N, P, run, TB, k, max_far, tiles, terms);
     if (cur.cost < 0) return false;
     // hill climbing: swap one tile site of one pass for a site outside it
     for (bool gain = true; gain;) {
         gain = false;
         for (size_t p = 0; p < tiles.size(); ++p) {
-            for (int a = 6 + P; a < N; ++a) {
-                if (!((tiles[p] >> a) & 1u)) continue;
-                for (int b = 6 + P; b < N; ++b) {
+            for (int a = 0; a < N; ++a) {
+                if (!((tiles[p] >> a) & 1u) || ((fixed >> a) & 1u)) continue;
+                for (int b = 0; b < N; ++b) {
                     if ((tiles[p] >> b) & 1u) continue;
                     std::vector<uint32_t> t2(tiles);
                     t2[p] = (t2[p] & ~(1u << a)) | (1u << b);
-                    const PlanEval ev = evaluate(N, P, TB, k, max_far, t2, terms);
+                    const PlanEval ev = evaluate(N, P, run, TB, k, max_far, t2, terms);
                     if (ev.cost >= 0 && ev.cost < cur.cost) {
                         tiles = t2;
                         cur = ev;
@@ -683,8 +735,7 @@ bool cube_plan(const CubeModel &m, const std::vector<CubeTerm> &terms, int tile_
     for (size_t p = 0; p < tiles.size(); ++p) {
         if (cur.assign[p].empty()) continue;
         CubePass cp;
-        for (int s = 0; s < 6 + P; ++s) cp.wb.push_back(s);
-        for (int s = 6 + P; s < N; ++s)
+        for (int s = 0; s < N; ++s)
             if ((tiles[p] >> s) & 1u) cp.wb.push_back(s);
         for (int s = 0; s < N; ++s)
             if ((cur.far[p] >> s) & 1u) cp.wb.push_back(s);

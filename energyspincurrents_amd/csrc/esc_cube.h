@@ -60,6 +60,9 @@ struct CubeModel {
     int waves;              // waves per workgroup
     bool paired;            // row pairs: site 0 is a register site under lanes on sites 1..6,
                             // so the lane moves rows 2y, 2y + 1 as one 16-byte access (complex64)
+    bool contiguous;        // lanes on the six sites above the pair bit (one run of 512 or 1024
+                            // bytes per wave access); else only a 64-byte run is fixed and the
+                            // planner picks the other lane sites
 };
 
 // coupling scalars of the kernels (double, device memory), in this order

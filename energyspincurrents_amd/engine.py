@@ -236,7 +236,9 @@ class Engine:
         reverse order; 7, 8: the generated sub-cube kernels over those plans)
         + 10000 * log2(waves per workgroup) + 100 * tile bits + window bits,
         plus, for the generated kernels, 1000000 * log2(rows per lane) and
-        10000000 when they run over row pairs (complex64, 16-byte accesses)."""
+        10000000 * layout: bit 0 over row pairs (complex64, 16-byte
+        accesses), bit 1 lanes on six contiguous sites (offered from 25
+        sites; otherwise only a 64-byte run of lane sites is fixed)."""
         n = ctypes.c_int32()
         check(lib().esc_kernel_configs(self._h, None, 0, ctypes.byref(n)))
         buf = (ctypes.c_int32 * max(n.value, 1))()
@@ -462,8 +464,10 @@ def default_kernel_config(n_sites, precision="c128"):
     default from 19 sites (default_tile_bits in esc_capi.cpp); also used for
     smaller lattices, where the window holds every site."""
     w = n_sites if n_sites <= 18 else (17 if n_sites <= 22 else 18)
-    if precision == "c64":   # 1xxxxxxx: over row pairs (16-byte accesses)
-        return (13731200 if n_sites <= 22 else (14831300 if n_sites <= 26 else 4831300)) + w
+    # layout digit (x0000000): bit 0 over row pairs (16-byte accesses), bit 1
+    # lanes on six contiguous sites
+    if precision == "c64":
+        return (13731200 if n_sites <= 22 else (14821200 if n_sites <= 26 else 13831300)) + w
     return 3731200 + w
 
 
@@ -472,10 +476,11 @@ def precompile_kernel(model: SpinModel, precision="c128", config=None):
     code-object cache without a device (esc_kernel_precompile) and return its
     HIP source.  config None: the configuration a context of this size uses
     by default (complex128: 2^3 rows per lane, 8 waves, 2^12-row tiles;
-    complex64: 2^3 rows per lane on 2^12-row tiles over row pairs up to 22
-    sites, 2^4 rows per lane on 2^13-row tiles with the windows in reverse
-    order beyond (over row pairs up to 26 sites), 8 waves; at most 17 window
-    sites up to 22 sites, 18 beyond)."""
+    complex64 over row pairs: 2^3 rows per lane, 8 waves, 2^12-row tiles up
+    to 22 sites; 2^4 rows per lane, 4 waves, 2^12-row tiles, windows in
+    reverse order up to 26 sites; 2^3 rows per lane, 8 waves, 2^13-row
+    tiles, windows in reverse order beyond; at most 17 window sites up to 22
+    sites, 18 beyond)."""
     N = model.lattice.N
     if config is None:
         config = default_kernel_config(N, precision)

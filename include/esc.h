@@ -112,10 +112,14 @@ int esc_ctx_info(const esc_ctx *ctx, int64_t *dim, int32_t *elem_bytes, int32_t 
  * width W allows; P = 6: the same windows run in reverse order; P = 7 / 8:
  * the plans of 5 / 6 run by kernels generated for the lattice at run time
  * (hiprtc) in which each lane holds 2^k rows over k tile sites, with
- * 1000000 * k added to the configuration.  d_out is overwritten. */
+ * 1000000 * k added to the configuration, and 10000000 * a layout digit:
+ * bit 0 row pairs (complex64, two adjacent rows per 16-byte lane access),
+ * bit 1 lanes on six contiguous sites (else only a 64-byte run is fixed).
+ * d_out is overwritten. */
 int esc_autotune(esc_ctx *ctx, const void *d_in, void *d_out, int reps, int *tile_bits, double *ms, void *stream);
-/* Select the H.psi kernel explicitly (a configuration as reported by
- * esc_autotune or listed by esc_kernel_configs). */
+/* Select the H.psi kernel explicitly: a configuration as reported by
+ * esc_autotune or listed by esc_kernel_configs, or any generated-kernel
+ * configuration (P = 7 / 8) that plans for the context's model. */
 int esc_set_tile_bits(esc_ctx *ctx, int tile_bits);
 /* List the kernel configurations of a context (*n = count; up to cap written). */
 int esc_kernel_configs(const esc_ctx *ctx, int32_t *configs, int32_t cap, int32_t *n);

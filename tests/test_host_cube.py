@@ -91,7 +91,7 @@ def test_closed_form_diagonal_counts_match_bond_by_bond():
     last = src[src.rindex("// pass "):]
     words = dict(re.findall(r"const unsigned ((?:zw|jw)\d+) = (0u(?: \| \(\(R0 \^ \(R0 >> \d+\)\) & 0x[0-9a-f]+u\))+);", last))
     calls = re.findall(r"esc_diag_c\(([^,]+), ([^,]+), pc0 \+ (\d+)u,", last)
-    slot_rows = [int(x, 16) for x in re.findall(r"out \+ \(base \| grw \| 0x([0-9a-f]+)u\) \+ lane\)\); ar\[\d+\] = v\.x", last)]
+    slot_rows = [int(x, 16) for x in re.findall(r"out \+ \(base \| grw \| 0x([0-9a-f]+)u \| lrow\)\)\); ar\[\d+\] = v\.x", last)]
     assert len(calls) == len(slot_rows) == 8 and words
     om = O.Model(3, 8, K=(-1.0, -1.0, -1.0), J=0.1, B=O.field111(0.2))
     px, py, pz = om.bonds            # families x, y, z (src/Hamiltonian.h:230-328)

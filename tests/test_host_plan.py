@@ -4,8 +4,10 @@ many lattices and configurations.
 
 Invariants: every off-diagonal term is applied by exactly one pass; its mask
 lies inside that pass's window; a pass's near terms are exactly its terms
-inside its tile; tiles hold the six lane sites and tile_bits sites and lie in
-their windows; windows hold w sites (generic) or at most w (generated); the
+inside its tile; tiles hold tile_bits sites, among them the six lane sites 0..5
+(generic) or the sites below and of the lanes' run (generated: 0..2 or 0..3
+with a 64-byte run, 0..5 or 0..6 with contiguous lanes), and lie in their
+windows; windows hold w sites (generic) or at most w (generated); the
 LDS of a workgroup fits the CU; the first pass, and only it, adds the
 diagonal.
 """
@@ -42,8 +44,9 @@ def configs(N, c64):
             for v in (2, 3):
                 for tb in (12, 13):
                     for w in (17, 18):
-                        for paired in ((0, 1) if c64 else (0,)):   # 1xxxxxxx: over row pairs
-                            out.append(10000000 * paired + 1000000 * k + 100000 * P + 10000 * v + 100 * tb + min(w, N))
+                        # layout digit: bit 0 over row pairs, bit 1 contiguous lanes
+                        for lay in ((0, 1, 2, 3) if c64 else (0, 2)):
+                            out.append(10000000 * lay + 1000000 * k + 100000 * P + 10000 * v + 100 * tb + min(w, N))
     return sorted(set(out))
 
 
@@ -68,7 +71,17 @@ def test_plan_invariants(name, model, prec):
         # generated sub-cube kernels in their last (with the output re-read)
         want = [0] * (len(passes) - 1) + [1] if cube else [1] + [0] * (len(passes) - 1)
         assert [p["diag"] for p in passes] == want, cfg
-        fixed = 127 if cfg // 10000000 == 1 else 63   # lane sites (and site 0 under them with row pairs)
+        # generic plans hold the six lane sites 0..5; generated plans the
+        # sites below and of the lanes' run: six contiguous lane sites (0..5,
+        # 0..6 with row pairs) or a 64-byte run and the next site (0..2 with
+        # 16-byte rows, 0..3 with 8-byte rows or row pairs)
+        lay = cfg // 10000000
+        if not cube:
+            fixed = 63
+        elif lay & 2:
+            fixed = 127 if lay & 1 else 63
+        else:
+            fixed = 7 if (prec == "c128" and not lay & 1) else 15
         for p in passes:
             W, T = p["window"], p["tile"]
             assert T & fixed == fixed and bin(T).count("1") == tb, (cfg, p)
