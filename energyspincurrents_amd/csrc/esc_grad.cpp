@@ -191,16 +191,20 @@ void grad_core(const esc_model &model, const std::vector<int> &base_ops, int osi
     void *t0 = V.get(), *t1 = V.get();
     void *cg_r = V.get(), *cg_p = V.get(), *cg_ap = V.get();
     // x = (H - e)^+ b on the complement of gs (b orthogonal to gs), by
-    // conjugate gradients; returns the relative residual reached
-    auto cg = [&](double e, const void *b, void *x, double tol) {
+    // conjugate gradients until the relative residual reaches tol, the
+    // curvature test fails, or (stall > 0) stall iterations bring no
+    // quartering of the best squared residual; returns the relative residual
+    // reached
+    auto cg = [&](double e, const void *b, void *x, double tol, int stall) {
         hip_check(hipMemsetAsync(x, 0, dim * c->esz, s), "clear");
         hip_check(hipMemcpyAsync(cg_r, b, dim * c->esz, hipMemcpyDeviceToDevice, s), "copy");
         hip_check(hipMemcpyAsync(cg_p, b, dim * c->esz, hipMemcpyDeviceToDevice, s), "copy");
         double rr = device_norm2(c, cg_r, s);
         const double bn = std::sqrt(rr);
         if (!(bn > 0.0)) return 0.0;
-        int iter = 0;
-        for (; iter < 5000 && std::sqrt(rr) > tol * bn; ++iter) {
+        int iter = 0, best_at = 0;
+        double best = rr;
+        for (; iter < 5000 && std::sqrt(rr) > tol * bn && (stall == 0 || iter - best_at < stall); ++iter) {
             run_hpsi(c, cg_p, cg_ap, s);
             axpby(-e, cg_p, 1.0, cg_ap);
             axpby(-dot(gs, cg_ap), gs, 1.0, cg_ap);
@@ -212,6 +216,10 @@ void grad_core(const esc_model &model, const std::vector<int> &base_ops, int osi
             const double rr2 = device_norm2(c, cg_r, s);
             axpby(1.0, cg_r, rr2 / rr, cg_p);
             rr = rr2;
+            if (rr < 0.25 * best) {
+                best = rr;
+                best_at = iter + 1;
+            }
         }
         progress("conjugate gradients: %d iterations, relative residual %.2e", iter, std::sqrt(rr) / bn);
         return std::sqrt(rr) / bn;
@@ -225,7 +233,7 @@ void grad_core(const esc_model &model, const std::vector<int> &base_ops, int osi
         resid = std::sqrt(device_norm2(c, t0, s));
         progress("gradient: Newton pass %d, E0 %.15f, residual %.3e", pass, e0, resid);
         if (resid < 1e-13 * std::max(1.0, std::fabs(e0))) break;
-        cg(e0, t0, t1, 1e-12);
+        cg(e0, t0, t1, 1e-12, 0);
         axpby(-1.0, t1, 1.0, gs);
         hip_check(esc::launch_scale(false, gs, 1.0 / std::sqrt(device_norm2(c, gs, s)), dim, s), "scale");
     }
@@ -443,6 +451,7 @@ void grad_core(const esc_model &model, const std::vector<int> &base_ops, int osi
         apply(tab(0, nb + 2 * r + 1), phibar, t0);
         axpby(1.0, t0, 1.0, gsbar);
     }
+    progress("gradient: recursion terms %.10g %.10g %.10g %.10g", g[0], g[1], g[2], g[3]);
     // the expectation values: d Re<gs|J_a|gs> = 2 Re<J_a gs|d gs> + <gs|dJ_a/dK|gs> dK
     for (int a = 0; a < nb; ++a) {
         if (up.exbar[a] == 0.0) continue;
@@ -457,17 +466,24 @@ void grad_core(const esc_model &model, const std::vector<int> &base_ops, int osi
     }
     // E0 through z: e0_bar dE0/dtheta_p (Hellmann-Feynman)
     for (int p = 0; p < 4; ++p) g[p] += e0bar * hf[p];
+    progress("gradient: E0 terms %.10g %.10g %.10g %.10g", e0bar * hf[0], e0bar * hf[1], e0bar * hf[2], e0bar * hf[3]);
     // the ground state: mu = (H - E0)^+ P gs_bar by CG on the complement of
-    // gs; theta_bar_p -= Re<mu|H_p gs>
+    // gs; theta_bar_p -= Re<mu|H_p gs>.  gs_bar is large (|gs_bar| / |grad|
+    // reaches 1e9 at 120 steps, in directions far above E0) and its part of
+    // the gradient small, so the solve runs to the rounding level: a residual
+    // of 1e-11 |gs_bar| moved 120-step gradients by 1e-3 relative.
     const cplx pg = dot(gs, gsbar);
     axpby(-pg, gs, 1.0, gsbar);
     void *mu = V.get();
-    if (cg(e0, gsbar, mu, 1e-11) > 1e-8)
+    if (cg(e0, gsbar, mu, 1e-15, 50) > 1e-8)
         throw EscError(ESC_ERR_NUMERIC, "CG with H - E0 did not converge (ground state not separated by a gap?)");
+    double gpart[4];
     for (int p = 0; p < 4; ++p) {
         run_hpsi(P[p].c, gs, t0, s);
-        g[p] -= dot(mu, t0).real();
+        gpart[p] = -dot(mu, t0).real();
+        g[p] += gpart[p];
     }
+    progress("gradient: ground-state terms %.10g %.10g %.10g %.10g", gpart[0], gpart[1], gpart[2], gpart[3]);
     hip_check(hipStreamSynchronize(s), "sync");
     progress("gradient: done");
     for (int p = 0; p < 4; ++p) grad[p] = g[p];
