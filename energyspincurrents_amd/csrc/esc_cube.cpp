@@ -798,6 +798,10 @@ std::string cube_source(const CubeModel &m, const std::vector<CubePass> &passes)
 // Compilation and caches
 // ---------------------------------------------------------------------------
 struct CubeProgram {
+    // the code object stays alive as long as its module: the runtime and
+    // tools attached to it (a profiler's code-object tracing) may read the
+    // image passed to hipModuleLoadData after the call returns
+    std::vector<char> image;
     hipModule_t module = nullptr;
     std::vector<hipFunction_t> fn, fnb;   // per pass: one system, a batch of systems
     int tile_bits = 0, threads = 0;
@@ -939,10 +943,9 @@ std::shared_ptr<CubeProgram> cube_program(const CubeModel &m, const std::vector<
         if (it != g_prog.end())
             if (auto sp = it->second.lock()) return sp;
     }
-    std::vector<char> code;
-    if (!code_for(src, key, code, err)) return nullptr;
     auto prog = std::make_shared<CubeProgram>();
-    if (hipModuleLoadData(&prog->module, code.data()) != hipSuccess) {
+    if (!code_for(src, key, prog->image, err)) return nullptr;
+    if (hipModuleLoadData(&prog->module, prog->image.data()) != hipSuccess) {
         prog->module = nullptr;
         if (err) *err = "hipModuleLoadData failed for the generated H.psi kernels";
         return nullptr;
