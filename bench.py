@@ -49,6 +49,8 @@ def parse():
                     help="skip the reference-algorithm (COO lists) CPU H.psi, which holds 29 GB of host memory")
     ap.add_argument("--no-spectrum", action="store_true", help="skip the JeJe spectrum timing")
     ap.add_argument("--no-n32", action="store_true", help="skip the 32-site complex64 H.psi timing")
+    ap.add_argument("--n32-config", type=int, default=None,
+                    help="32-site H.psi on this kernel configuration instead of the autotuned one")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batched scan")
     ap.add_argument("--kernel-config", default=None,
                     help="C128,C64: H.psi kernel configurations (default: the ones profiled under profiles/)")
@@ -312,7 +314,7 @@ def main():
         spec["c128"]["gs"] = None
         out["spectrum"] = spectrum_summary(spec)
     if not args.no_n32:
-        out["n32"] = n32_run(esc)
+        out["n32"] = n32_run(esc, args.n32_config)
         progress("32-site H.psi")
     if not args.no_batch:
         out["batch"] = batch_run(esc)
@@ -335,16 +337,20 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def n32_run(esc):
+def n32_run(esc, config=None):
     """H.psi at N=32 (4x4 cells, K=-1, h[111]=0.3; 2^32 rows, complex64, 34.4 GB
-    per vector): autotuned configuration, 5 timed applications."""
+    per vector): the autotuned configuration (or `config`), 5 timed
+    applications."""
     import torch
     lat = esc.Honeycomb(4, 8)
     model = esc.SpinModel(lat, K=(-1.0, -1.0, -1.0), B=esc.field111(0.3))
     eng = esc.Engine(model, "c64")
     x = eng.start_vector(1234, "Gaussian")
     y = eng.empty()
-    config, _ = eng.autotune(x, y, reps=1)
+    if config is None:
+        config, _ = eng.autotune(x, y, reps=1)
+    else:
+        eng.set_tile_bits(config)
     ms, _ = time_hpsi(eng, x, y, 5, 1)
     bytes_alg = 2 * eng.dim * eng.elem_bytes
     del x, y, eng
