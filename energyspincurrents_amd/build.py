@@ -96,7 +96,8 @@ def warm_jit_cache():
     """Compile the generated H.psi kernels of the benchmark lattices into
     jit_cache/ beside libesc.so, so that contexts of these models load them
     instead of compiling (the couplings are not part of the cache key): the
-    default configurations of the single-system workloads, and every
+    default configurations of the single-system workloads, the lane layouts
+    the 24-site GPU tests select, and every
     candidate of the lattices that bench.py autotunes (the batched scan's
     16-, 18-, 20- and 24-site clusters, the 32-site cluster)."""
     sys.path.insert(0, ROOT)
@@ -109,6 +110,9 @@ def warm_jit_cache():
     n32 = dict(K=(-1.0, -1.0, -1.0), B=esc.field111(0.3))
     jobs = [((3, 8), kjgh, "c128", None), ((3, 8), kjgh, "c64", None), ((2, 10), kjgh, "c128", None),
             ((2, 10), kjgh, "c64", None)]
+    # the lane layouts the 24-site GPU tests set explicitly
+    jobs += [((3, 8), kjgh, "c128", cfg) for cfg in (23731218, 23831318)]
+    jobs += [((3, 8), kjgh, "c64", cfg) for cfg in (24831318, 34831318, 4831318)]
     for (lx, ly), kw in (((2, 8), scan), ((3, 6), scan), ((2, 10), scan), ((3, 8), scan), ((4, 8), n32)):
         jobs += [((lx, ly), kw, "c64", cfg) for cfg in cube_candidates(lx * ly)]
 

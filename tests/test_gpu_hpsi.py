@@ -203,3 +203,25 @@ def test_tiled_kernels_random_vectors(prec, tol, Lx, Ly):
     tb, ms = eng.autotune(x, reps=3)
     assert tb in configs and ms > 0
     assert rel(eng.download(eng.hpsi(x)), ref) < tol
+
+
+@pytest.mark.parametrize("prec,tol,configs", [
+    ("c128", 1e-14, [3731218, 23731218, 23831318]),
+    ("c64", 1e-6, [14821218, 24831318, 34831318, 4831318])])
+def test_generated_lane_layouts_n24(prec, tol, configs):
+    """The generated kernels' lane layouts at 24 sites: free lane sites (a
+    64-byte run fixed: the defaults, and 4831318 with 8-byte rows) and lanes
+    on six contiguous sites (layout bit 1, 2xxxxxxx; with row pairs
+    3xxxxxxx), each against the oracle's gather on one random vector."""
+    m = O.Model(3, 8, K=(-1.0, -1.0, -1.0), J=0.1, G=0.2, B=O.field111(0.2))
+    rng = np.random.default_rng(24)
+    v = rng.standard_normal(m.dim) + 1j * rng.standard_normal(m.dim)
+    if prec == "c64":
+        v = v.astype(np.complex64).astype(np.complex128)
+    ref = O.hpsi_gather(m, v)
+    eng = engine_for(m, prec)
+    x = eng.upload(v)
+    for cfg in configs:
+        print(f"[lane layout {cfg}] {prec} N=24", flush=True)
+        eng.set_tile_bits(cfg)
+        assert rel(eng.download(eng.hpsi(x)), ref) < tol, f"config {cfg}"
