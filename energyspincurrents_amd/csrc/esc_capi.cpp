@@ -1184,7 +1184,7 @@ bool tile_config_fits(const esc_ctx *c, int config) {
         }
         const int TB = cfg_tile_bits(config), k = cfg_regs(config), waves = cfg_waves(config);
         const int P = cfg_paired(config) ? 1 : 0;
-        ok = ok && k >= 1 && TB - 6 - k >= 0 && ((1 << (TB - 6 - k)) % waves) == 0 &&
+        ok = ok && k >= 1 && TB - 6 - k >= 0 && waves <= 16 && ((1 << (TB - 6 - k)) % waves) == 0 &&
              ((1 << (TB - 6 - P)) % waves) == 0 && ((size_t)1 << TB) * c->esz <= 131072;
         c->tile_fits[config] = ok;
         return ok;

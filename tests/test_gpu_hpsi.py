@@ -124,6 +124,10 @@ def test_error_paths():
         eng.hpsi(x, out=x)                 # in-place is rejected
     with pytest.raises(esc.EscError):
         eng.apply_op("Jex", 99, x)         # site out of range
+    big = engine_for(O.Model(2, 8))
+    for cfg in (1751316, 3751316, 43731216, 99999999, -3731216, 123):
+        with pytest.raises(esc.EscError):   # 32-wave workgroups, a layout digit above 3, no such shape
+            big.set_tile_bits(cfg)
 
 
 def test_vector_arguments_checked_before_launch():
