@@ -29,7 +29,9 @@ DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", f"-I{INC}", f"-I{CSRC}",
           "-Wno-unused-command-line-argument", "-pthread"]
 
-LIB_SOURCES = ["esc_kernels.hip", "esc_capi.cpp", "esc_batch.cpp", "esc_grad.cpp", "esc_host.cpp", "esc_cube.cpp"]
+LIB_SOURCES = ["esc_kernels.hip", "esc_capi.cpp", "esc_plan.cpp", "esc_spectra.cpp", "esc_batch.cpp", "esc_grad.cpp",
+               "esc_host.cpp",
+               "esc_cube.cpp"]
 CLI_SOURCES = ["facade/lanczoscond_main.cpp"]
 
 

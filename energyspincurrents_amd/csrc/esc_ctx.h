@@ -1,6 +1,8 @@
 // esc_ctx.h -- the device context behind the C ABI (esc_ctx) and the host
-// helpers its source files share: esc_capi.cpp (contexts, kernels, Lanczos,
-// spectra, small solves), esc_batch.cpp (lockstep batches) and esc_grad.cpp
+// helpers its source files share: esc_capi.cpp (contexts, kernel choice,
+// Lanczos, small solves, the entry points), esc_plan.cpp (term tables, pass
+// plans, kernel configurations, operator tables), esc_spectra.cpp (spectra
+// from a ground state), esc_batch.cpp (lockstep batches) and esc_grad.cpp
 // (spectrum gradients).  Internal to the library.
 #pragma once
 #include "esc.h"
@@ -174,9 +176,65 @@ void sync_get(hipStream_t s, void *dst, const void *src, size_t n);
 void device_dot(esc_ctx *c, const void *x, const void *y, double out[2], hipStream_t s);
 double device_norm2(esc_ctx *c, const void *x, hipStream_t s);
 
+// H.psi kernel configurations
+constexpr int kCoverFirst = 5;   // pass digit of the cover-first planner
+constexpr int kCoverLast = 6;    // cover-first, windows in reverse order
+
+// A kernel configuration is 100000 * P + 10000 * V + 100 * TB + W: P planned
+// passes (kCoverFirst: the cover-first planner), 2^V waves per workgroup,
+// 2^TB-row tiles, 2^W-row windows; 0 = one row per lane.
+// Configurations 7 and 8 in the pass digit run the cover-first plan (5) or
+// its reverse (6) with the generated sub-cube kernels of esc_cube.cpp; the
+// millions digit is then k (2^k rows per lane).
+constexpr int kCubeFirst = 7;
+constexpr int kCubeLast = 8;
+inline int cfg_passes(int config) { return (config / 100000) % 10; }
+inline int cfg_regs(int config) { return (config / 1000000) % 10; }
+// the layout digit of the generated kernels (0..3): bit 0 = over row pairs
+// (complex64: 16-byte lane accesses), bit 1 = lanes on six contiguous sites
+inline bool cfg_paired(int config) { return ((config / 10000000) % 10) & 1; }
+inline bool cfg_contiguous(int config) { return ((config / 10000000) % 10) & 2; }
+inline bool cfg_cube(int config) { return cfg_passes(config) == kCubeFirst || cfg_passes(config) == kCubeLast; }
+inline int cfg_waves(int config) { return 1 << ((config / 10000) % 10); }
+inline int cfg_tile_bits(int config) { return (config / 100) % 100; }
+inline int cfg_window(int config) { return config % 100; }
+inline int make_config(int P, int waves, int tb, int w, int k = 0, bool paired = false, bool contiguous = false) {
+    int v = 0;
+    while ((1 << v) < waves) ++v;
+    return 10000000 * ((paired ? 1 : 0) | (contiguous ? 2 : 0)) + 1000000 * k + 100000 * P + 10000 * v + 100 * tb + w;
+}
+
+// the model on the device (esc_plan.cpp): term tables into c->htab and the
+// device copies
+void build_tables(esc_ctx *c);
+// an off-diagonal term of H in the 8-entry-table form: partner r ^ mask,
+// coefficient coef[index of bits b0, b1, b2 of r] (re, im)
+struct HostTerm {
+    uint32_t mask;
+    int b0, b1, b2;
+    double coef[8][2];
+    bool cplx;
+};
+std::vector<HostTerm> host_terms(const esc_ctx *c);
+// One planned pass: window sites W, tile sites T (lane sites 0..5 included)
+// and the terms (indices into host_terms) it applies from the tile (near) and
+// from the rest of the window (far).
+struct Plan {
+    uint32_t W, T;
+    std::vector<size_t> near, far;
+};
+std::vector<Plan> plan_windows(const esc_ctx *c, const std::vector<HostTerm> &terms, int TB, int w, int P);
+// the tiled kernel's passes of a configuration (custom: other terms than H's)
+std::vector<esc::GTables> build_passes(const esc_ctx *c, int TB, int w, int P, int WPB,
+                                       const std::vector<HostTerm> *custom = nullptr);
+// the generated kernels' model and passes of a configuration (do_plan false:
+// the model only); false with *why when the configuration does not apply
+bool cube_spec(const esc_ctx *c, int cfg, esc::CubeModel &m, std::vector<esc::CubePass> &passes, std::string *why,
+               bool do_plan = true);
+// coupling scalars of the generated kernels (esc::CubeScalar order)
+void cube_scalars(const esc::HTables &h, double *cf);
+
 // H.psi with a given kernel configuration / the context's choice
-int cfg_tile_bits(int config);
-bool cfg_cube(int config);
 // the generated program of a configuration (compiled or fetched from the
 // caches), its scalars uploaded to c->d_cube_cf
 esc::CubeProgram &cube_for(esc_ctx *c, int config);
@@ -225,6 +283,19 @@ void host_parallel_for(int n, int grain, const F &fn) {
 }
 
 int tridiag_ql(int n, const double *a, const double *b, double *evals, double *Z);
+// all eigenpairs of a tridiagonal matrix, on the host or the device
+void tridiag_eigenpairs(int n, const double *a, const double *b, double *ev, double *Z, bool on_device,
+                        hipStream_t s);
+// one continued-fraction recursion from d_phi (work: three device vectors);
+// alpha[0..iters-1], beta[0..iters], |phi|^2
+void cf_run(esc_ctx *ctx, const void *d_phi, int steps, void *work[3], double *alpha, double *beta, int *iters,
+            double *norm2, hipStream_t s, bool reortho = false);
+// the three-vector Lanczos recursion from start with esc_lanczos's exit
+// rule (returns the number of alphas); with coef, the same recursion
+// replayed for n_fixed vectors, accumulating acc += coef[k] v_k
+int plain_recursion(esc_ctx *c, const void *start, int steps, double eps, const std::complex<double> *coef,
+                    int n_fixed, void *acc, std::vector<double> &ar, std::vector<double> &bt, hipStream_t s,
+                    double resid_floor = 0.0);
 double tridiag_lowest(int n, const double *a, const double *b);
 
 // spectra from continued fractions: one run per channel / polarisation pair
@@ -241,5 +312,12 @@ std::map<std::pair<int, int>, std::vector<cplx>> cf_assemble(bool js, int nc, co
 void write_spectrum_table(bool js, int nc, const std::vector<cplx> &ex,
                           std::map<std::pair<int, int>, std::vector<cplx>> &F, const double *omega, int nw,
                           double eta, double *table);
+// LocalSzSz tables (ESC_DYN_LOCALSZSZ), the Ritz amplitudes <psi_k|vec> of
+// the kept basis, and the RIXS state sums (esc_spectra.cpp)
+void local_szsz_spectra(esc_ctx *ctx, int method, bool reortho, const void *d_gs, double e0, int cf_steps,
+                        double eta, const double *omega, int nw, double *table, double *expect, hipStream_t s);
+std::vector<cplx> ritz_amplitudes(esc_ctx *ctx, const void *d_vec, hipStream_t s);
+void rixs_spectra(esc_ctx *g, esc_ctx *q, int method, int site, const void *d_gs, double e0, int cf_steps,
+                  double gamma, double eta, const double *omega, int nw, double *table, hipStream_t s);
 
 }  // namespace esc_impl
