@@ -115,8 +115,9 @@ void split_mask(const Layout &L, uint32_t mask, Outer &o, uint32_t &rho) {
 // 6 - run more; the register sites (site 0 and k - 1 others with row pairs);
 // the rest are group sites.  `sel` picks the registers among `cand` (the tile
 // sites that are neither site 0 with row pairs nor a run site); the lanes take
-// the four lowest of the others (a lane or a group bit costs the same LDS
-// load, so that choice does not change the plan's cost).
+// the 6 - run lowest of the others (a lane or a group bit costs the same LDS
+// load, so that choice does not change the plan's cost, and low lane sites
+// keep a wave's accesses close together in memory).
 void set_layout(Layout &L, const std::vector<int> &cand, uint32_t sel) {
     std::vector<int> regs, rest;
     for (size_t q = 0; q < cand.size(); ++q) ((sel >> q) & 1u ? regs : rest).push_back(cand[q]);
