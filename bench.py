@@ -9,7 +9,7 @@ the JeJe spectrum pipeline in both precisions, the 32-site complex64 H.psi
 oracle's OpenMP gather H.psi on the host cores, a bounded sample).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-spectrum] [--no-n32] [--no-batch]
-                  [--pipelines]
+                  [--pipelines] [--n32-config CONFIG]
 
 --pipelines adds the whole 12-site and 32-site runs (ground state, JsJs and
 JeJe spectra), timed stage by stage; it is not part of the default run.
