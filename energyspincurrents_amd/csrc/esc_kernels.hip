@@ -673,7 +673,8 @@ __global__ void k_rec_init(double *__restrict__ rec, uint32_t *__restrict__ step
 
 // NT: streamed (non-temporal) loads and stores, used for complex128, where
 // they measured faster (about 15 % at 2^24 rows); complex64 keeps the cached
-// accesses, which measured faster there
+// accesses, which measured faster there and at 2^32 rows (a recursion step
+// 0.259 against 0.268 ms and 93.8 against 95.6 ms)
 template <typename R, bool NT>
 __global__ __launch_bounds__(kThreads) void k_lanczos_update_rec(typename Vec2<R>::T *__restrict__ w,
                                                                  const typename Vec2<R>::T *__restrict__ v,
