@@ -101,10 +101,10 @@ PROFILED_CONFIGS = {"c128": [3731218], "c64": [14821218]}
 # make a plain one-vector read / write report its own size), from the per-pass
 # rows of the named profile summary.
 MEASURED_TRAFFIC = {
-    # (184446 + 385158) KiB fetched x 2.0 + (262166 + 262183) KiB written
-    ("c128", 3731218): (1703482368, "profiles/hpsi_cube_n24_hbm_traffic.md: esc_cube c128 (config 3731218)"),
-    # (88583 + 160886) KiB fetched x 2.0 + (131105 + 131079) KiB written
-    ("c64", 14821218): (779388928, "profiles/hpsi_cube_n24_hbm_traffic.md: esc_cube c64 (config 14821218)"),
+    # (167610 + 301711) KiB fetched x 2.0 + (262151 + 262171) KiB written
+    ("c128", 3731218): (1498075136, "profiles/hpsi_cube_n24_hbm_traffic.md: esc_cube c128 (config 3731218)"),
+    # (90593 + 160587) KiB fetched x 2.0 + (131082 + 131074) KiB written
+    ("c64", 14821218): (782864384, "profiles/hpsi_cube_n24_hbm_traffic.md: esc_cube c64 (config 14821218)"),
 }
 
 

@@ -373,7 +373,7 @@ std::vector<int> tile_candidates(const esc_ctx *c) {
 // with 2^12-row tiles (16 waves for c128, which moves the least HBM traffic of
 // the equally fast shapes at N = 24, 8 waves for c64).  A shape the context
 // cannot run falls back to a single pass over 2^11- or 2^10-row tiles.
-// From 19 sites the generated sub-cube kernels come first (N = 24: 0.29 ms
+// From 19 sites the generated sub-cube kernels come first (N = 24: 0.27 ms
 // against 0.52 ms for the best generic shape in complex128, 0.15 against
 // 0.32 ms in complex64; N = 20: 18.8 against 40.1 us and 13.9 against
 // 28.8 us): complex128 2^3 rows per lane in 8-wave workgroups on 2^12-row
@@ -383,8 +383,8 @@ std::vector<int> tile_candidates(const esc_ctx *c) {
 // 2^12-row tiles with the windows in reverse order up to 26 sites (N = 24:
 // 154 against 162 us for 8 waves on 2^13-row tiles), and 2^3 rows per lane
 // in 8-wave workgroups on 2^13-row tiles, windows in reverse order, beyond
-// (N = 32: 66.6 ms; 72 ms without row pairs with contiguous lanes, 84 ms
-// with 2^4 rows per lane).  A model they do not cover, or a missing run-time
+// (N = 32: 61.3 ms; with one search per plan 66.6 ms, 72 ms without row
+// pairs with contiguous lanes, 84 ms with 2^4 rows per lane).  A model they do not cover, or a missing run-time
 // compiler, falls through to the generic shapes.
 int default_tile_bits(esc_ctx *c) {
     const int N = c->n_sites;

@@ -13,6 +13,8 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <thread>
+#include <random>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -28,6 +30,12 @@ namespace {
 
 constexpr uint32_t kCubeMaxGrid = 1u << 22;
 constexpr int kFarAhead = 1;   // far sub-cubes in flight ahead of their terms
+// local searches per plan: several for complex128 and for complex64 beyond
+// 26 sites (N = 24 c128: 291 -> 279 us; N = 32 c64: 66.7 -> 63.5 ms), one for
+// complex64 up to 26 sites, whose single-search plans measured 1.5-3 %
+// faster (N = 24: 155 against 157 us; N = 20: 13.6 against 14.1 us)
+constexpr int kPlanStarts = 10;
+int plan_starts(const CubeModel &m) { return (m.c64 && m.n_sites <= 26) ? 1 : kPlanStarts; }
 
 // ---------------------------------------------------------------------------
 // Layout of one pass
@@ -667,11 +675,16 @@ void emit_pass(std::ostringstream &o, const CubeModel &m, const CubePass &p, int
         o << "      dot_part[2 * (size_t)b] = x; dot_part[2 * (size_t)b + 1] = y;\n    }\n  }\n";
     }
     o << "}\n\n";
-    o << "extern \"C\" __global__ void __launch_bounds__(" << WPB * 64 << ") esc_cube_p" << index
+    // the first of several passes (no previous output, no diagonal) is
+    // compiled for four waves per SIMD, two 8-wave workgroups per CU (N = 24
+    // c128: 137 -> 128 VGPRs, the pass 118 -> 106 us); the last pass keeps
+    // its registers (capped it measured slower)
+    const std::string cap = (p.first && !p.last) ? " __attribute__((amdgpu_waves_per_eu(4)))" : "";
+    o << "extern \"C\" __global__ void __launch_bounds__(" << WPB * 64 << ")" << cap << " esc_cube_p" << index
       << "(const double *__restrict__ cf, const V2 *__restrict__ in, V2 *__restrict__ out, "
          "double *__restrict__ dot_part, unsigned block_base) {\n";
     o << "  esc_cube_body" << index << "(cf, in, out, dot_part, blockIdx.x + block_base);\n}\n\n";
-    o << "extern \"C\" __global__ void __launch_bounds__(" << WPB * 64 << ") esc_cube_b" << index
+    o << "extern \"C\" __global__ void __launch_bounds__(" << WPB * 64 << ")" << cap << " esc_cube_b" << index
       << "(const double *__restrict__ cfs, const V2 *const *__restrict__ ins, V2 *const *__restrict__ outs, "
          "double *const *__restrict__ dots, unsigned block_base) {\n";
     o << "  const unsigned inst = blockIdx.y;\n";
@@ -684,6 +697,7 @@ void emit_pass(std::ostringstream &o, const CubeModel &m, const CubePass &p, int
 bool cube_plan(const CubeModel &m, const std::vector<CubeTerm> &terms, int tile_bits, std::vector<uint32_t> tiles,
                int max_far, std::vector<CubePass> &passes) {
     const int N = m.n_sites, TB = tile_bits, k = m.reg_sites, P = m.paired ? 1 : 0, run = lane_run(m);
+    const int nstarts = plan_starts(m);
     // every tile holds the sites below and of the lanes' run; a 64-byte run
     // also the next site, so its 128-byte line lies in the tile (sites 0..2
     // with 16-byte rows, else 0..3); a seed tile that gains one of them drops
@@ -697,7 +711,8 @@ bool cube_plan(const CubeModel &m, const std::vector<CubeTerm> &terms, int tile_
     }
     // plans are cached per lattice, term set, shape and seed tiles
     std::string key = std::to_string(N) + ":" + std::to_string(TB) + ":" + std::to_string(k) + ":" +
-                      std::to_string(max_far) + ":" + std::to_string(P) + ":" + std::to_string(run);
+                      std::to_string(max_far) + ":" + std::to_string(P) + ":" + std::to_string(run) + ":" +
+                      std::to_string(nstarts);
     for (uint32_t t : tiles) key += ":" + std::to_string(t);
     for (const CubeTerm &t : terms)
         key += "," + std::to_string(t.mask) + "/" + std::to_string(t.kind) + "/" + std::to_string(t.f) + "/" +
@@ -709,29 +724,63 @@ bool cube_plan(const CubeModel &m, const std::vector<CubeTerm> &terms, int tile_
         auto it = cache.find(key);
         if (it != cache.end()) { passes = it->second; return !passes.empty(); }
     }
-    PlanEval cur = evaluate(N, P, run, TB, k, max_far, tiles, terms);
-    if (cur.cost < 0) return false;
-    // hill climbing: swap one tile site of one pass for a site outside it
-    for (bool gain = true; gain;) {
-        gain = false;
-        for (size_t p = 0; p < tiles.size(); ++p) {
-            for (int a = 0; a < N; ++a) {
-                if (!((tiles[p] >> a) & 1u) || ((fixed >> a) & 1u)) continue;
-                for (int b = 0; b < N; ++b) {
-                    if ((tiles[p] >> b) & 1u) continue;
-                    std::vector<uint32_t> t2(tiles);
-                    t2[p] = (t2[p] & ~(1u << a)) | (1u << b);
-                    const PlanEval ev = evaluate(N, P, run, TB, k, max_far, t2, terms);
-                    if (ev.cost >= 0 && ev.cost < cur.cost) {
-                        tiles = t2;
-                        cur = ev;
-                        gain = true;
-                        break;
+    // local search from several site orders (start 0: ascending sites, the
+    // others fixed shuffles), each on its own thread; the plan of lowest cost
+    // is kept (ties: the earliest start).  A single greedy search stops in
+    // local minima that depend on the order it tries swaps in.
+    struct Start {
+        std::vector<uint32_t> tiles;
+        PlanEval ev;
+    };
+    std::vector<Start> starts(nstarts);
+    auto search = [&](int si) {
+        std::vector<int> ord(N);
+        for (int s = 0; s < N; ++s) ord[s] = s;
+        if (si > 0) {
+            std::mt19937 g((unsigned)si);
+            std::shuffle(ord.begin(), ord.end(), g);
+        }
+        std::vector<uint32_t> t = tiles;
+        PlanEval cur = evaluate(N, P, run, TB, k, max_far, t, terms);
+        if (cur.cost < 0) { starts[si].ev = cur; return; }
+        // hill climbing: swap one tile site of one pass for a site outside it
+        for (bool gain = true; gain;) {
+            gain = false;
+            for (size_t p = 0; p < t.size(); ++p) {
+                for (int ai = 0; ai < N; ++ai) {
+                    const int a = ord[ai];
+                    if (!((t[p] >> a) & 1u) || ((fixed >> a) & 1u)) continue;
+                    for (int bi = 0; bi < N; ++bi) {
+                        const int b = si > 0 ? ord[(bi + ai) % N] : bi;
+                        if ((t[p] >> b) & 1u) continue;
+                        std::vector<uint32_t> t2(t);
+                        t2[p] = (t2[p] & ~(1u << a)) | (1u << b);
+                        const PlanEval ev = evaluate(N, P, run, TB, k, max_far, t2, terms);
+                        if (ev.cost >= 0 && ev.cost < cur.cost) {
+                            t = t2;
+                            cur = ev;
+                            gain = true;
+                            break;
+                        }
                     }
                 }
             }
         }
+        starts[si].tiles = t;
+        starts[si].ev = cur;
+    };
+    {
+        std::vector<std::thread> th;
+        for (int si = 1; si < nstarts; ++si) th.emplace_back(search, si);
+        search(0);
+        for (auto &x : th) x.join();
     }
+    int best = -1;
+    for (int si = 0; si < nstarts; ++si)
+        if (starts[si].ev.cost >= 0 && (best < 0 || starts[si].ev.cost < starts[best].ev.cost)) best = si;
+    if (best < 0) return false;
+    tiles = starts[best].tiles;
+    const PlanEval cur = starts[best].ev;
     passes.clear();
     for (size_t p = 0; p < tiles.size(); ++p) {
         if (cur.assign[p].empty()) continue;
