@@ -76,7 +76,8 @@ def cube_candidates(n_sites, c64=True):
     list of cube_candidates in esc_capi.cpp): 2^12- and 2^13-row tiles, 2^3
     or 2^4 rows per lane, 4 or 8 waves, both window orders; complex64 also
     over row pairs (layout bit 0: 10000000 + ...); from 25 sites also with
-    lanes on six contiguous sites (layout bit 1: 20000000 + ...)."""
+    lanes on six contiguous sites, without row pairs (layout bit 1:
+    20000000 + ...)."""
     w = n_sites if n_sites <= 18 else (17 if n_sites <= 22 else 18)
     out = []
     for tb in (12, 13):
@@ -86,7 +87,7 @@ def cube_candidates(n_sites, c64=True):
             for v in (2, 3):
                 for P in (7, 8):
                     for paired in ((0, 1) if c64 else (0,)):
-                        for contiguous in ((0, 2) if n_sites >= 25 else (0,)):
+                        for contiguous in ((0, 2) if n_sites >= 25 and not paired else (0,)):
                             if w == n_sites and P == 8:
                                 continue
                             out.append(10000000 * (paired | contiguous) + 1000000 * k + 100000 * P + 10000 * v

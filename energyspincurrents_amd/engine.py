@@ -238,7 +238,8 @@ class Engine:
         plus, for the generated kernels, 1000000 * log2(rows per lane) and
         10000000 * layout: bit 0 over row pairs (complex64, 16-byte
         accesses), bit 1 lanes on six contiguous sites (offered from 25
-        sites; otherwise only a 64-byte run of lane sites is fixed)."""
+        sites without row pairs; otherwise only a 64-byte run of lane sites
+        is fixed)."""
         n = ctypes.c_int32()
         check(lib().esc_kernel_configs(self._h, None, 0, ctypes.byref(n)))
         buf = (ctypes.c_int32 * max(n.value, 1))()

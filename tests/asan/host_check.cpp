@@ -103,6 +103,9 @@ int main(int argc, char **argv) {
                     for (int w : {17, 18}) {
                         const int ww = w > N ? N : w;
                         if ((P == 1) != (ww == N)) continue;
+                        // generated-kernel plans (ten local searches each for complex128):
+                        // one shape per lattice and order
+                        if (P >= 7 && (tb != 12 || w != 18)) continue;
                         const int k = P >= 7 ? 3 : 0;
                         const int cfg = 1000000 * k + 100000 * P + 30000 + 100 * tb + ww;
                         std::vector<int64_t> rows(6 * 16);

@@ -39,9 +39,11 @@ def configs(N, c64):
                     if (ww == N) != (P == 1):
                         continue
                     out.append(100000 * P + 10000 * v + 100 * tb + ww)
+    # generated-kernel plans: the full grid up to 26 sites, k = 3 with 8 waves
+    # beyond (each plan there runs ten local searches)
     for P in (7, 8):
-        for k in (3, 4):
-            for v in (2, 3):
+        for k in ((3, 4) if N <= 26 else (3,)):
+            for v in ((2, 3) if N <= 26 else (3,)):
                 for tb in (12, 13):
                     for w in (17, 18):
                         # layout digit: bit 0 over row pairs, bit 1 contiguous lanes
