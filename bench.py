@@ -50,7 +50,7 @@ def parse():
     ap.add_argument("--no-spectrum", action="store_true", help="skip the JeJe spectrum timing")
     ap.add_argument("--no-n32", action="store_true", help="skip the 32-site complex64 H.psi timing")
     ap.add_argument("--n32-config", type=int, default=None,
-                    help="32-site H.psi on this kernel configuration instead of the autotuned one")
+                    help="32-site H.psi on this kernel configuration instead of the default one")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batched scan")
     ap.add_argument("--kernel-config", default=None,
                     help="C128,C64: H.psi kernel configurations (default: the ones profiled under profiles/)")
@@ -314,7 +314,7 @@ def main():
         spec["c128"]["gs"] = None
         out["spectrum"] = spectrum_summary(spec)
     if not args.no_n32:
-        out["n32"] = n32_run(esc, args.n32_config)
+        out["n32"] = n32_run(esc, args.n32_config, args.autotune)
         progress("32-site H.psi")
     if not args.no_batch:
         out["batch"] = batch_run(esc)
@@ -337,19 +337,23 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def n32_run(esc, config=None):
+def n32_run(esc, config=None, autotune=False):
     """H.psi at N=32 (4x4 cells, K=-1, h[111]=0.3; 2^32 rows, complex64, 34.4 GB
-    per vector): the autotuned configuration (or `config`), 5 timed
-    applications."""
+    per vector): the library's default configuration (13831318, the one the
+    autotuner picks on this machine; `config` or `autotune` to override),
+    5 timed applications."""
     import torch
     lat = esc.Honeycomb(4, 8)
     model = esc.SpinModel(lat, K=(-1.0, -1.0, -1.0), B=esc.field111(0.3))
     eng = esc.Engine(model, "c64")
     x = eng.start_vector(1234, "Gaussian")
     y = eng.empty()
-    if config is None:
+    if autotune and config is None:
         config, _ = eng.autotune(x, y, reps=1)
     else:
+        if config is None:
+            from energyspincurrents_amd.engine import default_kernel_config
+            config = default_kernel_config(32, "c64")
         eng.set_tile_bits(config)
     ms, _ = time_hpsi(eng, x, y, 5, 1)
     bytes_alg = 2 * eng.dim * eng.elem_bytes
